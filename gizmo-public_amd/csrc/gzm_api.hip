@@ -1,0 +1,843 @@
+// gzm_api.hip -- C ABI of libgzm.so: context, device memory, grid build and
+// the three stage drivers.  See include/gzm.h for the contract.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <cstring>
+
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
+#include <rocprim/device/device_select.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/gzm.h"
+#include "gzm_internal.h"
+
+using namespace gzm;
+
+namespace {
+
+thread_local char g_last_error[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+}
+
+#define GZM_HIP(call)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return GZM_ERR_HIP;                                                                 \
+        }                                                                                       \
+    } while (0)
+
+enum TimerKind { kDensity = 0, kGradients = 1, kForce = 2, kOther = 3 };
+
+struct TimedLaunch {
+    hipEvent_t a, b;
+    int kind;
+};
+
+}  // namespace
+
+struct gzm_ctx {
+    gzm_params p;
+    hipStream_t stream = nullptr;
+    int n = 0;
+    int ncell = 0;
+    Grid grid;
+    BoxInfo box;
+    GammaConsts G;
+    double hmax = 0;
+
+    // caller-order staging
+    double *d_stage = nullptr;
+    size_t stage_doubles = 0;
+    // sorting
+    uint32_t *d_keys = nullptr, *d_keys_sorted = nullptr;
+    int *d_iota = nullptr, *d_perm = nullptr, *d_inv = nullptr;
+    void *d_temp = nullptr;
+    size_t temp_bytes = 0;
+    int2 *d_range = nullptr;
+    // sorted particle state
+    double4 *d_posm = nullptr, *d_velu = nullptr;
+    double *d_hsml = nullptr, *d_hsml0 = nullptr, *d_left = nullptr, *d_right = nullptr;
+    double *d_cn = nullptr, *d_cn0 = nullptr, *d_vsig = nullptr, *d_vsig0 = nullptr;
+    double *d_num_ngb = nullptr, *d_dhsml = nullptr, *d_divvel = nullptr, *d_rho = nullptr, *d_pres = nullptr;
+    double *d_nvt = nullptr, *d_fce = nullptr, *d_grad = nullptr, *d_acc = nullptr, *d_dtu = nullptr;
+    uint8_t *d_wakeup = nullptr, *d_is_active = nullptr;
+    int *d_active_caller = nullptr, *d_active = nullptr, *d_live = nullptr, *d_live2 = nullptr, *d_flags = nullptr;
+    int *d_count = nullptr;
+    double *d_hmax = nullptr;
+    GradRec *d_grec = nullptr;
+    ForceRec *d_frec = nullptr;
+    Counters *d_ctr = nullptr;
+    int n_active = 0;
+    bool all_active = true;
+    bool have_density = false, have_gradients = false;
+
+    gzm_timing timing;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+template <class T>
+gzm_status dev_alloc(T **p, size_t count)
+{
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        *p = nullptr;
+        return GZM_ERR_NOMEM;
+    }
+    return GZM_OK;
+}
+
+template <class T>
+void dev_free(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+hipEvent_t take_event(gzm_ctx *c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Bracket a launch with events on the context stream.
+struct Timed {
+    gzm_ctx *c;
+    TimedLaunch t;
+    Timed(gzm_ctx *ctx, int kind) : c(ctx)
+    {
+        t.kind = kind;
+        t.a = take_event(c);
+        t.b = take_event(c);
+        if (t.a) (void)hipEventRecord(t.a, c->stream);
+    }
+    ~Timed()
+    {
+        if (t.b) (void)hipEventRecord(t.b, c->stream);
+        c->pending.push_back(t);
+    }
+};
+
+// Called after a stream synchronize: fold pending event pairs into the totals.
+void collect_timings(gzm_ctx *c)
+{
+    for (auto &t : c->pending) {
+        float ms = 0;
+        if (t.a && t.b && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            switch (t.kind) {
+            case kDensity: c->timing.density_ms += ms; c->timing.density_launches++; break;
+            case kGradients: c->timing.gradients_ms += ms; c->timing.gradients_launches++; break;
+            case kForce: c->timing.force_ms += ms; c->timing.force_launches++; break;
+            default: c->timing.other_ms += ms; c->timing.other_launches++; break;
+            }
+        }
+        if (t.a) c->event_pool.push_back(t.a);
+        if (t.b) c->event_pool.push_back(t.b);
+    }
+    c->pending.clear();
+}
+
+gzm_status ensure_temp(gzm_ctx *c, size_t bytes)
+{
+    if (bytes <= c->temp_bytes) return GZM_OK;
+    dev_free(c->d_temp);
+    c->temp_bytes = 0;
+    gzm_status s = dev_alloc((char **)&c->d_temp, bytes);
+    if (s == GZM_OK) c->temp_bytes = bytes;
+    return s;
+}
+
+gzm_status ensure_stage(gzm_ctx *c, size_t doubles)
+{
+    if (doubles <= c->stage_doubles) return GZM_OK;
+    dev_free(c->d_stage);
+    c->stage_doubles = 0;
+    gzm_status s = dev_alloc(&c->d_stage, doubles);
+    if (s == GZM_OK) c->stage_doubles = doubles;
+    return s;
+}
+
+void free_particles(gzm_ctx *c)
+{
+    dev_free(c->d_keys); dev_free(c->d_keys_sorted); dev_free(c->d_iota); dev_free(c->d_perm); dev_free(c->d_inv);
+    dev_free(c->d_range); dev_free(c->d_posm); dev_free(c->d_velu);
+    dev_free(c->d_hsml); dev_free(c->d_hsml0); dev_free(c->d_left); dev_free(c->d_right);
+    dev_free(c->d_cn); dev_free(c->d_cn0); dev_free(c->d_vsig); dev_free(c->d_vsig0);
+    dev_free(c->d_num_ngb); dev_free(c->d_dhsml); dev_free(c->d_divvel); dev_free(c->d_rho); dev_free(c->d_pres);
+    dev_free(c->d_nvt); dev_free(c->d_fce); dev_free(c->d_grad); dev_free(c->d_acc); dev_free(c->d_dtu);
+    dev_free(c->d_wakeup); dev_free(c->d_is_active);
+    dev_free(c->d_active_caller); dev_free(c->d_active); dev_free(c->d_live); dev_free(c->d_live2);
+    dev_free(c->d_flags); dev_free(c->d_grec); dev_free(c->d_frec);
+    c->n = 0;
+}
+
+// host array (caller order, `width` doubles per particle) -> sorted device array
+gzm_status upload_sorted(gzm_ctx *c, const double *host, double *dst, int width)
+{
+    size_t cnt = (size_t)c->n * width;
+    gzm_status s = ensure_stage(c, cnt);
+    if (s) return s;
+    GZM_HIP(hipMemcpyAsync(c->d_stage, host, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    launch_gather_scalar(c->stream, c->n, c->d_perm, c->d_stage, dst, width);
+    GZM_HIP(hipGetLastError());
+    return GZM_OK;
+}
+
+// sorted device array -> host array in caller order
+gzm_status download_sorted(gzm_ctx *c, const double *src, double *host, int width)
+{
+    size_t cnt = (size_t)c->n * width;
+    gzm_status s = ensure_stage(c, cnt);
+    if (s) return s;
+    launch_scatter_scalar(c->stream, c->n, c->d_perm, src, c->d_stage, width);
+    GZM_HIP(hipGetLastError());
+    GZM_HIP(hipMemcpyAsync(host, c->d_stage, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GZM_HIP(hipStreamSynchronize(c->stream));
+    return GZM_OK;
+}
+
+gzm_status reset_counters(gzm_ctx *c)
+{
+    Counters z;
+    memset(&z, 0, sizeof(z));
+    GZM_HIP(hipMemcpyAsync(c->d_ctr, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+    return GZM_OK;
+}
+
+gzm_status read_counters(gzm_ctx *c, Counters *h)
+{
+    GZM_HIP(hipMemcpyAsync(h, c->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    GZM_HIP(hipStreamSynchronize(c->stream));
+    return GZM_OK;
+}
+
+// Map a caller active list to sorted slots, sorted ascending so that
+// consecutive threads hold spatially close particles.
+gzm_status set_active(gzm_ctx *c, const int32_t *active, int64_t n_active)
+{
+    if (active == nullptr) {
+        if (!c->all_active || c->n_active != c->n) {
+            GZM_HIP(hipMemcpyAsync(c->d_active, c->d_iota, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+        }
+        c->n_active = c->n;
+        c->all_active = true;
+        return GZM_OK;
+    }
+    if (n_active < 0 || n_active > c->n) {
+        set_error("n_active=%lld out of range [0, %d]", (long long)n_active, c->n);
+        return GZM_ERR_ARG;
+    }
+    // validate indices on the host before any kernel uses them
+    for (int64_t k = 0; k < n_active; k++) {
+        if (active[k] < 0 || active[k] >= c->n) {
+            set_error("active[%lld]=%d out of range", (long long)k, active[k]);
+            return GZM_ERR_ARG;
+        }
+    }
+    GZM_HIP(hipMemcpyAsync(c->d_active_caller, active, sizeof(int) * n_active, hipMemcpyHostToDevice, c->stream));
+    {
+        Timed tm(c, kOther);
+        launch_map_active(c->stream, (int)n_active, c->d_active_caller, c->d_inv, c->d_live);
+        size_t bytes = 0;
+        GZM_HIP(rocprim::radix_sort_keys(nullptr, bytes, c->d_live, c->d_active, (int)n_active, 0, 32, c->stream));
+        gzm_status s = ensure_temp(c, bytes);
+        if (s) return s;
+        GZM_HIP(rocprim::radix_sort_keys(c->d_temp, bytes, c->d_live, c->d_active, (int)n_active, 0, 32,
+                                         c->stream));
+    }
+    c->n_active = (int)n_active;
+    c->all_active = false;
+    return GZM_OK;
+}
+
+gzm_status compute_hmax(gzm_ctx *c)
+{
+    size_t bytes = 0;
+    GZM_HIP(rocprim::reduce(nullptr, bytes, c->d_hsml, c->d_hmax, 0.0, (size_t)c->n, rocprim::maximum<double>(),
+                            c->stream));
+    gzm_status s = ensure_temp(c, bytes);
+    if (s) return s;
+    GZM_HIP(rocprim::reduce(c->d_temp, bytes, c->d_hsml, c->d_hmax, 0.0, (size_t)c->n, rocprim::maximum<double>(),
+                            c->stream));
+    GZM_HIP(hipMemcpyAsync(&c->hmax, c->d_hmax, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GZM_HIP(hipStreamSynchronize(c->stream));
+    return GZM_OK;
+}
+
+// Ordered stream compaction: out = in[flags != 0], count to host.
+gzm_status compact(gzm_ctx *c, const int *in, const int *flags, int n, int *out, int *count)
+{
+    size_t bytes = 0;
+    GZM_HIP(rocprim::select(nullptr, bytes, in, flags, out, c->d_count, (size_t)n, c->stream));
+    gzm_status s = ensure_temp(c, bytes);
+    if (s) return s;
+    GZM_HIP(rocprim::select(c->d_temp, bytes, in, flags, out, c->d_count, (size_t)n, c->stream));
+    GZM_HIP(hipMemcpyAsync(count, c->d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GZM_HIP(hipStreamSynchronize(c->stream));
+    return GZM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gzm_status_string(int status)
+{
+    switch (status) {
+    case GZM_OK: return "ok";
+    case GZM_ERR_ARG: return "invalid argument";
+    case GZM_ERR_HIP: return "HIP runtime error";
+    case GZM_ERR_NOMEM: return "device allocation failed";
+    case GZM_ERR_STATE: return "call out of order";
+    case GZM_ERR_NO_DEVICE: return "no usable GPU";
+    case GZM_ERR_BRACKET: return "hsml bracket collapsed to zero (density.c:780-783)";
+    case GZM_ERR_HSML_NOT_CONVERGED: return "hsml iteration did not converge (endrun 1155)";
+    case GZM_ERR_RIEMANN: return "no positive Riemann pressure (endrun 1234)";
+    case GZM_ERR_RIEMANN_NEG_DENSITY: return "negative density into the exact Riemann solver (endrun 1234545)";
+    default: return "unknown status";
+    }
+}
+
+const char *gzm_last_error(void) { return g_last_error; }
+
+gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
+{
+    if (!out || !params) { set_error("NULL argument"); return GZM_ERR_ARG; }
+    *out = nullptr;
+    for (int a = 0; a < 3; a++) {
+        if (!(params->box[a] > 0)) { set_error("box[%d] must be > 0", a); return GZM_ERR_ARG; }
+    }
+    if (!(params->gamma > 1)) { set_error("gamma must be > 1"); return GZM_ERR_ARG; }
+    if (params->precision != 0) { set_error("only precision 0 (plain FP64) is available"); return GZM_ERR_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { set_error("no HIP device"); return GZM_ERR_NO_DEVICE; }
+    if (params->device < 0 || params->device >= ndev) { set_error("device %d not present", params->device); return GZM_ERR_ARG; }
+    GZM_HIP(hipSetDevice(params->device));
+    gzm_ctx *c = new gzm_ctx();
+    c->p = *params;
+    memset(&c->timing, 0, sizeof(c->timing));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        delete c;
+        return GZM_ERR_HIP;
+    }
+    double g = params->gamma;
+    c->G.G0 = g;
+    c->G.G1 = ((g - 1.0) / (2.0 * g));
+    c->G.G2 = ((g + 1.0) / (2.0 * g));
+    c->G.G3 = ((2.0 * g / (g - 1.0)));
+    c->G.G4 = (2.0 / (g - 1.0));
+    c->G.G5 = (2.0 / (g + 1.0));
+    c->G.G6 = ((g - 1.0) / (g + 1.0));
+    c->G.G7 = (0.5 * (g - 1.0));
+    c->G.G8 = (1.0 / g);
+    c->G.G9 = (g - 1.0);
+    for (int a = 0; a < 3; a++) {
+        c->box.L[a] = params->box[a];
+        c->box.half[a] = 0.5 * params->box[a];
+    }
+    if (dev_alloc(&c->d_ctr, 1) || dev_alloc(&c->d_count, 1) || dev_alloc(&c->d_hmax, 1)) {
+        gzm_ctx_destroy(c);
+        return GZM_ERR_NOMEM;
+    }
+    *out = c;
+    return GZM_OK;
+}
+
+gzm_status gzm_ctx_destroy(gzm_ctx *c)
+{
+    if (!c) return GZM_OK;
+    (void)hipSetDevice(c->p.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    collect_timings(c);
+    free_particles(c);
+    dev_free(c->d_stage);
+    dev_free(c->d_temp);
+    dev_free(c->d_ctr);
+    dev_free(c->d_count);
+    dev_free(c->d_hmax);
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GZM_OK;
+}
+
+gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
+{
+    if (!c || !v) { set_error("NULL argument"); return GZM_ERR_ARG; }
+    if (v->n <= 0 || v->n > (int64_t)0x7fffffff / 16) { set_error("n=%lld out of range", (long long)v->n); return GZM_ERR_ARG; }
+    if (!v->pos || !v->vel || !v->mass || !v->u || !v->hsml) {
+        set_error("pos, vel, mass, u and hsml are required");
+        return GZM_ERR_ARG;
+    }
+    GZM_HIP(hipSetDevice(c->p.device));
+    const int n = (int)v->n;
+    // positions must lie inside the box: the grid maps [0, L) to cells
+    for (int64_t i = 0; i < v->n; i++) {
+        for (int a = 0; a < 3; a++) {
+            double x = v->pos[3 * i + a];
+            if (!(x >= 0.0 && x < c->p.box[a])) {
+                set_error("pos[%lld][%d]=%g outside [0, %g)", (long long)i, a, x, c->p.box[a]);
+                return GZM_ERR_ARG;
+            }
+        }
+        if (!(v->hsml[i] >= 0) && !isnan(v->hsml[i])) { /* negative hsml is clamped by density() */ }
+    }
+    if (n != c->n) {
+        free_particles(c);
+        c->n = n;
+        gzm_status s = GZM_OK;
+        s = s ? s : dev_alloc(&c->d_keys, n);
+        s = s ? s : dev_alloc(&c->d_keys_sorted, n);
+        s = s ? s : dev_alloc(&c->d_iota, n);
+        s = s ? s : dev_alloc(&c->d_perm, n);
+        s = s ? s : dev_alloc(&c->d_inv, n);
+        s = s ? s : dev_alloc(&c->d_posm, n);
+        s = s ? s : dev_alloc(&c->d_velu, n);
+        double **scal[] = {&c->d_hsml, &c->d_hsml0, &c->d_left, &c->d_right, &c->d_cn, &c->d_cn0, &c->d_vsig,
+                           &c->d_vsig0, &c->d_num_ngb, &c->d_dhsml, &c->d_divvel, &c->d_rho, &c->d_pres, &c->d_fce,
+                           &c->d_dtu};
+        for (auto pp : scal) s = s ? s : dev_alloc(pp, n);
+        s = s ? s : dev_alloc(&c->d_nvt, (size_t)n * 6);
+        s = s ? s : dev_alloc(&c->d_grad, (size_t)n * 15);
+        s = s ? s : dev_alloc(&c->d_acc, (size_t)n * 3);
+        s = s ? s : dev_alloc(&c->d_wakeup, n);
+        s = s ? s : dev_alloc(&c->d_is_active, n);
+        s = s ? s : dev_alloc(&c->d_active_caller, n);
+        s = s ? s : dev_alloc(&c->d_active, n);
+        s = s ? s : dev_alloc(&c->d_live, n);
+        s = s ? s : dev_alloc(&c->d_live2, n);
+        s = s ? s : dev_alloc(&c->d_flags, n);
+        s = s ? s : dev_alloc(&c->d_grec, n);
+        s = s ? s : dev_alloc(&c->d_frec, n);
+        if (s) { free_particles(c); return s; }
+    }
+
+    // grid: cell side from the parameter or ~ the mean initial hsml
+    double s_cell = c->p.cell_size;
+    if (!(s_cell > 0)) {
+        double hsum = 0;
+        int64_t cnt = 0;
+        for (int64_t i = 0; i < v->n; i++) {
+            double h = v->hsml[i];
+            if (h > 0 && isfinite(h)) { hsum += h; cnt++; }
+        }
+        double vol = c->p.box[0] * c->p.box[1] * c->p.box[2];
+        s_cell = cnt ? hsum / cnt : cbrt(vol * 8.0 / n);
+    }
+    int64_t ncell = 1;
+    for (int a = 0; a < 3; a++) {
+        int nc = (int)(c->p.box[a] / s_cell);
+        nc = std::max(1, std::min(nc, 1024));
+        c->grid.nc[a] = nc;
+        c->grid.inv_s[a] = nc / c->p.box[a];
+        c->grid.s[a] = c->p.box[a] / nc;
+        ncell *= nc;
+    }
+    if (ncell > (int64_t)1 << 28) { set_error("grid of %lld cells too large", (long long)ncell); return GZM_ERR_ARG; }
+    if (ncell != c->ncell) {
+        dev_free(c->d_range);
+        gzm_status s = dev_alloc(&c->d_range, (size_t)ncell);
+        if (s) return s;
+        c->ncell = (int)ncell;
+    }
+    c->grid.range = c->d_range;
+
+    // inputs to staging (caller order), keys, stable sort by cell key
+    gzm_status s = ensure_stage(c, (size_t)n * 8);
+    if (s) return s;
+    double *st_pos = c->d_stage, *st_vel = c->d_stage + 3 * (size_t)n, *st_mass = c->d_stage + 6 * (size_t)n,
+           *st_u = c->d_stage + 7 * (size_t)n;
+    GZM_HIP(hipMemcpyAsync(st_pos, v->pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    GZM_HIP(hipMemcpyAsync(st_vel, v->vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    GZM_HIP(hipMemcpyAsync(st_mass, v->mass, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    GZM_HIP(hipMemcpyAsync(st_u, v->u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    {
+        Timed tm(c, kOther);
+        launch_cell_keys(c->stream, n, st_pos, c->grid, c->d_keys, c->d_iota);
+        size_t bytes = 0;
+        GZM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, c->d_keys, c->d_keys_sorted, c->d_iota, c->d_perm, n, 0, 30,
+                                          c->stream));
+        s = ensure_temp(c, bytes);
+        if (s) return s;
+        GZM_HIP(rocprim::radix_sort_pairs(c->d_temp, bytes, c->d_keys, c->d_keys_sorted, c->d_iota, c->d_perm, n, 0,
+                                          30, c->stream));
+        launch_gather_sorted(c->stream, n, c->d_perm, st_pos, st_vel, st_mass, st_u, c->d_posm, c->d_velu, c->d_inv);
+        launch_cell_ranges(c->stream, n, c->d_posm, c->grid, c->d_range, c->ncell);
+        GZM_HIP(hipGetLastError());
+    }
+    // d_iota was consumed as sort values; rebuild it as the identity for "all active"
+    std::vector<int> iota(n);
+    for (int i = 0; i < n; i++) iota[i] = i;
+    GZM_HIP(hipMemcpyAsync(c->d_iota, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+
+    // carried state
+    s = upload_sorted(c, v->hsml, c->d_hsml, 1);
+    if (s) return s;
+    if (v->cn) { s = upload_sorted(c, v->cn, c->d_cn, 1); if (s) return s; }
+    else launch_fill_double(c->stream, c->d_cn, n, 1.0);  // init.c:514
+    if (v->vsig) { s = upload_sorted(c, v->vsig, c->d_vsig, 1); if (s) return s; }
+    else launch_fill_double(c->stream, c->d_vsig, n, kMinReal);
+    GZM_HIP(hipStreamSynchronize(c->stream));
+    GZM_HIP(hipMemcpyAsync(c->d_hsml0, c->d_hsml, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    GZM_HIP(hipMemcpyAsync(c->d_cn0, c->d_cn, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    GZM_HIP(hipMemcpyAsync(c->d_vsig0, c->d_vsig, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+
+    // optional state of particles that may stay inactive
+    struct Opt { const double *h; double *d; int w; } opts[] = {
+        {v->num_ngb, c->d_num_ngb, 1}, {v->dhsml, c->d_dhsml, 1}, {v->divvel, c->d_divvel, 1},
+        {v->density, c->d_rho, 1}, {v->pressure, c->d_pres, 1}, {v->fce, c->d_fce, 1}};
+    for (auto &o : opts) {
+        if (o.h) { s = upload_sorted(c, o.h, o.d, o.w); if (s) return s; }
+        else launch_fill_double(c->stream, o.d, (long long)n * o.w, 0.0);
+    }
+    if (v->nvt) {
+        std::vector<double> t6((size_t)n * 6);
+        for (int i = 0; i < n; i++) {
+            const double *t = v->nvt + 9 * (size_t)i;
+            double *o = t6.data() + 6 * (size_t)i;
+            o[0] = t[0]; o[1] = t[1]; o[2] = t[2]; o[3] = t[4]; o[4] = t[5]; o[5] = t[8];
+        }
+        s = upload_sorted(c, t6.data(), c->d_nvt, 6);
+        if (s) return s;
+    } else launch_fill_double(c->stream, c->d_nvt, (long long)n * 6, 0.0);
+    if (v->grad_rho && v->grad_p && v->grad_v) {
+        std::vector<double> g((size_t)n * 15);
+        for (int i = 0; i < n; i++) {
+            double *o = g.data() + 15 * (size_t)i;
+            for (int k = 0; k < 3; k++) o[k] = v->grad_rho[3 * (size_t)i + k];
+            for (int k = 0; k < 3; k++) o[3 + k] = v->grad_p[3 * (size_t)i + k];
+            for (int k = 0; k < 9; k++) o[6 + k] = v->grad_v[9 * (size_t)i + k];
+        }
+        s = upload_sorted(c, g.data(), c->d_grad, 15);
+        if (s) return s;
+    } else launch_fill_double(c->stream, c->d_grad, (long long)n * 15, 0.0);
+    launch_fill_double(c->stream, c->d_acc, (long long)n * 3, 0.0);
+    launch_fill_double(c->stream, c->d_dtu, n, 0.0);
+    launch_fill_u8(c->stream, c->d_wakeup, n, 0);
+    GZM_HIP(hipGetLastError());
+    GZM_HIP(hipStreamSynchronize(c->stream));
+    collect_timings(c);
+    c->have_density = (v->density != nullptr && v->nvt != nullptr);
+    c->have_gradients = (v->grad_rho != nullptr);
+    c->all_active = false;
+    c->n_active = 0;
+    return GZM_OK;
+}
+
+gzm_status gzm_restore_initial_state(gzm_ctx *c)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    GZM_HIP(hipMemcpyAsync(c->d_hsml, c->d_hsml0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    GZM_HIP(hipMemcpyAsync(c->d_cn, c->d_cn0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    GZM_HIP(hipMemcpyAsync(c->d_vsig, c->d_vsig0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    return GZM_OK;
+}
+
+gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_pass_stats *stats)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    GZM_HIP(hipSetDevice(c->p.device));
+    gzm_status s = set_active(c, active, n_active);
+    if (s) return s;
+    s = reset_counters(c);
+    if (s) return s;
+    const int na = c->n_active;
+    int n_live = 0;
+    if (na > 0) {
+        {
+            Timed tm(c, kOther);
+            launch_density_init(c->stream, na, c->d_active, c->d_posm, c->d_hsml, c->d_left, c->d_right,
+                                c->p.max_hsml, c->d_flags);
+        }
+        s = compact(c, c->d_active, c->d_flags, na, c->d_live, &n_live);
+        if (s) return s;
+    }
+    DensitySweepArgs a;
+    a.grid = c->grid;
+    a.box = c->box;
+    a.hp.des_num_ngb = c->p.des_num_ngb;
+    a.hp.max_num_ngb_dev = c->p.max_num_ngb_dev;
+    a.hp.min_hsml = c->p.min_hsml;
+    a.hp.max_hsml = c->p.max_hsml;
+    a.hp.first_step = c->p.first_step;
+    a.posm = c->d_posm;
+    a.velu = c->d_velu;
+    a.hsml = c->d_hsml; a.left = c->d_left; a.right = c->d_right; a.cn = c->d_cn;
+    a.num_ngb = c->d_num_ngb; a.dhsml = c->d_dhsml; a.divvel = c->d_divvel; a.rho = c->d_rho;
+    a.fce = c->d_fce; a.nvt = c->d_nvt;
+    a.redo = c->d_flags;
+    a.ctr = c->d_ctr;
+    int iter = 0, sweeps = 0;
+    int status = GZM_OK;
+    while (n_live > 0) {
+        a.hp.iter = iter;
+        a.live = c->d_live;
+        a.n_live = n_live;
+        {
+            Timed tm(c, kDensity);
+            launch_density_sweep(c->stream, a);
+        }
+        GZM_HIP(hipGetLastError());
+        sweeps++;
+        int n_next = 0;
+        s = compact(c, c->d_live, c->d_flags, n_live, c->d_live2, &n_next);
+        if (s) return s;
+        std::swap(c->d_live, c->d_live2);
+        n_live = n_next;
+        if (n_live > 0) {
+            iter++;
+            if (iter > c->p.maxiter) { status = GZM_ERR_HSML_NOT_CONVERGED; break; }
+        }
+    }
+    if (na > 0) {
+        Timed tm(c, kOther);
+        launch_density_finalize(c->stream, na, c->d_active, c->d_posm, c->d_velu, c->d_hsml, c->d_num_ngb,
+                                c->d_rho, c->d_pres, c->G.G9);
+    }
+    GZM_HIP(hipGetLastError());
+    Counters h;
+    s = read_counters(c, &h);
+    if (s) return s;
+    collect_timings(c);
+    if (h.status == 6 && status == GZM_OK) status = GZM_ERR_BRACKET;
+    if (stats) {
+        stats->density_sweeps = sweeps;
+        stats->density_interactions = (int64_t)h.interactions;
+        stats->status = status;
+    }
+    c->have_density = true;
+    c->have_gradients = false;
+    if (status != GZM_OK) set_error("%s", gzm_status_string(status));
+    return (gzm_status)status;
+}
+
+gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_pass_stats *stats)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    if (!c->have_density) { set_error("gradients need density results (run gzm_density or upload them)"); return GZM_ERR_STATE; }
+    GZM_HIP(hipSetDevice(c->p.device));
+    gzm_status s = set_active(c, active, n_active);
+    if (s) return s;
+    s = reset_counters(c);
+    if (s) return s;
+    s = compute_hmax(c);
+    if (s) return s;
+    {
+        Timed tm(c, kOther);
+        launch_pack_grad(c->stream, c->n, c->d_posm, c->d_velu, c->d_hsml, c->d_rho, c->d_pres, c->d_grec);
+    }
+    GradientArgs a;
+    a.grid = c->grid;
+    a.box = c->box;
+    a.active = c->d_active;
+    a.n_active = c->n_active;
+    a.hmax = c->hmax;
+    a.rec = c->d_grec;
+    a.nvt = c->d_nvt;
+    a.cn = c->d_cn;
+    a.grad = c->d_grad;
+    a.ctr = c->d_ctr;
+    if (c->n_active > 0) {
+        Timed tm(c, kGradients);
+        launch_gradients(c->stream, a);
+    }
+    GZM_HIP(hipGetLastError());
+    Counters h;
+    s = read_counters(c, &h);
+    if (s) return s;
+    collect_timings(c);
+    if (stats) stats->gradient_interactions = (int64_t)h.interactions;
+    c->have_gradients = true;
+    return GZM_OK;
+}
+
+gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_pass_stats *stats)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    if (!c->have_density) { set_error("force needs density results"); return GZM_ERR_STATE; }
+    GZM_HIP(hipSetDevice(c->p.device));
+    gzm_status s = set_active(c, active, n_active);
+    if (s) return s;
+    s = reset_counters(c);
+    if (s) return s;
+    s = compute_hmax(c);
+    if (s) return s;
+    {
+        Timed tm(c, kOther);
+        launch_pack_force(c->stream, c->n, c->d_posm, c->d_velu, c->d_hsml, c->d_rho, c->d_pres, c->d_dhsml,
+                          c->d_cn, c->d_fce, c->d_nvt, c->d_grad, c->p.gamma, c->d_frec);
+        if (!c->all_active) {
+            launch_fill_u8(c->stream, c->d_is_active, c->n, 0);
+            launch_set_flags(c->stream, c->n_active, c->d_active, c->d_is_active);
+            launch_fill_u8(c->stream, c->d_wakeup, c->n, 0);
+        }
+    }
+    ForceArgs a;
+    a.grid = c->grid;
+    a.box = c->box;
+    a.G = c->G;
+    a.active = c->d_active;
+    a.n_active = c->n_active;
+    a.hmax = c->hmax;
+    a.max_hsml = c->p.max_hsml;
+    a.rec = c->d_frec;
+    a.is_active = c->all_active ? nullptr : c->d_is_active;
+    a.vsig_old = c->d_vsig;
+    a.acc = c->d_acc;
+    a.dtu = c->d_dtu;
+    a.vsig = c->d_vsig;
+    a.wakeup = c->d_wakeup;
+    a.ctr = c->d_ctr;
+    if (c->n_active > 0) {
+        Timed tm(c, kForce);
+        launch_force(c->stream, a);
+    }
+    GZM_HIP(hipGetLastError());
+    Counters h;
+    s = read_counters(c, &h);
+    if (s) return s;
+    collect_timings(c);
+    if (stats) {
+        stats->force_interactions = (int64_t)h.interactions;
+        stats->riemann_retries = (int64_t)h.retries;
+        stats->exact_solver_calls = (int64_t)h.exact_calls;
+        stats->status = h.status;
+    }
+    if (h.status != 0) {
+        set_error("%s", gzm_status_string(h.status));
+        return (gzm_status)h.status;
+    }
+    return GZM_OK;
+}
+
+gzm_status gzm_download(gzm_ctx *c, gzm_host_view *v)
+{
+    if (!c || !v) { set_error("NULL argument"); return GZM_ERR_ARG; }
+    if (c->n == 0 || v->n != c->n) { set_error("view size %lld does not match uploaded %d", (long long)v->n, c->n); return GZM_ERR_ARG; }
+    GZM_HIP(hipSetDevice(c->p.device));
+    GZM_HIP(hipStreamSynchronize(c->stream));
+    const int n = c->n;
+    struct Out { double *h; const double *d; int w; } outs[] = {
+        {v->hsml, c->d_hsml, 1}, {v->cn, c->d_cn, 1}, {v->vsig, c->d_vsig, 1}, {v->num_ngb, c->d_num_ngb, 1},
+        {v->dhsml, c->d_dhsml, 1}, {v->divvel, c->d_divvel, 1}, {v->density, c->d_rho, 1},
+        {v->pressure, c->d_pres, 1}, {v->fce, c->d_fce, 1}, {v->acc, c->d_acc, 3}, {v->dtu, c->d_dtu, 1}};
+    for (auto &o : outs) {
+        if (!o.h) continue;
+        gzm_status s = download_sorted(c, o.d, o.h, o.w);
+        if (s) return s;
+    }
+    if (v->nvt) {
+        std::vector<double> t6((size_t)n * 6);
+        gzm_status s = download_sorted(c, c->d_nvt, t6.data(), 6);
+        if (s) return s;
+        for (int i = 0; i < n; i++) {
+            const double *t = t6.data() + 6 * (size_t)i;
+            double *o = v->nvt + 9 * (size_t)i;
+            o[0] = t[0]; o[1] = t[1]; o[2] = t[2];
+            o[3] = t[1]; o[4] = t[3]; o[5] = t[4];
+            o[6] = t[2]; o[7] = t[4]; o[8] = t[5];
+        }
+    }
+    if (v->grad_rho || v->grad_p || v->grad_v) {
+        std::vector<double> g((size_t)n * 15);
+        gzm_status s = download_sorted(c, c->d_grad, g.data(), 15);
+        if (s) return s;
+        for (int i = 0; i < n; i++) {
+            const double *o = g.data() + 15 * (size_t)i;
+            if (v->grad_rho) for (int k = 0; k < 3; k++) v->grad_rho[3 * (size_t)i + k] = o[k];
+            if (v->grad_p) for (int k = 0; k < 3; k++) v->grad_p[3 * (size_t)i + k] = o[3 + k];
+            if (v->grad_v) for (int k = 0; k < 9; k++) v->grad_v[9 * (size_t)i + k] = o[6 + k];
+        }
+    }
+    if (v->wakeup) {
+        std::vector<uint8_t> w(n);
+        GZM_HIP(hipMemcpyAsync(w.data(), c->d_wakeup, n, hipMemcpyDeviceToHost, c->stream));
+        std::vector<int> perm(n);
+        GZM_HIP(hipMemcpyAsync(perm.data(), c->d_perm, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+        GZM_HIP(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < n; k++) v->wakeup[perm[k]] = w[k];
+    }
+    return GZM_OK;
+}
+
+gzm_status gzm_neighbor_counts(gzm_ctx *c, int32_t mode, const int32_t *active, int64_t n_active, int32_t *counts,
+                               int64_t *total)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    if (mode != 0 && mode != 1) { set_error("mode must be 0 or 1"); return GZM_ERR_ARG; }
+    GZM_HIP(hipSetDevice(c->p.device));
+    // counts come back in the order of `active`, so map without re-sorting
+    int na;
+    if (active == nullptr) {
+        na = c->n;
+        GZM_HIP(hipMemcpyAsync(c->d_live, c->d_inv, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        if (n_active < 0 || n_active > c->n) { set_error("n_active out of range"); return GZM_ERR_ARG; }
+        for (int64_t k = 0; k < n_active; k++)
+            if (active[k] < 0 || active[k] >= c->n) { set_error("active[%lld] out of range", (long long)k); return GZM_ERR_ARG; }
+        na = (int)n_active;
+        GZM_HIP(hipMemcpyAsync(c->d_active_caller, active, sizeof(int) * na, hipMemcpyHostToDevice, c->stream));
+        launch_map_active(c->stream, na, c->d_active_caller, c->d_inv, c->d_live);
+    }
+    gzm_status s = reset_counters(c);
+    if (s) return s;
+    s = compute_hmax(c);
+    if (s) return s;
+    launch_neighbor_counts(c->stream, c->grid, c->box, mode, na, c->d_live, c->d_posm, c->d_hsml, c->hmax,
+                           counts ? c->d_flags : nullptr, c->d_ctr);
+    GZM_HIP(hipGetLastError());
+    if (counts) GZM_HIP(hipMemcpyAsync(counts, c->d_flags, sizeof(int) * na, hipMemcpyDeviceToHost, c->stream));
+    Counters h;
+    s = read_counters(c, &h);
+    if (s) return s;
+    if (total) *total = (int64_t)h.interactions;
+    c->all_active = false;  // d_live was used as scratch; the active list is rebuilt on the next stage call
+    c->n_active = 0;
+    return GZM_OK;
+}
+
+gzm_status gzm_timing_get(gzm_ctx *c, gzm_timing *t)
+{
+    if (!c || !t) { set_error("NULL argument"); return GZM_ERR_ARG; }
+    (void)hipStreamSynchronize(c->stream);
+    collect_timings(c);
+    *t = c->timing;
+    return GZM_OK;
+}
+
+gzm_status gzm_timing_reset(gzm_ctx *c)
+{
+    if (!c) { set_error("NULL argument"); return GZM_ERR_ARG; }
+    (void)hipStreamSynchronize(c->stream);
+    collect_timings(c);
+    memset(&c->timing, 0, sizeof(c->timing));
+    return GZM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// gzm_internal.h -- device data layout and kernel launch interface shared by
+// gzm_kernels.hip (kernels) and gzm_api.hip (context, memory, C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mfm_device.h"
+
+namespace gzm {
+
+// Cell grid over the periodic box.  Particles are stored sorted by the Morton
+// key of their cell, so each cell is one contiguous slot range; `range` is
+// indexed by the linear cell id (cx*ny + cy)*nz + cz.
+struct Grid {
+    int nc[3];
+    double inv_s[3];  // cells per unit length
+    double s[3];      // cell side
+    const int2 *range;
+};
+
+// Per-particle record read by the force loop for the neighbour j (and for i).
+// 320 bytes: one aligned block so a gather touches 2.5 cache lines.
+struct alignas(64) ForceRec {
+    double4 xh;    // x, y, z, hsml
+    double4 vc;    // vx, vy, vz, sound speed sqrt(gamma P / rho)
+    double4 th;    // rho, P, V = m / rho, DhsmlNgbFactor
+    double4 mc;    // m, ConditionNumber, FaceClosureError, 0
+    double4 t0;    // NV_T: i00 i01 i02 i11
+    double4 t1;    // NV_T: i12 i22 0 0
+    double4 g0;    // grad rho (3), grad P x
+    double4 g1;    // grad P y, z, grad v row 0 (x, y)
+    double4 g2;    // grad v row 0 z, row 1 (x, y, z)
+    double4 g3;    // grad v row 2 (x, y, z), 0
+};
+
+// Per-particle record read by the gradient loop for the neighbour j.
+struct alignas(32) GradRec {
+    double4 xh;    // x, y, z, hsml
+    double4 vr;    // vx, vy, vz, rho
+    double2 pm;    // P, m
+};
+
+// Counters accumulated by the kernels (device memory, zeroed per stage).
+struct Counters {
+    unsigned long long interactions;
+    unsigned long long retries;
+    unsigned long long exact_calls;
+    int status;
+    int n_next;
+};
+
+struct DensitySweepArgs {
+    Grid grid;
+    BoxInfo box;
+    HsmlParams hp;
+    const int *live;
+    int n_live;
+    const double4 *posm;   // x, y, z, m
+    const double4 *velu;   // vx, vy, vz, u
+    double *hsml, *left, *right, *cn;
+    double *num_ngb, *dhsml, *divvel, *rho, *fce, *nvt;  // nvt: 6 per particle
+    int *redo;             // per live entry: 1 = sweep again
+    Counters *ctr;
+};
+
+struct GradientArgs {
+    Grid grid;
+    BoxInfo box;
+    const int *active;
+    int n_active;
+    double hmax;
+    const GradRec *rec;
+    const double *nvt;     // 6 per particle
+    const double *cn;
+    double *grad;          // 15 per particle: rho(3), P(3), v(3x3 row-major)
+    Counters *ctr;
+};
+
+struct ForceArgs {
+    Grid grid;
+    BoxInfo box;
+    GammaConsts G;
+    const int *active;
+    int n_active;
+    double hmax;
+    double max_hsml;
+    const ForceRec *rec;
+    const uint8_t *is_active;  // NULL when every particle is active (no wakeup test)
+    const double *vsig_old;    // MaxSignalVel of inactive particles
+    double *acc;               // 3 per particle
+    double *dtu;
+    double *vsig;
+    uint8_t *wakeup;
+    Counters *ctr;
+};
+
+// launchers (gzm_kernels.hip); all asynchronous on `stream`
+void launch_cell_keys(hipStream_t s, int n, const double *pos, Grid g, uint32_t *keys, int *iota);
+void launch_gather_sorted(hipStream_t s, int n, const int *perm, const double *pos, const double *vel,
+                          const double *mass, const double *u, double4 *posm, double4 *velu, int *inv);
+void launch_gather_scalar(hipStream_t s, int n, const int *perm, const double *src, double *dst, int width);
+void launch_scatter_scalar(hipStream_t s, int n, const int *perm, const double *src, double *dst, int width);
+void launch_cell_ranges(hipStream_t s, int n, const double4 *posm, Grid g, int2 *range, int ncell);
+void launch_map_active(hipStream_t s, int n_active, const int *active_caller, const int *inv, int *active_sorted);
+void launch_density_init(hipStream_t s, int n_active, const int *active, const double4 *posm, double *hsml,
+                         double *left, double *right, double max_hsml, int *live_flag);
+void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a);
+void launch_density_finalize(hipStream_t s, int n_active, const int *active, const double4 *posm,
+                             const double4 *velu, const double *hsml, double *num_ngb, double *rho,
+                             double *pres, double gamma_minus_1);
+void launch_pack_grad(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
+                      const double *rho, const double *pres, GradRec *rec);
+void launch_gradients(hipStream_t s, const GradientArgs &a);
+void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
+                       const double *rho, const double *pres, const double *dhsml, const double *cn,
+                       const double *fce, const double *nvt, const double *grad, double gamma, ForceRec *rec);
+void launch_force(hipStream_t s, const ForceArgs &a);
+void launch_neighbor_counts(hipStream_t s, Grid g, BoxInfo box, int mode, int n_active, const int *active,
+                            const double4 *posm, const double *hsml, double hmax, int *counts, Counters *ctr);
+void launch_fill_int(hipStream_t s, int *p, int n, int value);
+void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value);
+void launch_fill_double(hipStream_t s, double *p, long long n, double value);
+void launch_set_flags(hipStream_t s, int n_active, const int *active, uint8_t *flags);
+
+}  // namespace gzm

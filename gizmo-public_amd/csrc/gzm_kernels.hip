@@ -1,0 +1,813 @@
+// gzm_kernels.hip -- HIP kernels of the MFM hydro pass for gfx950 (FP64).
+//
+// One thread per active particle i ("gather form"): the thread walks the
+// cells of the grid that can hold neighbours, evaluates every pair (i, j) for
+// its own particle only, and finishes the per-particle work (hsml update,
+// gradient construction + limiter, flux sums -> dv/dt, du/dt) in registers.
+// Consecutive threads hold consecutive particles of the Morton-sorted order,
+// so the lanes of a wavefront walk almost the same cells and their neighbour
+// loads hit the same cache lines.
+//
+// No pair writes to j (the reference disables its pair-symmetric writes too,
+// hydro_evaluate.h:159-168, gradients.c:1554-1565), so every kernel is free of
+// races and deterministic for a fixed particle order.
+#include "gzm_internal.h"
+
+namespace gzm {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ int cell_coord(double x, double inv_s, int nc)
+{
+    int c = (int)floor(x * inv_s);
+    if (c < 0) c = 0;
+    if (c >= nc) c = nc - 1;
+    return c;
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton3(int cx, int cy, int cz)
+{
+    return (spread10((uint32_t)cx) << 2) | (spread10((uint32_t)cy) << 1) | spread10((uint32_t)cz);
+}
+
+__device__ __forceinline__ double axis_gap(double x, double c0, double s)
+{
+    double g = c0 - x;
+    double g2 = x - (c0 + s);
+    g = (g > g2) ? g : g2;
+    return (g > 0) ? g : 0.0;
+}
+
+// Visit every sorted slot k whose cell may hold a particle within distance R
+// of (x, y, z), periodic.  Cells whose box lies farther than R are skipped.
+template <typename F>
+__device__ __forceinline__ void for_each_candidate(const Grid &g, const double xs[3], double R, F &&f)
+{
+    int lo[3], hi[3];
+    bool full[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        // a radius that spans the axis (or is not finite) walks the whole axis once
+        full[a] = !(R * g.inv_s[a] < 0.5 * g.nc[a] - 2.0);
+        if (!full[a]) {
+            lo[a] = (int)floor((xs[a] - R) * g.inv_s[a] - 1e-6);
+            hi[a] = (int)floor((xs[a] + R) * g.inv_s[a] + 1e-6);
+        } else {
+            lo[a] = 0;
+            hi[a] = g.nc[a] - 1;
+        }
+    }
+    const double R2 = R * R * (1.0 + 1e-9);
+    for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+        const int wx = (cx < 0) ? cx + g.nc[0] : ((cx >= g.nc[0]) ? cx - g.nc[0] : cx);
+        const double gx = full[0] ? 0.0 : axis_gap(xs[0], cx * g.s[0], g.s[0]);
+        const double gx2 = gx * gx;
+        if (gx2 > R2) continue;
+        for (int cy = lo[1]; cy <= hi[1]; ++cy) {
+            const int wy = (cy < 0) ? cy + g.nc[1] : ((cy >= g.nc[1]) ? cy - g.nc[1] : cy);
+            const double gy = full[1] ? 0.0 : axis_gap(xs[1], cy * g.s[1], g.s[1]);
+            const double gxy2 = gx2 + gy * gy;
+            if (gxy2 > R2) continue;
+            const int rowbase = (wx * g.nc[1] + wy) * g.nc[2];
+            for (int cz = lo[2]; cz <= hi[2]; ++cz) {
+                const int wz = (cz < 0) ? cz + g.nc[2] : ((cz >= g.nc[2]) ? cz - g.nc[2] : cz);
+                const double gz = full[2] ? 0.0 : axis_gap(xs[2], cz * g.s[2], g.s[2]);
+                if (gxy2 + gz * gz > R2) continue;
+                const int2 r = g.range[rowbase + wz];
+                for (int k = r.x; k < r.y; ++k) f(k);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void wrap_dp(const BoxInfo &box, double dp[3])
+{
+    dp[1] = wrap_axis(dp[1], box.L[1], box.half[1]);
+    dp[0] = wrap_axis(dp[0], box.L[0], box.half[0]);
+    dp[2] = wrap_axis(dp[2], box.L[2], box.half[2]);
+}
+
+// wavefront-wide sum, then one atomic per wavefront
+__device__ __forceinline__ void wave_add_u64(unsigned long long *dst, unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+// ---------------------------------------------------------------------------
+// grid build
+// ---------------------------------------------------------------------------
+__global__ void cell_keys_kernel(int n, const double *pos, Grid g, uint32_t *keys, int *iota)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int cx = cell_coord(pos[3 * i + 0], g.inv_s[0], g.nc[0]);
+    int cy = cell_coord(pos[3 * i + 1], g.inv_s[1], g.nc[1]);
+    int cz = cell_coord(pos[3 * i + 2], g.inv_s[2], g.nc[2]);
+    keys[i] = morton3(cx, cy, cz);
+    iota[i] = i;
+}
+
+__global__ void gather_sorted_kernel(int n, const int *perm, const double *pos, const double *vel,
+                                     const double *mass, const double *u, double4 *posm, double4 *velu, int *inv)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = perm[k];
+    posm[k] = make_double4(pos[3 * i + 0], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
+    velu[k] = make_double4(vel[3 * i + 0], vel[3 * i + 1], vel[3 * i + 2], u[i]);
+    inv[i] = k;
+}
+
+__global__ void gather_scalar_kernel(int n, const int *perm, const double *src, double *dst, int width)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * width) return;
+    int k = (int)(t / width), c = (int)(t % width);
+    dst[t] = src[(long long)perm[k] * width + c];
+}
+
+__global__ void scatter_scalar_kernel(int n, const int *perm, const double *src, double *dst, int width)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * width) return;
+    int k = (int)(t / width), c = (int)(t % width);
+    dst[(long long)perm[k] * width + c] = src[t];
+}
+
+__global__ void cell_ranges_kernel(int n, const double4 *posm, Grid g, int2 *range)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double4 p = posm[k];
+    int c = (cell_coord(p.x, g.inv_s[0], g.nc[0]) * g.nc[1] + cell_coord(p.y, g.inv_s[1], g.nc[1])) * g.nc[2] +
+            cell_coord(p.z, g.inv_s[2], g.nc[2]);
+    int cprev = -1, cnext = -1;
+    if (k > 0) {
+        double4 q = posm[k - 1];
+        cprev = (cell_coord(q.x, g.inv_s[0], g.nc[0]) * g.nc[1] + cell_coord(q.y, g.inv_s[1], g.nc[1])) * g.nc[2] +
+                cell_coord(q.z, g.inv_s[2], g.nc[2]);
+    }
+    if (k + 1 < n) {
+        double4 q = posm[k + 1];
+        cnext = (cell_coord(q.x, g.inv_s[0], g.nc[0]) * g.nc[1] + cell_coord(q.y, g.inv_s[1], g.nc[1])) * g.nc[2] +
+                cell_coord(q.z, g.inv_s[2], g.nc[2]);
+    }
+    if (c != cprev) range[c].x = k;
+    if (c != cnext) range[c].y = k + 1;
+}
+
+__global__ void map_active_kernel(int n_active, const int *active_caller, const int *inv, int *active_sorted)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_active) return;
+    active_sorted[t] = inv[active_caller[t]];
+}
+
+__global__ void fill_int_kernel(int *p, int n, int v)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+__global__ void fill_u8_kernel(uint8_t *p, int n, uint8_t v)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+__global__ void fill_double_kernel(double *p, long long n, double v)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+__global__ void set_flags_kernel(int n_active, const int *active, uint8_t *flags)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_active) flags[active[t]] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// density (density.c:252-347 pair sums + 505-850 iteration, fused)
+// ---------------------------------------------------------------------------
+__global__ void density_init_kernel(int n_active, const int *active, const double4 *posm, double *hsml,
+                                    double *left, double *right, double max_hsml, int *live_flag)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_active) return;
+    int i = active[t];
+    int live = posm[i].w > 0;  // density_isactive: gas with mass > 0 (density.c:85)
+    live_flag[t] = live;
+    if (!live) return;
+    left[i] = 0;
+    right[i] = 0;
+    double h = hsml[i];
+    if ((h < 0) || !isfinite(h) || (h > 0.99 * max_hsml)) hsml[i] = 0.99 * max_hsml;  // density.c:484
+}
+
+__global__ __launch_bounds__(kBlock) void density_sweep_kernel(DensitySweepArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < a.n_live;
+    unsigned long long cnt = 0;
+    if (valid) {
+        const int i = a.live[t];
+        const double4 pi = a.posm[i];
+        const double4 vi = a.velu[i];
+        double h = a.hsml[i];
+        double hinv, hinv3, hinv4;
+        kernel_hinv(h, hinv, hinv3, hinv4);
+        const double h2 = h * h;
+        DensitySums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const double xs[3] = {pi.x, pi.y, pi.z};
+        for_each_candidate(a.grid, xs, h, [&](int j) {
+            const double4 pj = a.posm[j];
+            double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+            wrap_dp(a.box, dp);
+            const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+            if (r2 < h2 && pj.w > 0) {
+                cnt++;
+                const double r = sqrt(r2);
+                const double u = r * hinv;
+                double wk, dwk;
+                kernel_wdw(u, hinv3, hinv4, wk, dwk);
+                S.ngb += wk;
+                S.rho += pj.w * wk;
+                S.dhsml += -(3 * hinv * wk + u * dwk);
+                if (r > 0) {
+                    S.t00 += wk * dp[0] * dp[0];
+                    S.t01 += wk * dp[0] * dp[1];
+                    S.t02 += wk * dp[0] * dp[2];
+                    S.t11 += wk * dp[1] * dp[1];
+                    S.t12 += wk * dp[1] * dp[2];
+                    S.t22 += wk * dp[2] * dp[2];
+                    S.f0 += wk * dp[0];
+                    S.f1 += wk * dp[1];
+                    S.f2 += wk * dp[2];
+                    const double4 vj = a.velu[j];
+                    const double dv0 = vi.x - vj.x, dv1 = vi.y - vj.y, dv2 = vi.z - vj.z;
+                    S.divvel -= dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2) / r;
+                }
+            }
+        });
+        double left = a.left[i], right = a.right[i];
+        DensityResult R;
+        const int res = hsml_update(a.hp, S, h, left, right, a.cn[i], R);
+        a.hsml[i] = h;
+        a.left[i] = left;
+        a.right[i] = right;
+        a.cn[i] = R.cn_stored;
+        a.num_ngb[i] = R.num_ngb;
+        a.dhsml[i] = R.dhsml;
+        a.divvel[i] = R.divvel;
+        a.rho[i] = R.rho_sum;
+        a.fce[i] = R.fce;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a.nvt[6 * (size_t)i + k] = R.nvt[k];
+        a.redo[t] = (res == 1) ? 1 : 0;
+        if (res < 0) atomicMax(&a.ctr->status, (int)6);
+    }
+    wave_add_u64(&a.ctr->interactions, cnt);
+}
+
+__global__ void density_finalize_kernel(int n_active, const int *active, const double4 *posm, const double4 *velu,
+                                        const double *hsml, double *num_ngb, double *rho, double *pres, double gm1)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_active) return;
+    int i = active[t];
+    double m = posm[i].w;
+    if (!(m > 0)) return;
+    double h = hsml[i], nn = num_ngb[i], d;
+    // density.c:967-978 (MFM: own mass times the kernel number density)
+    if ((h > 0) && (nn > 0)) d = m * nn / (kNormCoeff * pow(h, 3.0));
+    else if (h <= 0) d = 0;
+    else d = m / (kNormCoeff * pow(h, 3.0));
+    rho[i] = d;
+    pres[i] = gm1 * velu[i].w * d;  // eos.c:58-61
+    num_ngb[i] = (nn > 0) ? pow(nn, 1. / 3.) : 0;  // density.c:1020
+}
+
+// ---------------------------------------------------------------------------
+// gradients (gradients.c:1477-2089 pair loop, 1052-1253 finalize)
+// ---------------------------------------------------------------------------
+__global__ void pack_grad_kernel(int n, const double4 *posm, const double4 *velu, const double *hsml,
+                                 const double *rho, const double *pres, GradRec *rec)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double4 p = posm[k], v = velu[k];
+    GradRec r;
+    r.xh = make_double4(p.x, p.y, p.z, hsml[k]);
+    r.vr = make_double4(v.x, v.y, v.z, rho[k]);
+    r.pm = make_double2(pres[k], p.w);
+    rec[k] = r;
+}
+
+__device__ __forceinline__ void minmax_check(double x, double &mn, double &mx)
+{
+    if (x < mn) mn = x;
+    else if (x > mx) mx = x;
+}
+
+__global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < a.n_active;
+    unsigned long long cnt = 0;
+    if (valid) {
+        const int i = a.active[t];
+        const GradRec ri = a.rec[i];
+        double gr[3] = {0, 0, 0}, gp[3] = {0, 0, 0}, gv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        double maxd = 0, mn_r = 0, mx_r = 0, mn_p = 0, mx_p = 0;
+        double mn_v[3] = {0, 0, 0}, mx_v[3] = {0, 0, 0};
+        const double h_i = ri.xh.w, rho_i = ri.vr.w, p_i = ri.pm.x, m_i = ri.pm.y;
+        if ((m_i > 0) && (rho_i > 0) && (h_i > 0)) {  // GasGrad_isactive (gradients.c:66-75)
+            double hinv, hinv3, hinv4;
+            kernel_hinv(h_i, hinv, hinv3, hinv4);
+            const double h2_i = h_i * h_i;
+            const double xs[3] = {ri.xh.x, ri.xh.y, ri.xh.z};
+            const double R = (h_i > a.hmax) ? h_i : a.hmax;
+            for_each_candidate(a.grid, xs, R, [&](int j) {
+                const double4 xj = a.rec[j].xh;
+                double dp[3] = {xs[0] - xj.x, xs[1] - xj.y, xs[2] - xj.z};
+                wrap_dp(a.box, dp);
+                const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                const double h_j = xj.w;
+                if (r2 <= 0) return;
+                if ((r2 >= h2_i) && (r2 >= h_j * h_j)) return;
+                const double4 vr = a.rec[j].vr;
+                const double2 pm = a.rec[j].pm;
+                if (!((pm.y > 0) && (vr.w > 0) && (h_j > 0))) return;
+                cnt++;
+                const double r = sqrt(r2);
+                const double wk_i = (r < h_i) ? kernel_w(r * hinv, hinv3) : 0.0;
+                if (r > maxd) maxd = r;
+                const double d_rho = vr.w - rho_i;
+                minmax_check(d_rho, mn_r, mx_r);
+                const double dpp = pm.x - p_i;
+                minmax_check(dpp, mn_p, mx_p);
+                const double dv[3] = {vr.x - ri.vr.x, vr.y - ri.vr.y, vr.z - ri.vr.z};
+                minmax_check(dv[0], mn_v[0], mx_v[0]);
+                minmax_check(dv[1], mn_v[1], mx_v[1]);
+                minmax_check(dv[2], mn_v[2], mx_v[2]);
+                if (r < h_i) {
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const double w = -wk_i * dp[k];
+                        gr[k] += w * d_rho;
+                        gp[k] += w * dpp;
+                        gv[0][k] += w * dv[0];
+                        gv[1][k] += w * dv[1];
+                        gv[2][k] += w * dv[2];
+                    }
+                }
+            });
+        }
+        double nvt[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) nvt[k] = a.nvt[6 * (size_t)i + k];
+        apply_nvt(nvt, gr);
+        apply_nvt(nvt, gp);
+        apply_nvt(nvt, gv[0]);
+        apply_nvt(nvt, gv[1]);
+        apply_nvt(nvt, gv[2]);
+        // SELFGRAVITY_OFF, no MAGNETIC/GALSF: h_lim = h, stol = 0.1 (gradients.c:1219-1232)
+        const double cn = a.cn[i];
+        const double d_max = (h_i > maxd) ? h_i : maxd;
+        double a_lim = 0.25;
+        if (cn > 100) a_lim = dmin(0.5, 0.25 + 0.25 * (cn - 100) / 100);
+        slope_limit(gr, mx_r, mn_r, a_lim, h_i, 0, 1, d_max, rho_i);
+        slope_limit(gp, mx_p, mn_p, a_lim, h_i, 0.1, 1, d_max, p_i);
+        slope_limit(gv[0], mx_v[0], mn_v[0], a_lim, h_i, 0.1, 0, 0, 0);
+        slope_limit(gv[1], mx_v[1], mn_v[1], a_lim, h_i, 0.1, 0, 0, 0);
+        slope_limit(gv[2], mx_v[2], mn_v[2], a_lim, h_i, 0.1, 0, 0, 0);
+        double *o = a.grad + 15 * (size_t)i;
+        o[0] = gr[0]; o[1] = gr[1]; o[2] = gr[2];
+        o[3] = gp[0]; o[4] = gp[1]; o[5] = gp[2];
+#pragma unroll
+        for (int k = 0; k < 3; k++) { o[6 + 3 * k] = gv[k][0]; o[7 + 3 * k] = gv[k][1]; o[8 + 3 * k] = gv[k][2]; }
+    }
+    wave_add_u64(&a.ctr->interactions, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// force (hydro_evaluate.h:22-516, hydro_core_meshless.h, compute_finitevol_faces.h,
+// final operations hydro_toplevel.c:652-767)
+// ---------------------------------------------------------------------------
+__global__ void pack_force_kernel(int n, const double4 *posm, const double4 *velu, const double *hsml,
+                                  const double *rho, const double *pres, const double *dhsml, const double *cn,
+                                  const double *fce, const double *nvt, const double *grad, double gamma,
+                                  ForceRec *rec)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double4 p = posm[k], v = velu[k];
+    double d = rho[k], P = pres[k];
+    const double *t = nvt + 6 * (size_t)k;
+    const double *g = grad + 15 * (size_t)k;
+    ForceRec r;
+    r.xh = make_double4(p.x, p.y, p.z, hsml[k]);
+    r.vc = make_double4(v.x, v.y, v.z, sqrt(gamma * P / d));  // eos.c:245-254
+    r.th = make_double4(d, P, p.w / d, dhsml[k]);
+    r.mc = make_double4(p.w, cn[k], fce[k], 0.0);
+    r.t0 = make_double4(t[0], t[1], t[2], t[3]);
+    r.t1 = make_double4(t[4], t[5], 0.0, 0.0);
+    r.g0 = make_double4(g[0], g[1], g[2], g[3]);
+    r.g1 = make_double4(g[4], g[5], g[6], g[7]);
+    r.g2 = make_double4(g[8], g[9], g[10], g[11]);
+    r.g3 = make_double4(g[12], g[13], g[14], 0.0);
+    rec[k] = r;
+}
+
+struct ForceI {
+    double x[3], v[3], h, hinv, hinv3, hinv4, rho, P, cs, V, dhsml, cn, cnumcrit2, fce;
+    double nvt[6], gr[3], gp[3], gv[9];
+};
+
+__device__ __forceinline__ void unpack_grads(const ForceRec &r, double gr[3], double gp[3], double gv[9])
+{
+    gr[0] = r.g0.x; gr[1] = r.g0.y; gr[2] = r.g0.z;
+    gp[0] = r.g0.w; gp[1] = r.g1.x; gp[2] = r.g1.y;
+    gv[0] = r.g1.z; gv[1] = r.g1.w; gv[2] = r.g2.x;
+    gv[3] = r.g2.y; gv[4] = r.g2.z; gv[5] = r.g2.w;
+    gv[6] = r.g3.x; gv[7] = r.g3.y; gv[8] = r.g3.z;
+}
+
+// one (i, j) interaction of the MFM force loop; returns the status code
+__device__ __forceinline__ int force_pair(const ForceI &I, const ForceRec &J, const double dp[3], double r2,
+                                          const GammaConsts &G, ROut &ro, double Fv[3], double &Fp,
+                                          double &vsig, int &mask)
+{
+    int status = 0;
+    const double r = sqrt(r2);
+    const double rinv = 1 / r;
+    const double vj[3] = {J.vc.x, J.vc.y, J.vc.z};
+    const double dv[3] = {I.v[0] - vj[0], I.v[1] - vj[1], I.v[2] - vj[2]};
+    const double rho_j = J.th.x, P_j = J.th.y, V_j = J.th.z, cs_j = J.vc.w, h_j = J.xh.w;
+    vsig = I.cs + cs_j;
+    const double vdotr2 = dp[0] * dv[0] + dp[1] * dv[1] + dp[2] * dv[2];
+    if (vdotr2 < 0) vsig -= 1.0 * vdotr2 * rinv;  // MFM: fac_mu = 1 (hydro_evaluate.h:235-245)
+    double wk_i = 0, dwk_i = 0, wk_j = 0, dwk_j = 0;
+    if (r < I.h) kernel_wdw(r * I.hinv, I.hinv3, I.hinv4, wk_i, dwk_i);
+    if (r < h_j) {
+        double hinv_j, hinv3_j, hinv4_j;
+        kernel_hinv(h_j, hinv_j, hinv3_j, hinv4_j);
+        kernel_wdw(r * hinv_j, hinv3_j, hinv4_j, wk_j, dwk_j);
+    }
+    Fv[0] = Fv[1] = Fv[2] = 0;
+    Fp = 0;
+    mask = 0;
+    const double leak = 0.5 * (I.fce + J.mc.z);
+    const double cn_j = J.mc.y;
+    // effective face (compute_finitevol_faces.h:14-106)
+    double wt_i = I.V, wt_j = V_j;
+    if ((fabs(I.V - V_j) / dmin(I.V, V_j)) / 3 > 1.25) {
+        wt_i = wt_j = I.V * V_j * (wk_i + wk_j) / (I.V * wk_i + V_j * wk_j);
+    }
+    const double tj[6] = {J.t0.x, J.t0.y, J.t0.z, J.t0.w, J.t1.x, J.t1.y};
+    double A[3];
+    double Anorm = 0, fdp = 0;
+    {
+        const double ti_d[3] = {I.nvt[0] * dp[0] + I.nvt[1] * dp[1] + I.nvt[2] * dp[2],
+                                I.nvt[1] * dp[0] + I.nvt[3] * dp[1] + I.nvt[4] * dp[2],
+                                I.nvt[2] * dp[0] + I.nvt[4] * dp[1] + I.nvt[5] * dp[2]};
+        const double tj_d[3] = {tj[0] * dp[0] + tj[1] * dp[1] + tj[2] * dp[2],
+                                tj[1] * dp[0] + tj[3] * dp[1] + tj[4] * dp[2],
+                                tj[2] * dp[0] + tj[4] * dp[1] + tj[5] * dp[2]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            A[k] = wk_i * wt_i * ti_d[k] + wk_j * wt_j * tj_d[k];
+            Anorm += A[k] * A[k];
+            fdp += A[k] * dp[k];
+        }
+    }
+    if ((cn_j * cn_j > 1.0e12 + I.cnumcrit2) || (fdp < 0)) {
+        double An = -(wt_i * I.V * dwk_i + wt_j * V_j * dwk_j) / r;
+        A[0] = An * dp[0];
+        A[1] = An * dp[1];
+        A[2] = An * dp[2];
+        Anorm = An * An * r2;
+    }
+    Anorm = sqrt(Anorm);
+    if (Anorm == 0) return 0;
+
+    const double n[3] = {A[0] / Anorm, A[1] / Anorm, A[2] / Anorm};
+    // faces at the midpoint: s_i = -r/2, s_j = +r/2 (hydro_core_meshless.h:70-90)
+    const double s_i = 0 - 0.5 * r, s_j = 0 - (-0.5 * r);
+    double di[3], dj[3], vf[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double e = dp[k] * rinv;
+        dj[k] = e * s_j;
+        di[k] = e * s_i;
+        vf[k] = rinv * (-s_i * vj[k] + s_j * I.v[k]);
+    }
+    double fvi = 0, fvj = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) { fvi += I.v[k] * n[k]; fvj += vj[k] * n[k]; }
+    const double fadv = rinv * (-s_i * fvj + s_j * fvi);
+    double v2a = 0;
+    double vdotr2_phys = vdotr2;
+    vdotr2_phys *= 1 / (r * 1.0);
+    if (vdotr2_phys < 0) v2a = vdotr2_phys * vdotr2_phys;
+    const double vdotf2 = fvi - fvj;
+    if (vdotf2 < 0) v2a = dmax(v2a, vdotf2 * vdotf2);
+    const int recon = (leak > 1) ? 0 : 1;
+    RState SL, SR;
+    double gr_j[3], gp_j[3], gv_j[9];
+    unpack_grads(J, gr_j, gp_j, gv_j);
+    reconstruct(I.rho, I.gr, rho_j, gr_j, di, dj, SL.rho, SR.rho, recon);
+    reconstruct(I.P, I.gp, P_j, gp_j, di, dj, SL.p, SR.p, recon);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        reconstruct(I.v[k], I.gv + 3 * k, vj[k], gv_j + 3 * k, di, dj, SL.v[k], SR.v[k], recon);
+        SL.v[k] -= vf[k];
+        SR.v[k] -= vf[k];
+    }
+    const double pti = I.P + I.rho * v2a;
+    const double ptj = P_j + rho_j * v2a;
+    double lim = 1.1 * 1.0 * dmax(pti, ptj);
+    if (recon == 0) lim = dmax(lim, dmax(dmax(I.P, P_j), 2. * dmax(I.rho, rho_j) * v2a));
+    int err = 0;
+    mask = riemann_solve(SL, SR, n, lim, G, ro, err);
+    if ((ro.P_M < 0) || isnan(ro.P_M) || (ro.P_M > 1.4 * lim)) {
+        mask |= 8;  // first-order retry (hydro_core_meshless.h:209-265)
+        SR.p = I.P; SL.p = P_j;
+        SR.rho = I.rho; SL.rho = rho_j;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { SR.v[k] = I.v[k] - vf[k]; SL.v[k] = vj[k] - vf[k]; }
+        mask |= riemann_solve(SL, SR, n, 1.4 * lim, G, ro, err);
+        if ((ro.P_M < 0) || isnan(ro.P_M)) {
+            SR.p = I.P; SL.p = P_j;
+            SR.rho = I.rho; SL.rho = rho_j;
+#pragma unroll
+            for (int k = 0; k < 3; k++) { SR.v[k] = 0; SL.v[k] = 0; }
+            mask |= riemann_solve(SL, SR, n, 2.0 * lim, G, ro, err);
+            if ((ro.P_M < 0) || isnan(ro.P_M)) status = 1234;
+        }
+    }
+    if (err) status = err;
+    if ((ro.P_M > 0) && !isnan(ro.P_M)) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) ro.Fp += vf[k] * ro.Fv[k];
+        Fp = Anorm * ro.Fp;
+#pragma unroll
+        for (int k = 0; k < 3; k++) Fv[k] = Anorm * ro.Fv[k];
+        // MFM adiabatic face correction (hydro_core_meshless.h:335-374)
+        const double sm_c = fabs(ro.S_M) / dmin(I.cs, cs_j);
+        if ((sm_c < 0.5) || (leak > 1)) {
+            int use = 1;
+            const double fpm = ro.P_M * Anorm;
+            const double pdv_fac = ro.P_M * vdotr2_phys / 1.0;
+            const double pdv_i = dwk_i * I.V * I.V * I.dhsml * pdv_fac;
+            const double pdv_j = dwk_j * V_j * V_j * J.th.w * pdv_fac;
+            const double du_old = fpm * (ro.S_M + fadv);
+            const double du_new = 0.5 * (pdv_i - pdv_j + fpm * (fvi + fvj));
+            const double cnum2 = cn_j * cn_j;
+            if (sm_c > 1.e-3 && cnum2 < I.cnumcrit2) {
+                const double ei = I.P / I.rho, ej = P_j / rho_j;
+                if (ei != ej) {
+                    if (ei > ej) {
+                        const double dtoj = -du_old + fpm * fvj;
+                        if (dtoj > 0) use = 0;
+                        else if (dtoj < 0) { if (dtoj > -du_new + fpm * fvj) use = 0; }
+                    } else {
+                        const double dtoi = +du_old - fpm * fvi;
+                        if (dtoi > 0) use = 0;
+                        else if (dtoi < 0) { if (dtoi > +du_new - fpm * fvi) use = 0; }
+                    }
+                }
+            }
+            if (cnum2 >= I.cnumcrit2) use = 1;
+            if (use) Fp += du_new - du_old;
+        }
+    } else {
+        Fp = 0;
+        Fv[0] = Fv[1] = Fv[2] = 0;
+    }
+    return status;
+}
+
+__global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < a.n_active;
+    unsigned long long cnt = 0, nretry = 0, nexact = 0;
+    if (valid) {
+        const int i = a.active[t];
+        const ForceRec ri = a.rec[i];
+        double acc[3] = {0, 0, 0}, dte = 0, vmax = kMinReal;
+        const double m_i = ri.mc.x, rho_i = ri.th.x;
+        int status = 0;
+        if ((m_i > 0) && (rho_i > 0)) {
+            ForceI I;
+            I.x[0] = ri.xh.x; I.x[1] = ri.xh.y; I.x[2] = ri.xh.z;
+            I.v[0] = ri.vc.x; I.v[1] = ri.vc.y; I.v[2] = ri.vc.z;
+            I.h = ri.xh.w;
+            kernel_hinv(I.h, I.hinv, I.hinv3, I.hinv4);
+            I.rho = rho_i; I.P = ri.th.y; I.cs = ri.vc.w; I.V = ri.th.z; I.dhsml = ri.th.w;
+            I.cn = ri.mc.y; I.fce = ri.mc.z;
+            I.cnumcrit2 = kCondDanger * kCondDanger - I.cn * I.cn;
+            I.nvt[0] = ri.t0.x; I.nvt[1] = ri.t0.y; I.nvt[2] = ri.t0.z; I.nvt[3] = ri.t0.w;
+            I.nvt[4] = ri.t1.x; I.nvt[5] = ri.t1.y;
+            unpack_grads(ri, I.gr, I.gp, I.gv);
+            double out_vsig = I.cs;
+            ROut ro = {0, 0, {0, 0, 0}, 0};
+            const double h2_i = I.h * I.h;
+            const double R = (I.h > a.hmax) ? I.h : a.hmax;
+            for_each_candidate(a.grid, I.x, R, [&](int j) {
+                const double4 xj = a.rec[j].xh;
+                double dp[3] = {I.x[0] - xj.x, I.x[1] - xj.y, I.x[2] - xj.z};
+                wrap_dp(a.box, dp);
+                const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                const double h_j = xj.w;
+                if ((r2 >= h2_i) && (r2 >= h_j * h_j)) return;
+                if (r2 <= 0) return;
+                const ForceRec J = a.rec[j];
+                if (!(J.mc.x > 0) || !(J.th.x > 0)) return;  // hydro_evaluate.h:149-150
+                cnt++;
+                double Fv[3], Fp, vsig;
+                int mask;
+                const int e = force_pair(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+                if (e > status) status = e;
+                nretry += (mask & 8) ? 1 : 0;
+                nexact += (mask & 4) ? 1 : 0;
+                acc[0] += Fv[0];
+                acc[1] += Fv[1];
+                acc[2] += Fv[2];
+                dte += Fp;
+                if (vsig > out_vsig) out_vsig = vsig;
+                if (a.is_active && !a.is_active[j]) {
+                    if (vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;  // WAKEUP (hydro_evaluate.h:485-495)
+                }
+            });
+            if (vmax < out_vsig) vmax = out_vsig;
+            // hydro_final_operations_and_cleanup (hydro_toplevel.c:738-764)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                dte -= (I.v[k] / 1.0) * acc[k];
+                acc[k] /= m_i;
+            }
+            dte /= m_i;
+            if (I.h >= 0.99 * a.max_hsml) dte = 0;
+        } else if (m_i > 0) {
+            dte /= m_i;
+        }
+        a.acc[3 * (size_t)i + 0] = acc[0];
+        a.acc[3 * (size_t)i + 1] = acc[1];
+        a.acc[3 * (size_t)i + 2] = acc[2];
+        a.dtu[i] = dte;
+        a.vsig[i] = vmax;
+        if (status) atomicMax(&a.ctr->status, status);
+    }
+    wave_add_u64(&a.ctr->interactions, cnt);
+    wave_add_u64(&a.ctr->retries, nretry);
+    wave_add_u64(&a.ctr->exact_calls, nexact);
+}
+
+// neighbour-set sizes for a fixed hsml (same predicates as the stage kernels)
+__global__ __launch_bounds__(kBlock) void neighbor_count_kernel(Grid g, BoxInfo box, int mode, int n_active,
+                                                                const int *active, const double4 *posm,
+                                                                const double *hsml, double hmax, int *counts,
+                                                                Counters *ctr)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cnt = 0;
+    if (t < n_active) {
+        const int i = active[t];
+        const double4 pi = posm[i];
+        const double h = hsml[i], h2 = h * h;
+        const double xs[3] = {pi.x, pi.y, pi.z};
+        const double R = (mode == 0) ? h : ((h > hmax) ? h : hmax);
+        for_each_candidate(g, xs, R, [&](int j) {
+            const double4 pj = posm[j];
+            double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+            wrap_dp(box, dp);
+            const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+            if (mode == 0) {
+                if (r2 < h2) cnt++;
+            } else {
+                const double h_j = hsml[j];
+                if ((r2 >= h2) && (r2 >= h_j * h_j)) return;
+                if (r2 <= 0) return;
+                cnt++;
+            }
+        });
+        if (counts) counts[t] = (int)cnt;
+    }
+    wave_add_u64(&ctr->interactions, cnt);
+}
+
+inline unsigned nblocks(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_cell_keys(hipStream_t s, int n, const double *pos, Grid g, uint32_t *keys, int *iota)
+{
+    if ((n) > 0) hipLaunchKernelGGL(cell_keys_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pos, g, keys, iota);
+}
+void launch_gather_sorted(hipStream_t s, int n, const int *perm, const double *pos, const double *vel,
+                          const double *mass, const double *u, double4 *posm, double4 *velu, int *inv)
+{
+    if ((n) > 0) hipLaunchKernelGGL(gather_sorted_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, perm, pos, vel, mass, u,
+                       posm, velu, inv);
+}
+void launch_gather_scalar(hipStream_t s, int n, const int *perm, const double *src, double *dst, int width)
+{
+    if (((long long)n * width) > 0) hipLaunchKernelGGL(gather_scalar_kernel, dim3(nblocks((long long)n * width)), dim3(kBlock), 0, s, n, perm, src,
+                       dst, width);
+}
+void launch_scatter_scalar(hipStream_t s, int n, const int *perm, const double *src, double *dst, int width)
+{
+    if (((long long)n * width) > 0) hipLaunchKernelGGL(scatter_scalar_kernel, dim3(nblocks((long long)n * width)), dim3(kBlock), 0, s, n, perm,
+                       src, dst, width);
+}
+void launch_cell_ranges(hipStream_t s, int n, const double4 *posm, Grid g, int2 *range, int ncell)
+{
+    if ((2LL * ncell) > 0) hipLaunchKernelGGL(fill_int_kernel, dim3(nblocks(2LL * ncell)), dim3(kBlock), 0, s, (int *)range, 2 * ncell, 0);
+    if ((n) > 0) hipLaunchKernelGGL(cell_ranges_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, posm, g, range);
+}
+void launch_map_active(hipStream_t s, int n_active, const int *active_caller, const int *inv, int *active_sorted)
+{
+    if ((n_active) > 0) hipLaunchKernelGGL(map_active_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, n_active, active_caller, inv,
+                       active_sorted);
+}
+void launch_density_init(hipStream_t s, int n_active, const int *active, const double4 *posm, double *hsml,
+                         double *left, double *right, double max_hsml, int *live_flag)
+{
+    if ((n_active) > 0) hipLaunchKernelGGL(density_init_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, n_active, active, posm,
+                       hsml, left, right, max_hsml, live_flag);
+}
+void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a)
+{
+    if ((a.n_live) > 0) hipLaunchKernelGGL(density_sweep_kernel, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+}
+void launch_density_finalize(hipStream_t s, int n_active, const int *active, const double4 *posm,
+                             const double4 *velu, const double *hsml, double *num_ngb, double *rho, double *pres,
+                             double gm1)
+{
+    if ((n_active) > 0) hipLaunchKernelGGL(density_finalize_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, n_active, active, posm,
+                       velu, hsml, num_ngb, rho, pres, gm1);
+}
+void launch_pack_grad(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
+                      const double *rho, const double *pres, GradRec *rec)
+{
+    if ((n) > 0) hipLaunchKernelGGL(pack_grad_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, posm, velu, hsml, rho, pres, rec);
+}
+void launch_gradients(hipStream_t s, const GradientArgs &a)
+{
+    if ((a.n_active) > 0) hipLaunchKernelGGL(gradients_kernel, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+}
+void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
+                       const double *rho, const double *pres, const double *dhsml, const double *cn,
+                       const double *fce, const double *nvt, const double *grad, double gamma, ForceRec *rec)
+{
+    if ((n) > 0) hipLaunchKernelGGL(pack_force_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, posm, velu, hsml, rho, pres,
+                       dhsml, cn, fce, nvt, grad, gamma, rec);
+}
+void launch_force(hipStream_t s, const ForceArgs &a)
+{
+    if ((a.n_active) > 0) hipLaunchKernelGGL(force_kernel, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+}
+void launch_neighbor_counts(hipStream_t s, Grid g, BoxInfo box, int mode, int n_active, const int *active,
+                            const double4 *posm, const double *hsml, double hmax, int *counts, Counters *ctr)
+{
+    if ((n_active) > 0) hipLaunchKernelGGL(neighbor_count_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, g, box, mode, n_active,
+                       active, posm, hsml, hmax, counts, ctr);
+}
+void launch_fill_int(hipStream_t s, int *p, int n, int value)
+{
+    if ((n) > 0) hipLaunchKernelGGL(fill_int_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, value);
+}
+void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value)
+{
+    if ((n) > 0) hipLaunchKernelGGL(fill_u8_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, value);
+}
+void launch_fill_double(hipStream_t s, double *p, long long n, double value)
+{
+    if ((n) > 0) hipLaunchKernelGGL(fill_double_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, value);
+}
+void launch_set_flags(hipStream_t s, int n_active, const int *active, uint8_t *flags)
+{
+    if ((n_active) > 0) hipLaunchKernelGGL(set_flags_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, n_active, active, flags);
+}
+
+}  // namespace gzm

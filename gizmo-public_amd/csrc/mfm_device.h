@@ -1,0 +1,569 @@
+// mfm_device.h -- device-side physics of the MFM hydro pass (FP64), gfx950.
+//
+// Pure per-particle / per-pair functions used by the stage kernels in
+// gzm_kernels.hip.  Each one cites the reference lines whose behaviour it
+// reproduces.  Arithmetic keeps the reference's operation order; the FP64
+// translation unit is compiled with -ffp-contract=off so that products and
+// sums round exactly as in the CPU pass.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace gzm {
+
+// allvars.h:1443 (the literal, not 4*pi/3), kernel.h:63, allvars.h:1106-1112, allvars.h:78
+constexpr double kNormCoeff = 4.188790204786;
+constexpr double kKernelNorm = 8.0 / 3.14159265358979323846;
+constexpr double kCondDanger = 1.0e3;
+constexpr double kMinReal = 1e-56;
+constexpr double kWakeup = 4.1;
+
+// proto.h:123-124 (NaN behaviour matters: DMAX(NaN,b) = b)
+__device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
+
+// reimann.h:1-10 for the run's adiabatic index
+struct GammaConsts {
+    double G0, G1, G2, G3, G4, G5, G6, G7, G8, G9;
+};
+
+struct BoxInfo {
+    double L[3];
+    double half[3];
+};
+
+// ---------------------------------------------------------------------------
+// cubic-spline kernel (kernel.h:182-197, 225-242, 373-374)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void kernel_hinv(double h, double &hinv, double &hinv3, double &hinv4)
+{
+    hinv = 1.0 / h;
+    hinv3 = hinv * hinv * hinv;
+    hinv4 = hinv3 * hinv;
+}
+
+__device__ __forceinline__ void kernel_wdw(double u, double hinv3, double hinv4, double &wk, double &dwk)
+{
+    if (u >= 1) { wk = 0; dwk = 0; return; }
+    double w, dw;
+    if (u < 0.5) {
+        dw = u * (18.0 * u - 12.0);
+        w = (1.0 + 6.0 * (u - 1.0) * u * u);
+    } else {
+        double t1 = (1.0 - u);
+        double t2 = t1 * t1;
+        dw = -6.0 * t2;
+        w = 2.0 * t2 * t1;
+    }
+    dwk = dw * (kKernelNorm * hinv4);
+    wk = w * (kKernelNorm * hinv3);
+}
+
+__device__ __forceinline__ double kernel_w(double u, double hinv3)
+{
+    if (u >= 1) return 0;
+    double w;
+    if (u < 0.5) {
+        w = (1.0 + 6.0 * (u - 1.0) * u * u);
+    } else {
+        double t1 = (1.0 - u);
+        double t2 = t1 * t1;
+        w = 2.0 * t2 * t1;
+    }
+    return w * (kKernelNorm * hinv3);
+}
+
+// signed nearest-image wrap of one component (allvars.h:1519, NEAREST_XYZ)
+__device__ __forceinline__ double wrap_axis(double x, double L, double half)
+{
+    return (x > half) ? (x - L) : ((x < -half) ? (x + L) : x);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 symmetric inverse with the cofactor formula and Frobenius condition
+// number (system/system.c:355-391).  The input is the symmetric matrix
+// (a b c / b d e / c e f); element order inside every product follows the
+// general 3x3 expression so the rounding is the reference's.
+// Returns CN; inv = (i00 i01 i02 i11 i12 i22).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double invert_sym3(double a, double b, double c, double d, double e, double f,
+                                              double inv[6])
+{
+    // T00=a T01=b T02=c T10=b T11=d T12=e T20=c T21=e T22=f
+    double frob = 0;
+    frob += a * a; frob += b * b; frob += c * c;
+    frob += b * b; frob += d * d; frob += e * e;
+    frob += c * c; frob += e * e; frob += f * f;
+    double det = a * d * f + b * e * c + c * b * e - c * d * c - b * b * f - a * e * e;
+    double i00 = 0, i01 = 0, i02 = 0, i11 = 0, i12 = 0, i22 = 0;
+    if ((det != 0) && !isnan(det)) {
+        i00 = (d * f - e * e) / det;
+        i01 = (c * e - b * f) / det;
+        i02 = (b * e - c * d) / det;
+        i11 = (a * f - c * c) / det;
+        i12 = (c * b - a * e) / det;
+        i22 = (a * d - b * b) / det;
+    }
+    // Tinv row-major: i00 i01 i02 / i10=i01 i11 i12 / i20=i02 i21=i12 i22
+    double frob_inv = 0;
+    frob_inv += i00 * i00; frob_inv += i01 * i01; frob_inv += i02 * i02;
+    frob_inv += i01 * i01; frob_inv += i11 * i11; frob_inv += i12 * i12;
+    frob_inv += i02 * i02; frob_inv += i12 * i12; frob_inv += i22 * i22;
+    inv[0] = i00; inv[1] = i01; inv[2] = i02; inv[3] = i11; inv[4] = i12; inv[5] = i22;
+    return dmax(sqrt(frob * frob_inv) / 3, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Newton/bisection update of one particle after a density sweep
+// (density.c:505-850).  Reads the raw sums of the sweep, writes the
+// normalised per-particle fields; returns 1 if the particle needs another
+// sweep, 0 if it is done, -1 on the Left==Right==0 failure.
+// ---------------------------------------------------------------------------
+struct HsmlParams {
+    double des_num_ngb, max_num_ngb_dev, min_hsml, max_hsml;
+    int first_step;
+    int iter;
+};
+
+struct DensitySums {
+    double ngb, rho, dhsml, divvel;
+    double t00, t01, t02, t11, t12, t22;  // second moments
+    double f0, f1, f2;                    // first moments (face terms, NV_T[1][0],[2][0],[2][1])
+};
+
+struct DensityResult {
+    double num_ngb, dhsml, divvel, rho_sum, fce, cn_stored;
+    double nvt[6];
+};
+
+__device__ __forceinline__ double ncorr_from_cn(double cn)
+{
+    double c0 = 0.1 * kCondDanger;
+    double nc = 1;
+    if (cn > c0) nc = sqrt(1.0 + (cn - c0) / kCondDanger);
+    if (nc > 2) nc = 2;
+    return nc;
+}
+
+__device__ inline int hsml_update(const HsmlParams &P, const DensitySums &S, double &h, double &Left,
+                                  double &Right, double cn_stored_in, DensityResult &R)
+{
+    const int iter = P.iter;
+    double desnumngb = P.des_num_ngb, desnumngbdev = P.max_num_ngb_dev;
+    if (P.first_step) { if (P.max_num_ngb_dev > 0.05) desnumngbdev = 0.05; }
+    const double desnumngbdev_0 = desnumngbdev;
+    double NumNgb = S.ngb, Dhsml = S.dhsml, DivVel = S.divvel;
+    if (NumNgb > 0) {
+        Dhsml *= h / (3 * NumNgb);
+        DivVel /= NumNgb;
+        NumNgb *= kNormCoeff * pow(h, 3.0);
+    } else {
+        NumNgb = Dhsml = DivVel = 0;
+    }
+    if (Dhsml > -0.9) Dhsml = 1 / (1 + Dhsml); else Dhsml = 1;
+    DivVel *= Dhsml;
+
+    // NV_T inversion with diagonal preconditioning, face-closure error
+    double ConditionNumber;
+    {
+        double V_i = kNormCoeff * pow(h, 3.0) / NumNgb;
+        double dx_i = sqrt(V_i * (S.t00 + S.t11 + S.t22));
+        double norm = pow(h, -1.0);
+        double a = S.t00 / norm, b = S.t01 / norm, c = S.t02 / norm;
+        double d = S.t11 / norm, e = S.t12 / norm, f = S.t22 / norm;
+        const double cn_thresh = 10. * kCondDanger;
+        double trace = a + d + f;
+        double add = 1.05 * (trace / 3) / cn_thresh;
+        double inv[6];
+        // the reference loops until CN < 1e4 (density.c:543-552); the cap only
+        // guarantees termination on a GPU (1.2^4096 overflows long before)
+        for (int guard = 0; guard < 4096; guard++) {
+            ConditionNumber = invert_sym3(a, b, c, d, e, f, inv);
+            if (ConditionNumber < cn_thresh) break;
+            a += add; d += add; f += add;
+            add *= 1.2;
+        }
+        for (int k = 0; k < 6; k++) R.nvt[k] = inv[k] / norm;
+        // Face_Area_OneSided_Estimator_out = 2 V_i NV_T^-1 . (f0 f1 f2), row by row
+        const double fin[3] = {S.f0, S.f1, S.f2};
+        const double row[3][3] = {{R.nvt[0], R.nvt[1], R.nvt[2]},
+                                  {R.nvt[1], R.nvt[3], R.nvt[4]},
+                                  {R.nvt[2], R.nvt[4], R.nvt[5]}};
+        double leak = 0;
+        for (int k1 = 0; k1 < 3; k1++) {
+            double o = 0;
+            for (int k2 = 0; k2 < 3; k2++) o += 2. * V_i * row[k1][k2] * fin[k2];
+            leak += fabs(o) / 3;
+        }
+        R.fce = leak / (2. * 3 * pow(dx_i, 2.0));
+    }
+
+    // neighbour-number target (density.c:565-595)
+    double cn_stored = cn_stored_in;
+    if ((iter == 0) && (ConditionNumber > cn_stored) && (cn_stored > 0)) {
+        double nc = ncorr_from_cn(cn_stored);
+        double dn_ngb = fabs(NumNgb - P.des_num_ngb * nc) / (desnumngbdev_0 * nc);
+        nc = ncorr_from_cn(ConditionNumber);
+        double dn_ngb_alt = fabs(NumNgb - P.des_num_ngb * nc) / (desnumngbdev_0 * nc);
+        dn_ngb = dmin(dn_ngb, dn_ngb_alt);
+        if (dn_ngb < 10.0) cn_stored = ConditionNumber;
+    }
+    double ncorr_ngb = ncorr_from_cn(cn_stored);
+    if (R.fce > 0.35) ncorr_ngb = dmax(ncorr_ngb, dmin(R.fce / 0.35, 2.));
+    desnumngb = P.des_num_ngb * ncorr_ngb;
+    desnumngbdev = desnumngbdev_0 * ncorr_ngb;
+    if (iter > 1)
+        desnumngbdev = dmin(0.25 * desnumngb, desnumngbdev * exp(0.1 * log(desnumngb / (16. * desnumngbdev)) * (double)iter));
+
+    const double maxsoft = P.max_hsml, minsoft = P.min_hsml;
+    int redo = 0, maxflag = 0, minflag = 0;
+    if ((NumNgb < (desnumngb - desnumngbdev) && h < 0.999 * maxsoft) ||
+        (NumNgb > (desnumngb + desnumngbdev) && h > 1.001 * minsoft)) redo = 1;
+    if ((h >= 0.999 * maxsoft) && (NumNgb < (desnumngb - desnumngbdev))) {
+        redo = 0;
+        if (h != maxsoft) { redo = 1; h = maxsoft; maxflag = 1; }
+    }
+    if ((h <= 1.001 * minsoft) && (NumNgb > (desnumngb + desnumngbdev))) {
+        redo = 0;
+        if (h != minsoft) { redo = 1; h = minsoft; minflag = 1; }
+    }
+    if (redo == 0) cn_stored = ConditionNumber;
+
+    R.num_ngb = NumNgb;
+    R.dhsml = Dhsml;
+    R.divvel = DivVel;
+    R.rho_sum = S.rho;
+    int result = redo;
+    if (redo) {
+        if (Left > 0 && Right > 0 && (Right - Left) < 1.0e-3 * Left) {
+            R.cn_stored = ConditionNumber;  // bracket narrow enough: accept (density.c:732-740)
+            return 0;
+        }
+        if ((maxflag == 0) && (minflag == 0)) {
+            if (NumNgb < (desnumngb - desnumngbdev)) Left = dmax(h, Left);
+            else {
+                if (Right != 0) { if (h < Right) Right = h; }
+                else Right = h;
+            }
+            if (Right > 0 && Left > 0) {
+                double maxjump = 0;
+                if (iter > 1) maxjump = 0.2 * log(Right / Left);
+                if (NumNgb > 1) {
+                    double jumpvar = Dhsml * log(desnumngb / NumNgb) / 3;
+                    if (iter > 1) { if (fabs(jumpvar) < maxjump) { jumpvar = (jumpvar < 0) ? -maxjump : maxjump; } }
+                    h *= exp(jumpvar);
+                } else {
+                    h *= 2.0;
+                }
+                if ((h < Right) && (h > Left)) {
+                    if (iter > 1) {
+                        double hfac = exp(maxjump);
+                        if (h > Right / hfac) h = Right / hfac;
+                        if (h < Left * hfac) h = Left * hfac;
+                    }
+                } else {
+                    if (h > Right) h = Right;
+                    if (h < Left) h = Left;
+                    h = pow(h * Left * Right, 1.0 / 3.0);
+                }
+            } else {
+                if (Right == 0 && Left == 0) { R.cn_stored = cn_stored; return -1; }
+                if (Right == 0 && Left > 0) {
+                    double fac_lim = (NumNgb > 1) ? log(desnumngb / NumNgb) / 3 : 1.4;
+                    if ((NumNgb < 2 * desnumngb) && (NumNgb > 0.1 * desnumngb)) {
+                        double slope = Dhsml;
+                        if (iter > 2 && slope < 1) slope = 0.5 * (slope + 1);
+                        double fac = fac_lim * slope;
+                        if (iter >= 4) { if (Dhsml == 1) fac *= 10; }
+                        if (fac < fac_lim + 0.231) h *= exp(fac);
+                        else h *= exp(fac_lim + 0.231);
+                    } else h *= exp(fac_lim);
+                }
+                if (Right > 0 && Left == 0) {
+                    double fac_lim = (NumNgb > 1) ? log(desnumngb / NumNgb) / 3 : 1.4;
+                    if (fac_lim < -1.535) fac_lim = -1.535;
+                    if ((NumNgb < 2 * desnumngb) && (NumNgb > 0.1 * desnumngb)) {
+                        double slope = Dhsml;
+                        if (iter > 2 && slope < 1) slope = 0.5 * (slope + 1);
+                        double fac = fac_lim * slope;
+                        if (iter >= 4) { if (Dhsml == 1) fac *= 10; }
+                        if (fac > fac_lim - 0.231) h *= exp(fac);
+                        else h *= exp(fac_lim - 0.231);
+                    } else h *= exp(fac_lim);
+                }
+            }
+        }
+        if (h < minsoft) h = minsoft;
+        if (minflag == 1) h = minsoft;
+        if (h > maxsoft) h = maxsoft;
+        if (maxflag == 1) h = maxsoft;
+    }
+    R.cn_stored = cn_stored;
+    return result;
+}
+
+// ---------------------------------------------------------------------------
+// slope limiter (gradients.c:537-556)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void slope_limit(double g[3], double valmax, double valmin, double alim, double h,
+                                            double shoot_tol, int pos_preserve, double d_max, double val_cen)
+{
+    double d_abs = 0.0;
+    for (int k = 0; k < 3; k++) d_abs += g[k] * g[k];
+    if (d_abs > 0) {
+        d_abs = sqrt(d_abs);
+        double cfac = 1 / (alim * h * d_abs);
+        double fabs_max = fabs(valmax), fabs_min = fabs(valmin), abs_max = fabs_max, abs_min = fabs_min;
+        if (abs_max < abs_min) { abs_max = fabs_min; abs_min = fabs_max; }
+        double f_corr = dmin(abs_min + shoot_tol * abs_max, abs_max);
+        cfac *= f_corr;
+        if (pos_preserve == 1) {
+            double fmin = dmin(val_cen, dmax(0, dmax(kMinReal * val_cen, dmin(0.5 * (val_cen + valmin), val_cen - f_corr))));
+            cfac = dmin((((val_cen - fmin) / d_max) / d_abs), cfac);
+        }
+        if (cfac < 1) { for (int k = 0; k < 3; k++) g[k] *= cfac; }
+    }
+}
+
+// NV_T . g (gradients.c:559-572, matrix branch; nvt = i00 i01 i02 i11 i12 i22)
+__device__ __forceinline__ void apply_nvt(const double nvt[6], double g[3])
+{
+    double v0 = g[0], v1 = g[1], v2 = g[2];
+    g[0] = nvt[0] * v0 + nvt[1] * v1 + nvt[2] * v2;
+    g[1] = nvt[1] * v0 + nvt[3] * v1 + nvt[4] * v2;
+    g[2] = nvt[2] * v0 + nvt[4] * v1 + nvt[5] * v2;
+}
+
+// ---------------------------------------------------------------------------
+// face reconstruction and limiter (reimann.h:129-248), fac_minmax 0.5,
+// fac_meddev 0.375.  R side is particle i, L side is particle j.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void reconstruct(double Qi, const double gi[3], double Qj, const double gj[3],
+                                            const double di[3], const double dj[3], double &QL, double &QR,
+                                            int mode)
+{
+    if (mode == 0) { QR = Qi; QL = Qj; return; }
+    if (Qi == Qj) { QL = QR = Qi; return; }
+    QR = Qi + gi[0] * di[0] + gi[1] * di[1] + gi[2] * di[2];
+    QL = Qj + gj[0] * dj[0] + gj[1] * dj[1] + gj[2] * dj[2];
+    double Qmax, Qmin;
+    double Qmed = 0.5 * (Qi + Qj);
+    if (Qi < Qj) { Qmax = Qj; Qmin = Qi; } else { Qmax = Qi; Qmin = Qj; }
+    double fac = 0.5 * (Qmax - Qmin);
+    double Qmax_eff = Qmax + fac;
+    double Qmin_eff = Qmin - fac;
+    if (Qmax < 0) { if (Qmax_eff > 0) Qmax_eff = Qmax * Qmax / (Qmax - (Qmax_eff - Qmax)); }
+    if (Qmin > 0) { if (Qmin_eff < 0) Qmin_eff = Qmin * Qmin / (Qmin + (Qmin - Qmin_eff)); }
+    fac = 0.375 * (Qmax - Qmin);
+    double Qmed_max = Qmed + fac;
+    double Qmed_min = Qmed - fac;
+    if (Qmed_max > Qmax_eff) Qmed_max = Qmax_eff;
+    if (Qmed_min < Qmin_eff) Qmed_min = Qmin_eff;
+    if (Qi < Qj) {
+        if (QR < Qmin_eff) QR = Qmin_eff;
+        if (QR > Qmed_max) QR = Qmed_max;
+        if (QL > Qmax_eff) QL = Qmax_eff;
+        if (QL < Qmed_min) QL = Qmed_min;
+    } else {
+        if (QR > Qmax_eff) QR = Qmax_eff;
+        if (QR < Qmed_min) QR = Qmed_min;
+        if (QL < Qmin_eff) QL = Qmin_eff;
+        if (QL > Qmed_max) QL = Qmed_max;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Riemann problem for MFM hydro (reimann.h:286-593, 685-1138).
+// ---------------------------------------------------------------------------
+struct RState {
+    double rho, p, v[3];
+};
+
+struct ROut {
+    double P_M, S_M, Fv[3], Fp;
+};
+
+// get_wavespeeds_and_pressure_star: Gaburov HLLC estimate, Roe, then PVRS
+// (reimann.h:527-593).  Sets P_M and S_M.
+__device__ inline void wavespeeds_pstar(const RState &L, const RState &R, const GammaConsts &G, const double n[3],
+                                        double vL, double vR, double cL, double cR, double hL, double hR,
+                                        double limiter, ROut &o)
+{
+    if ((vR - vL) > dmax(cL, cR)) {
+        o.P_M = kMinReal; o.S_M = 0;
+        return;
+    }
+    double PT_L = L.p, PT_R = R.p;
+    double S_L = dmin(vL, vR) - dmax(cL, cR);
+    double S_R = dmax(vL, vR) + dmax(cL, cR);
+    double rwL = L.rho * (S_L - vL);
+    double rwR = R.rho * (S_R - vR);
+    o.S_M = ((PT_R - PT_L) + rwL * vL - rwR * vR) / (rwL - rwR);
+    o.P_M = (PT_L * rwR - PT_R * rwL + rwL * rwR * (vR - vL)) / (rwR - rwL);
+    if (o.P_M <= kMinReal) { o.P_M = kMinReal; o.S_M = 0; }
+    if ((o.P_M <= 0) || isnan(o.P_M) || (o.P_M > limiter)) {
+        double sL = sqrt(L.rho), sR = sqrt(R.rho);
+        double sinv = 1 / (sL + sR);
+        double vx = (sL * L.v[0] + sR * R.v[0]) * sinv;
+        double vy = (sL * L.v[1] + sR * R.v[1]) * sinv;
+        double vz = (sL * L.v[2] + sR * R.v[2]) * sinv;
+        double v_roe = vx * n[0] + vy * n[1] + vz * n[2];
+        double h_roe = (sL * hL + sR * hR) * sinv;
+        double c_roe = sqrt(dmax(1.e-30, G.G9 * (h_roe - 0.5 * (vx * vx + vy * vy + vz * vz))));
+        S_R = dmax(vR + cR, v_roe + c_roe);
+        S_L = dmin(vL - cL, v_roe - c_roe);
+        rwR = R.rho * (S_R - vR);
+        rwL = -L.rho * (S_L - vL);
+        o.S_M = ((rwR * vR + rwL * vL) + (PT_L - PT_R)) / (rwR + rwL);
+        o.P_M = L.rho * (vL - S_L) * (vL - o.S_M) + PT_L;
+        if (o.P_M <= kMinReal) { o.P_M = kMinReal; o.S_M = 0; }
+        if ((o.P_M <= 0) || isnan(o.P_M) || (o.P_M > limiter)) {
+            o.P_M = 0.5 * ((PT_L + PT_R) + (vL - vR) * 0.25 * (L.rho + R.rho) * (cL + cR));
+            o.S_M = 0.5 * (vR + vL) + 2.0 * (PT_L - PT_R) / ((L.rho + R.rho) * (cL + cR));
+            double S_plus = dmax(dmax(fabs(vL - cL), fabs(vR - cR)), dmax(fabs(vL + cL), fabs(vR + cR)));
+            if (o.S_M < -S_plus) o.S_M = -S_plus;
+            if (o.S_M > S_plus) o.S_M = S_plus;
+            if (o.P_M <= kMinReal) { o.P_M = kMinReal; o.S_M = 0; }
+        }
+    }
+}
+
+// Kurganov-Tadmor/PWK flux, MFM branch (reimann.h:484-519)
+__device__ __noinline__ void riemann_kt(const RState &L, const RState &R, const double n[3], double vL, double vR,
+                                        double cL, double cR, double hL, double hR, ROut &o)
+{
+    double thr = 0.001 * 0.5 * (L.rho + R.rho) * 0.5 * (cL + cR);
+    double dv2 = 0;
+    for (int k = 0; k < 3; k++) { double dvk = R.rho * R.v[k] - L.rho * L.v[k]; dv2 += dvk * dvk; }
+    double dvL = fabs(R.rho * vR - L.rho * vL);
+    double alpha = dvL / sqrt(thr * thr + dv2);
+    double S_L = alpha * cL; if (vL > 0) S_L += vL; else S_L -= vL;
+    double S_R = alpha * cR; if (vR > 0) S_R += vR; else S_R -= vR;
+    double S_M = dmax(S_L, S_R);
+    o.S_M = dmax(cL + fabs(vL), cR + fabs(vR));
+    double frl = L.rho * (vL + S_M);
+    double frr = R.rho * (vR - S_M);
+    double denom = 1 / (L.rho * vL - R.rho * vR + S_M * (L.rho + R.rho));
+    o.P_M = (frl * R.p - frr * L.p) * denom;
+    for (int k = 0; k < 3; k++) o.Fv[k] = (R.v[k] - L.v[k]) * frl * frr * denom + (o.P_M) * n[k];
+    o.Fp = (S_M * (frl * R.p + frr * L.p) + (hR - hL) * frl * frr) * denom;
+}
+
+// exact iterative solver (reimann.h:685-724, 1034-1138); for MFM only P_M and
+// S_M change (sample_reimann_standard returns at once, reimann.h:873-877).
+// Returns 0 or 1234545.
+__device__ __noinline__ int riemann_exact(const RState &L, const RState &R, const GammaConsts &G, double vL,
+                                          double vR, double cL, double cR, ROut &o)
+{
+    if ((L.p == 0 && R.p == 0) || (L.rho == 0 && R.rho == 0)) { o.P_M = o.S_M = 0; return 0; }
+    if ((L.rho > 0) && (R.rho > 0)) {
+        double dvel = vR - vL;
+        double check_vel = G.G4 * (cR + cL) - dvel;
+        bool converged = false;
+        if (!(check_vel < 0)) {
+            // guess_for_pressure (reimann.h:1101-1138)
+            double Pg;
+            {
+                double pv = 0.5 * (L.p + R.p) - 0.125 * (vR - vL) * (L.p + R.p) * (cL + cR);
+                double pmin = dmin(L.p, R.p), pmax = dmax(L.p, R.p);
+                if (pmin <= 0) Pg = 0.5 * (pmin + pmax);
+                else {
+                    double qrat = pmax / pmin;
+                    if (qrat <= 2.0 && (pmin <= pv && pv <= pmax)) Pg = pv;
+                    else if (pv < pmin) {
+                        double pnu = (cL + cR) - G.G7 * (vR - vL);
+                        double pde = cL / pow(L.p, G.G1) + cR / pow(R.p, G.G1);
+                        Pg = pow(pnu / pde, G.G3);
+                    } else {
+                        double gel = sqrt((G.G5 / L.rho) / (G.G6 * L.p + pv));
+                        double ger = sqrt((G.G5 / R.rho) / (G.G6 * R.p + pv));
+                        double x = (gel * L.p + ger * R.p - (vR - vL)) / (gel + ger);
+                        if (x < pmin || x > pmax) x = pmin;
+                        Pg = x;
+                    }
+                }
+            }
+            double tol = 100.0, W_L = 0, W_R = 0, Z_L, Z_R;
+            int niter = 0;
+            while ((tol > 1.e-6) && (niter < 1000)) {
+                double Pg_prev = Pg;
+                if (Pg > L.p) {
+                    double a0 = G.G5 / L.rho, a1 = G.G6 * L.p, a2 = sqrt(a0 / (Pg + a1));
+                    W_L = (Pg - L.p) * a2;
+                    Z_L = a2 * (1.0 - 0.5 * (Pg - L.p) / (a1 + Pg));
+                } else {
+                    double pratio = Pg / L.p;
+                    W_L = G.G4 * cL * (pow(pratio, G.G1) - 1);
+                    Z_L = 1 / (L.rho * cL) * pow(Pg / L.p, -G.G2);
+                }
+                if (Pg > R.p) {
+                    double a0 = G.G5 / R.rho, a1 = G.G6 * R.p, a2 = sqrt(a0 / (Pg + a1));
+                    W_R = (Pg - R.p) * a2;
+                    Z_R = a2 * (1.0 - 0.5 * (Pg - R.p) / (a1 + Pg));
+                } else {
+                    double pratio = Pg / R.p;
+                    W_R = G.G4 * cR * (pow(pratio, G.G1) - 1);
+                    Z_R = 1 / (R.rho * cR) * pow(pratio, -G.G2);
+                }
+                if (niter < 500) Pg -= (W_L + W_R + dvel) / (Z_L + Z_R);
+                else Pg -= 0.5 * (W_L + W_R + dvel) / (Z_L + Z_R);
+                if (Pg < 0.1 * Pg_prev) Pg = 0.1 * Pg_prev;
+                tol = 2.0 * fabs((Pg - Pg_prev) / (Pg + Pg_prev));
+                niter++;
+            }
+            if (niter < 1000) {
+                o.P_M = Pg;
+                o.S_M = 0.5 * (vL + vR) + 0.5 * (W_R - W_L);
+                converged = true;
+            }
+        }
+        if (!converged) {
+            // vacuum inside the fan, sampled at S = 0 (reimann.h:818-846)
+            double S_L = vL + G.G4 * cL, S_R = vR - G.G4 * cR;
+            o.P_M = 0;
+            if (0.0 <= S_L) o.S_M = S_L;
+            else if (0.0 >= S_R) o.S_M = S_R;
+            else o.S_M = 0.0;
+        }
+        return 0;
+    }
+    if ((L.rho < 0) || (R.rho < 0)) return 1234545;
+    if (L.rho > 0) { o.P_M = 0; o.S_M = vL + G.G4 * cL; }
+    if (R.rho > 0) { o.P_M = 0; o.S_M = vR - G.G4 * cR; }
+    return 0;
+}
+
+// Riemann_solver() for MFM hydro (reimann.h:286-388).  Returns a stage mask:
+// bit0 HLLC, bit1 KT, bit2 exact; *err gets 1234545 on the exact solver's
+// negative-density exit.
+__device__ inline int riemann_solve(const RState &L, const RState &R, const double n[3], double limiter,
+                                    const GammaConsts &G, ROut &o, int &err)
+{
+    if ((L.p < 0 && R.p < 0) || (L.rho < 0) || (R.rho < 0)) { o.P_M = 0; return 0; }
+    double cL = sqrt(G.G0 * L.p / L.rho);
+    double cR = sqrt(G.G0 * R.p / R.rho);
+    double uL = L.p / (G.G9 * L.rho);
+    double uR = R.p / (G.G9 * R.rho);
+    double hL = L.p / L.rho + uL + 0.5 * (L.v[0] * L.v[0] + L.v[1] * L.v[1] + L.v[2] * L.v[2]);
+    double hR = R.p / R.rho + uR + 0.5 * (R.v[0] * R.v[0] + R.v[1] * R.v[1] + R.v[2] * R.v[2]);
+    double vL = L.v[0] * n[0] + L.v[1] * n[1] + L.v[2] * n[2];
+    double vR = R.v[0] * n[0] + R.v[1] * n[1] + R.v[2] * n[2];
+    int mask = 1;
+    wavespeeds_pstar(L, R, G, n, vL, vR, cL, cR, hL, hR, limiter, o);
+    for (int k = 0; k < 3; k++) o.Fv[k] = o.P_M * n[k];
+    o.Fp = o.P_M * o.S_M;
+    if ((o.P_M < 0) || isnan(o.P_M)) {
+        riemann_kt(L, R, n, vL, vR, cL, cR, hL, hR, o);
+        mask |= 2;
+    }
+    if ((o.P_M < 0) || isnan(o.P_M) || (o.P_M > limiter)) {
+        int e = riemann_exact(L, R, G, vL, vR, cL, cR, o);
+        if (e) err = e;
+        mask |= 4;
+    }
+    return mask;
+}
+
+}  // namespace gzm

@@ -1,0 +1,323 @@
+/*
+ * gizmo_host.c -- GIZMO's hydro entry points over P[]/SphP[] for one rank,
+ * running on the MI355X library (include/gzm.h).  See
+ * include/gizmo_mfm_host.h for the contract.
+ *
+ * Each entry point gathers the fields the stage reads from the particle
+ * structures into contiguous arrays, calls the library, and scatters the
+ * results back for the active particles -- the role the reference's
+ * particle2in / out2particle functions play around its neighbour loops
+ * (density.c:116-244, gradients.c:254-532, hydro_toplevel.c:397-638).
+ */
+#include "gizmo_mfm_host.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gzm.h"
+
+struct particle_data *P = NULL;
+struct gas_cell_data *SphP = NULL;
+struct global_data_all_processes All;
+int NumPart = 0, N_gas = 0;
+int FirstActiveParticle = -1;
+int *NextActiveParticle = NULL;
+
+static gzm_ctx *g_ctx = NULL;
+static int g_device = 0;
+static int g_status = 0;
+static int g_resident = 0;       /* density() results live on the device */
+static gizmo_endrun_handler g_endrun = NULL;
+static char g_err[512];
+
+/* host staging arrays (caller = P[] index order) */
+static int g_cap = 0;
+static double *h_pos, *h_vel, *h_mass, *h_u, *h_hsml, *h_cn, *h_vsig, *h_num_ngb, *h_dhsml, *h_divvel;
+static double *h_rho, *h_pres, *h_nvt, *h_fce, *h_grho, *h_gp, *h_gv, *h_acc, *h_dtu;
+static unsigned char *h_wakeup;
+static int *h_active;
+
+void gizmo_host_set_endrun_handler(gizmo_endrun_handler handler) { g_endrun = handler; }
+
+void endrun(int code)
+{
+    if (g_endrun) {
+        g_endrun(code, g_err);
+        return;
+    }
+    fprintf(stderr, "endrun called with an error level of %d: %s\n", code, g_err);
+    fflush(stderr);
+    exit(code);
+}
+
+int gizmo_host_last_status(void) { return g_status; }
+const char *gizmo_host_last_error(void) { return g_err; }
+int gizmo_host_set_device(int device)
+{
+    g_device = device;
+    return 0;
+}
+
+static void free_staging(void)
+{
+    double **arrs[] = {&h_pos, &h_vel, &h_mass, &h_u, &h_hsml, &h_cn, &h_vsig, &h_num_ngb, &h_dhsml, &h_divvel,
+                       &h_rho, &h_pres, &h_nvt, &h_fce, &h_grho, &h_gp, &h_gv, &h_acc, &h_dtu};
+    for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) { free(*arrs[k]); *arrs[k] = NULL; }
+    free(h_wakeup); h_wakeup = NULL;
+    free(h_active); h_active = NULL;
+    g_cap = 0;
+}
+
+static int ensure_staging(int n)
+{
+    if (n <= g_cap) return 0;
+    free_staging();
+    size_t N = (size_t)n;
+    h_pos = malloc(3 * N * sizeof(double)); h_vel = malloc(3 * N * sizeof(double));
+    h_mass = malloc(N * sizeof(double)); h_u = malloc(N * sizeof(double)); h_hsml = malloc(N * sizeof(double));
+    h_cn = malloc(N * sizeof(double)); h_vsig = malloc(N * sizeof(double)); h_num_ngb = malloc(N * sizeof(double));
+    h_dhsml = malloc(N * sizeof(double)); h_divvel = malloc(N * sizeof(double)); h_rho = malloc(N * sizeof(double));
+    h_pres = malloc(N * sizeof(double)); h_nvt = malloc(9 * N * sizeof(double)); h_fce = malloc(N * sizeof(double));
+    h_grho = malloc(3 * N * sizeof(double)); h_gp = malloc(3 * N * sizeof(double)); h_gv = malloc(9 * N * sizeof(double));
+    h_acc = malloc(3 * N * sizeof(double)); h_dtu = malloc(N * sizeof(double));
+    h_wakeup = malloc(N); h_active = malloc(N * sizeof(int));
+    if (!h_pos || !h_vel || !h_mass || !h_u || !h_hsml || !h_cn || !h_vsig || !h_num_ngb || !h_dhsml || !h_divvel ||
+        !h_rho || !h_pres || !h_nvt || !h_fce || !h_grho || !h_gp || !h_gv || !h_acc || !h_dtu || !h_wakeup || !h_active) {
+        free_staging();
+        return -1;
+    }
+    g_cap = n;
+    return 0;
+}
+
+int gizmo_host_allocate(int n_gas)
+{
+    gizmo_host_free();
+    if (n_gas <= 0) return -1;
+    P = calloc((size_t)n_gas, sizeof(struct particle_data));
+    SphP = calloc((size_t)n_gas, sizeof(struct gas_cell_data));
+    NextActiveParticle = malloc((size_t)n_gas * sizeof(int));
+    if (!P || !SphP || !NextActiveParticle) { gizmo_host_free(); return -1; }
+    NumPart = N_gas = n_gas;
+    All.MaxPart = n_gas;
+    for (int i = 0; i < n_gas; i++) {
+        SphP[i].ConditionNumber = 1;   /* init.c:514 */
+        SphP[i].FaceClosureError = 0;  /* init.c:516 */
+        SphP[i].MaxSignalVel = 1e-56;
+    }
+    gizmo_host_all_active();
+    return 0;
+}
+
+void gizmo_host_free(void)
+{
+    free(P); P = NULL;
+    free(SphP); SphP = NULL;
+    free(NextActiveParticle); NextActiveParticle = NULL;
+    NumPart = N_gas = 0;
+    FirstActiveParticle = -1;
+    g_resident = 0;
+}
+
+void gizmo_host_all_active(void)
+{
+    for (int i = 0; i < NumPart; i++) NextActiveParticle[i] = (i + 1 < NumPart) ? i + 1 : -1;
+    FirstActiveParticle = (NumPart > 0) ? 0 : -1;
+}
+
+int gizmo_host_set_active(const int *idx, int n)
+{
+    for (int k = 0; k < n; k++)
+        if (idx[k] < 0 || idx[k] >= NumPart) return -1;
+    FirstActiveParticle = (n > 0) ? idx[0] : -1;
+    for (int k = 0; k < n; k++) NextActiveParticle[idx[k]] = (k + 1 < n) ? idx[k + 1] : -1;
+    return 0;
+}
+
+void gizmo_host_release(void)
+{
+    if (g_ctx) gzm_ctx_destroy(g_ctx);
+    g_ctx = NULL;
+    g_resident = 0;
+}
+
+/* FirstActiveParticle/NextActiveParticle -> gas indices (the loops' condition
+ * P[i].Type==0; mass/density tests happen inside the library) */
+static int collect_active(void)
+{
+    int n = 0;
+    for (int i = FirstActiveParticle; i >= 0; i = NextActiveParticle[i]) {
+        if (i < N_gas && P[i].Type == 0) h_active[n++] = i;
+        if (n > N_gas) break; /* malformed list */
+    }
+    return n;
+}
+
+static int fail(int status, const char *where)
+{
+    g_status = status;
+    snprintf(g_err, sizeof(g_err), "%s: %s (%s)", where, gzm_status_string(status), gzm_last_error());
+    return status;
+}
+
+static int ensure_ctx(void)
+{
+    gzm_params p;
+    memset(&p, 0, sizeof(p));
+    p.box[0] = p.box[1] = p.box[2] = All.BoxSize;
+    p.gamma = 5.0 / 3.0; /* EOS_GAMMA (allvars.h:1092) */
+    p.des_num_ngb = All.DesNumNgb;
+    p.max_num_ngb_dev = All.MaxNumNgbDeviation;
+    p.min_hsml = All.MinHsml;
+    p.max_hsml = All.MaxHsml;
+    p.first_step = (All.Time == All.TimeBegin);
+    p.maxiter = 150; /* MAXITER (allvars.h:1294) */
+    p.device = g_device;
+    p.precision = 0;
+    p.cell_size = 0;
+    if (g_ctx) gzm_ctx_destroy(g_ctx);
+    g_ctx = NULL;
+    int s = gzm_ctx_create(&g_ctx, &p);
+    if (s) return fail(s, "gzm_ctx_create");
+    return 0;
+}
+
+/* P[]/SphP[] -> staging; with_state also sends the density/gradient results
+ * that inactive particles contribute as neighbours */
+static int upload(int with_state)
+{
+    int n = N_gas;
+    if (ensure_staging(n)) return fail(GZM_ERR_NOMEM, "host staging");
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) {
+            h_pos[3 * i + k] = P[i].Pos[k];
+            h_vel[3 * i + k] = SphP[i].VelPred[k];
+        }
+        h_mass[i] = P[i].Mass;
+        h_u[i] = SphP[i].InternalEnergyPred;
+        h_hsml[i] = P[i].Hsml;
+        h_cn[i] = SphP[i].ConditionNumber;
+        h_vsig[i] = SphP[i].MaxSignalVel;
+        h_num_ngb[i] = P[i].NumNgb;
+        h_dhsml[i] = P[i].DhsmlNgbFactor;
+        h_divvel[i] = P[i].Particle_DivVel;
+        h_rho[i] = SphP[i].Density;
+        h_pres[i] = SphP[i].Pressure;
+        h_fce[i] = SphP[i].FaceClosureError;
+        memcpy(h_nvt + 9 * (size_t)i, SphP[i].NV_T, 9 * sizeof(double));
+        memcpy(h_grho + 3 * (size_t)i, SphP[i].Gradients.Density, 3 * sizeof(double));
+        memcpy(h_gp + 3 * (size_t)i, SphP[i].Gradients.Pressure, 3 * sizeof(double));
+        memcpy(h_gv + 9 * (size_t)i, SphP[i].Gradients.Velocity, 9 * sizeof(double));
+    }
+    gzm_host_view v;
+    memset(&v, 0, sizeof(v));
+    v.n = n;
+    v.pos = h_pos; v.vel = h_vel; v.mass = h_mass; v.u = h_u;
+    v.hsml = h_hsml; v.cn = h_cn; v.vsig = h_vsig;
+    v.num_ngb = h_num_ngb; v.dhsml = h_dhsml; v.divvel = h_divvel; v.fce = h_fce;
+    if (with_state) {
+        v.density = h_rho; v.pressure = h_pres; v.nvt = h_nvt;
+        v.grad_rho = h_grho; v.grad_p = h_gp; v.grad_v = h_gv;
+    }
+    int s = gzm_upload(g_ctx, &v);
+    if (s) return fail(s, "gzm_upload");
+    return 0;
+}
+
+static int download(void)
+{
+    gzm_host_view v;
+    memset(&v, 0, sizeof(v));
+    v.n = N_gas;
+    v.hsml = h_hsml; v.cn = h_cn; v.vsig = h_vsig; v.num_ngb = h_num_ngb; v.dhsml = h_dhsml; v.divvel = h_divvel;
+    v.density = h_rho; v.pressure = h_pres; v.nvt = h_nvt; v.fce = h_fce;
+    v.grad_rho = h_grho; v.grad_p = h_gp; v.grad_v = h_gv; v.acc = h_acc; v.dtu = h_dtu; v.wakeup = h_wakeup;
+    int s = gzm_download(g_ctx, &v);
+    if (s) return fail(s, "gzm_download");
+    return 0;
+}
+
+void density(void)
+{
+    g_status = 0;
+    if (N_gas <= 0) return;
+    if (ensure_ctx() || upload(0)) { endrun(g_status); return; }
+    int na = collect_active();
+    gzm_pass_stats st;
+    memset(&st, 0, sizeof(st));
+    int s = gzm_density(g_ctx, h_active, na, &st);
+    if (s) { fail(s, "gzm_density"); endrun(s); return; }
+    if (download()) { endrun(g_status); return; }
+    for (int a = 0; a < na; a++) {
+        int i = h_active[a];
+        if (!(P[i].Mass > 0)) continue;
+        P[i].Hsml = h_hsml[i];
+        P[i].NumNgb = h_num_ngb[i];
+        P[i].DhsmlNgbFactor = h_dhsml[i];
+        P[i].Particle_DivVel = h_divvel[i];
+        SphP[i].Density = h_rho[i];
+        SphP[i].Pressure = h_pres[i];
+        SphP[i].ConditionNumber = h_cn[i];
+        SphP[i].FaceClosureError = h_fce[i];
+        memcpy(SphP[i].NV_T, h_nvt + 9 * (size_t)i, 9 * sizeof(double));
+    }
+    g_resident = 1;
+}
+
+void hydro_gradient_calc(void)
+{
+    g_status = 0;
+    if (N_gas <= 0) return;
+    if (!g_resident) {
+        if (ensure_ctx() || upload(1)) { endrun(g_status); return; }
+        g_resident = 1;
+    }
+    int na = collect_active();
+    gzm_pass_stats st;
+    memset(&st, 0, sizeof(st));
+    int s = gzm_gradients(g_ctx, h_active, na, &st);
+    if (s) { fail(s, "gzm_gradients"); endrun(s); return; }
+    if (download()) { endrun(g_status); return; }
+    for (int a = 0; a < na; a++) {
+        int i = h_active[a];
+        memcpy(SphP[i].Gradients.Density, h_grho + 3 * (size_t)i, 3 * sizeof(double));
+        memcpy(SphP[i].Gradients.Pressure, h_gp + 3 * (size_t)i, 3 * sizeof(double));
+        memcpy(SphP[i].Gradients.Velocity, h_gv + 9 * (size_t)i, 9 * sizeof(double));
+    }
+}
+
+void hydro_force(void)
+{
+    g_status = 0;
+    if (N_gas <= 0) return;
+    if (!g_resident) {
+        if (ensure_ctx() || upload(1)) { endrun(g_status); return; }
+        g_resident = 1;
+    }
+    int na = collect_active();
+    gzm_pass_stats st;
+    memset(&st, 0, sizeof(st));
+    int s = gzm_hydro_force(g_ctx, h_active, na, &st);
+    if (s) { fail(s, "gzm_hydro_force"); endrun(s); return; }
+    if (download()) { endrun(g_status); return; }
+    for (int a = 0; a < na; a++) {
+        int i = h_active[a];
+        for (int k = 0; k < 3; k++) SphP[i].HydroAccel[k] = h_acc[3 * (size_t)i + k];
+        SphP[i].DtInternalEnergy = h_dtu[i];
+        SphP[i].MaxSignalVel = h_vsig[i];
+    }
+    for (int i = 0; i < N_gas; i++)
+        if (h_wakeup[i]) SphP[i].wakeup = 1;
+}
+
+void compute_hydro_densities_and_forces(void)
+{
+    /* accel.c:57-111: density, force_update_hmax (the library derives the
+     * per-cell search bound from the current hsml), gradients, force */
+    density();
+    if (g_status) return;
+    hydro_gradient_calc();
+    if (g_status) return;
+    hydro_force();
+}

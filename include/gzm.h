@@ -1,0 +1,165 @@
+/*
+ * gzm.h -- C ABI of the MI355X-native MFM hydro pass (libgzm.so).
+ *
+ * The library replaces the arithmetic and neighbour search behind GIZMO's
+ * three hydro entry points for one rank's particle set on one GPU:
+ *
+ *   gzm_density()      replaces density()              (hydro/density.c:451-1044,
+ *                                                        proto.h:723)
+ *   gzm_gradients()    replaces hydro_gradient_calc()  (hydro/gradients.c:577-1469,
+ *                                                        proto.h:962)
+ *   gzm_hydro_force()  replaces hydro_force()          (hydro/hydro_toplevel.c:1045-1057,
+ *                                                        proto.h:749)
+ *
+ * The reference runs each as an instance of its neighbour-operator template
+ * (system/code_block_xchange_perform_ops.h) over the active list
+ * FirstActiveParticle/NextActiveParticle[] (allvars.h:1652-1654) with an
+ * octree search (ngb.c:102-130).  Here the search is a cell grid over
+ * particles sorted by cell key, and every stage is a HIP kernel for gfx950.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Host arrays are in the caller's particle
+ *    order, FP64, contiguous: pos/vel/acc [n][3], nvt [n][9] row-major,
+ *    grad_rho/grad_p [n][3], grad_v [n][3][3] with grad_v[i][k][d] = dv_k/dx_d
+ *    (the reference's SphP.Gradients.Velocity[k][d]).
+ *  - Active lists hold caller indices; NULL means "all particles".
+ *  - Calls are synchronous with respect to the host; device state stays
+ *    resident between calls.  A context is not re-entrant; use one per host
+ *    thread.  The library never aborts: failures the reference turns into
+ *    endrun(code) come back as that code (1155, 1234, 1234545).
+ */
+#ifndef GZM_H
+#define GZM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gzm_status {
+    GZM_OK = 0,
+    GZM_ERR_ARG = 1,            /* bad argument (NULL pointer, n <= 0, size mismatch) */
+    GZM_ERR_HIP = 2,            /* HIP runtime error (message via gzm_last_error) */
+    GZM_ERR_NOMEM = 3,          /* device allocation failed */
+    GZM_ERR_STATE = 4,          /* call out of order (e.g. force before density) */
+    GZM_ERR_NO_DEVICE = 5,      /* no usable GPU */
+    GZM_ERR_BRACKET = 6,        /* Left == Right == 0 in the hsml iteration (density.c:780-783) */
+    GZM_ERR_HSML_NOT_CONVERGED = 1155, /* density.c:859 */
+    GZM_ERR_RIEMANN = 1234,            /* hydro_core_meshless.h:262 */
+    GZM_ERR_RIEMANN_NEG_DENSITY = 1234545 /* reimann.h:714 */
+} gzm_status;
+
+/* Run-time parameters: the All.* fields the three stages read
+ * (begrun.c:1227-1242 tags, allvars.h:1091-1113 compile-time defaults). */
+typedef struct gzm_params {
+    double box[3];          /* boxSize_X/Y/Z; the box is periodic in all axes */
+    double gamma;           /* EOS_GAMMA, 5/3 by default */
+    double des_num_ngb;     /* All.DesNumNgb */
+    double max_num_ngb_dev; /* All.MaxNumNgbDeviation */
+    double min_hsml;        /* All.MinHsml */
+    double max_hsml;        /* All.MaxHsml */
+    int32_t first_step;     /* 1 when All.Time == All.TimeBegin (density.c:509) */
+    int32_t maxiter;        /* MAXITER, 150 (allvars.h:1294) */
+    int32_t device;         /* HIP device ordinal */
+    int32_t precision;      /* 0: plain FP64 pass (the only mode of this release) */
+    double cell_size;       /* grid cell side; <= 0 picks one from the initial hsml */
+} gzm_params;
+
+/* Host arrays of one particle set, caller order.  Inputs are read by
+ * gzm_upload; every non-NULL output is written by gzm_download. */
+typedef struct gzm_host_view {
+    int64_t n;
+    /* inputs: P[].Pos, SphP[].VelPred, P[].Mass, SphP[].InternalEnergyPred */
+    const double *pos;
+    const double *vel;
+    const double *mass;
+    const double *u;
+    /* state carried between passes (in at upload, out at download):
+     * P[].Hsml, SphP[].ConditionNumber, SphP[].MaxSignalVel */
+    double *hsml;
+    double *cn;
+    double *vsig;
+    /* state of inactive particles that active ones read (in at upload when
+     * non-NULL; recomputed for active particles by the stages) */
+    double *num_ngb;   /* P[].NumNgb (NumNgb^(1/3) after density) */
+    double *dhsml;     /* P[].DhsmlNgbFactor */
+    double *divvel;    /* P[].Particle_DivVel */
+    double *density;   /* SphP[].Density */
+    double *pressure;  /* SphP[].Pressure */
+    double *nvt;       /* SphP[].NV_T (inverse, after density) [n][9] */
+    double *fce;       /* SphP[].FaceClosureError */
+    double *grad_rho;  /* SphP[].Gradients.Density [n][3] */
+    double *grad_p;    /* SphP[].Gradients.Pressure [n][3] */
+    double *grad_v;    /* SphP[].Gradients.Velocity [n][3][3] */
+    /* force outputs: SphP[].HydroAccel, SphP[].DtInternalEnergy, wakeup flags */
+    double *acc;
+    double *dtu;
+    uint8_t *wakeup;
+} gzm_host_view;
+
+typedef struct gzm_pass_stats {
+    int32_t density_sweeps;          /* iterations of the hsml loop */
+    int32_t status;
+    int64_t density_interactions;    /* sum over sweeps of |{j : r_ij^2 < h_i^2}| */
+    int64_t gradient_interactions;   /* sum over active i of the pair-set size */
+    int64_t force_interactions;      /* idem for the force loop */
+    int64_t riemann_retries;         /* pairs that needed the first-order retry */
+    int64_t exact_solver_calls;      /* pairs that reached the exact solver */
+} gzm_pass_stats;
+
+/* Per-kernel device time (HIP events on the context stream), accumulated
+ * since the last gzm_timing_reset. */
+typedef struct gzm_timing {
+    double density_ms;   int64_t density_launches;
+    double gradients_ms; int64_t gradients_launches;
+    double force_ms;     int64_t force_launches;
+    double other_ms;     int64_t other_launches;   /* grid build, packing, finalize */
+} gzm_timing;
+
+typedef struct gzm_ctx gzm_ctx;
+
+const char *gzm_status_string(int status);
+const char *gzm_last_error(void);
+
+/* Context: owns the stream, the cell grid and every device array. */
+gzm_status gzm_ctx_create(gzm_ctx **ctx, const gzm_params *params);
+gzm_status gzm_ctx_destroy(gzm_ctx *ctx);
+
+/* Copy one particle set to the device, sort it by cell and build the grid
+ * (the role of domain_Decomposition + force_treebuild, domain.c:106,
+ * gravity/forcetree.c:109, for this rank). */
+gzm_status gzm_upload(gzm_ctx *ctx, const gzm_host_view *view);
+
+/* density(): hsml iteration to convergence over the active list, then the
+ * final density, pressure and NumNgb^(1/3) (density.c:451-1044). */
+gzm_status gzm_density(gzm_ctx *ctx, const int32_t *active, int64_t n_active, gzm_pass_stats *stats);
+
+/* hydro_gradient_calc(): least-squares gradients of rho, P, v with the slope
+ * limiter (gradients.c:577-1469). */
+gzm_status gzm_gradients(gzm_ctx *ctx, const int32_t *active, int64_t n_active, gzm_pass_stats *stats);
+
+/* hydro_force(): MFM faces, HLLC fluxes, dv/dt, du/dt, v_sig, wakeup flags
+ * (hydro_toplevel.c:961-1057, hydro_evaluate.h:22-516). */
+gzm_status gzm_hydro_force(gzm_ctx *ctx, const int32_t *active, int64_t n_active, gzm_pass_stats *stats);
+
+/* Copy results back into the caller-order host arrays of `view`. */
+gzm_status gzm_download(gzm_ctx *ctx, gzm_host_view *view);
+
+/* Restore hsml, cn and vsig to the values given at upload (repeated passes
+ * over the same input, e.g. for timing). Device-side copy only. */
+gzm_status gzm_restore_initial_state(gzm_ctx *ctx);
+
+/* Neighbour-set sizes for the current hsml, caller order of `active`:
+ * mode 0 density set {j : r2 < h_i^2}, mode 1 pair set
+ * {j : r2 > 0 and (r2 < h_i^2 or r2 < h_j^2)}.  counts may be NULL. */
+gzm_status gzm_neighbor_counts(gzm_ctx *ctx, int32_t mode, const int32_t *active, int64_t n_active,
+                               int32_t *counts, int64_t *total);
+
+gzm_status gzm_timing_get(gzm_ctx *ctx, gzm_timing *timing);
+gzm_status gzm_timing_reset(gzm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
