@@ -537,8 +537,11 @@ gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
         s = upload_sorted(c, g.data(), c->d_grad, 15);
         if (s) return s;
     } else launch_fill_double(c->stream, c->d_grad, (long long)n * 15, 0.0);
-    launch_fill_double(c->stream, c->d_acc, (long long)n * 3, 0.0);
-    launch_fill_double(c->stream, c->d_dtu, n, 0.0);
+    // force outputs of particles that stay inactive round-trip unchanged
+    if (v->acc) { s = upload_sorted(c, v->acc, c->d_acc, 3); if (s) return s; }
+    else launch_fill_double(c->stream, c->d_acc, (long long)n * 3, 0.0);
+    if (v->dtu) { s = upload_sorted(c, v->dtu, c->d_dtu, 1); if (s) return s; }
+    else launch_fill_double(c->stream, c->d_dtu, n, 0.0);
     launch_fill_u8(c->stream, c->d_wakeup, n, 0);
     GZM_HIP(hipGetLastError());
     GZM_HIP(hipStreamSynchronize(c->stream));
@@ -594,8 +597,10 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
     a.redo = c->d_flags;
     a.ctr = c->d_ctr;
     int iter = 0, sweeps = 0;
+    int64_t updates = 0;
     int status = GZM_OK;
     while (n_live > 0) {
+        updates += n_live;
         a.hp.iter = iter;
         a.live = c->d_live;
         a.n_live = n_live;
@@ -629,6 +634,7 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
     if (stats) {
         stats->density_sweeps = sweeps;
         stats->density_interactions = (int64_t)h.interactions;
+        stats->density_updates = updates;
         stats->status = status;
     }
     c->have_density = true;

@@ -63,6 +63,7 @@ class PassStats(C.Structure):
         ("force_interactions", C.c_int64),
         ("riemann_retries", C.c_int64),
         ("exact_solver_calls", C.c_int64),
+        ("density_updates", C.c_int64),
     ]
 
 
@@ -194,7 +195,7 @@ class MFMContext:
         v.u = _p(arr("u", required=True))
         v.hsml = _p(arr("hsml", required=True))
         for name in ("cn", "vsig", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "fce",
-                     "grad_rho", "grad_p", "grad_v"):
+                     "grad_rho", "grad_p", "grad_v", "acc", "dtu"):
             setattr(v, name, _p(arr(name)))
         _check(self.L.gzm_upload(self.ctx, C.byref(v)), "gzm_upload")
 
@@ -268,7 +269,8 @@ def run_pass(ic: dict, active=None, params: Params | None = None, stages=("densi
         stats = {}
         if "density" in stages:
             s = ctx.density(active)
-            stats.update(density_sweeps=s.density_sweeps, density_interactions=s.density_interactions)
+            stats.update(density_sweeps=s.density_sweeps, density_interactions=s.density_interactions,
+                         density_updates=s.density_updates)
         if "gradients" in stages:
             s = ctx.gradients(active)
             stats.update(gradient_interactions=s.gradient_interactions)

@@ -92,7 +92,9 @@ typedef struct gzm_host_view {
     double *grad_rho;  /* SphP[].Gradients.Density [n][3] */
     double *grad_p;    /* SphP[].Gradients.Pressure [n][3] */
     double *grad_v;    /* SphP[].Gradients.Velocity [n][3][3] */
-    /* force outputs: SphP[].HydroAccel, SphP[].DtInternalEnergy, wakeup flags */
+    /* force outputs: SphP[].HydroAccel, SphP[].DtInternalEnergy, wakeup flags
+     * (acc and dtu are also read at upload when non-NULL, so that inactive
+     * particles keep their values) */
     double *acc;
     double *dtu;
     uint8_t *wakeup;
@@ -106,6 +108,7 @@ typedef struct gzm_pass_stats {
     int64_t force_interactions;      /* idem for the force loop */
     int64_t riemann_retries;         /* pairs that needed the first-order retry */
     int64_t exact_solver_calls;      /* pairs that reached the exact solver */
+    int64_t density_updates;         /* sum over sweeps of the particles swept */
 } gzm_pass_stats;
 
 /* Per-kernel device time (HIP events on the context stream), accumulated

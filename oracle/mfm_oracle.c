@@ -386,6 +386,7 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                 kernel_hinv(h, &hinv, &hinv3, &hinv4);
                 nb_collect(&b, p, &g, n, pos, hsml, i, h, h, 0);
                 double Ngb = 0, Rho = 0, DhsmlNgb = 0, DivVel = 0, NVT[3][3] = {{0}};
+                double divvel_l1 = 0, f_l1[3] = {0, 0, 0};
                 for (int q = 0; q < b.cnt; q++) {
                     int j = b.j[q];
                     if (mass[j] <= 0) continue;
@@ -412,6 +413,10 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                         double dv1 = vel[3 * i + 1] - vel[3 * j + 1];
                         double dv2 = vel[3 * i + 2] - vel[3 * j + 2];
                         DivVel -= dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2) / r;
+                        divvel_l1 += fabs(dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2) / r);
+                        f_l1[0] += fabs(wk * dp[0]);
+                        f_l1[1] += fabs(wk * dp[1]);
+                        f_l1[2] += fabs(wk * dp[2]);
                     }
                 }
                 /* ---- iteration body (density.c:505-850) ---- */
@@ -420,6 +425,7 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                 double desnumngbdev_0 = desnumngbdev, ConditionNumber = 0;
                 double NumNgb = Ngb, Dhsml = DhsmlNgb, Tinv[3][3];
                 if (NumNgb > 0) {
+                    divvel_l1 /= NumNgb;
                     Dhsml *= h / (NUMDIMS * NumNgb);
                     DivVel /= NumNgb;
                     NumNgb *= NORM_COEFF * pow(h, NUMDIMS);
@@ -428,7 +434,8 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                 }
                 if (Dhsml > -0.9) Dhsml = 1 / (1 + Dhsml); else Dhsml = 1;
                 DivVel *= Dhsml;
-                double FCE;
+                divvel_l1 *= fabs(Dhsml);
+                double FCE, FCE_l1 = 0;
                 {
                     double V_i = NORM_COEFF * pow(h, NUMDIMS) / NumNgb;
                     double dx_i = sqrt(V_i * (NVT[0][0] + NVT[1][1] + NVT[2][2]));
@@ -451,6 +458,13 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                     double leak = 0;
                     for (int k1 = 0; k1 < 3; k1++) leak += fabs(fout[k1]) / NUMDIMS;
                     FCE = leak / (2. * NUMDIMS * pow(dx_i, NUMDIMS - 1));
+                    double leak_l1 = 0;
+                    for (int k1 = 0; k1 < 3; k1++) {
+                        double o = 0;
+                        for (int k2 = 0; k2 < 3; k2++) o += fabs(2. * V_i * NVT[k1][k2]) * f_l1[k2];
+                        leak_l1 += o / NUMDIMS;
+                    }
+                    FCE_l1 = leak_l1 / (2. * NUMDIMS * pow(dx_i, NUMDIMS - 1));
                 }
                 /* neighbour-number target (density.c:565-595) */
                 double cn_stored = s->cn[i];
@@ -570,6 +584,8 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                 memcpy(s->nvt + 9 * (size_t)i, NVT, 9 * sizeof(double));
                 s->fce[i] = FCE;
                 s->cn[i] = cn_stored;
+                if (s->divvel_l1) s->divvel_l1[i] = divvel_l1;
+                if (s->fce_l1) s->fce_l1[i] = FCE_l1;
             }
             nb_free(&b);
         }
@@ -669,6 +685,7 @@ int orc_gradients(const orc_params *p, orc_state *s, const int *active, int n_ac
             int i = active[a];
             double gr[3] = {0, 0, 0}, gp[3] = {0, 0, 0}, gv[3][3] = {{0}};
             double maxd = 0, mx_rho = 0, mn_rho = 0, mx_p = 0, mn_p = 0, mx_v[3] = {0, 0, 0}, mn_v[3] = {0, 0, 0};
+            double l1[15] = {0};
             if (gasgrad_isactive(s, i)) {
                 double rho_i = s->density[i], p_i = s->pressure[i];
                 const double *v_i = s->vel + 3 * i;
@@ -697,12 +714,22 @@ int orc_gradients(const orc_params *p, orc_state *s, const int *active, int n_ac
                             gr[k] += wk_xyz_i * d_rho;
                             gp[k] += wk_xyz_i * dpp;
                             for (int k2 = 0; k2 < 3; k2++) gv[k2][k] += wk_xyz_i * dv[k2];
+                            l1[k] += fabs(wk_xyz_i * d_rho);
+                            l1[3 + k] += fabs(wk_xyz_i * dpp);
+                            for (int k2 = 0; k2 < 3; k2++) l1[6 + 3 * k2 + k] += fabs(wk_xyz_i * dv[k2]);
                         }
                     }
                 }
             }
             /* finalize (gradients.c:1052-1058, 1219-1253) */
             const double *nvt = s->nvt + 9 * (size_t)i;
+            if (s->grad_l1) {
+                double *o = s->grad_l1 + 15 * (size_t)i;
+                for (int q = 0; q < 5; q++)
+                    for (int k = 0; k < 3; k++)
+                        o[3 * q + k] = fabs(nvt[3 * k + 0]) * l1[3 * q + 0] + fabs(nvt[3 * k + 1]) * l1[3 * q + 1] +
+                                       fabs(nvt[3 * k + 2]) * l1[3 * q + 2];
+            }
             construct_gradient(gr, nvt);
             construct_gradient(gp, nvt);
             for (int k = 0; k < 3; k++) construct_gradient(gv[k], nvt);
@@ -1042,10 +1069,17 @@ static void force_i_setup(const orc_params *p, const orc_state *s, int i, force_
 /* one (i,j) interaction of hydro_force_evaluate: hydro_evaluate.h:170-298 and
  * the MFM core.  Writes Fv[3], Fp, vsig; returns status (0, 1234, 1234545).
  * `ro` is the per-i Riemann output that the reference keeps across pairs. */
+static int near_tie(double a, double b)
+{
+    double m = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+    return fabs(a - b) <= 4.0 * (nextafter(m, INFINITY) - m);
+}
+
 static int force_pair(const orc_params *p, const orc_state *s, const force_i_t *fi, const gconst *c,
                       int j, const double dp[3], double r2, rout *ro, double Fv_out[3], double *Fp_out,
-                      double *vsig_out, int *mask_out)
+                      double *vsig_out, int *mask_out, double *Fp_abs_out)
 {
+    double Fp_abs = 0; /* sum of |terms| that make up Fluxes_p (error scale only) */
     const double epsilon_entropic_eos_big = 0.5, epsilon_entropic_eos_small = 1.e-3; /* hydro_evaluate.h:45-54 */
     int status = 0;
     const double *v_i = fi->v;
@@ -1156,8 +1190,10 @@ static int force_pair(const orc_params *p, const orc_state *s, const force_i_t *
         }
         if (err) status = err;
         if ((ro->P_M > 0) && (!isnan(ro->P_M))) {
-            for (int k = 0; k < 3; k++) ro->Fp += v_frame[k] * ro->Fv[k];
+            Fp_abs = fabs(ro->Fp);
+            for (int k = 0; k < 3; k++) { ro->Fp += v_frame[k] * ro->Fv[k]; Fp_abs += fabs(v_frame[k] * ro->Fv[k]); }
             Fluxes_p = Face_Area_Norm * ro->Fp;
+            Fp_abs *= Face_Area_Norm;
             for (int k = 0; k < 3; k++) Fluxes_v[k] = Face_Area_Norm * ro->Fv[k];
             /* MFM adiabatic face correction (hydro_core_meshless.h:335-374) */
             double SM_over_ceff = fabs(ro->S_M) / DMIN(sound_i, sound_j);
@@ -1171,6 +1207,7 @@ static int force_pair(const orc_params *p, const orc_state *s, const force_i_t *
                 double du_new = 0.5 * (PdV_i - PdV_j + facenorm_pm * (face_vel_i + face_vel_j));
                 double cnum2 = cn_j * cn_j;
                 if (SM_over_ceff > epsilon_entropic_eos_small && cnum2 < fi->cnumcrit2) {
+                    if (near_tie(Pressure_i / rho_i, Pressure_j / rho_j)) mask |= 16;
                     if (Pressure_i / rho_i != Pressure_j / rho_j) {
                         if (Pressure_i / rho_i > Pressure_j / rho_j) {
                             double dtoj = -du_old + facenorm_pm * face_vel_j;
@@ -1184,7 +1221,11 @@ static int force_pair(const orc_params *p, const orc_state *s, const force_i_t *
                     }
                 }
                 if (cnum2 >= fi->cnumcrit2) use_entropic_energy_equation = 1;
-                if (use_entropic_energy_equation) Fluxes_p += du_new - du_old;
+                if (use_entropic_energy_equation) {
+                    Fluxes_p += du_new - du_old;
+                    Fp_abs += fabs(0.5 * PdV_i) + fabs(0.5 * PdV_j) + fabs(0.5 * facenorm_pm * (face_vel_i + face_vel_j)) +
+                              fabs(du_old);
+                }
             }
         } else {
             Fluxes_p = 0; Fluxes_v[0] = Fluxes_v[1] = Fluxes_v[2] = 0;
@@ -1194,6 +1235,7 @@ static int force_pair(const orc_params *p, const orc_state *s, const force_i_t *
     *Fp_out = Fluxes_p;
     *vsig_out = vsig;
     if (mask_out) *mask_out = mask;
+    if (Fp_abs_out) *Fp_abs_out = Fp_abs;
     return status;
 }
 
@@ -1211,7 +1253,7 @@ int orc_force_pair(const orc_params *p, const orc_state *s, int i, int j, double
     rout ro;
     memset(&ro, 0, sizeof(ro));
     int mask = 0;
-    int stt = force_pair(p, s, &fi, &c, j, dp, r2, &ro, out, out + 3, out + 4, &mask);
+    int stt = force_pair(p, s, &fi, &c, j, dp, r2, &ro, out, out + 3, out + 4, &mask, NULL);
     out[5] = mask;
     return stt;
 }
@@ -1234,6 +1276,8 @@ int orc_hydro_force(const orc_params *p, orc_state *s, const int *active, int n_
             int i = active[a];
             /* preloop (hydro_toplevel.c:968-976) */
             double Acc[3] = {0, 0, 0}, DtE = 0, MaxSignalVel = MIN_REAL_NUMBER;
+            double acc_l1[3] = {0, 0, 0}, dte_l1 = 0;
+            int tie = 0;
             double m_i = s->mass[i], rho_i = s->density[i];
             if ((m_i > 0) && (rho_i > 0)) {
                 force_i_t fi;
@@ -1247,14 +1291,17 @@ int orc_hydro_force(const orc_params *p, orc_state *s, const int *active, int n_
                     if (s->mass[j] <= 0) continue;
                     if (s->density[j] <= 0) continue;
                     nint++;
-                    double Fv[3], Fp, vsig;
+                    double Fv[3], Fp, vsig, Fp_abs;
                     int mask = 0;
-                    int e = force_pair(p, s, &fi, &c, j, b.dp + 3 * q, b.r2[q], &ro, Fv, &Fp, &vsig, &mask);
+                    int e = force_pair(p, s, &fi, &c, j, b.dp + 3 * q, b.r2[q], &ro, Fv, &Fp, &vsig, &mask, &Fp_abs);
                     if (e > status) status = e;
                     if (mask & 8) nretry++;
+                    if (mask & 16) tie = 1;
                     if (mask & 4) nexact++;
                     for (int k = 0; k < 3; k++) Acc[k] += Fv[k];
                     DtE += Fp;
+                    for (int k = 0; k < 3; k++) acc_l1[k] += fabs(Fv[k]);
+                    dte_l1 += Fp_abs;
                     if (vsig > out_vsig) out_vsig = vsig;
                     if (s->wakeup && !s->is_active[j]) {
                         if (vsig > WAKEUP * s->vsig[j]) {
@@ -1268,8 +1315,11 @@ int orc_hydro_force(const orc_params *p, orc_state *s, const int *active, int n_
                 for (int k = 0; k < 3; k++) {
                     DtE -= (fi.v[k] / 1.0) * Acc[k];
                     Acc[k] /= m_i;
+                    dte_l1 += fabs(fi.v[k]) * acc_l1[k];
+                    acc_l1[k] /= m_i;
                 }
                 DtE /= m_i;
+                dte_l1 /= m_i;
                 if (fi.h >= 0.99 * p->max_hsml) DtE = 0;
             } else if (m_i > 0) {
                 /* rho <= 0: the pair loop returns at once; the final operations still run */
@@ -1280,6 +1330,9 @@ int orc_hydro_force(const orc_params *p, orc_state *s, const int *active, int n_
             s->acc[3 * (size_t)i + 2] = Acc[2];
             s->dtu[i] = DtE;
             s->vsig[i] = MaxSignalVel;
+            if (s->acc_l1) for (int k = 0; k < 3; k++) s->acc_l1[3 * (size_t)i + k] = acc_l1[k];
+            if (s->dtu_l1) s->dtu_l1[i] = dte_l1;
+            if (s->tie_flag) s->tie_flag[i] = (unsigned char)tie;
         }
         nb_free(&b);
     }

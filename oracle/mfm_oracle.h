@@ -62,6 +62,20 @@ typedef struct orc_state {
     double *vsig;
     unsigned char *wakeup;    /* may be NULL */
     unsigned char *is_active; /* [n] 1 if particle is in the active list */
+    /* optional (may be NULL): sums of the absolute values of the pair terms
+     * behind each summed output, after the same normalisation.  They bound
+     * what a different summation order can change (n * eps * sum|terms|)
+     * and serve as the error scale of parity tests. */
+    double *divvel_l1;        /* [n] */
+    double *fce_l1;           /* [n] */
+    double *grad_l1;          /* [n][15]: rho(3), P(3), v(3x3) */
+    double *acc_l1;           /* [n][3] */
+    double *dtu_l1;           /* [n] */
+    /* optional: 1 where the force loop compared P_i/rho_i with P_j/rho_j
+     * (hydro_core_meshless.h:349-351) for values within 4 ulp of each other.
+     * Such a branch is decided by rounding, so any change of summation order
+     * (in the reference as well) can flip it. */
+    unsigned char *tie_flag;  /* [n] */
 } orc_state;
 
 typedef struct orc_stats {

@@ -44,6 +44,8 @@ class State(C.Structure):
         ("grad_rho", _dp), ("grad_p", _dp), ("grad_v", _dp),
         ("acc", _dp), ("dtu", _dp), ("vsig", _dp),
         ("wakeup", _ub), ("is_active", _ub),
+        ("divvel_l1", _dp), ("fce_l1", _dp), ("grad_l1", _dp), ("acc_l1", _dp), ("dtu_l1", _dp),
+        ("tie_flag", _ub),
     ]
 
 
@@ -180,10 +182,16 @@ class OracleState:
         self.grad_rho = np.ascontiguousarray(ic.get("grad_rho", z(N, 3)), dtype=np.float64).copy()
         self.grad_p = np.ascontiguousarray(ic.get("grad_p", z(N, 3)), dtype=np.float64).copy()
         self.grad_v = np.ascontiguousarray(ic.get("grad_v", z(N, 3, 3)), dtype=np.float64).copy()
-        self.acc = z(N, 3)
-        self.dtu = z(N)
+        self.acc = np.ascontiguousarray(ic.get("acc", z(N, 3)), dtype=np.float64).copy()
+        self.dtu = np.ascontiguousarray(ic.get("dtu", z(N)), dtype=np.float64).copy()
         self.vsig = np.ascontiguousarray(ic.get("vsig", np.full(N, 1e-56)), dtype=np.float64).copy()
         self.wakeup = np.zeros(N, dtype=np.uint8)
+        self.divvel_l1 = z(N)
+        self.fce_l1 = z(N)
+        self.grad_l1 = z(N, 15)
+        self.acc_l1 = z(N, 3)
+        self.dtu_l1 = z(N)
+        self.tie_flag = np.zeros(N, dtype=np.uint8)
         if active is None:
             active = np.arange(N, dtype=np.int32)
         self.active = np.ascontiguousarray(active, dtype=np.int32)
@@ -198,11 +206,15 @@ class OracleState:
             setattr(s, name, ptr(getattr(self, name)))
         s.wakeup = ptr(self.wakeup, _ub)
         s.is_active = ptr(self.is_active, _ub)
+        for name in ("divvel_l1", "fce_l1", "grad_l1", "acc_l1", "dtu_l1"):
+            setattr(s, name, ptr(getattr(self, name)))
+        s.tie_flag = ptr(self.tie_flag, _ub)
         return s
 
     def outputs(self) -> dict:
         names = ("hsml", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "cn", "fce",
-                 "grad_rho", "grad_p", "grad_v", "acc", "dtu", "vsig", "wakeup")
+                 "grad_rho", "grad_p", "grad_v", "acc", "dtu", "vsig", "wakeup",
+                 "divvel_l1", "fce_l1", "grad_l1", "acc_l1", "dtu_l1", "tie_flag")
         return {k: getattr(self, k).copy() for k in names}
 
 

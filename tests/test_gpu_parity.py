@@ -1,0 +1,238 @@
+"""Parity of the HIP pass (libgzm.so through its C ABI) with the FP64 oracle.
+
+Bar: neighbour sets and interaction counts bit-exact; density, hsml, NumNgb,
+pressure, signal speed within 1e-12 relative; gradients, dv/dt, du/dt within
+1e-10 of the sum of |pair terms| (see tests/parity.py).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import gzm
+import ics
+import oracle as O
+from parity import TOL_DENSITY, TOL_FLUX, check_pass
+
+pytestmark = pytest.mark.gpu
+
+def _untied(ic, seed=1):
+    """Same particle set with u perturbed by ~1e-9 relative: no two particles
+    share P/rho, so the entropic-switch comparison is never a rounding tie and
+    du/dt is held to the strict tolerance everywhere."""
+    ic = dict(ic)
+    ic["u"] = ic["u"] * (1.0 + 1e-9 * ics.uniform(seed, 77, ic["u"].size))
+    return ic
+
+
+CONFIGS = {
+    "lattice_gaussian_8": lambda: ics.lattice_gaussian(8),
+    "lattice_gaussian_16": lambda: ics.lattice_gaussian(16),
+    "lattice_gaussian_32": lambda: ics.lattice_gaussian(32),   # the 32^3 CPU reference case
+    "blast_16": lambda: ics.blast(16),
+    "blast_32": lambda: ics.blast(32),
+    "random_state_12": lambda: ics.random_state(12),
+    "random_state_20": lambda: ics.random_state(20),
+    "perfect_lattice_12": lambda: ics.perfect_lattice(12, velocity=(0.2, 0.0, -0.1)),
+    "blast_16_untied": lambda: _untied(ics.blast(16)),
+    "lattice_gaussian_16_untied": lambda: _untied(ics.lattice_gaussian(16)),
+}
+
+
+def run_both(ic, active=None, params=None, stages=("density", "gradients", "force")):
+    ref, st, _ = O.run_pass(ic, active=active, stages=stages)
+    out, gst = gzm.run_pass(ic, active=active, stages=stages, params=params)
+    return ref, st, out, gst
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_pass_matches_oracle(name):
+    ic = CONFIGS[name]()
+    ref, st, out, gst = run_both(ic)
+    print(name, {k: v for k, v in gst.items() if k != "timing"})
+    assert gst["density_sweeps"] == st.iterations
+    assert gst["density_interactions"] == st.dens_interactions
+    assert gst["gradient_interactions"] == st.grad_interactions
+    assert gst["force_interactions"] == st.force_interactions
+    assert gst["exact_solver_calls"] == st.exact_solver_calls
+    assert gst["riemann_retries"] == st.riemann_retries
+    errs, bad = check_pass(out, ref)
+    print(name, {k: f"{v:.2e}" for k, v in errs.items()})
+    assert not bad, bad
+    if name.endswith("_untied") or name.startswith("random_state"):
+        assert errs["n_tied"] == 0
+
+
+@pytest.mark.parametrize("cell_size", [0.0, 0.03, 0.11, 0.4])
+def test_neighbor_sets_bit_exact_for_any_grid(cell_size):
+    rng = np.random.default_rng(9)
+    N = 6000
+    pos = rng.uniform(0, 1, size=(N, 3))
+    pos[:300] = np.mod(0.5 + rng.normal(scale=0.01, size=(300, 3)), 1.0)  # clump
+    pos[300:330] = np.mod(rng.uniform(-0.002, 0.002, size=(30, 3)), 1.0)  # across the corner seam
+    pos[pos >= 1.0] = 0.0
+    hsml = rng.uniform(0.02, 0.09, size=N)
+    ic = dict(pos=pos, vel=np.zeros((N, 3)), mass=np.full(N, 1.0 / N), u=np.ones(N), hsml=hsml, box=np.ones(3))
+    with gzm.MFMContext(gzm.make_params(cell_size=cell_size)) as ctx:
+        ctx.upload(ic)
+        for mode in (0, 1):
+            counts, total = ctx.neighbor_counts(mode)
+            ref_counts, ref_total = O.neighbor_counts(pos, hsml, mode)
+            np.testing.assert_array_equal(counts, ref_counts)
+            assert total == ref_total
+        sub = np.sort(rng.choice(N, 500, replace=False)).astype(np.int32)
+        counts, total = ctx.neighbor_counts(1, active=sub)
+        ref_counts, _ = O.neighbor_counts(pos, hsml, 1, active=sub)
+        np.testing.assert_array_equal(counts, ref_counts)
+
+
+def test_partial_active_set_and_wakeup_flags():
+    ic = ics.random_state(14)
+    N = ic["pos"].shape[0]
+    ref_full, _, _ = O.run_pass(ic)
+    state = dict(ic)
+    for k in ("hsml", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "cn", "fce",
+              "grad_rho", "grad_p", "grad_v", "acc", "dtu", "vsig"):
+        state[k] = ref_full[k]
+    state["hsml"] = ref_full["hsml"] * (1 + 0.1 * np.sin(np.arange(N)))
+    state["vsig"] = ref_full["vsig"] * np.where(np.arange(N) % 3 == 0, 0.1, 1.0)  # some inactive get woken
+    active = np.sort(np.random.default_rng(4).choice(N, N // 10, replace=False)).astype(np.int32)
+    ref, st, out, gst = run_both(state, active=active)
+    assert gst["density_interactions"] == st.dens_interactions
+    assert gst["force_interactions"] == st.force_interactions
+    errs, bad = check_pass(out, ref, active=active)
+    assert not bad, bad
+    np.testing.assert_array_equal(out["wakeup"], ref["wakeup"])
+    assert ref["wakeup"].sum() > 0
+    inactive = np.setdiff1d(np.arange(N), active)
+    for k in ("hsml", "density", "pressure", "grad_v", "acc", "dtu"):
+        np.testing.assert_array_equal(out[k][inactive], state[k][inactive])
+
+
+def test_conservation_on_gpu():
+    ic = ics.random_state(24)
+    out, _ = gzm.run_pass(ic)
+    m = ic["mass"]
+    p = (m[:, None] * out["acc"]).sum(0)
+    e = (m * (out["dtu"] + (ic["vel"] * out["acc"]).sum(1))).sum()
+    assert np.abs(p).max() <= 1e-13 * np.abs(m[:, None] * out["acc"]).sum()
+    assert abs(e) <= 1e-13 * (np.abs(m * out["dtu"]).sum() + np.abs(m[:, None] * ic["vel"] * out["acc"]).sum())
+
+
+def test_repeated_pass_is_bit_identical():
+    ic = ics.blast(16)
+    with gzm.MFMContext(gzm.make_params()) as ctx:
+        ctx.upload(ic)
+        outs = []
+        for _ in range(3):
+            ctx.restore_initial_state()
+            ctx.density()
+            ctx.gradients()
+            ctx.hydro_force()
+            outs.append(ctx.download())
+    for o in outs[1:]:
+        for k in ("hsml", "density", "grad_p", "acc", "dtu", "vsig"):
+            np.testing.assert_array_equal(o[k], outs[0][k])
+
+
+def test_hsml_not_converged_reports_1155():
+    ic = dict(ics.random_state(8))
+    ic["hsml"] = ic["hsml"] * 3.0
+    with pytest.raises(gzm.GzmError) as e:
+        gzm.run_pass(ic, params=gzm.make_params(maxiter=1), stages=("density",))
+    assert e.value.status == 1155
+
+
+def test_blast_128_full_size():
+    """The 128^3 blast-wave workload of the benchmark, against the oracle."""
+    ic = ics.blast(128)
+    ref, st, out, gst = run_both(ic)
+    assert gst["density_interactions"] == st.dens_interactions
+    assert gst["force_interactions"] == st.force_interactions
+    errs, bad = check_pass(out, ref)
+    print({k: f"{v:.2e}" for k, v in errs.items()})
+    assert not bad, bad
+
+
+# ------------------------------------------------------- host mirror -------
+class ParticleData(C.Structure):
+    _fields_ = [("Pos", C.c_double * 3), ("Vel", C.c_double * 3), ("Mass", C.c_double), ("Hsml", C.c_double),
+                ("NumNgb", C.c_double), ("DhsmlNgbFactor", C.c_double), ("Particle_DivVel", C.c_double),
+                ("Type", C.c_int), ("TimeBin", C.c_int), ("ID", C.c_ulonglong)]
+
+
+class GasCellData(C.Structure):
+    _fields_ = [("Density", C.c_double), ("Pressure", C.c_double), ("InternalEnergy", C.c_double),
+                ("InternalEnergyPred", C.c_double), ("VelPred", C.c_double * 3), ("HydroAccel", C.c_double * 3),
+                ("DtInternalEnergy", C.c_double), ("MaxSignalVel", C.c_double), ("ConditionNumber", C.c_double),
+                ("FaceClosureError", C.c_double), ("NV_T", (C.c_double * 3) * 3),
+                ("GradDensity", C.c_double * 3), ("GradPressure", C.c_double * 3),
+                ("GradVelocity", (C.c_double * 3) * 3), ("wakeup", C.c_int)]
+
+
+class AllData(C.Structure):
+    _fields_ = [("BoxSize", C.c_double), ("DesNumNgb", C.c_double), ("MaxNumNgbDeviation", C.c_double),
+                ("MinHsml", C.c_double), ("MaxHsml", C.c_double), ("Time", C.c_double), ("TimeBegin", C.c_double),
+                ("MaxPart", C.c_int)]
+
+
+def host_lib():
+    lib = C.CDLL(os.path.join(gzm.LIB_DIR, "libgizmo_host.so"))
+    lib.gizmo_host_allocate.argtypes = [C.c_int]
+    lib.gizmo_host_last_status.restype = C.c_int
+    return lib
+
+
+def fill_host(lib, ic):
+    N = ic["pos"].shape[0]
+    assert lib.gizmo_host_allocate(N) == 0
+    P = C.cast(C.c_void_p.in_dll(lib, "P"), C.POINTER(ParticleData))
+    S = C.cast(C.c_void_p.in_dll(lib, "SphP"), C.POINTER(GasCellData))
+    A = AllData.in_dll(lib, "All")
+    A.BoxSize, A.DesNumNgb, A.MaxNumNgbDeviation = 1.0, 32.0, 0.05
+    A.MinHsml, A.MaxHsml, A.Time, A.TimeBegin = 0.0, 1e56, 0.0, 0.0
+    for i in range(N):
+        p, s = P[i], S[i]
+        p.Pos[:] = list(ic["pos"][i])
+        p.Vel[:] = list(ic["vel"][i])
+        s.VelPred[:] = list(ic["vel"][i])
+        p.Mass = ic["mass"][i]
+        p.Hsml = ic["hsml"][i]
+        p.Type = 0
+        s.InternalEnergy = s.InternalEnergyPred = ic["u"][i]
+    return P, S
+
+
+def test_host_mirror_is_a_drop_in():
+    ic = ics.random_state(10)
+    N = ic["pos"].shape[0]
+    lib = host_lib()
+    codes = []
+    handler_t = C.CFUNCTYPE(None, C.c_int, C.c_char_p)
+    handler = handler_t(lambda code, where: codes.append(code))
+    lib.gizmo_host_set_endrun_handler(handler)
+    P, S = fill_host(lib, ic)
+    lib.compute_hydro_densities_and_forces()
+    assert lib.gizmo_host_last_status() == 0 and not codes
+    out, _ = gzm.run_pass(ic)
+    for i in range(0, N, 7):
+        assert P[i].Hsml == out["hsml"][i]
+        assert S[i].Density == out["density"][i]
+        assert list(S[i].HydroAccel) == list(out["acc"][i])
+        assert S[i].DtInternalEnergy == out["dtu"][i]
+        assert list(S[i].GradPressure) == list(out["grad_p"][i])
+        assert [S[i].NV_T[a][b] for a in range(3) for b in range(3)] == list(out["nvt"][i])
+    # the reference's failure path: endrun(1155) when hsml cannot converge
+    lib.gizmo_host_release()
+    fill_host(lib, dict(ic, hsml=ic["hsml"] * 1e-3))
+    A = AllData.in_dll(lib, "All")
+    A.MaxHsml = 1e56
+    lib.gizmo_host_set_device(0)
+    # MAXITER is compiled in (150); a collapsed bracket cannot occur here, so
+    # check the status plumbing with an out-of-box position instead
+    P[0].Pos[0] = 2.0
+    lib.density()
+    assert codes and codes[-1] == 1  # GZM_ERR_ARG from gzm_upload
+    lib.gizmo_host_release()
+    lib.gizmo_host_free()
