@@ -85,6 +85,14 @@ struct gzm_ctx {
     int n_active = 0;
     bool all_active = true;
     bool have_density = false, have_gradients = false;
+    // pair list written by the gradient kernel, read by the force kernel
+    int *d_list = nullptr, *d_list_count = nullptr;
+    uint8_t *d_list_over = nullptr;
+    size_t list_entries = 0;
+    int list_cap = 64;
+    bool list_valid = false;
+    int list_overflow_slices = 0;
+    uint64_t active_version = 0, list_version = 0;
 
     gzm_timing timing;
     std::vector<TimedLaunch> pending;
@@ -193,6 +201,9 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_wakeup); dev_free(c->d_is_active);
     dev_free(c->d_active_caller); dev_free(c->d_active); dev_free(c->d_live); dev_free(c->d_live2);
     dev_free(c->d_flags); dev_free(c->d_grec); dev_free(c->d_frec);
+    dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
+    c->list_entries = 0;
+    c->list_valid = false;
     c->n = 0;
 }
 
@@ -243,6 +254,7 @@ gzm_status set_active(gzm_ctx *c, const int32_t *active, int64_t n_active)
     if (active == nullptr) {
         if (!c->all_active || c->n_active != c->n) {
             GZM_HIP(hipMemcpyAsync(c->d_active, c->d_iota, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+            c->active_version++;
         }
         c->n_active = c->n;
         c->all_active = true;
@@ -272,6 +284,23 @@ gzm_status set_active(gzm_ctx *c, const int32_t *active, int64_t n_active)
     }
     c->n_active = (int)n_active;
     c->all_active = false;
+    c->active_version++;
+    return GZM_OK;
+}
+
+// the gradient kernel writes a pair list sized for the active list
+gzm_status ensure_list(gzm_ctx *c)
+{
+    size_t slices = ((size_t)c->n_active + 63) / 64;
+    size_t entries = slices * (size_t)c->list_cap * 64;
+    if (entries <= c->list_entries && c->d_list_count) return GZM_OK;
+    dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
+    c->list_entries = 0;
+    gzm_status s = dev_alloc(&c->d_list, entries);
+    s = s ? s : dev_alloc(&c->d_list_count, (size_t)c->n + 64);
+    s = s ? s : dev_alloc(&c->d_list_over, slices + 1);
+    if (s) return s;
+    c->list_entries = entries;
     return GZM_OK;
 }
 
@@ -340,6 +369,7 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
     GZM_HIP(hipSetDevice(params->device));
     gzm_ctx *c = new gzm_ctx();
     c->p = *params;
+    c->list_cap = params->pair_list_cap == 0 ? 64 : params->pair_list_cap;
     memset(&c->timing, 0, sizeof(c->timing));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
@@ -550,6 +580,7 @@ gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
     c->have_gradients = (v->grad_rho != nullptr);
     c->all_active = false;
     c->n_active = 0;
+    c->list_valid = false;
     return GZM_OK;
 }
 
@@ -639,6 +670,7 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
     }
     c->have_density = true;
     c->have_gradients = false;
+    c->list_valid = false;
     if (status != GZM_OK) set_error("%s", gzm_status_string(status));
     return (gzm_status)status;
 }
@@ -669,6 +701,15 @@ gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gz
     a.cn = c->d_cn;
     a.grad = c->d_grad;
     a.ctr = c->d_ctr;
+    memset(&a.list, 0, sizeof(a.list));
+    if (c->n_active > 0 && c->list_cap > 0) {
+        s = ensure_list(c);
+        if (s) return s;
+        a.list.idx = c->d_list;
+        a.list.count = c->d_list_count;
+        a.list.overflow = c->d_list_over;
+        a.list.cap = c->list_cap;
+    }
     if (c->n_active > 0) {
         Timed tm(c, kGradients);
         launch_gradients(c->stream, a);
@@ -680,6 +721,9 @@ gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gz
     collect_timings(c);
     if (stats) stats->gradient_interactions = (int64_t)h.interactions;
     c->have_gradients = true;
+    c->list_valid = (c->n_active > 0 && c->list_cap > 0);
+    c->list_version = c->active_version;
+    c->list_overflow_slices = h.overflow_slices;
     return GZM_OK;
 }
 
@@ -720,9 +764,22 @@ gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, 
     a.vsig = c->d_vsig;
     a.wakeup = c->d_wakeup;
     a.ctr = c->d_ctr;
+    memset(&a.list, 0, sizeof(a.list));
+    a.fallback_only = 0;
+    const bool use_list = c->list_valid && c->list_version == c->active_version;
+    if (use_list) {
+        a.list.idx = c->d_list;
+        a.list.count = c->d_list_count;
+        a.list.overflow = c->d_list_over;
+        a.list.cap = c->list_cap;
+    }
     if (c->n_active > 0) {
         Timed tm(c, kForce);
         launch_force(c->stream, a);
+        if (use_list && c->list_overflow_slices > 0) {
+            a.fallback_only = 1;  // slices whose list overflowed scan the grid
+            launch_force(c->stream, a);
+        }
     }
     GZM_HIP(hipGetLastError());
     Counters h;

@@ -47,7 +47,7 @@ struct Counters {
     unsigned long long retries;
     unsigned long long exact_calls;
     int status;
-    int n_next;
+    int overflow_slices;
 };
 
 struct DensitySweepArgs {
@@ -64,6 +64,18 @@ struct DensitySweepArgs {
     Counters *ctr;
 };
 
+// Pair list in sliced-ELL form: the 64 consecutive entries of the active list
+// that one wavefront handles form a slice; entry k of lane l of slice s is
+// list[(s * cap + k) * 64 + l], so a wavefront reads 64 neighbours with one
+// coalesced 256-byte load.  A slice whose longest list exceeds `cap` is
+// flagged and handled by the scanning fallback.
+struct PairList {
+    int *idx;              // (n_slices * cap * 64)
+    int *count;            // per active entry
+    uint8_t *overflow;     // per slice
+    int cap;
+};
+
 struct GradientArgs {
     Grid grid;
     BoxInfo box;
@@ -74,6 +86,7 @@ struct GradientArgs {
     const double *nvt;     // 6 per particle
     const double *cn;
     double *grad;          // 15 per particle: rho(3), P(3), v(3x3 row-major)
+    PairList list;         // written when list.idx != NULL
     Counters *ctr;
 };
 
@@ -92,6 +105,8 @@ struct ForceArgs {
     double *dtu;
     double *vsig;
     uint8_t *wakeup;
+    PairList list;         // list.idx == NULL: scan the grid for every particle
+    int fallback_only;     // scan only the slices flagged in list.overflow
     Counters *ctr;
 };
 
@@ -115,7 +130,7 @@ void launch_gradients(hipStream_t s, const GradientArgs &a);
 void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
                        const double *rho, const double *pres, const double *dhsml, const double *cn,
                        const double *fce, const double *nvt, const double *grad, double gamma, ForceRec *rec);
-void launch_force(hipStream_t s, const ForceArgs &a);
+void launch_force(hipStream_t s, const ForceArgs &a);  // list-driven if a.list.idx, else scanning
 void launch_neighbor_counts(hipStream_t s, Grid g, BoxInfo box, int mode, int n_active, const int *active,
                             const double4 *posm, const double *hsml, double hmax, int *counts, Counters *ctr);
 void launch_fill_int(hipStream_t s, int *p, int n, int value);

@@ -328,13 +328,16 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = t < a.n_active;
     unsigned long long cnt = 0;
+    int listed = 0;
     if (valid) {
         const int i = a.active[t];
         const GradRec ri = a.rec[i];
-        double gr[3] = {0, 0, 0}, gp[3] = {0, 0, 0}, gv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        double gr0 = 0, gr1 = 0, gr2 = 0, gp0 = 0, gp1 = 0, gp2 = 0;
+        double gv00 = 0, gv01 = 0, gv02 = 0, gv10 = 0, gv11 = 0, gv12 = 0, gv20 = 0, gv21 = 0, gv22 = 0;
         double maxd = 0, mn_r = 0, mx_r = 0, mn_p = 0, mx_p = 0;
-        double mn_v[3] = {0, 0, 0}, mx_v[3] = {0, 0, 0};
+        double mn_v0 = 0, mx_v0 = 0, mn_v1 = 0, mx_v1 = 0, mn_v2 = 0, mx_v2 = 0;
         const double h_i = ri.xh.w, rho_i = ri.vr.w, p_i = ri.pm.x, m_i = ri.pm.y;
+        int *list_row = a.list.idx ? a.list.idx + ((size_t)(t >> 6) * a.list.cap) * 64 + (t & 63) : nullptr;
         if ((m_i > 0) && (rho_i > 0) && (h_i > 0)) {  // GasGrad_isactive (gradients.c:66-75)
             double hinv, hinv3, hinv4;
             kernel_hinv(h_i, hinv, hinv3, hinv4);
@@ -351,7 +354,14 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
                 if ((r2 >= h2_i) && (r2 >= h_j * h_j)) return;
                 const double4 vr = a.rec[j].vr;
                 const double2 pm = a.rec[j].pm;
-                if (!((pm.y > 0) && (vr.w > 0) && (h_j > 0))) return;
+                if (!((pm.y > 0) && (vr.w > 0))) return;
+                // the list holds the force loop's pair set (m_j > 0, rho_j > 0,
+                // hydro_evaluate.h:149-150); gradients also need h_j > 0
+                if (list_row) {
+                    if (listed < a.list.cap) list_row[(size_t)listed * 64] = j;
+                    listed++;
+                }
+                if (!(h_j > 0)) return;
                 cnt++;
                 const double r = sqrt(r2);
                 const double wk_i = (r < h_i) ? kernel_w(r * hinv, hinv3) : 0.0;
@@ -360,31 +370,32 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
                 minmax_check(d_rho, mn_r, mx_r);
                 const double dpp = pm.x - p_i;
                 minmax_check(dpp, mn_p, mx_p);
-                const double dv[3] = {vr.x - ri.vr.x, vr.y - ri.vr.y, vr.z - ri.vr.z};
-                minmax_check(dv[0], mn_v[0], mx_v[0]);
-                minmax_check(dv[1], mn_v[1], mx_v[1]);
-                minmax_check(dv[2], mn_v[2], mx_v[2]);
+                const double dv0 = vr.x - ri.vr.x, dv1 = vr.y - ri.vr.y, dv2 = vr.z - ri.vr.z;
+                minmax_check(dv0, mn_v0, mx_v0);
+                minmax_check(dv1, mn_v1, mx_v1);
+                minmax_check(dv2, mn_v2, mx_v2);
                 if (r < h_i) {
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        const double w = -wk_i * dp[k];
-                        gr[k] += w * d_rho;
-                        gp[k] += w * dpp;
-                        gv[0][k] += w * dv[0];
-                        gv[1][k] += w * dv[1];
-                        gv[2][k] += w * dv[2];
-                    }
+                    const double w0 = -wk_i * dp[0], w1 = -wk_i * dp[1], w2 = -wk_i * dp[2];
+                    gr0 += w0 * d_rho; gr1 += w1 * d_rho; gr2 += w2 * d_rho;
+                    gp0 += w0 * dpp; gp1 += w1 * dpp; gp2 += w2 * dpp;
+                    gv00 += w0 * dv0; gv01 += w1 * dv0; gv02 += w2 * dv0;
+                    gv10 += w0 * dv1; gv11 += w1 * dv1; gv12 += w2 * dv1;
+                    gv20 += w0 * dv2; gv21 += w1 * dv2; gv22 += w2 * dv2;
                 }
             });
+        } else if ((m_i > 0) && (rho_i > 0)) {
+            listed = a.list.cap + 1;  // the force loop still runs for i: send its slice to the scan
         }
         double nvt[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) nvt[k] = a.nvt[6 * (size_t)i + k];
+        double gr[3] = {gr0, gr1, gr2}, gp[3] = {gp0, gp1, gp2};
+        double gv0[3] = {gv00, gv01, gv02}, gv1[3] = {gv10, gv11, gv12}, gv2[3] = {gv20, gv21, gv22};
         apply_nvt(nvt, gr);
         apply_nvt(nvt, gp);
-        apply_nvt(nvt, gv[0]);
-        apply_nvt(nvt, gv[1]);
-        apply_nvt(nvt, gv[2]);
+        apply_nvt(nvt, gv0);
+        apply_nvt(nvt, gv1);
+        apply_nvt(nvt, gv2);
         // SELFGRAVITY_OFF, no MAGNETIC/GALSF: h_lim = h, stol = 0.1 (gradients.c:1219-1232)
         const double cn = a.cn[i];
         const double d_max = (h_i > maxd) ? h_i : maxd;
@@ -392,14 +403,24 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
         if (cn > 100) a_lim = dmin(0.5, 0.25 + 0.25 * (cn - 100) / 100);
         slope_limit(gr, mx_r, mn_r, a_lim, h_i, 0, 1, d_max, rho_i);
         slope_limit(gp, mx_p, mn_p, a_lim, h_i, 0.1, 1, d_max, p_i);
-        slope_limit(gv[0], mx_v[0], mn_v[0], a_lim, h_i, 0.1, 0, 0, 0);
-        slope_limit(gv[1], mx_v[1], mn_v[1], a_lim, h_i, 0.1, 0, 0, 0);
-        slope_limit(gv[2], mx_v[2], mn_v[2], a_lim, h_i, 0.1, 0, 0, 0);
+        slope_limit(gv0, mx_v0, mn_v0, a_lim, h_i, 0.1, 0, 0, 0);
+        slope_limit(gv1, mx_v1, mn_v1, a_lim, h_i, 0.1, 0, 0, 0);
+        slope_limit(gv2, mx_v2, mn_v2, a_lim, h_i, 0.1, 0, 0, 0);
         double *o = a.grad + 15 * (size_t)i;
         o[0] = gr[0]; o[1] = gr[1]; o[2] = gr[2];
         o[3] = gp[0]; o[4] = gp[1]; o[5] = gp[2];
-#pragma unroll
-        for (int k = 0; k < 3; k++) { o[6 + 3 * k] = gv[k][0]; o[7 + 3 * k] = gv[k][1]; o[8 + 3 * k] = gv[k][2]; }
+        o[6] = gv0[0]; o[7] = gv0[1]; o[8] = gv0[2];
+        o[9] = gv1[0]; o[10] = gv1[1]; o[11] = gv1[2];
+        o[12] = gv2[0]; o[13] = gv2[1]; o[14] = gv2[2];
+        if (a.list.idx) a.list.count[t] = listed;
+    }
+    if (a.list.idx) {
+        // a slice overflows when any of its lanes listed more than cap neighbours
+        const bool over = __any(listed > a.list.cap);
+        if ((threadIdx.x & 63) == 0 && t < a.n_active) {
+            a.list.overflow[t >> 6] = over ? 1 : 0;
+            if (over) atomicAdd(&a.ctr->overflow_slices, 1);
+        }
     }
     wave_add_u64(&a.ctr->interactions, cnt);
 }
@@ -542,22 +563,31 @@ __device__ __forceinline__ int force_pair(const ForceI &I, const ForceRec &J, co
     const double ptj = P_j + rho_j * v2a;
     double lim = 1.1 * 1.0 * dmax(pti, ptj);
     if (recon == 0) lim = dmax(lim, dmax(dmax(I.P, P_j), 2. * dmax(I.rho, rho_j) * v2a));
+    // Riemann problem with the reference's two retries (hydro_core_meshless.h:209-265):
+    // attempt 0 reconstructed states, 1 first-order states, 2 first-order with
+    // zero velocities; one solver instance in the code
     int err = 0;
-    mask = riemann_solve(SL, SR, n, lim, G, ro, err);
-    if ((ro.P_M < 0) || isnan(ro.P_M) || (ro.P_M > 1.4 * lim)) {
-        mask |= 8;  // first-order retry (hydro_core_meshless.h:209-265)
+    double lim_a = lim;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        mask |= riemann_solve(SL, SR, n, lim_a, G, ro, err);
+        const bool bad = (ro.P_M < 0) || isnan(ro.P_M);
+        if (attempt == 0) {
+            if (!(bad || (ro.P_M > 1.4 * lim))) break;
+            mask |= 8;
+            lim_a = 1.4 * lim;
+        } else if (attempt == 1) {
+            if (!bad) break;
+            lim_a = 2.0 * lim;
+        } else {
+            if (bad) status = 1234;
+            break;
+        }
         SR.p = I.P; SL.p = P_j;
         SR.rho = I.rho; SL.rho = rho_j;
 #pragma unroll
-        for (int k = 0; k < 3; k++) { SR.v[k] = I.v[k] - vf[k]; SL.v[k] = vj[k] - vf[k]; }
-        mask |= riemann_solve(SL, SR, n, 1.4 * lim, G, ro, err);
-        if ((ro.P_M < 0) || isnan(ro.P_M)) {
-            SR.p = I.P; SL.p = P_j;
-            SR.rho = I.rho; SL.rho = rho_j;
-#pragma unroll
-            for (int k = 0; k < 3; k++) { SR.v[k] = 0; SL.v[k] = 0; }
-            mask |= riemann_solve(SL, SR, n, 2.0 * lim, G, ro, err);
-            if ((ro.P_M < 0) || isnan(ro.P_M)) status = 1234;
+        for (int k = 0; k < 3; k++) {
+            SR.v[k] = (attempt == 0) ? I.v[k] - vf[k] : 0.0;
+            SL.v[k] = (attempt == 0) ? vj[k] - vf[k] : 0.0;
         }
     }
     if (err) status = err;
@@ -602,15 +632,24 @@ __device__ __forceinline__ int force_pair(const ForceI &I, const ForceRec &J, co
     return status;
 }
 
+// USE_LIST: walk the pair list written by the gradient kernel (slices that
+// overflowed it are skipped and left to the scanning variant); otherwise scan
+// the grid (every slice, or only the overflowed ones when a.fallback_only).
+template <bool USE_LIST>
 __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = t < a.n_active;
+    bool valid = t < a.n_active;
+    if (USE_LIST) {
+        if (valid && a.list.overflow[t >> 6]) valid = false;
+    } else if (a.fallback_only) {
+        if (valid && !a.list.overflow[t >> 6]) valid = false;
+    }
     unsigned long long cnt = 0, nretry = 0, nexact = 0;
     if (valid) {
         const int i = a.active[t];
         const ForceRec ri = a.rec[i];
-        double acc[3] = {0, 0, 0}, dte = 0, vmax = kMinReal;
+        double acc0 = 0, acc1 = 0, acc2 = 0, dte = 0, vmax = kMinReal;
         const double m_i = ri.mc.x, rho_i = ri.th.x;
         int status = 0;
         if ((m_i > 0) && (rho_i > 0)) {
@@ -628,17 +667,18 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
             double out_vsig = I.cs;
             ROut ro = {0, 0, {0, 0, 0}, 0};
             const double h2_i = I.h * I.h;
-            const double R = (I.h > a.hmax) ? I.h : a.hmax;
-            for_each_candidate(a.grid, I.x, R, [&](int j) {
+            auto visit = [&](int j, bool check) {
                 const double4 xj = a.rec[j].xh;
                 double dp[3] = {I.x[0] - xj.x, I.x[1] - xj.y, I.x[2] - xj.z};
                 wrap_dp(a.box, dp);
                 const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-                const double h_j = xj.w;
-                if ((r2 >= h2_i) && (r2 >= h_j * h_j)) return;
-                if (r2 <= 0) return;
+                if (check) {
+                    const double h_j = xj.w;
+                    if ((r2 >= h2_i) && (r2 >= h_j * h_j)) return;
+                    if (r2 <= 0) return;
+                    if (!(a.rec[j].mc.x > 0) || !(a.rec[j].th.x > 0)) return;  // hydro_evaluate.h:149-150
+                }
                 const ForceRec J = a.rec[j];
-                if (!(J.mc.x > 0) || !(J.th.x > 0)) return;  // hydro_evaluate.h:149-150
                 cnt++;
                 double Fv[3], Fp, vsig;
                 int mask;
@@ -646,30 +686,41 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
                 if (e > status) status = e;
                 nretry += (mask & 8) ? 1 : 0;
                 nexact += (mask & 4) ? 1 : 0;
-                acc[0] += Fv[0];
-                acc[1] += Fv[1];
-                acc[2] += Fv[2];
+                acc0 += Fv[0];
+                acc1 += Fv[1];
+                acc2 += Fv[2];
                 dte += Fp;
                 if (vsig > out_vsig) out_vsig = vsig;
                 if (a.is_active && !a.is_active[j]) {
                     if (vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;  // WAKEUP (hydro_evaluate.h:485-495)
                 }
-            });
+            };
+            if (USE_LIST) {
+                // the list holds the pair set in the gradient kernel's scan order,
+                // which is this kernel's scan order as well (same predicate, same R)
+                const int n = a.list.count[t];
+                const int *row = a.list.idx + ((size_t)(t >> 6) * a.list.cap) * 64 + (t & 63);
+                for (int k = 0; k < n; k++) visit(row[(size_t)k * 64], false);
+            } else {
+                const double R = (I.h > a.hmax) ? I.h : a.hmax;
+                for_each_candidate(a.grid, I.x, R, [&](int j) { visit(j, true); });
+            }
             if (vmax < out_vsig) vmax = out_vsig;
             // hydro_final_operations_and_cleanup (hydro_toplevel.c:738-764)
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                dte -= (I.v[k] / 1.0) * acc[k];
-                acc[k] /= m_i;
-            }
+            dte -= (I.v[0] / 1.0) * acc0;
+            acc0 /= m_i;
+            dte -= (I.v[1] / 1.0) * acc1;
+            acc1 /= m_i;
+            dte -= (I.v[2] / 1.0) * acc2;
+            acc2 /= m_i;
             dte /= m_i;
             if (I.h >= 0.99 * a.max_hsml) dte = 0;
         } else if (m_i > 0) {
             dte /= m_i;
         }
-        a.acc[3 * (size_t)i + 0] = acc[0];
-        a.acc[3 * (size_t)i + 1] = acc[1];
-        a.acc[3 * (size_t)i + 2] = acc[2];
+        a.acc[3 * (size_t)i + 0] = acc0;
+        a.acc[3 * (size_t)i + 1] = acc1;
+        a.acc[3 * (size_t)i + 2] = acc2;
         a.dtu[i] = dte;
         a.vsig[i] = vmax;
         if (status) atomicMax(&a.ctr->status, status);
@@ -785,7 +836,11 @@ void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 
 }
 void launch_force(hipStream_t s, const ForceArgs &a)
 {
-    if ((a.n_active) > 0) hipLaunchKernelGGL(force_kernel, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    if (a.n_active <= 0) return;
+    if (a.list.idx && !a.fallback_only)
+        hipLaunchKernelGGL(force_kernel<true>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(force_kernel<false>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
 }
 void launch_neighbor_counts(hipStream_t s, Grid g, BoxInfo box, int mode, int n_active, const int *active,
                             const double4 *posm, const double *hsml, double hmax, int *counts, Counters *ctr)

@@ -430,35 +430,53 @@ __device__ inline void wavespeeds_pstar(const RState &L, const RState &R, const 
     }
 }
 
-// Kurganov-Tadmor/PWK flux, MFM branch (reimann.h:484-519)
-__device__ __noinline__ void riemann_kt(const RState &L, const RState &R, const double n[3], double vL, double vR,
-                                        double cL, double cR, double hL, double hR, ROut &o)
+// Kurganov-Tadmor/PWK flux, MFM branch (reimann.h:484-519).  Reached only on
+// a NaN/negative HLLC pressure, so it lives out of line with scalar arguments.
+__device__ __noinline__ ROut riemann_kt(double rL, double pL, double vLx, double vLy, double vLz, double rR,
+                                        double pR, double vRx, double vRy, double vRz, double n0, double n1,
+                                        double n2, double vL, double vR, double cL, double cR, double hL,
+                                        double hR)
 {
-    double thr = 0.001 * 0.5 * (L.rho + R.rho) * 0.5 * (cL + cR);
+    ROut o;
+    double thr = 0.001 * 0.5 * (rL + rR) * 0.5 * (cL + cR);
+    double d0 = rR * vRx - rL * vLx, d1 = rR * vRy - rL * vLy, d2 = rR * vRz - rL * vLz;
     double dv2 = 0;
-    for (int k = 0; k < 3; k++) { double dvk = R.rho * R.v[k] - L.rho * L.v[k]; dv2 += dvk * dvk; }
-    double dvL = fabs(R.rho * vR - L.rho * vL);
+    dv2 += d0 * d0;
+    dv2 += d1 * d1;
+    dv2 += d2 * d2;
+    double dvL = fabs(rR * vR - rL * vL);
     double alpha = dvL / sqrt(thr * thr + dv2);
     double S_L = alpha * cL; if (vL > 0) S_L += vL; else S_L -= vL;
     double S_R = alpha * cR; if (vR > 0) S_R += vR; else S_R -= vR;
     double S_M = dmax(S_L, S_R);
     o.S_M = dmax(cL + fabs(vL), cR + fabs(vR));
-    double frl = L.rho * (vL + S_M);
-    double frr = R.rho * (vR - S_M);
-    double denom = 1 / (L.rho * vL - R.rho * vR + S_M * (L.rho + R.rho));
-    o.P_M = (frl * R.p - frr * L.p) * denom;
-    for (int k = 0; k < 3; k++) o.Fv[k] = (R.v[k] - L.v[k]) * frl * frr * denom + (o.P_M) * n[k];
-    o.Fp = (S_M * (frl * R.p + frr * L.p) + (hR - hL) * frl * frr) * denom;
+    double frl = rL * (vL + S_M);
+    double frr = rR * (vR - S_M);
+    double denom = 1 / (rL * vL - rR * vR + S_M * (rL + rR));
+    o.P_M = (frl * pR - frr * pL) * denom;
+    o.Fv[0] = (vRx - vLx) * frl * frr * denom + (o.P_M) * n0;
+    o.Fv[1] = (vRy - vLy) * frl * frr * denom + (o.P_M) * n1;
+    o.Fv[2] = (vRz - vLz) * frl * frr * denom + (o.P_M) * n2;
+    o.Fp = (S_M * (frl * pR + frr * pL) + (hR - hL) * frl * frr) * denom;
+    return o;
 }
 
 // exact iterative solver (reimann.h:685-724, 1034-1138); for MFM only P_M and
 // S_M change (sample_reimann_standard returns at once, reimann.h:873-877).
-// Returns 0 or 1234545.
-__device__ __noinline__ int riemann_exact(const RState &L, const RState &R, const GammaConsts &G, double vL,
-                                          double vR, double cL, double cR, ROut &o)
+// Rare (HLLC pressure above the limiter), so out of line with scalar
+// arguments; returns (P_M, S_M, status) with status 0 or 1234545.
+struct ExactOut {
+    double P_M, S_M;
+    int status;
+};
+
+__device__ __noinline__ ExactOut riemann_exact(double rL, double pL, double rR, double pR, double vL, double vR,
+                                               double cL, double cR, double P_M_in, double S_M_in,
+                                               GammaConsts G)
 {
-    if ((L.p == 0 && R.p == 0) || (L.rho == 0 && R.rho == 0)) { o.P_M = o.S_M = 0; return 0; }
-    if ((L.rho > 0) && (R.rho > 0)) {
+    ExactOut o = {P_M_in, S_M_in, 0};
+    if ((pL == 0 && pR == 0) || (rL == 0 && rR == 0)) { o.P_M = o.S_M = 0; return o; }
+    if ((rL > 0) && (rR > 0)) {
         double dvel = vR - vL;
         double check_vel = G.G4 * (cR + cL) - dvel;
         bool converged = false;
@@ -466,20 +484,20 @@ __device__ __noinline__ int riemann_exact(const RState &L, const RState &R, cons
             // guess_for_pressure (reimann.h:1101-1138)
             double Pg;
             {
-                double pv = 0.5 * (L.p + R.p) - 0.125 * (vR - vL) * (L.p + R.p) * (cL + cR);
-                double pmin = dmin(L.p, R.p), pmax = dmax(L.p, R.p);
+                double pv = 0.5 * (pL + pR) - 0.125 * (vR - vL) * (pL + pR) * (cL + cR);
+                double pmin = dmin(pL, pR), pmax = dmax(pL, pR);
                 if (pmin <= 0) Pg = 0.5 * (pmin + pmax);
                 else {
                     double qrat = pmax / pmin;
                     if (qrat <= 2.0 && (pmin <= pv && pv <= pmax)) Pg = pv;
                     else if (pv < pmin) {
                         double pnu = (cL + cR) - G.G7 * (vR - vL);
-                        double pde = cL / pow(L.p, G.G1) + cR / pow(R.p, G.G1);
+                        double pde = cL / pow(pL, G.G1) + cR / pow(pR, G.G1);
                         Pg = pow(pnu / pde, G.G3);
                     } else {
-                        double gel = sqrt((G.G5 / L.rho) / (G.G6 * L.p + pv));
-                        double ger = sqrt((G.G5 / R.rho) / (G.G6 * R.p + pv));
-                        double x = (gel * L.p + ger * R.p - (vR - vL)) / (gel + ger);
+                        double gel = sqrt((G.G5 / rL) / (G.G6 * pL + pv));
+                        double ger = sqrt((G.G5 / rR) / (G.G6 * pR + pv));
+                        double x = (gel * pL + ger * pR - (vR - vL)) / (gel + ger);
                         if (x < pmin || x > pmax) x = pmin;
                         Pg = x;
                     }
@@ -489,23 +507,23 @@ __device__ __noinline__ int riemann_exact(const RState &L, const RState &R, cons
             int niter = 0;
             while ((tol > 1.e-6) && (niter < 1000)) {
                 double Pg_prev = Pg;
-                if (Pg > L.p) {
-                    double a0 = G.G5 / L.rho, a1 = G.G6 * L.p, a2 = sqrt(a0 / (Pg + a1));
-                    W_L = (Pg - L.p) * a2;
-                    Z_L = a2 * (1.0 - 0.5 * (Pg - L.p) / (a1 + Pg));
+                if (Pg > pL) {
+                    double a0 = G.G5 / rL, a1 = G.G6 * pL, a2 = sqrt(a0 / (Pg + a1));
+                    W_L = (Pg - pL) * a2;
+                    Z_L = a2 * (1.0 - 0.5 * (Pg - pL) / (a1 + Pg));
                 } else {
-                    double pratio = Pg / L.p;
+                    double pratio = Pg / pL;
                     W_L = G.G4 * cL * (pow(pratio, G.G1) - 1);
-                    Z_L = 1 / (L.rho * cL) * pow(Pg / L.p, -G.G2);
+                    Z_L = 1 / (rL * cL) * pow(Pg / pL, -G.G2);
                 }
-                if (Pg > R.p) {
-                    double a0 = G.G5 / R.rho, a1 = G.G6 * R.p, a2 = sqrt(a0 / (Pg + a1));
-                    W_R = (Pg - R.p) * a2;
-                    Z_R = a2 * (1.0 - 0.5 * (Pg - R.p) / (a1 + Pg));
+                if (Pg > pR) {
+                    double a0 = G.G5 / rR, a1 = G.G6 * pR, a2 = sqrt(a0 / (Pg + a1));
+                    W_R = (Pg - pR) * a2;
+                    Z_R = a2 * (1.0 - 0.5 * (Pg - pR) / (a1 + Pg));
                 } else {
-                    double pratio = Pg / R.p;
+                    double pratio = Pg / pR;
                     W_R = G.G4 * cR * (pow(pratio, G.G1) - 1);
-                    Z_R = 1 / (R.rho * cR) * pow(pratio, -G.G2);
+                    Z_R = 1 / (rR * cR) * pow(pratio, -G.G2);
                 }
                 if (niter < 500) Pg -= (W_L + W_R + dvel) / (Z_L + Z_R);
                 else Pg -= 0.5 * (W_L + W_R + dvel) / (Z_L + Z_R);
@@ -527,19 +545,19 @@ __device__ __noinline__ int riemann_exact(const RState &L, const RState &R, cons
             else if (0.0 >= S_R) o.S_M = S_R;
             else o.S_M = 0.0;
         }
-        return 0;
+        return o;
     }
-    if ((L.rho < 0) || (R.rho < 0)) return 1234545;
-    if (L.rho > 0) { o.P_M = 0; o.S_M = vL + G.G4 * cL; }
-    if (R.rho > 0) { o.P_M = 0; o.S_M = vR - G.G4 * cR; }
-    return 0;
+    if ((rL < 0) || (rR < 0)) { o.status = 1234545; return o; }
+    if (rL > 0) { o.P_M = 0; o.S_M = vL + G.G4 * cL; }
+    if (rR > 0) { o.P_M = 0; o.S_M = vR - G.G4 * cR; }
+    return o;
 }
 
 // Riemann_solver() for MFM hydro (reimann.h:286-388).  Returns a stage mask:
-// bit0 HLLC, bit1 KT, bit2 exact; *err gets 1234545 on the exact solver's
+// bit0 HLLC, bit1 KT, bit2 exact; err gets 1234545 on the exact solver's
 // negative-density exit.
-__device__ inline int riemann_solve(const RState &L, const RState &R, const double n[3], double limiter,
-                                    const GammaConsts &G, ROut &o, int &err)
+__device__ __forceinline__ int riemann_solve(const RState &L, const RState &R, const double n[3], double limiter,
+                                             const GammaConsts &G, ROut &o, int &err)
 {
     if ((L.p < 0 && R.p < 0) || (L.rho < 0) || (R.rho < 0)) { o.P_M = 0; return 0; }
     double cL = sqrt(G.G0 * L.p / L.rho);
@@ -555,12 +573,15 @@ __device__ inline int riemann_solve(const RState &L, const RState &R, const doub
     for (int k = 0; k < 3; k++) o.Fv[k] = o.P_M * n[k];
     o.Fp = o.P_M * o.S_M;
     if ((o.P_M < 0) || isnan(o.P_M)) {
-        riemann_kt(L, R, n, vL, vR, cL, cR, hL, hR, o);
+        o = riemann_kt(L.rho, L.p, L.v[0], L.v[1], L.v[2], R.rho, R.p, R.v[0], R.v[1], R.v[2], n[0], n[1], n[2], vL,
+                       vR, cL, cR, hL, hR);
         mask |= 2;
     }
     if ((o.P_M < 0) || isnan(o.P_M) || (o.P_M > limiter)) {
-        int e = riemann_exact(L, R, G, vL, vR, cL, cR, o);
-        if (e) err = e;
+        ExactOut e = riemann_exact(L.rho, L.p, R.rho, R.p, vL, vR, cL, cR, o.P_M, o.S_M, G);
+        o.P_M = e.P_M;
+        o.S_M = e.S_M;
+        if (e.status) err = e.status;
         mask |= 4;
     }
     return mask;

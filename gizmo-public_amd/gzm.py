@@ -40,6 +40,8 @@ class Params(C.Structure):
         ("device", C.c_int32),
         ("precision", C.c_int32),
         ("cell_size", C.c_double),
+        ("pair_list_cap", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -126,7 +128,8 @@ def _p(a, t=_dp):
 
 
 def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_ngb_dev=0.05,
-                min_hsml=0.0, max_hsml=1e56, first_step=1, maxiter=150, device=0, cell_size=0.0) -> Params:
+                min_hsml=0.0, max_hsml=1e56, first_step=1, maxiter=150, device=0, cell_size=0.0,
+                pair_list_cap=0) -> Params:
     p = Params()
     p.box[:] = list(box)
     p.gamma = gamma
@@ -139,6 +142,7 @@ def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_
     p.device = device
     p.precision = 0
     p.cell_size = cell_size
+    p.pair_list_cap = pair_list_cap
     return p
 
 

@@ -176,6 +176,7 @@ static int ensure_ctx(void)
     p.device = g_device;
     p.precision = 0;
     p.cell_size = 0;
+    p.pair_list_cap = 0;
     if (g_ctx) gzm_ctx_destroy(g_ctx);
     g_ctx = NULL;
     int s = gzm_ctx_create(&g_ctx, &p);

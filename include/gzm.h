@@ -64,6 +64,11 @@ typedef struct gzm_params {
     int32_t device;         /* HIP device ordinal */
     int32_t precision;      /* 0: plain FP64 pass (the only mode of this release) */
     double cell_size;       /* grid cell side; <= 0 picks one from the initial hsml */
+    int32_t pair_list_cap;  /* neighbours kept per particle in the pair list that the
+                               gradient loop hands to the force loop: 0 = default (64),
+                               < 0 = no list (the force loop scans the grid); particles
+                               with more pairs fall back to the scan */
+    int32_t reserved;
 } gzm_params;
 
 /* Host arrays of one particle set, caller order.  Inputs are read by

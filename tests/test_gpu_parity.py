@@ -64,6 +64,19 @@ def test_pass_matches_oracle(name):
         assert errs["n_tied"] == 0
 
 
+@pytest.mark.parametrize("cap", [-1, 4, 24])
+def test_pair_list_fallbacks_give_identical_results(cap):
+    """The force loop reads the gradient loop's pair list; capacity 4 and 24
+    send all / some slices to the scanning fallback, -1 disables the list.
+    Every variant must equal the default bit for bit (same summation order)."""
+    ic = ics.random_state(14)
+    base, bst = gzm.run_pass(ic)
+    out, st = gzm.run_pass(ic, params=gzm.make_params(pair_list_cap=cap))
+    assert st["force_interactions"] == bst["force_interactions"]
+    for k in ("acc", "dtu", "vsig", "grad_v"):
+        np.testing.assert_array_equal(out[k], base[k])
+
+
 @pytest.mark.parametrize("cell_size", [0.0, 0.03, 0.11, 0.4])
 def test_neighbor_sets_bit_exact_for_any_grid(cell_size):
     rng = np.random.default_rng(9)
