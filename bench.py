@@ -49,7 +49,24 @@ def parse():
     ap.add_argument("--n", type=int, default=128, help="lattice side (default 128 -> 2,097,152 particles)")
     ap.add_argument("--cpu-repeats", type=int, default=3, help="oracle passes for the CPU baseline (0 = skip)")
     ap.add_argument("--cell-size", type=float, default=0.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="instead of the pass, copy a buffer of this many MiB three times and exit: the "
+                         "known-size reference that calibrates the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """Device-to-device copies of exactly `mib` MiB (read mib, write mib each)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dst[-16:], src[-16:]))
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 8, "bytes_written_per_copy": n * 8,
+                      "copies": 3, "ok": ok}))
 
 
 def cpu_cores():
@@ -69,6 +86,10 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
 
     import gzm
     import ics

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wall", f"-I{INC}",
 ]
 
-HIP_SOURCES = ["csrc/gzm_api.hip", "csrc/gzm_kernels.hip"]
+HIP_SOURCES = ["csrc/gzm_api.hip", "csrc/gzm_kernels.hip", "csrc/gzm_kernels_f32.hip"]
 HOST_SOURCES = ["host/gizmo_host.c"]
 
 

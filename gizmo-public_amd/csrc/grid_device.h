@@ -1,0 +1,149 @@
+// grid_device.h -- device helpers shared by the stage kernels: launch
+// geometry, the cell-grid walk, the periodic wrap and wavefront reductions.
+#pragma once
+
+#include "gzm_internal.h"
+
+namespace gzm {
+
+constexpr int kBlock = 256;
+constexpr int kXcds = 8;         // MI355X: 8 XCDs, each with its own L2
+constexpr int kListChunk = 16;   // pair-list entries a lane stages before one 64-byte store
+
+// Workgroups are dealt to the XCDs round-robin (block b runs on XCD b % 8).
+// Renumber them so that each XCD receives one contiguous run of the
+// Morton-ordered particles: its L2 then holds the cells its waves share,
+// instead of every eighth block of the whole box.  A bijection on
+// [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block()
+{
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
+
+__device__ __forceinline__ int cell_coord(double x, double inv_s, int nc)
+{
+    int c = (int)floor(x * inv_s);
+    if (c < 0) c = 0;
+    if (c >= nc) c = nc - 1;
+    return c;
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton3(int cx, int cy, int cz)
+{
+    return (spread10((uint32_t)cx) << 2) | (spread10((uint32_t)cy) << 1) | spread10((uint32_t)cz);
+}
+
+__device__ __forceinline__ double axis_gap(double x, double c0, double s)
+{
+    double g = c0 - x;
+    double g2 = x - (c0 + s);
+    g = (g > g2) ? g : g2;
+    return (g > 0) ? g : 0.0;
+}
+
+// Visit every cell that may hold a particle within distance R of (x, y, z),
+// periodic; cells whose box lies farther than R are skipped.  f(cx, cy, cz,
+// begin, end) receives the cell's unwrapped coordinates (outside [0, nc) when
+// the walk crosses the periodic boundary; the image of a particle of that cell
+// lies at x + (c - c_wrapped) * s) and its sorted slot range.  `full` reports
+// the axes walked whole because R spans half the box.
+template <typename F>
+__device__ __forceinline__ void for_each_cell(const Grid &g, const double xs[3], double R, bool full[3], F &&f)
+{
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        // a radius that spans the axis (or is not finite) walks the whole axis once
+        full[a] = !(R * g.inv_s[a] < 0.5 * g.nc[a] - 2.0);
+        if (!full[a]) {
+            lo[a] = (int)floor((xs[a] - R) * g.inv_s[a] - 1e-6);
+            hi[a] = (int)floor((xs[a] + R) * g.inv_s[a] + 1e-6);
+        } else {
+            lo[a] = 0;
+            hi[a] = g.nc[a] - 1;
+        }
+    }
+    const double R2 = R * R * (1.0 + 1e-9);
+    for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+        const int wx = (cx < 0) ? cx + g.nc[0] : ((cx >= g.nc[0]) ? cx - g.nc[0] : cx);
+        const double gx = full[0] ? 0.0 : axis_gap(xs[0], cx * g.s[0], g.s[0]);
+        const double gx2 = gx * gx;
+        if (gx2 > R2) continue;
+        for (int cy = lo[1]; cy <= hi[1]; ++cy) {
+            const int wy = (cy < 0) ? cy + g.nc[1] : ((cy >= g.nc[1]) ? cy - g.nc[1] : cy);
+            const double gy = full[1] ? 0.0 : axis_gap(xs[1], cy * g.s[1], g.s[1]);
+            const double gxy2 = gx2 + gy * gy;
+            if (gxy2 > R2) continue;
+            const int rowbase = (wx * g.nc[1] + wy) * g.nc[2];
+            for (int cz = lo[2]; cz <= hi[2]; ++cz) {
+                const int wz = (cz < 0) ? cz + g.nc[2] : ((cz >= g.nc[2]) ? cz - g.nc[2] : cz);
+                const double gz = full[2] ? 0.0 : axis_gap(xs[2], cz * g.s[2], g.s[2]);
+                if (gxy2 + gz * gz > R2) continue;
+                const int2 r = g.range[rowbase + wz];
+                f(cx, cy, cz, r.x, r.y);
+            }
+        }
+    }
+}
+
+// Visit every sorted slot k whose cell may hold a particle within distance R
+// of (x, y, z), periodic.
+template <typename F>
+__device__ __forceinline__ void for_each_candidate(const Grid &g, const double xs[3], double R, F &&f)
+{
+    bool full[3];
+    for_each_cell(g, xs, R, full, [&](int, int, int, int begin, int end) {
+        for (int k = begin; k < end; ++k) f(k);
+    });
+}
+
+__device__ __forceinline__ void wrap_dp(const BoxInfo &box, double dp[3])
+{
+    dp[1] = wrap_axis(dp[1], box.L[1], box.half[1]);
+    dp[0] = wrap_axis(dp[0], box.L[0], box.half[0]);
+    dp[2] = wrap_axis(dp[2], box.L[2], box.half[2]);
+}
+
+// wavefront-wide sum, then one atomic per wavefront
+__device__ __forceinline__ void wave_add_u64(unsigned long long *dst, unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+// MINMAX_CHECK of the reference (else-if: a new minimum is not also tested
+// as a maximum, gradients.c:29).  Written with selects: as two conditional
+// stores the compiler may merge them into one store through a selected
+// address, which moves the accumulators to scratch memory.
+template <typename T>
+__device__ __forceinline__ void minmax_check(T x, T &mn, T &mx)
+{
+    const bool lt = x < mn;
+    const bool gt = !lt && (x > mx);
+    mn = lt ? x : mn;
+    mx = gt ? x : mx;
+}
+
+// One lane's staged list chunk (column `lane` of `stage`) -> 64 bytes of its row.
+__device__ __forceinline__ void flush_list_chunk(const int *stage, int *dst)
+{
+    int4 *d = reinterpret_cast<int4 *>(dst);
+#pragma unroll
+    for (int q = 0; q < kListChunk / 4; q++)
+        d[q] = make_int4(stage[(4 * q + 0) * kBlock + threadIdx.x], stage[(4 * q + 1) * kBlock + threadIdx.x],
+                         stage[(4 * q + 2) * kBlock + threadIdx.x], stage[(4 * q + 3) * kBlock + threadIdx.x]);
+}
+
+}  // namespace gzm

@@ -81,6 +81,11 @@ struct gzm_ctx {
     double *d_hmax = nullptr;
     GradRec *d_grec = nullptr;
     ForceRec *d_frec = nullptr;
+    // FP32-bulk records (precision 1)
+    float4 *d_offm = nullptr, *d_velf = nullptr;
+    uint32_t *d_cellc = nullptr;
+    GradRecF *d_grecf = nullptr;
+    ForceRecF *d_frecf = nullptr;
     Counters *d_ctr = nullptr;
     int n_active = 0;
     bool all_active = true;
@@ -201,6 +206,7 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_wakeup); dev_free(c->d_is_active);
     dev_free(c->d_active_caller); dev_free(c->d_active); dev_free(c->d_live); dev_free(c->d_live2);
     dev_free(c->d_flags); dev_free(c->d_grec); dev_free(c->d_frec);
+    dev_free(c->d_offm); dev_free(c->d_velf); dev_free(c->d_cellc); dev_free(c->d_grecf); dev_free(c->d_frecf);
     dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
     c->list_entries = 0;
     c->list_valid = false;
@@ -288,6 +294,16 @@ gzm_status set_active(gzm_ctx *c, const int32_t *active, int64_t n_active)
     return GZM_OK;
 }
 
+F32Geom f32_geom(const gzm_ctx *c)
+{
+    F32Geom g;
+    g.offm = c->d_offm;
+    g.velf = c->d_velf;
+    g.cellc = c->d_cellc;
+    for (int a = 0; a < 3; a++) g.s[a] = (float)c->grid.s[a];
+    return g;
+}
+
 // the gradient kernel writes a pair list sized for the active list
 gzm_status ensure_list(gzm_ctx *c)
 {
@@ -362,14 +378,20 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
         if (!(params->box[a] > 0)) { set_error("box[%d] must be > 0", a); return GZM_ERR_ARG; }
     }
     if (!(params->gamma > 1)) { set_error("gamma must be > 1"); return GZM_ERR_ARG; }
-    if (params->precision != 0) { set_error("only precision 0 (plain FP64) is available"); return GZM_ERR_ARG; }
+    if (params->precision != 0 && params->precision != 1) {
+        set_error("precision must be 0 (plain FP64) or 1 (FP32 bulk)");
+        return GZM_ERR_ARG;
+    }
+    if (params->pair_list_cap > 4096) { set_error("pair_list_cap %d exceeds 4096", params->pair_list_cap); return GZM_ERR_ARG; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { set_error("no HIP device"); return GZM_ERR_NO_DEVICE; }
     if (params->device < 0 || params->device >= ndev) { set_error("device %d not present", params->device); return GZM_ERR_ARG; }
     GZM_HIP(hipSetDevice(params->device));
     gzm_ctx *c = new gzm_ctx();
     c->p = *params;
+    // whole 64-byte chunks per row (the gradient kernel stores 16 entries at a time)
     c->list_cap = params->pair_list_cap == 0 ? 64 : params->pair_list_cap;
+    if (c->list_cap > 0) c->list_cap = (c->list_cap + 15) / 16 * 16;
     memset(&c->timing, 0, sizeof(c->timing));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
@@ -463,8 +485,16 @@ gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
         s = s ? s : dev_alloc(&c->d_live, n);
         s = s ? s : dev_alloc(&c->d_live2, n);
         s = s ? s : dev_alloc(&c->d_flags, n);
-        s = s ? s : dev_alloc(&c->d_grec, n);
-        s = s ? s : dev_alloc(&c->d_frec, n);
+        if (c->p.precision == 0) {
+            s = s ? s : dev_alloc(&c->d_grec, n);
+            s = s ? s : dev_alloc(&c->d_frec, n);
+        } else {
+            s = s ? s : dev_alloc(&c->d_offm, n);
+            s = s ? s : dev_alloc(&c->d_velf, n);
+            s = s ? s : dev_alloc(&c->d_cellc, n);
+            s = s ? s : dev_alloc(&c->d_grecf, n);
+            s = s ? s : dev_alloc(&c->d_frecf, n);
+        }
         if (s) { free_particles(c); return s; }
     }
 
@@ -519,6 +549,8 @@ gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
                                           30, c->stream));
         launch_gather_sorted(c->stream, n, c->d_perm, st_pos, st_vel, st_mass, st_u, c->d_posm, c->d_velu, c->d_inv);
         launch_cell_ranges(c->stream, n, c->d_posm, c->grid, c->d_range, c->ncell);
+        if (c->p.precision == 1)
+            launch_pack_geom_f32(c->stream, n, c->d_posm, c->d_velu, c->grid, c->d_offm, c->d_velf, c->d_cellc);
         GZM_HIP(hipGetLastError());
     }
     // d_iota was consumed as sort values; rebuild it as the identity for "all active"
@@ -613,6 +645,7 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
         if (s) return s;
     }
     DensitySweepArgs a;
+    memset(&a, 0, sizeof(a));
     a.grid = c->grid;
     a.box = c->box;
     a.hp.des_num_ngb = c->p.des_num_ngb;
@@ -626,6 +659,7 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
     a.num_ngb = c->d_num_ngb; a.dhsml = c->d_dhsml; a.divvel = c->d_divvel; a.rho = c->d_rho;
     a.fce = c->d_fce; a.nvt = c->d_nvt;
     a.redo = c->d_flags;
+    a.f32 = f32_geom(c);
     a.ctr = c->d_ctr;
     int iter = 0, sweeps = 0;
     int64_t updates = 0;
@@ -637,7 +671,8 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
         a.n_live = n_live;
         {
             Timed tm(c, kDensity);
-            launch_density_sweep(c->stream, a);
+            if (c->p.precision == 1) launch_density_sweep_f32(c->stream, a);
+            else launch_density_sweep(c->stream, a);
         }
         GZM_HIP(hipGetLastError());
         sweeps++;
@@ -688,9 +723,13 @@ gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gz
     if (s) return s;
     {
         Timed tm(c, kOther);
-        launch_pack_grad(c->stream, c->n, c->d_posm, c->d_velu, c->d_hsml, c->d_rho, c->d_pres, c->d_grec);
+        if (c->p.precision == 1)
+            launch_pack_grad_f32(c->stream, c->n, c->d_offm, c->d_velf, c->d_hsml, c->d_rho, c->d_pres, c->d_grecf);
+        else
+            launch_pack_grad(c->stream, c->n, c->d_posm, c->d_velu, c->d_hsml, c->d_rho, c->d_pres, c->d_grec);
     }
     GradientArgs a;
+    memset(&a, 0, sizeof(a));
     a.grid = c->grid;
     a.box = c->box;
     a.active = c->d_active;
@@ -700,8 +739,13 @@ gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gz
     a.nvt = c->d_nvt;
     a.cn = c->d_cn;
     a.grad = c->d_grad;
+    a.posm = c->d_posm;
+    a.hsml = c->d_hsml;
+    a.rho = c->d_rho;
+    a.pres = c->d_pres;
+    a.recf = c->d_grecf;
+    a.f32 = f32_geom(c);
     a.ctr = c->d_ctr;
-    memset(&a.list, 0, sizeof(a.list));
     if (c->n_active > 0 && c->list_cap > 0) {
         s = ensure_list(c);
         if (s) return s;
@@ -712,7 +756,8 @@ gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gz
     }
     if (c->n_active > 0) {
         Timed tm(c, kGradients);
-        launch_gradients(c->stream, a);
+        if (c->p.precision == 1) launch_gradients_f32(c->stream, a);
+        else launch_gradients(c->stream, a);
     }
     GZM_HIP(hipGetLastError());
     Counters h;
@@ -740,8 +785,13 @@ gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, 
     if (s) return s;
     {
         Timed tm(c, kOther);
-        launch_pack_force(c->stream, c->n, c->d_posm, c->d_velu, c->d_hsml, c->d_rho, c->d_pres, c->d_dhsml,
-                          c->d_cn, c->d_fce, c->d_nvt, c->d_grad, c->p.gamma, c->d_frec);
+        if (c->p.precision == 1)
+            launch_pack_force_f32(c->stream, c->n, c->d_offm, c->d_velf, c->d_cellc, c->d_posm, c->d_hsml, c->d_rho,
+                                  c->d_pres, c->d_dhsml, c->d_cn, c->d_fce, c->d_nvt, c->d_grad, c->p.gamma,
+                                  c->d_frecf);
+        else
+            launch_pack_force(c->stream, c->n, c->d_posm, c->d_velu, c->d_hsml, c->d_rho, c->d_pres, c->d_dhsml,
+                              c->d_cn, c->d_fce, c->d_nvt, c->d_grad, c->p.gamma, c->d_frec);
         if (!c->all_active) {
             launch_fill_u8(c->stream, c->d_is_active, c->n, 0);
             launch_set_flags(c->stream, c->n_active, c->d_active, c->d_is_active);
@@ -749,6 +799,7 @@ gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, 
         }
     }
     ForceArgs a;
+    memset(&a, 0, sizeof(a));
     a.grid = c->grid;
     a.box = c->box;
     a.G = c->G;
@@ -763,8 +814,16 @@ gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, 
     a.dtu = c->d_dtu;
     a.vsig = c->d_vsig;
     a.wakeup = c->d_wakeup;
+    a.posm = c->d_posm;
+    a.velu = c->d_velu;
+    a.hsml = c->d_hsml;
+    a.recf = c->d_frecf;
+    a.f32 = f32_geom(c);
+    // the FP32 list kernel takes pair displacements from cell differences
+    // unless the search radius spans half the box on some axis
+    for (int ax = 0; ax < 3; ax++)
+        if (!(c->hmax * c->grid.inv_s[ax] < 0.5 * c->grid.nc[ax] - 2.0)) a.exact_geom = 1;
     a.ctr = c->d_ctr;
-    memset(&a.list, 0, sizeof(a.list));
     a.fallback_only = 0;
     const bool use_list = c->list_valid && c->list_version == c->active_version;
     if (use_list) {
@@ -775,10 +834,11 @@ gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, 
     }
     if (c->n_active > 0) {
         Timed tm(c, kForce);
-        launch_force(c->stream, a);
+        auto launch = (c->p.precision == 1) ? launch_force_f32 : launch_force;
+        launch(c->stream, a);
         if (use_list && c->list_overflow_slices > 0) {
             a.fallback_only = 1;  // slices whose list overflowed scan the grid
-            launch_force(c->stream, a);
+            launch(c->stream, a);
         }
     }
     GZM_HIP(hipGetLastError());

@@ -25,9 +25,9 @@ struct alignas(64) ForceRec {
     double4 xh;    // x, y, z, hsml
     double4 vc;    // vx, vy, vz, sound speed sqrt(gamma P / rho)
     double4 th;    // rho, P, V = m / rho, DhsmlNgbFactor
-    double4 mc;    // m, ConditionNumber, FaceClosureError, 0
+    double4 mc;    // m, ConditionNumber, FaceClosureError, P / rho
     double4 t0;    // NV_T: i00 i01 i02 i11
-    double4 t1;    // NV_T: i12 i22 0 0
+    double4 t1;    // NV_T: i12 i22, 1 / hsml, 0
     double4 g0;    // grad rho (3), grad P x
     double4 g1;    // grad P y, z, grad v row 0 (x, y)
     double4 g2;    // grad v row 0 z, row 1 (x, y, z)
@@ -39,6 +39,43 @@ struct alignas(32) GradRec {
     double4 xh;    // x, y, z, hsml
     double4 vr;    // vx, vy, vz, rho
     double2 pm;    // P, m
+};
+
+// ---------------------------------------------------------------------------
+// FP32-bulk records.  A position is stored as the packed integer coordinates
+// of its grid cell plus an FP32 offset from that cell's FP64 origin
+// (x = c * s + off), so a pair displacement is (c_i - c_j) * s +
+// (off_i - off_j): exact cell arithmetic plus an FP32 term of the size of a
+// cell, whatever the box size.
+// ---------------------------------------------------------------------------
+struct F32Geom {
+    const float4 *offm;        // off x, y, z, m (FP32), sorted order
+    const float4 *velf;        // v x, y, z, 0 (FP32)
+    const uint32_t *cellc;     // cx | cy << 10 | cz << 20
+    float s[3];                // cell side in FP32
+};
+
+// Per-particle record of the FP32 gradient loop (48 bytes).
+struct alignas(16) GradRecF {
+    float4 xh;     // off x, y, z, hsml
+    float4 vr;     // vx, vy, vz, rho
+    float4 pm;     // P, m, 0, 0
+};
+
+// Per-particle record of the FP32 force loop (160 bytes); P / rho and the
+// condition number stay FP64 because the loop compares them exactly.
+struct alignas(16) ForceRecF {
+    float4 xh;     // off x, y, z, hsml
+    float4 vc;     // vx, vy, vz, sound speed
+    float4 th;     // rho, P, V = m / rho, DhsmlNgbFactor
+    float4 t0;     // NV_T: i00 i01 i02 i11
+    float4 t1;     // NV_T: i12 i22, 1 / hsml, FaceClosureError
+    float4 g0;     // grad rho (3), grad P x
+    float4 g1;     // grad P y, z, grad v row 0 (x, y)
+    float4 g2;     // grad v row 0 z, row 1 (x, y, z)
+    float4 g3;     // grad v row 2 (x, y, z), packed cell coordinates (bits)
+    double cn;     // ConditionNumber
+    double e;      // P / rho
 };
 
 // Counters accumulated by the kernels (device memory, zeroed per stage).
@@ -61,16 +98,18 @@ struct DensitySweepArgs {
     double *hsml, *left, *right, *cn;
     double *num_ngb, *dhsml, *divvel, *rho, *fce, *nvt;  // nvt: 6 per particle
     int *redo;             // per live entry: 1 = sweep again
+    F32Geom f32;           // FP32-bulk kernels only
     Counters *ctr;
 };
 
-// Pair list in sliced-ELL form: the 64 consecutive entries of the active list
-// that one wavefront handles form a slice; entry k of lane l of slice s is
-// list[(s * cap + k) * 64 + l], so a wavefront reads 64 neighbours with one
-// coalesced 256-byte load.  A slice whose longest list exceeds `cap` is
-// flagged and handled by the scanning fallback.
+// Pair list: row t (entry t of the active list) holds the force-loop pair
+// set of that particle in scan order, idx[t * cap + k]; cap is a multiple of
+// 16 so each row is a whole number of 64-byte chunks.  The 64 consecutive
+// entries of the active list that one wavefront handles form a slice; a
+// slice in which any particle has more than `cap` pairs is flagged and
+// handled by the scanning fallback.
 struct PairList {
-    int *idx;              // (n_slices * cap * 64)
+    int *idx;              // (n_slices * 64 * cap)
     int *count;            // per active entry
     uint8_t *overflow;     // per slice
     int cap;
@@ -87,6 +126,12 @@ struct GradientArgs {
     const double *cn;
     double *grad;          // 15 per particle: rho(3), P(3), v(3x3 row-major)
     PairList list;         // written when list.idx != NULL
+    const double4 *posm;   // FP64 positions and masses (FP32 kernels: exact re-tests)
+    const double *hsml;    // FP64 hsml (FP32 kernels: exact re-tests)
+    const double *rho;     // FP64 density and pressure of i (FP32 kernels: limiter)
+    const double *pres;
+    const GradRecF *recf;  // FP32 kernels
+    F32Geom f32;
     Counters *ctr;
 };
 
@@ -107,6 +152,13 @@ struct ForceArgs {
     uint8_t *wakeup;
     PairList list;         // list.idx == NULL: scan the grid for every particle
     int fallback_only;     // scan only the slices flagged in list.overflow
+    const double4 *posm;   // FP64 positions and masses (FP32 kernels)
+    const double4 *velu;   // FP64 velocities (FP32 kernels: final operations)
+    const double *hsml;    // FP64 hsml (FP32 kernels: exact re-tests)
+    const ForceRecF *recf; // FP32 kernels
+    F32Geom f32;
+    int exact_geom;        // FP32 list kernel: pair displacements from FP64 positions
+                           // (the search radius spans half the box on some axis)
     Counters *ctr;
 };
 
@@ -131,6 +183,19 @@ void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 
                        const double *rho, const double *pres, const double *dhsml, const double *cn,
                        const double *fce, const double *nvt, const double *grad, double gamma, ForceRec *rec);
 void launch_force(hipStream_t s, const ForceArgs &a);  // list-driven if a.list.idx, else scanning
+// FP32-bulk kernels (gzm_kernels_f32.hip)
+void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const double4 *velu, Grid g, float4 *offm,
+                          float4 *velf, uint32_t *cellc);
+void launch_density_sweep_f32(hipStream_t s, const DensitySweepArgs &a);
+void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const double *hsml,
+                          const double *rho, const double *pres, GradRecF *rec);
+void launch_gradients_f32(hipStream_t s, const GradientArgs &a);
+void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const uint32_t *cellc,
+                           const double4 *posm, const double *hsml, const double *rho, const double *pres,
+                           const double *dhsml, const double *cn, const double *fce, const double *nvt,
+                           const double *grad, double gamma, ForceRecF *rec);
+void launch_force_f32(hipStream_t s, const ForceArgs &a);
+
 void launch_neighbor_counts(hipStream_t s, Grid g, BoxInfo box, int mode, int n_active, const int *active,
                             const double4 *posm, const double *hsml, double hmax, int *counts, Counters *ctr);
 void launch_fill_int(hipStream_t s, int *p, int n, int value);

@@ -11,100 +11,12 @@
 // No pair writes to j (the reference disables its pair-symmetric writes too,
 // hydro_evaluate.h:159-168, gradients.c:1554-1565), so every kernel is free of
 // races and deterministic for a fixed particle order.
+#include "grid_device.h"
 #include "gzm_internal.h"
 
 namespace gzm {
 
 namespace {
-
-constexpr int kBlock = 256;
-
-__device__ __forceinline__ int cell_coord(double x, double inv_s, int nc)
-{
-    int c = (int)floor(x * inv_s);
-    if (c < 0) c = 0;
-    if (c >= nc) c = nc - 1;
-    return c;
-}
-
-__device__ __forceinline__ uint32_t spread10(uint32_t v)
-{
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
-__device__ __forceinline__ uint32_t morton3(int cx, int cy, int cz)
-{
-    return (spread10((uint32_t)cx) << 2) | (spread10((uint32_t)cy) << 1) | spread10((uint32_t)cz);
-}
-
-__device__ __forceinline__ double axis_gap(double x, double c0, double s)
-{
-    double g = c0 - x;
-    double g2 = x - (c0 + s);
-    g = (g > g2) ? g : g2;
-    return (g > 0) ? g : 0.0;
-}
-
-// Visit every sorted slot k whose cell may hold a particle within distance R
-// of (x, y, z), periodic.  Cells whose box lies farther than R are skipped.
-template <typename F>
-__device__ __forceinline__ void for_each_candidate(const Grid &g, const double xs[3], double R, F &&f)
-{
-    int lo[3], hi[3];
-    bool full[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        // a radius that spans the axis (or is not finite) walks the whole axis once
-        full[a] = !(R * g.inv_s[a] < 0.5 * g.nc[a] - 2.0);
-        if (!full[a]) {
-            lo[a] = (int)floor((xs[a] - R) * g.inv_s[a] - 1e-6);
-            hi[a] = (int)floor((xs[a] + R) * g.inv_s[a] + 1e-6);
-        } else {
-            lo[a] = 0;
-            hi[a] = g.nc[a] - 1;
-        }
-    }
-    const double R2 = R * R * (1.0 + 1e-9);
-    for (int cx = lo[0]; cx <= hi[0]; ++cx) {
-        const int wx = (cx < 0) ? cx + g.nc[0] : ((cx >= g.nc[0]) ? cx - g.nc[0] : cx);
-        const double gx = full[0] ? 0.0 : axis_gap(xs[0], cx * g.s[0], g.s[0]);
-        const double gx2 = gx * gx;
-        if (gx2 > R2) continue;
-        for (int cy = lo[1]; cy <= hi[1]; ++cy) {
-            const int wy = (cy < 0) ? cy + g.nc[1] : ((cy >= g.nc[1]) ? cy - g.nc[1] : cy);
-            const double gy = full[1] ? 0.0 : axis_gap(xs[1], cy * g.s[1], g.s[1]);
-            const double gxy2 = gx2 + gy * gy;
-            if (gxy2 > R2) continue;
-            const int rowbase = (wx * g.nc[1] + wy) * g.nc[2];
-            for (int cz = lo[2]; cz <= hi[2]; ++cz) {
-                const int wz = (cz < 0) ? cz + g.nc[2] : ((cz >= g.nc[2]) ? cz - g.nc[2] : cz);
-                const double gz = full[2] ? 0.0 : axis_gap(xs[2], cz * g.s[2], g.s[2]);
-                if (gxy2 + gz * gz > R2) continue;
-                const int2 r = g.range[rowbase + wz];
-                for (int k = r.x; k < r.y; ++k) f(k);
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ void wrap_dp(const BoxInfo &box, double dp[3])
-{
-    dp[1] = wrap_axis(dp[1], box.L[1], box.half[1]);
-    dp[0] = wrap_axis(dp[0], box.L[0], box.half[0]);
-    dp[2] = wrap_axis(dp[2], box.L[2], box.half[2]);
-}
-
-// wavefront-wide sum, then one atomic per wavefront
-__device__ __forceinline__ void wave_add_u64(unsigned long long *dst, unsigned long long v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
-}
 
 // ---------------------------------------------------------------------------
 // grid build
@@ -220,7 +132,7 @@ __global__ void density_init_kernel(int n_active, const int *active, const doubl
 
 __global__ __launch_bounds__(kBlock) void density_sweep_kernel(DensitySweepArgs a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block() * kBlock + threadIdx.x;
     const bool valid = t < a.n_live;
     unsigned long long cnt = 0;
     if (valid) {
@@ -317,15 +229,14 @@ __global__ void pack_grad_kernel(int n, const double4 *posm, const double4 *velu
     rec[k] = r;
 }
 
-__device__ __forceinline__ void minmax_check(double x, double &mn, double &mx)
-{
-    if (x < mn) mn = x;
-    else if (x > mx) mx = x;
-}
-
 __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // Pair-list entries are staged in LDS, column-major (entry c of thread x
+    // at stage[c * kBlock + x], so same-c stores of a wave hit distinct banks),
+    // and leave as whole 64-byte chunks: single 4-byte stores spread over the
+    // loop would reach HBM as partial-line write-backs.
+    __shared__ int stage[kListChunk * kBlock];
+    const int t = xcd_block() * kBlock + threadIdx.x;
     const bool valid = t < a.n_active;
     unsigned long long cnt = 0;
     int listed = 0;
@@ -337,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
         double maxd = 0, mn_r = 0, mx_r = 0, mn_p = 0, mx_p = 0;
         double mn_v0 = 0, mx_v0 = 0, mn_v1 = 0, mx_v1 = 0, mn_v2 = 0, mx_v2 = 0;
         const double h_i = ri.xh.w, rho_i = ri.vr.w, p_i = ri.pm.x, m_i = ri.pm.y;
-        int *list_row = a.list.idx ? a.list.idx + ((size_t)(t >> 6) * a.list.cap) * 64 + (t & 63) : nullptr;
+        int *list_row = a.list.idx ? a.list.idx + (size_t)t * a.list.cap : nullptr;
         if ((m_i > 0) && (rho_i > 0) && (h_i > 0)) {  // GasGrad_isactive (gradients.c:66-75)
             double hinv, hinv3, hinv4;
             kernel_hinv(h_i, hinv, hinv3, hinv4);
@@ -358,7 +269,11 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
                 // the list holds the force loop's pair set (m_j > 0, rho_j > 0,
                 // hydro_evaluate.h:149-150); gradients also need h_j > 0
                 if (list_row) {
-                    if (listed < a.list.cap) list_row[(size_t)listed * 64] = j;
+                    if (listed < a.list.cap) {
+                        const int c = listed & (kListChunk - 1);
+                        stage[c * kBlock + threadIdx.x] = j;
+                        if (c == kListChunk - 1) flush_list_chunk(stage, list_row + (listed - c));
+                    }
                     listed++;
                 }
                 if (!(h_j > 0)) return;
@@ -383,6 +298,9 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
                     gv20 += w0 * dv2; gv21 += w1 * dv2; gv22 += w2 * dv2;
                 }
             });
+            // the partly filled last chunk (entries past `listed` are never read)
+            if (list_row && listed < a.list.cap && (listed & (kListChunk - 1)))
+                flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)));
         } else if ((m_i > 0) && (rho_i > 0)) {
             listed = a.list.cap + 1;  // the force loop still runs for i: send its slice to the scan
         }
@@ -437,16 +355,16 @@ __global__ void pack_force_kernel(int n, const double4 *posm, const double4 *vel
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double4 p = posm[k], v = velu[k];
-    double d = rho[k], P = pres[k];
+    double d = rho[k], P = pres[k], h = hsml[k];
     const double *t = nvt + 6 * (size_t)k;
     const double *g = grad + 15 * (size_t)k;
     ForceRec r;
-    r.xh = make_double4(p.x, p.y, p.z, hsml[k]);
+    r.xh = make_double4(p.x, p.y, p.z, h);
     r.vc = make_double4(v.x, v.y, v.z, sqrt(gamma * P / d));  // eos.c:245-254
     r.th = make_double4(d, P, p.w / d, dhsml[k]);
-    r.mc = make_double4(p.w, cn[k], fce[k], 0.0);
+    r.mc = make_double4(p.w, cn[k], fce[k], P / d);
     r.t0 = make_double4(t[0], t[1], t[2], t[3]);
-    r.t1 = make_double4(t[4], t[5], 0.0, 0.0);
+    r.t1 = make_double4(t[4], t[5], 1.0 / h, 0.0);
     r.g0 = make_double4(g[0], g[1], g[2], g[3]);
     r.g1 = make_double4(g[4], g[5], g[6], g[7]);
     r.g2 = make_double4(g[8], g[9], g[10], g[11]);
@@ -454,182 +372,20 @@ __global__ void pack_force_kernel(int n, const double4 *posm, const double4 *vel
     rec[k] = r;
 }
 
-struct ForceI {
-    double x[3], v[3], h, hinv, hinv3, hinv4, rho, P, cs, V, dhsml, cn, cnumcrit2, fce;
-    double nvt[6], gr[3], gp[3], gv[9];
-};
-
-__device__ __forceinline__ void unpack_grads(const ForceRec &r, double gr[3], double gp[3], double gv[9])
+// ForceRec -> the pair-side quantities (moves only)
+__device__ __forceinline__ void unpack_side(const ForceRec &r, PairSide<double> &S)
 {
-    gr[0] = r.g0.x; gr[1] = r.g0.y; gr[2] = r.g0.z;
-    gp[0] = r.g0.w; gp[1] = r.g1.x; gp[2] = r.g1.y;
-    gv[0] = r.g1.z; gv[1] = r.g1.w; gv[2] = r.g2.x;
-    gv[3] = r.g2.y; gv[4] = r.g2.z; gv[5] = r.g2.w;
-    gv[6] = r.g3.x; gv[7] = r.g3.y; gv[8] = r.g3.z;
-}
-
-// one (i, j) interaction of the MFM force loop; returns the status code
-__device__ __forceinline__ int force_pair(const ForceI &I, const ForceRec &J, const double dp[3], double r2,
-                                          const GammaConsts &G, ROut &ro, double Fv[3], double &Fp,
-                                          double &vsig, int &mask)
-{
-    int status = 0;
-    const double r = sqrt(r2);
-    const double rinv = 1 / r;
-    const double vj[3] = {J.vc.x, J.vc.y, J.vc.z};
-    const double dv[3] = {I.v[0] - vj[0], I.v[1] - vj[1], I.v[2] - vj[2]};
-    const double rho_j = J.th.x, P_j = J.th.y, V_j = J.th.z, cs_j = J.vc.w, h_j = J.xh.w;
-    vsig = I.cs + cs_j;
-    const double vdotr2 = dp[0] * dv[0] + dp[1] * dv[1] + dp[2] * dv[2];
-    if (vdotr2 < 0) vsig -= 1.0 * vdotr2 * rinv;  // MFM: fac_mu = 1 (hydro_evaluate.h:235-245)
-    double wk_i = 0, dwk_i = 0, wk_j = 0, dwk_j = 0;
-    if (r < I.h) kernel_wdw(r * I.hinv, I.hinv3, I.hinv4, wk_i, dwk_i);
-    if (r < h_j) {
-        double hinv_j, hinv3_j, hinv4_j;
-        kernel_hinv(h_j, hinv_j, hinv3_j, hinv4_j);
-        kernel_wdw(r * hinv_j, hinv3_j, hinv4_j, wk_j, dwk_j);
-    }
-    Fv[0] = Fv[1] = Fv[2] = 0;
-    Fp = 0;
-    mask = 0;
-    const double leak = 0.5 * (I.fce + J.mc.z);
-    const double cn_j = J.mc.y;
-    // effective face (compute_finitevol_faces.h:14-106)
-    double wt_i = I.V, wt_j = V_j;
-    if ((fabs(I.V - V_j) / dmin(I.V, V_j)) / 3 > 1.25) {
-        wt_i = wt_j = I.V * V_j * (wk_i + wk_j) / (I.V * wk_i + V_j * wk_j);
-    }
-    const double tj[6] = {J.t0.x, J.t0.y, J.t0.z, J.t0.w, J.t1.x, J.t1.y};
-    double A[3];
-    double Anorm = 0, fdp = 0;
-    {
-        const double ti_d[3] = {I.nvt[0] * dp[0] + I.nvt[1] * dp[1] + I.nvt[2] * dp[2],
-                                I.nvt[1] * dp[0] + I.nvt[3] * dp[1] + I.nvt[4] * dp[2],
-                                I.nvt[2] * dp[0] + I.nvt[4] * dp[1] + I.nvt[5] * dp[2]};
-        const double tj_d[3] = {tj[0] * dp[0] + tj[1] * dp[1] + tj[2] * dp[2],
-                                tj[1] * dp[0] + tj[3] * dp[1] + tj[4] * dp[2],
-                                tj[2] * dp[0] + tj[4] * dp[1] + tj[5] * dp[2]};
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            A[k] = wk_i * wt_i * ti_d[k] + wk_j * wt_j * tj_d[k];
-            Anorm += A[k] * A[k];
-            fdp += A[k] * dp[k];
-        }
-    }
-    if ((cn_j * cn_j > 1.0e12 + I.cnumcrit2) || (fdp < 0)) {
-        double An = -(wt_i * I.V * dwk_i + wt_j * V_j * dwk_j) / r;
-        A[0] = An * dp[0];
-        A[1] = An * dp[1];
-        A[2] = An * dp[2];
-        Anorm = An * An * r2;
-    }
-    Anorm = sqrt(Anorm);
-    if (Anorm == 0) return 0;
-
-    const double n[3] = {A[0] / Anorm, A[1] / Anorm, A[2] / Anorm};
-    // faces at the midpoint: s_i = -r/2, s_j = +r/2 (hydro_core_meshless.h:70-90)
-    const double s_i = 0 - 0.5 * r, s_j = 0 - (-0.5 * r);
-    double di[3], dj[3], vf[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const double e = dp[k] * rinv;
-        dj[k] = e * s_j;
-        di[k] = e * s_i;
-        vf[k] = rinv * (-s_i * vj[k] + s_j * I.v[k]);
-    }
-    double fvi = 0, fvj = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) { fvi += I.v[k] * n[k]; fvj += vj[k] * n[k]; }
-    const double fadv = rinv * (-s_i * fvj + s_j * fvi);
-    double v2a = 0;
-    double vdotr2_phys = vdotr2;
-    vdotr2_phys *= 1 / (r * 1.0);
-    if (vdotr2_phys < 0) v2a = vdotr2_phys * vdotr2_phys;
-    const double vdotf2 = fvi - fvj;
-    if (vdotf2 < 0) v2a = dmax(v2a, vdotf2 * vdotf2);
-    const int recon = (leak > 1) ? 0 : 1;
-    RState SL, SR;
-    double gr_j[3], gp_j[3], gv_j[9];
-    unpack_grads(J, gr_j, gp_j, gv_j);
-    reconstruct(I.rho, I.gr, rho_j, gr_j, di, dj, SL.rho, SR.rho, recon);
-    reconstruct(I.P, I.gp, P_j, gp_j, di, dj, SL.p, SR.p, recon);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        reconstruct(I.v[k], I.gv + 3 * k, vj[k], gv_j + 3 * k, di, dj, SL.v[k], SR.v[k], recon);
-        SL.v[k] -= vf[k];
-        SR.v[k] -= vf[k];
-    }
-    const double pti = I.P + I.rho * v2a;
-    const double ptj = P_j + rho_j * v2a;
-    double lim = 1.1 * 1.0 * dmax(pti, ptj);
-    if (recon == 0) lim = dmax(lim, dmax(dmax(I.P, P_j), 2. * dmax(I.rho, rho_j) * v2a));
-    // Riemann problem with the reference's two retries (hydro_core_meshless.h:209-265):
-    // attempt 0 reconstructed states, 1 first-order states, 2 first-order with
-    // zero velocities; one solver instance in the code
-    int err = 0;
-    double lim_a = lim;
-    for (int attempt = 0; attempt < 3; attempt++) {
-        mask |= riemann_solve(SL, SR, n, lim_a, G, ro, err);
-        const bool bad = (ro.P_M < 0) || isnan(ro.P_M);
-        if (attempt == 0) {
-            if (!(bad || (ro.P_M > 1.4 * lim))) break;
-            mask |= 8;
-            lim_a = 1.4 * lim;
-        } else if (attempt == 1) {
-            if (!bad) break;
-            lim_a = 2.0 * lim;
-        } else {
-            if (bad) status = 1234;
-            break;
-        }
-        SR.p = I.P; SL.p = P_j;
-        SR.rho = I.rho; SL.rho = rho_j;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            SR.v[k] = (attempt == 0) ? I.v[k] - vf[k] : 0.0;
-            SL.v[k] = (attempt == 0) ? vj[k] - vf[k] : 0.0;
-        }
-    }
-    if (err) status = err;
-    if ((ro.P_M > 0) && !isnan(ro.P_M)) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) ro.Fp += vf[k] * ro.Fv[k];
-        Fp = Anorm * ro.Fp;
-#pragma unroll
-        for (int k = 0; k < 3; k++) Fv[k] = Anorm * ro.Fv[k];
-        // MFM adiabatic face correction (hydro_core_meshless.h:335-374)
-        const double sm_c = fabs(ro.S_M) / dmin(I.cs, cs_j);
-        if ((sm_c < 0.5) || (leak > 1)) {
-            int use = 1;
-            const double fpm = ro.P_M * Anorm;
-            const double pdv_fac = ro.P_M * vdotr2_phys / 1.0;
-            const double pdv_i = dwk_i * I.V * I.V * I.dhsml * pdv_fac;
-            const double pdv_j = dwk_j * V_j * V_j * J.th.w * pdv_fac;
-            const double du_old = fpm * (ro.S_M + fadv);
-            const double du_new = 0.5 * (pdv_i - pdv_j + fpm * (fvi + fvj));
-            const double cnum2 = cn_j * cn_j;
-            if (sm_c > 1.e-3 && cnum2 < I.cnumcrit2) {
-                const double ei = I.P / I.rho, ej = P_j / rho_j;
-                if (ei != ej) {
-                    if (ei > ej) {
-                        const double dtoj = -du_old + fpm * fvj;
-                        if (dtoj > 0) use = 0;
-                        else if (dtoj < 0) { if (dtoj > -du_new + fpm * fvj) use = 0; }
-                    } else {
-                        const double dtoi = +du_old - fpm * fvi;
-                        if (dtoi > 0) use = 0;
-                        else if (dtoi < 0) { if (dtoi > +du_new - fpm * fvi) use = 0; }
-                    }
-                }
-            }
-            if (cnum2 >= I.cnumcrit2) use = 1;
-            if (use) Fp += du_new - du_old;
-        }
-    } else {
-        Fp = 0;
-        Fv[0] = Fv[1] = Fv[2] = 0;
-    }
-    return status;
+    S.v[0] = r.vc.x; S.v[1] = r.vc.y; S.v[2] = r.vc.z; S.cs = r.vc.w;
+    S.h = r.xh.w; S.hinv = r.t1.z;
+    S.rho = r.th.x; S.P = r.th.y; S.V = r.th.z; S.dhsml = r.th.w;
+    S.cn = r.mc.y; S.fce = r.mc.z; S.e = r.mc.w;
+    S.nvt[0] = r.t0.x; S.nvt[1] = r.t0.y; S.nvt[2] = r.t0.z; S.nvt[3] = r.t0.w;
+    S.nvt[4] = r.t1.x; S.nvt[5] = r.t1.y;
+    S.gr[0] = r.g0.x; S.gr[1] = r.g0.y; S.gr[2] = r.g0.z;
+    S.gp[0] = r.g0.w; S.gp[1] = r.g1.x; S.gp[2] = r.g1.y;
+    S.gv[0] = r.g1.z; S.gv[1] = r.g1.w; S.gv[2] = r.g2.x;
+    S.gv[3] = r.g2.y; S.gv[4] = r.g2.z; S.gv[5] = r.g2.w;
+    S.gv[6] = r.g3.x; S.gv[7] = r.g3.y; S.gv[8] = r.g3.z;
 }
 
 // USE_LIST: walk the pair list written by the gradient kernel (slices that
@@ -638,7 +394,7 @@ __device__ __forceinline__ int force_pair(const ForceI &I, const ForceRec &J, co
 template <bool USE_LIST>
 __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block() * kBlock + threadIdx.x;
     bool valid = t < a.n_active;
     if (USE_LIST) {
         if (valid && a.list.overflow[t >> 6]) valid = false;
@@ -653,23 +409,18 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
         const double m_i = ri.mc.x, rho_i = ri.th.x;
         int status = 0;
         if ((m_i > 0) && (rho_i > 0)) {
-            ForceI I;
-            I.x[0] = ri.xh.x; I.x[1] = ri.xh.y; I.x[2] = ri.xh.z;
-            I.v[0] = ri.vc.x; I.v[1] = ri.vc.y; I.v[2] = ri.vc.z;
-            I.h = ri.xh.w;
-            kernel_hinv(I.h, I.hinv, I.hinv3, I.hinv4);
-            I.rho = rho_i; I.P = ri.th.y; I.cs = ri.vc.w; I.V = ri.th.z; I.dhsml = ri.th.w;
-            I.cn = ri.mc.y; I.fce = ri.mc.z;
+            PairI<double> I;
+            unpack_side(ri, I);
+            const double x_i[3] = {ri.xh.x, ri.xh.y, ri.xh.z};
+            I.hinv3 = I.hinv * I.hinv * I.hinv;
+            I.hinv4 = I.hinv3 * I.hinv;
             I.cnumcrit2 = kCondDanger * kCondDanger - I.cn * I.cn;
-            I.nvt[0] = ri.t0.x; I.nvt[1] = ri.t0.y; I.nvt[2] = ri.t0.z; I.nvt[3] = ri.t0.w;
-            I.nvt[4] = ri.t1.x; I.nvt[5] = ri.t1.y;
-            unpack_grads(ri, I.gr, I.gp, I.gv);
             double out_vsig = I.cs;
-            ROut ro = {0, 0, {0, 0, 0}, 0};
+            ROut<double> ro = {0, 0, {0, 0, 0}, 0};
             const double h2_i = I.h * I.h;
             auto visit = [&](int j, bool check) {
                 const double4 xj = a.rec[j].xh;
-                double dp[3] = {I.x[0] - xj.x, I.x[1] - xj.y, I.x[2] - xj.z};
+                double dp[3] = {x_i[0] - xj.x, x_i[1] - xj.y, x_i[2] - xj.z};
                 wrap_dp(a.box, dp);
                 const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
                 if (check) {
@@ -678,7 +429,8 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
                     if (r2 <= 0) return;
                     if (!(a.rec[j].mc.x > 0) || !(a.rec[j].th.x > 0)) return;  // hydro_evaluate.h:149-150
                 }
-                const ForceRec J = a.rec[j];
+                PairSide<double> J;
+                unpack_side(a.rec[j], J);
                 cnt++;
                 double Fv[3], Fp, vsig;
                 int mask;
@@ -699,11 +451,11 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
                 // the list holds the pair set in the gradient kernel's scan order,
                 // which is this kernel's scan order as well (same predicate, same R)
                 const int n = a.list.count[t];
-                const int *row = a.list.idx + ((size_t)(t >> 6) * a.list.cap) * 64 + (t & 63);
-                for (int k = 0; k < n; k++) visit(row[(size_t)k * 64], false);
+                const int *row = a.list.idx + (size_t)t * a.list.cap;
+                for (int k = 0; k < n; k++) visit(row[k], false);
             } else {
                 const double R = (I.h > a.hmax) ? I.h : a.hmax;
-                for_each_candidate(a.grid, I.x, R, [&](int j) { visit(j, true); });
+                for_each_candidate(a.grid, x_i, R, [&](int j) { visit(j, true); });
             }
             if (vmax < out_vsig) vmax = out_vsig;
             // hydro_final_operations_and_cleanup (hydro_toplevel.c:738-764)
@@ -736,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void neighbor_count_kernel(Grid g, BoxInfo 
                                                                 const double *hsml, double hmax, int *counts,
                                                                 Counters *ctr)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block() * kBlock + threadIdx.x;
     unsigned long long cnt = 0;
     if (t < n_active) {
         const int i = active[t];

@@ -1,0 +1,583 @@
+// gzm_kernels_f32.hip -- FP32-bulk kernels of the MFM hydro pass (gfx950).
+//
+// Same loops as gzm_kernels.hip (one thread per active particle, gather
+// form), with the per-pair arithmetic in FP32:
+//   * geometry: FP32 offsets from FP64 cell origins plus exact integer cell
+//     differences (the F32Geom layout in gzm_internal.h);
+//   * neighbour predicates: decided on the FP32 r^2 unless it lies within a
+//     relative band of the threshold, where the pair is re-tested with the FP64
+//     expression of the FP64 kernels, so the neighbour sets are those of the
+//     FP64 pass for the same hsml, bit for bit;
+//   * kernel weights, MLS sums, face areas, reconstruction, HLLC in FP32;
+//   * FP64 kept for the hsml iteration (hsml_update), the NV_T inversion, the
+//     slope limiter, the exact Riemann solver, the entropic-switch comparison
+//     of P/rho, and the accumulation of the conserved fluxes (dv/dt, du/dt).
+#include "grid_device.h"
+#include "gzm_internal.h"
+
+namespace gzm {
+
+namespace {
+
+// Bound on |r2_fp32 - r2_fp64| relative to the threshold h^2.  The offsets
+// carry 2^-24 s rounding, a cell shift dc * s_f carries 2 * 2^-24 |dc| s, and
+// the component sums, squares and their sum add 2^-24 each; for the
+// neighbourhoods walked here (|dc| s ~ r ~ h) that is below 2e-6 h^2, so 2e-5
+// leaves a factor of ten.  Pairs inside the band are decided in FP64.
+constexpr double kBand = 2e-5;
+constexpr float kBandF = 2e-5f;
+
+struct Band {
+    float lo, hi;
+};
+
+__device__ __forceinline__ Band band_of(double t2)
+{
+    return {(float)(t2 * (1 - kBand)), (float)(t2 * (1 + kBand))};
+}
+
+// FP32 decision of r2 < t2: 1 inside, 0 outside, -1 undecided
+__device__ __forceinline__ int decide(float r2, Band b) { return (r2 < b.lo) ? 1 : ((r2 >= b.hi) ? 0 : -1); }
+
+// squared distance exactly as the FP64 kernels compute it
+__device__ __forceinline__ double exact_r2(const BoxInfo &box, const double4 &pi, const double4 &pj, double dp[3])
+{
+    dp[0] = pi.x - pj.x;
+    dp[1] = pi.y - pj.y;
+    dp[2] = pi.z - pj.z;
+    wrap_dp(box, dp);
+    return dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+}
+
+__device__ __forceinline__ void unpack_cell(uint32_t p, int c[3])
+{
+    c[0] = (int)(p & 1023u);
+    c[1] = (int)((p >> 10) & 1023u);
+    c[2] = (int)((p >> 20) & 1023u);
+}
+
+// nearest image of a cell-index difference (valid while the pair is closer
+// than half the box minus two cells, which the callers guarantee)
+__device__ __forceinline__ int wrap_cell(int d, int nc)
+{
+    const int half = nc >> 1;
+    return (d > half) ? d - nc : ((d < -half) ? d + nc : d);
+}
+
+__global__ void pack_geom_f32_kernel(int n, const double4 *posm, const double4 *velu, Grid g, float4 *offm,
+                                     float4 *velf, uint32_t *cellc)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double4 p = posm[k], v = velu[k];
+    const int cx = cell_coord(p.x, g.inv_s[0], g.nc[0]);
+    const int cy = cell_coord(p.y, g.inv_s[1], g.nc[1]);
+    const int cz = cell_coord(p.z, g.inv_s[2], g.nc[2]);
+    offm[k] = make_float4((float)(p.x - cx * g.s[0]), (float)(p.y - cy * g.s[1]), (float)(p.z - cz * g.s[2]),
+                          (float)p.w);
+    velf[k] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.0f);
+    cellc[k] = (uint32_t)cx | ((uint32_t)cy << 10) | ((uint32_t)cz << 20);
+}
+
+// ---------------------------------------------------------------------------
+// density (density.c:252-347 pair sums in FP32 + 505-850 iteration in FP64)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepArgs a)
+{
+    const int t = xcd_block() * kBlock + threadIdx.x;
+    const bool valid = t < a.n_live;
+    unsigned long long cnt = 0;
+    if (valid) {
+        const int i = a.live[t];
+        const double4 pi = a.posm[i];
+        const float4 oi = a.f32.offm[i];
+        const float4 vi = a.f32.velf[i];
+        int ci[3];
+        unpack_cell(a.f32.cellc[i], ci);
+        double h = a.hsml[i];
+        const double h2 = h * h;
+        const Band bh = band_of(h2);
+        const float hinv = (float)(1.0 / h);
+        const float hinv3 = hinv * hinv * hinv, hinv4 = hinv3 * hinv;
+        float s_ngb = 0, s_rho = 0, s_dh = 0, s_div = 0;
+        float t00 = 0, t01 = 0, t02 = 0, t11 = 0, t12 = 0, t22 = 0, f0 = 0, f1 = 0, f2 = 0;
+        const double xs[3] = {pi.x, pi.y, pi.z};
+        bool full[3];
+        for_each_cell(a.grid, xs, h, full, [&](int cx, int cy, int cz, int beg, int end) {
+            const bool exact_geom = full[0] || full[1] || full[2];
+            const float sh0 = (float)(ci[0] - cx) * a.f32.s[0];
+            const float sh1 = (float)(ci[1] - cy) * a.f32.s[1];
+            const float sh2 = (float)(ci[2] - cz) * a.f32.s[2];
+            for (int k = beg; k < end; ++k) {
+                const float4 oj = a.f32.offm[k];
+                float dp[3];
+                float r2;
+                int in;
+                if (!exact_geom) {
+                    dp[0] = sh0 + (oi.x - oj.x);
+                    dp[1] = sh1 + (oi.y - oj.y);
+                    dp[2] = sh2 + (oi.z - oj.z);
+                    r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                    in = decide(r2, bh);
+                    if (in < 0) {
+                        double d64[3];
+                        in = exact_r2(a.box, pi, a.posm[k], d64) < h2;
+                    }
+                } else {
+                    double d64[3];
+                    const double r2e = exact_r2(a.box, pi, a.posm[k], d64);
+                    in = r2e < h2;
+                    dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+                    r2 = (float)r2e;
+                }
+                if (!in || !(oj.w > 0)) continue;
+                cnt++;
+                const float r = Real<float>::root(r2);
+                const float u = r * hinv;
+                float wk, dwk;
+                kernel_wdw(u, hinv3, hinv4, wk, dwk);
+                s_ngb += wk;
+                s_rho += oj.w * wk;
+                s_dh += -(3 * hinv * wk + u * dwk);
+                if (r > 0) {
+                    t00 += wk * dp[0] * dp[0];
+                    t01 += wk * dp[0] * dp[1];
+                    t02 += wk * dp[0] * dp[2];
+                    t11 += wk * dp[1] * dp[1];
+                    t12 += wk * dp[1] * dp[2];
+                    t22 += wk * dp[2] * dp[2];
+                    f0 += wk * dp[0];
+                    f1 += wk * dp[1];
+                    f2 += wk * dp[2];
+                    const float4 vj = a.f32.velf[k];
+                    const float dv0 = vi.x - vj.x, dv1 = vi.y - vj.y, dv2 = vi.z - vj.z;
+                    s_div -= Real<float>::div(dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2), r);
+                }
+            }
+        });
+        DensitySums S = {s_ngb, s_rho, s_dh, s_div, t00, t01, t02, t11, t12, t22, f0, f1, f2};
+        double left = a.left[i], right = a.right[i];
+        DensityResult R;
+        const int res = hsml_update(a.hp, S, h, left, right, a.cn[i], R);
+        a.hsml[i] = h;
+        a.left[i] = left;
+        a.right[i] = right;
+        a.cn[i] = R.cn_stored;
+        a.num_ngb[i] = R.num_ngb;
+        a.dhsml[i] = R.dhsml;
+        a.divvel[i] = R.divvel;
+        a.rho[i] = R.rho_sum;
+        a.fce[i] = R.fce;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a.nvt[6 * (size_t)i + k] = R.nvt[k];
+        a.redo[t] = (res == 1) ? 1 : 0;
+        if (res < 0) atomicMax(&a.ctr->status, (int)6);
+    }
+    wave_add_u64(&a.ctr->interactions, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// gradients (gradients.c:1477-2089 pair loop in FP32, 1052-1253 finalize in FP64)
+// ---------------------------------------------------------------------------
+__global__ void pack_grad_f32_kernel(int n, const float4 *offm, const float4 *velf, const double *hsml,
+                                     const double *rho, const double *pres, GradRecF *rec)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 o = offm[k], v = velf[k];
+    GradRecF r;
+    r.xh = make_float4(o.x, o.y, o.z, (float)hsml[k]);
+    r.vr = make_float4(v.x, v.y, v.z, (float)rho[k]);
+    r.pm = make_float4((float)pres[k], o.w, 0.0f, 0.0f);
+    rec[k] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void gradients_f32_kernel(GradientArgs a)
+{
+    __shared__ int stage[kListChunk * kBlock];
+    const int t = xcd_block() * kBlock + threadIdx.x;
+    const bool valid = t < a.n_active;
+    unsigned long long cnt = 0;
+    int listed = 0;
+    if (valid) {
+        const int i = a.active[t];
+        const double4 pi = a.posm[i];
+        const GradRecF ri = a.recf[i];
+        int ci[3];
+        unpack_cell(a.f32.cellc[i], ci);
+        const double h_i = a.hsml[i], m_i = pi.w;
+        const double rho_i = a.rho[i], p_i = a.pres[i];
+        float gr0 = 0, gr1 = 0, gr2 = 0, gp0 = 0, gp1 = 0, gp2 = 0;
+        float gv00 = 0, gv01 = 0, gv02 = 0, gv10 = 0, gv11 = 0, gv12 = 0, gv20 = 0, gv21 = 0, gv22 = 0;
+        float maxd = 0, mn_r = 0, mx_r = 0, mn_p = 0, mx_p = 0;
+        float mn_v0 = 0, mx_v0 = 0, mn_v1 = 0, mx_v1 = 0, mn_v2 = 0, mx_v2 = 0;
+        int *list_row = a.list.idx ? a.list.idx + (size_t)t * a.list.cap : nullptr;
+        if ((m_i > 0) && (rho_i > 0) && (h_i > 0)) {  // GasGrad_isactive (gradients.c:66-75)
+            const float hinv = (float)(1.0 / h_i);
+            const float hinv3 = hinv * hinv * hinv;
+            const double h2_i = h_i * h_i;
+            const Band bi = band_of(h2_i);
+            const float pos_min = (float)(h2_i * kBand);  // r2 below this: re-test r2 > 0 in FP64
+            const float rho_f = ri.vr.w, p_f = ri.pm.x;
+            const double xs[3] = {pi.x, pi.y, pi.z};
+            const double R = (h_i > a.hmax) ? h_i : a.hmax;
+            bool full[3];
+            for_each_cell(a.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
+                const bool exact_geom = full[0] || full[1] || full[2];
+                const float sh0 = (float)(ci[0] - cx) * a.f32.s[0];
+                const float sh1 = (float)(ci[1] - cy) * a.f32.s[1];
+                const float sh2 = (float)(ci[2] - cz) * a.f32.s[2];
+                for (int k = beg; k < end; ++k) {
+                    if (k == i) continue;  // r2 = 0 exactly
+                    const float4 xj = a.recf[k].xh;
+                    const float hj2 = xj.w * xj.w;
+                    float dp[3];
+                    float r2;
+                    int in_i, in_j, pos;
+                    if (!exact_geom) {
+                        dp[0] = sh0 + (ri.xh.x - xj.x);
+                        dp[1] = sh1 + (ri.xh.y - xj.y);
+                        dp[2] = sh2 + (ri.xh.z - xj.z);
+                        r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                        if ((r2 >= bi.hi) && (r2 >= hj2 * (1 + kBandF))) continue;  // clearly outside both
+                        in_i = decide(r2, bi);
+                        in_j = decide(r2, Band{hj2 * (1 - kBandF), hj2 * (1 + kBandF)});
+                        pos = (r2 >= pos_min) ? 1 : -1;
+                        if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
+                            double d64[3];
+                            const double r2e = exact_r2(a.box, pi, a.posm[k], d64);
+                            const double h_j = a.hsml[k];
+                            in_i = r2e < h2_i;
+                            in_j = r2e < h_j * h_j;
+                            pos = r2e > 0;
+                        }
+                    } else {
+                        double d64[3];
+                        const double r2e = exact_r2(a.box, pi, a.posm[k], d64);
+                        const double h_j = a.hsml[k];
+                        in_i = r2e < h2_i;
+                        in_j = r2e < h_j * h_j;
+                        pos = r2e > 0;
+                        dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+                        r2 = (float)r2e;
+                    }
+                    if (!pos || (!in_i && !in_j)) continue;
+                    const float4 vr = a.recf[k].vr;
+                    const float4 pm = a.recf[k].pm;
+                    if (!((pm.y > 0) && (vr.w > 0))) continue;
+                    // the list holds the force loop's pair set (m_j > 0, rho_j > 0,
+                    // hydro_evaluate.h:149-150); gradients also need h_j > 0
+                    if (list_row) {
+                        if (listed < a.list.cap) {
+                            const int c = listed & (kListChunk - 1);
+                            stage[c * kBlock + threadIdx.x] = k;
+                            if (c == kListChunk - 1) flush_list_chunk(stage, list_row + (listed - c));
+                        }
+                        listed++;
+                    }
+                    if (!(xj.w > 0)) continue;
+                    cnt++;
+                    const float r = Real<float>::root(r2);
+                    if (r > maxd) maxd = r;
+                    const float d_rho = vr.w - rho_f;
+                    minmax_check(d_rho, mn_r, mx_r);
+                    const float dpp = pm.x - p_f;
+                    minmax_check(dpp, mn_p, mx_p);
+                    const float dv0 = vr.x - ri.vr.x, dv1 = vr.y - ri.vr.y, dv2 = vr.z - ri.vr.z;
+                    minmax_check(dv0, mn_v0, mx_v0);
+                    minmax_check(dv1, mn_v1, mx_v1);
+                    minmax_check(dv2, mn_v2, mx_v2);
+                    if (in_i) {  // r < h_i (the weight vanishes at the boundary)
+                        const float wk_i = kernel_w(r * hinv, hinv3);
+                        const float w0 = -wk_i * dp[0], w1 = -wk_i * dp[1], w2 = -wk_i * dp[2];
+                        gr0 += w0 * d_rho; gr1 += w1 * d_rho; gr2 += w2 * d_rho;
+                        gp0 += w0 * dpp; gp1 += w1 * dpp; gp2 += w2 * dpp;
+                        gv00 += w0 * dv0; gv01 += w1 * dv0; gv02 += w2 * dv0;
+                        gv10 += w0 * dv1; gv11 += w1 * dv1; gv12 += w2 * dv1;
+                        gv20 += w0 * dv2; gv21 += w1 * dv2; gv22 += w2 * dv2;
+                    }
+                }
+            });
+            if (list_row && listed < a.list.cap && (listed & (kListChunk - 1)))
+                flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)));
+        } else if ((m_i > 0) && (rho_i > 0)) {
+            listed = a.list.cap + 1;  // the force loop still runs for i: send its slice to the scan
+        }
+        double nvt[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) nvt[k] = a.nvt[6 * (size_t)i + k];
+        double gr[3] = {gr0, gr1, gr2}, gp[3] = {gp0, gp1, gp2};
+        double gv0[3] = {gv00, gv01, gv02}, gv1[3] = {gv10, gv11, gv12}, gv2[3] = {gv20, gv21, gv22};
+        apply_nvt(nvt, gr);
+        apply_nvt(nvt, gp);
+        apply_nvt(nvt, gv0);
+        apply_nvt(nvt, gv1);
+        apply_nvt(nvt, gv2);
+        // SELFGRAVITY_OFF, no MAGNETIC/GALSF: h_lim = h, stol = 0.1 (gradients.c:1219-1232)
+        const double cn = a.cn[i];
+        const double d_max = (h_i > (double)maxd) ? h_i : (double)maxd;
+        double a_lim = 0.25;
+        if (cn > 100) a_lim = dmin(0.5, 0.25 + 0.25 * (cn - 100) / 100);
+        slope_limit(gr, mx_r, mn_r, a_lim, h_i, 0, 1, d_max, rho_i);
+        slope_limit(gp, mx_p, mn_p, a_lim, h_i, 0.1, 1, d_max, p_i);
+        slope_limit(gv0, mx_v0, mn_v0, a_lim, h_i, 0.1, 0, 0, 0);
+        slope_limit(gv1, mx_v1, mn_v1, a_lim, h_i, 0.1, 0, 0, 0);
+        slope_limit(gv2, mx_v2, mn_v2, a_lim, h_i, 0.1, 0, 0, 0);
+        double *o = a.grad + 15 * (size_t)i;
+        o[0] = gr[0]; o[1] = gr[1]; o[2] = gr[2];
+        o[3] = gp[0]; o[4] = gp[1]; o[5] = gp[2];
+        o[6] = gv0[0]; o[7] = gv0[1]; o[8] = gv0[2];
+        o[9] = gv1[0]; o[10] = gv1[1]; o[11] = gv1[2];
+        o[12] = gv2[0]; o[13] = gv2[1]; o[14] = gv2[2];
+        if (a.list.idx) a.list.count[t] = listed;
+    }
+    if (a.list.idx) {
+        const bool over = __any(listed > a.list.cap);
+        if ((threadIdx.x & 63) == 0 && t < a.n_active) {
+            a.list.overflow[t >> 6] = over ? 1 : 0;
+            if (over) atomicAdd(&a.ctr->overflow_slices, 1);
+        }
+    }
+    wave_add_u64(&a.ctr->interactions, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// force (hydro_evaluate.h:22-516 with the pair math in FP32, FP64 flux sums)
+// ---------------------------------------------------------------------------
+__global__ void pack_force_f32_kernel(int n, const float4 *offm, const float4 *velf, const uint32_t *cellc,
+                                      const double4 *posm, const double *hsml, const double *rho,
+                                      const double *pres, const double *dhsml, const double *cn, const double *fce,
+                                      const double *nvt, const double *grad, double gamma, ForceRecF *rec)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 o = offm[k], v = velf[k];
+    const double d = rho[k], P = pres[k], h = hsml[k], m = posm[k].w;
+    const double *t = nvt + 6 * (size_t)k;
+    const double *g = grad + 15 * (size_t)k;
+    ForceRecF r;
+    r.xh = make_float4(o.x, o.y, o.z, (float)h);
+    r.vc = make_float4(v.x, v.y, v.z, (float)sqrt(gamma * P / d));  // eos.c:245-254
+    r.th = make_float4((float)d, (float)P, (float)(m / d), (float)dhsml[k]);
+    r.t0 = make_float4((float)t[0], (float)t[1], (float)t[2], (float)t[3]);
+    r.t1 = make_float4((float)t[4], (float)t[5], (float)(1.0 / h), (float)fce[k]);
+    r.g0 = make_float4((float)g[0], (float)g[1], (float)g[2], (float)g[3]);
+    r.g1 = make_float4((float)g[4], (float)g[5], (float)g[6], (float)g[7]);
+    r.g2 = make_float4((float)g[8], (float)g[9], (float)g[10], (float)g[11]);
+    r.g3 = make_float4((float)g[12], (float)g[13], (float)g[14], __uint_as_float(cellc[k]));
+    r.cn = cn[k];
+    r.e = P / d;
+    rec[k] = r;
+}
+
+__device__ __forceinline__ void unpack_side_f(const ForceRecF &r, PairSide<float> &S)
+{
+    S.v[0] = r.vc.x; S.v[1] = r.vc.y; S.v[2] = r.vc.z; S.cs = r.vc.w;
+    S.h = r.xh.w; S.hinv = r.t1.z;
+    S.rho = r.th.x; S.P = r.th.y; S.V = r.th.z; S.dhsml = r.th.w;
+    S.cn = r.cn; S.fce = r.t1.w; S.e = r.e;
+    S.nvt[0] = r.t0.x; S.nvt[1] = r.t0.y; S.nvt[2] = r.t0.z; S.nvt[3] = r.t0.w;
+    S.nvt[4] = r.t1.x; S.nvt[5] = r.t1.y;
+    S.gr[0] = r.g0.x; S.gr[1] = r.g0.y; S.gr[2] = r.g0.z;
+    S.gp[0] = r.g0.w; S.gp[1] = r.g1.x; S.gp[2] = r.g1.y;
+    S.gv[0] = r.g1.z; S.gv[1] = r.g1.w; S.gv[2] = r.g2.x;
+    S.gv[3] = r.g2.y; S.gv[4] = r.g2.z; S.gv[5] = r.g2.w;
+    S.gv[6] = r.g3.x; S.gv[7] = r.g3.y; S.gv[8] = r.g3.z;
+}
+
+// USE_LIST: walk the pair list of the FP32 gradient kernel (overflowed slices
+// are left to the scanning variant); otherwise scan the grid with the same
+// banded predicate as the gradient kernel.
+template <bool USE_LIST>
+__global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
+{
+    const int t = xcd_block() * kBlock + threadIdx.x;
+    bool valid = t < a.n_active;
+    if (USE_LIST) {
+        if (valid && a.list.overflow[t >> 6]) valid = false;
+    } else if (a.fallback_only) {
+        if (valid && !a.list.overflow[t >> 6]) valid = false;
+    }
+    unsigned long long cnt = 0, nretry = 0, nexact = 0;
+    if (valid) {
+        const int i = a.active[t];
+        const ForceRecF ri = a.recf[i];
+        const double4 pi = a.posm[i];
+        double acc0 = 0, acc1 = 0, acc2 = 0, dte = 0, vmax = kMinReal;
+        const double m_i = pi.w;
+        int status = 0;
+        if ((m_i > 0) && (ri.th.x > 0)) {
+            PairI<float> I;
+            unpack_side_f(ri, I);
+            I.hinv3 = I.hinv * I.hinv * I.hinv;
+            I.hinv4 = I.hinv3 * I.hinv;
+            I.cnumcrit2 = kCondDanger * kCondDanger - I.cn * I.cn;
+            int ci[3];
+            unpack_cell(__float_as_uint(ri.g3.w), ci);
+            double out_vsig = I.cs;
+            ROut<float> ro = {0, 0, {0, 0, 0}, 0};
+            auto eval = [&](int j, const PairSide<float> &J, const float dp[3], float r2) {
+                cnt++;
+                float Fv[3], Fp, vsig;
+                int mask;
+                const int e = force_pair(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+                if (e > status) status = e;
+                nretry += (mask & 8) ? 1 : 0;
+                nexact += (mask & 4) ? 1 : 0;
+                acc0 += (double)Fv[0];
+                acc1 += (double)Fv[1];
+                acc2 += (double)Fv[2];
+                dte += (double)Fp;
+                if ((double)vsig > out_vsig) out_vsig = vsig;
+                if (a.is_active && !a.is_active[j]) {
+                    if ((double)vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;  // WAKEUP (hydro_evaluate.h:485-495)
+                }
+            };
+            if (USE_LIST) {
+                const int n = a.list.count[t];
+                const int *row = a.list.idx + (size_t)t * a.list.cap;
+                for (int k = 0; k < n; k++) {
+                    const int j = row[k];
+                    const ForceRecF &Jr = a.recf[j];
+                    PairSide<float> J;
+                    unpack_side_f(Jr, J);
+                    const float4 xj = Jr.xh;
+                    float dp[3];
+                    if (!a.exact_geom) {
+                        int cj[3];
+                        unpack_cell(__float_as_uint(Jr.g3.w), cj);
+                        dp[0] = (float)wrap_cell(ci[0] - cj[0], a.grid.nc[0]) * a.f32.s[0] + (ri.xh.x - xj.x);
+                        dp[1] = (float)wrap_cell(ci[1] - cj[1], a.grid.nc[1]) * a.f32.s[1] + (ri.xh.y - xj.y);
+                        dp[2] = (float)wrap_cell(ci[2] - cj[2], a.grid.nc[2]) * a.f32.s[2] + (ri.xh.z - xj.z);
+                    } else {
+                        double d64[3];
+                        exact_r2(a.box, pi, a.posm[j], d64);
+                        dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+                    }
+                    const float r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                    eval(j, J, dp, r2);
+                }
+            } else {
+                const double h_i = a.hsml[i];
+                const double h2_i = h_i * h_i;
+                const Band bi = band_of(h2_i);
+                const float pos_min = (float)(h2_i * kBand);
+                const double xs[3] = {pi.x, pi.y, pi.z};
+                const double R = (h_i > a.hmax) ? h_i : a.hmax;
+                bool full[3];
+                for_each_cell(a.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
+                    const bool exact_geom = full[0] || full[1] || full[2];
+                    const float sh0 = (float)(ci[0] - cx) * a.f32.s[0];
+                    const float sh1 = (float)(ci[1] - cy) * a.f32.s[1];
+                    const float sh2 = (float)(ci[2] - cz) * a.f32.s[2];
+                    for (int j = beg; j < end; ++j) {
+                        if (j == i) continue;
+                        const float4 xj = a.recf[j].xh;
+                        const float hj2 = xj.w * xj.w;
+                        float dp[3];
+                        float r2;
+                        int in_i, in_j, pos;
+                        if (!exact_geom) {
+                            dp[0] = sh0 + (ri.xh.x - xj.x);
+                            dp[1] = sh1 + (ri.xh.y - xj.y);
+                            dp[2] = sh2 + (ri.xh.z - xj.z);
+                            r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                            if ((r2 >= bi.hi) && (r2 >= hj2 * (1 + kBandF))) continue;
+                            in_i = decide(r2, bi);
+                            in_j = decide(r2, Band{hj2 * (1 - kBandF), hj2 * (1 + kBandF)});
+                            pos = (r2 >= pos_min) ? 1 : -1;
+                            if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
+                                double d64[3];
+                                const double r2e = exact_r2(a.box, pi, a.posm[j], d64);
+                                const double h_j = a.hsml[j];
+                                in_i = r2e < h2_i;
+                                in_j = r2e < h_j * h_j;
+                                pos = r2e > 0;
+                            }
+                        } else {
+                            double d64[3];
+                            const double r2e = exact_r2(a.box, pi, a.posm[j], d64);
+                            const double h_j = a.hsml[j];
+                            in_i = r2e < h2_i;
+                            in_j = r2e < h_j * h_j;
+                            pos = r2e > 0;
+                            dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+                            r2 = (float)r2e;
+                        }
+                        if (!pos || (!in_i && !in_j)) continue;
+                        const ForceRecF &Jr = a.recf[j];
+                        if (!(a.posm[j].w > 0) || !(Jr.th.x > 0)) continue;  // hydro_evaluate.h:149-150
+                        PairSide<float> J;
+                        unpack_side_f(Jr, J);
+                        eval(j, J, dp, r2);
+                    }
+                });
+            }
+            if (vmax < out_vsig) vmax = out_vsig;
+            // hydro_final_operations_and_cleanup (hydro_toplevel.c:738-764), FP64
+            const double4 vi = a.velu[i];
+            const double v0 = vi.x, v1 = vi.y, v2 = vi.z;
+            dte -= (v0 / 1.0) * acc0;
+            acc0 /= m_i;
+            dte -= (v1 / 1.0) * acc1;
+            acc1 /= m_i;
+            dte -= (v2 / 1.0) * acc2;
+            acc2 /= m_i;
+            dte /= m_i;
+            if (a.hsml[i] >= 0.99 * a.max_hsml) dte = 0;
+        } else if (m_i > 0) {
+            dte /= m_i;
+        }
+        a.acc[3 * (size_t)i + 0] = acc0;
+        a.acc[3 * (size_t)i + 1] = acc1;
+        a.acc[3 * (size_t)i + 2] = acc2;
+        a.dtu[i] = dte;
+        a.vsig[i] = vmax;
+        if (status) atomicMax(&a.ctr->status, status);
+    }
+    wave_add_u64(&a.ctr->interactions, cnt);
+    wave_add_u64(&a.ctr->retries, nretry);
+    wave_add_u64(&a.ctr->exact_calls, nexact);
+}
+
+inline unsigned nblocks(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const double4 *velu, Grid g, float4 *offm,
+                          float4 *velf, uint32_t *cellc)
+{
+    if (n > 0) hipLaunchKernelGGL(pack_geom_f32_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, posm, velu, g, offm, velf, cellc);
+}
+void launch_density_sweep_f32(hipStream_t s, const DensitySweepArgs &a)
+{
+    if (a.n_live > 0) hipLaunchKernelGGL(density_sweep_f32_kernel, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+}
+void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const double *hsml,
+                          const double *rho, const double *pres, GradRecF *rec)
+{
+    if (n > 0) hipLaunchKernelGGL(pack_grad_f32_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, offm, velf, hsml, rho, pres, rec);
+}
+void launch_gradients_f32(hipStream_t s, const GradientArgs &a)
+{
+    if (a.n_active > 0) hipLaunchKernelGGL(gradients_f32_kernel, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+}
+void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const uint32_t *cellc,
+                           const double4 *posm, const double *hsml, const double *rho, const double *pres,
+                           const double *dhsml, const double *cn, const double *fce, const double *nvt,
+                           const double *grad, double gamma, ForceRecF *rec)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(pack_force_f32_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, offm, velf, cellc, posm, hsml,
+                           rho, pres, dhsml, cn, fce, nvt, grad, gamma, rec);
+}
+void launch_force_f32(hipStream_t s, const ForceArgs &a)
+{
+    if (a.n_active <= 0) return;
+    if (a.list.idx && !a.fallback_only)
+        hipLaunchKernelGGL(force_f32_kernel<true>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(force_f32_kernel<false>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+}
+
+}  // namespace gzm

@@ -1,10 +1,16 @@
-// mfm_device.h -- device-side physics of the MFM hydro pass (FP64), gfx950.
+// mfm_device.h -- device-side physics of the MFM hydro pass, gfx950.
 //
-// Pure per-particle / per-pair functions used by the stage kernels in
-// gzm_kernels.hip.  Each one cites the reference lines whose behaviour it
-// reproduces.  Arithmetic keeps the reference's operation order; the FP64
-// translation unit is compiled with -ffp-contract=off so that products and
-// sums round exactly as in the CPU pass.
+// Pure per-particle / per-pair functions used by the stage kernels.  Each one
+// cites the reference lines whose behaviour it reproduces.  Arithmetic keeps
+// the reference's operation order; the library is compiled with
+// -ffp-contract=off so that products and sums round exactly as in the CPU
+// pass.
+//
+// The per-pair functions are templates over the real type T: T = double is
+// the plain FP64 pass, T = float the FP32-bulk pass (same steps, FP32
+// arithmetic with the hardware reciprocal and square root).  Per-particle
+// steps (hsml iteration, NV_T inversion, slope limiter, exact Riemann solver)
+// are FP64 only.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,6 +29,26 @@ constexpr double kWakeup = 4.1;
 // proto.h:123-124 (NaN behaviour matters: DMAX(NaN,b) = b)
 __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return (a > b) ? a : b; }
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return (a < b) ? a : b; }
+
+// Arithmetic that differs between the two precisions.  FP64: IEEE division
+// and square root (the reference's).  FP32: v_rcp_f32 / v_sqrt_f32 (1 ulp),
+// and MIN_REAL_NUMBER (1e-56, allvars.h:78) replaced by the smallest power of
+// ten that stays a normal float, since 1e-56 underflows to 0 in FP32.
+template <typename T> struct Real;
+template <> struct Real<double> {
+    static constexpr double tiny = kMinReal;
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double rcp(double a) { return 1.0 / a; }
+    static __device__ __forceinline__ double root(double a) { return sqrt(a); }
+};
+template <> struct Real<float> {
+    static constexpr float tiny = 1e-37f;
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float rcp(float a) { return __builtin_amdgcn_rcpf(a); }
+    static __device__ __forceinline__ float root(float a) { return __builtin_amdgcn_sqrtf(a); }
+};
 
 // reimann.h:1-10 for the run's adiabatic index
 struct GammaConsts {
@@ -37,42 +63,45 @@ struct BoxInfo {
 // ---------------------------------------------------------------------------
 // cubic-spline kernel (kernel.h:182-197, 225-242, 373-374)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void kernel_hinv(double h, double &hinv, double &hinv3, double &hinv4)
+template <typename T>
+__device__ __forceinline__ void kernel_hinv(T h, T &hinv, T &hinv3, T &hinv4)
 {
-    hinv = 1.0 / h;
+    hinv = Real<T>::rcp(h);
     hinv3 = hinv * hinv * hinv;
     hinv4 = hinv3 * hinv;
 }
 
-__device__ __forceinline__ void kernel_wdw(double u, double hinv3, double hinv4, double &wk, double &dwk)
+template <typename T>
+__device__ __forceinline__ void kernel_wdw(T u, T hinv3, T hinv4, T &wk, T &dwk)
 {
     if (u >= 1) { wk = 0; dwk = 0; return; }
-    double w, dw;
-    if (u < 0.5) {
-        dw = u * (18.0 * u - 12.0);
-        w = (1.0 + 6.0 * (u - 1.0) * u * u);
+    T w, dw;
+    if (u < T(0.5)) {
+        dw = u * (T(18.0) * u - T(12.0));
+        w = (T(1.0) + T(6.0) * (u - T(1.0)) * u * u);
     } else {
-        double t1 = (1.0 - u);
-        double t2 = t1 * t1;
-        dw = -6.0 * t2;
-        w = 2.0 * t2 * t1;
+        T t1 = (T(1.0) - u);
+        T t2 = t1 * t1;
+        dw = T(-6.0) * t2;
+        w = T(2.0) * t2 * t1;
     }
-    dwk = dw * (kKernelNorm * hinv4);
-    wk = w * (kKernelNorm * hinv3);
+    dwk = dw * (T(kKernelNorm) * hinv4);
+    wk = w * (T(kKernelNorm) * hinv3);
 }
 
-__device__ __forceinline__ double kernel_w(double u, double hinv3)
+template <typename T>
+__device__ __forceinline__ T kernel_w(T u, T hinv3)
 {
     if (u >= 1) return 0;
-    double w;
-    if (u < 0.5) {
-        w = (1.0 + 6.0 * (u - 1.0) * u * u);
+    T w;
+    if (u < T(0.5)) {
+        w = (T(1.0) + T(6.0) * (u - T(1.0)) * u * u);
     } else {
-        double t1 = (1.0 - u);
-        double t2 = t1 * t1;
-        w = 2.0 * t2 * t1;
+        T t1 = (T(1.0) - u);
+        T t2 = t1 * t1;
+        w = T(2.0) * t2 * t1;
     }
-    return w * (kKernelNorm * hinv3);
+    return w * (T(kKernelNorm) * hinv3);
 }
 
 // signed nearest-image wrap of one component (allvars.h:1519, NEAREST_XYZ)
@@ -340,25 +369,25 @@ __device__ __forceinline__ void apply_nvt(const double nvt[6], double g[3])
 // face reconstruction and limiter (reimann.h:129-248), fac_minmax 0.5,
 // fac_meddev 0.375.  R side is particle i, L side is particle j.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void reconstruct(double Qi, const double gi[3], double Qj, const double gj[3],
-                                            const double di[3], const double dj[3], double &QL, double &QR,
-                                            int mode)
+template <typename T>
+__device__ __forceinline__ void reconstruct(T Qi, const T gi[3], T Qj, const T gj[3], const T di[3], const T dj[3],
+                                            T &QL, T &QR, int mode)
 {
     if (mode == 0) { QR = Qi; QL = Qj; return; }
     if (Qi == Qj) { QL = QR = Qi; return; }
     QR = Qi + gi[0] * di[0] + gi[1] * di[1] + gi[2] * di[2];
     QL = Qj + gj[0] * dj[0] + gj[1] * dj[1] + gj[2] * dj[2];
-    double Qmax, Qmin;
-    double Qmed = 0.5 * (Qi + Qj);
+    T Qmax, Qmin;
+    T Qmed = T(0.5) * (Qi + Qj);
     if (Qi < Qj) { Qmax = Qj; Qmin = Qi; } else { Qmax = Qi; Qmin = Qj; }
-    double fac = 0.5 * (Qmax - Qmin);
-    double Qmax_eff = Qmax + fac;
-    double Qmin_eff = Qmin - fac;
-    if (Qmax < 0) { if (Qmax_eff > 0) Qmax_eff = Qmax * Qmax / (Qmax - (Qmax_eff - Qmax)); }
-    if (Qmin > 0) { if (Qmin_eff < 0) Qmin_eff = Qmin * Qmin / (Qmin + (Qmin - Qmin_eff)); }
-    fac = 0.375 * (Qmax - Qmin);
-    double Qmed_max = Qmed + fac;
-    double Qmed_min = Qmed - fac;
+    T fac = T(0.5) * (Qmax - Qmin);
+    T Qmax_eff = Qmax + fac;
+    T Qmin_eff = Qmin - fac;
+    if (Qmax < 0) { if (Qmax_eff > 0) Qmax_eff = Real<T>::div(Qmax * Qmax, (Qmax - (Qmax_eff - Qmax))); }
+    if (Qmin > 0) { if (Qmin_eff < 0) Qmin_eff = Real<T>::div(Qmin * Qmin, (Qmin + (Qmin - Qmin_eff))); }
+    fac = T(0.375) * (Qmax - Qmin);
+    T Qmed_max = Qmed + fac;
+    T Qmed_min = Qmed - fac;
     if (Qmed_max > Qmax_eff) Qmed_max = Qmax_eff;
     if (Qmed_min < Qmin_eff) Qmed_min = Qmin_eff;
     if (Qi < Qj) {
@@ -377,82 +406,85 @@ __device__ __forceinline__ void reconstruct(double Qi, const double gi[3], doubl
 // ---------------------------------------------------------------------------
 // Riemann problem for MFM hydro (reimann.h:286-593, 685-1138).
 // ---------------------------------------------------------------------------
+template <typename T>
 struct RState {
-    double rho, p, v[3];
+    T rho, p, v[3];
 };
 
+template <typename T>
 struct ROut {
-    double P_M, S_M, Fv[3], Fp;
+    T P_M, S_M, Fv[3], Fp;
 };
 
 // get_wavespeeds_and_pressure_star: Gaburov HLLC estimate, Roe, then PVRS
 // (reimann.h:527-593).  Sets P_M and S_M.
-__device__ inline void wavespeeds_pstar(const RState &L, const RState &R, const GammaConsts &G, const double n[3],
-                                        double vL, double vR, double cL, double cR, double hL, double hR,
-                                        double limiter, ROut &o)
+template <typename T>
+__device__ inline void wavespeeds_pstar(const RState<T> &L, const RState<T> &R, const GammaConsts &G, const T n[3],
+                                        T vL, T vR, T cL, T cR, T hL, T hR, T limiter, ROut<T> &o)
 {
-    if ((vR - vL) > dmax(cL, cR)) {
-        o.P_M = kMinReal; o.S_M = 0;
+    using M = Real<T>;
+    if ((vR - vL) > tmax(cL, cR)) {
+        o.P_M = M::tiny; o.S_M = 0;
         return;
     }
-    double PT_L = L.p, PT_R = R.p;
-    double S_L = dmin(vL, vR) - dmax(cL, cR);
-    double S_R = dmax(vL, vR) + dmax(cL, cR);
-    double rwL = L.rho * (S_L - vL);
-    double rwR = R.rho * (S_R - vR);
-    o.S_M = ((PT_R - PT_L) + rwL * vL - rwR * vR) / (rwL - rwR);
-    o.P_M = (PT_L * rwR - PT_R * rwL + rwL * rwR * (vR - vL)) / (rwR - rwL);
-    if (o.P_M <= kMinReal) { o.P_M = kMinReal; o.S_M = 0; }
+    T PT_L = L.p, PT_R = R.p;
+    T S_L = tmin(vL, vR) - tmax(cL, cR);
+    T S_R = tmax(vL, vR) + tmax(cL, cR);
+    T rwL = L.rho * (S_L - vL);
+    T rwR = R.rho * (S_R - vR);
+    o.S_M = M::div(((PT_R - PT_L) + rwL * vL - rwR * vR), (rwL - rwR));
+    o.P_M = M::div((PT_L * rwR - PT_R * rwL + rwL * rwR * (vR - vL)), (rwR - rwL));
+    if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
     if ((o.P_M <= 0) || isnan(o.P_M) || (o.P_M > limiter)) {
-        double sL = sqrt(L.rho), sR = sqrt(R.rho);
-        double sinv = 1 / (sL + sR);
-        double vx = (sL * L.v[0] + sR * R.v[0]) * sinv;
-        double vy = (sL * L.v[1] + sR * R.v[1]) * sinv;
-        double vz = (sL * L.v[2] + sR * R.v[2]) * sinv;
-        double v_roe = vx * n[0] + vy * n[1] + vz * n[2];
-        double h_roe = (sL * hL + sR * hR) * sinv;
-        double c_roe = sqrt(dmax(1.e-30, G.G9 * (h_roe - 0.5 * (vx * vx + vy * vy + vz * vz))));
-        S_R = dmax(vR + cR, v_roe + c_roe);
-        S_L = dmin(vL - cL, v_roe - c_roe);
+        T sL = M::root(L.rho), sR = M::root(R.rho);
+        T sinv = M::rcp(sL + sR);
+        T vx = (sL * L.v[0] + sR * R.v[0]) * sinv;
+        T vy = (sL * L.v[1] + sR * R.v[1]) * sinv;
+        T vz = (sL * L.v[2] + sR * R.v[2]) * sinv;
+        T v_roe = vx * n[0] + vy * n[1] + vz * n[2];
+        T h_roe = (sL * hL + sR * hR) * sinv;
+        T c_roe = M::root(tmax(T(1.e-30), T(G.G9) * (h_roe - T(0.5) * (vx * vx + vy * vy + vz * vz))));
+        S_R = tmax(vR + cR, v_roe + c_roe);
+        S_L = tmin(vL - cL, v_roe - c_roe);
         rwR = R.rho * (S_R - vR);
         rwL = -L.rho * (S_L - vL);
-        o.S_M = ((rwR * vR + rwL * vL) + (PT_L - PT_R)) / (rwR + rwL);
+        o.S_M = M::div(((rwR * vR + rwL * vL) + (PT_L - PT_R)), (rwR + rwL));
         o.P_M = L.rho * (vL - S_L) * (vL - o.S_M) + PT_L;
-        if (o.P_M <= kMinReal) { o.P_M = kMinReal; o.S_M = 0; }
+        if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
         if ((o.P_M <= 0) || isnan(o.P_M) || (o.P_M > limiter)) {
-            o.P_M = 0.5 * ((PT_L + PT_R) + (vL - vR) * 0.25 * (L.rho + R.rho) * (cL + cR));
-            o.S_M = 0.5 * (vR + vL) + 2.0 * (PT_L - PT_R) / ((L.rho + R.rho) * (cL + cR));
-            double S_plus = dmax(dmax(fabs(vL - cL), fabs(vR - cR)), dmax(fabs(vL + cL), fabs(vR + cR)));
+            o.P_M = T(0.5) * ((PT_L + PT_R) + (vL - vR) * T(0.25) * (L.rho + R.rho) * (cL + cR));
+            o.S_M = T(0.5) * (vR + vL) + M::div(T(2.0) * (PT_L - PT_R), ((L.rho + R.rho) * (cL + cR)));
+            T S_plus = tmax(tmax(fabs(vL - cL), fabs(vR - cR)), tmax(fabs(vL + cL), fabs(vR + cR)));
             if (o.S_M < -S_plus) o.S_M = -S_plus;
             if (o.S_M > S_plus) o.S_M = S_plus;
-            if (o.P_M <= kMinReal) { o.P_M = kMinReal; o.S_M = 0; }
+            if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
         }
     }
 }
 
 // Kurganov-Tadmor/PWK flux, MFM branch (reimann.h:484-519).  Reached only on
 // a NaN/negative HLLC pressure, so it lives out of line with scalar arguments.
-__device__ __noinline__ ROut riemann_kt(double rL, double pL, double vLx, double vLy, double vLz, double rR,
-                                        double pR, double vRx, double vRy, double vRz, double n0, double n1,
-                                        double n2, double vL, double vR, double cL, double cR, double hL,
-                                        double hR)
+template <typename T>
+__device__ __noinline__ ROut<T> riemann_kt(T rL, T pL, T vLx, T vLy, T vLz, T rR, T pR, T vRx, T vRy, T vRz, T n0,
+                                           T n1, T n2, T vL, T vR, T cL, T cR, T hL, T hR)
 {
-    ROut o;
-    double thr = 0.001 * 0.5 * (rL + rR) * 0.5 * (cL + cR);
-    double d0 = rR * vRx - rL * vLx, d1 = rR * vRy - rL * vLy, d2 = rR * vRz - rL * vLz;
-    double dv2 = 0;
+    using M = Real<T>;
+    ROut<T> o;
+    T thr = T(0.001) * T(0.5) * (rL + rR) * T(0.5) * (cL + cR);
+    T d0 = rR * vRx - rL * vLx, d1 = rR * vRy - rL * vLy, d2 = rR * vRz - rL * vLz;
+    T dv2 = 0;
     dv2 += d0 * d0;
     dv2 += d1 * d1;
     dv2 += d2 * d2;
-    double dvL = fabs(rR * vR - rL * vL);
-    double alpha = dvL / sqrt(thr * thr + dv2);
-    double S_L = alpha * cL; if (vL > 0) S_L += vL; else S_L -= vL;
-    double S_R = alpha * cR; if (vR > 0) S_R += vR; else S_R -= vR;
-    double S_M = dmax(S_L, S_R);
-    o.S_M = dmax(cL + fabs(vL), cR + fabs(vR));
-    double frl = rL * (vL + S_M);
-    double frr = rR * (vR - S_M);
-    double denom = 1 / (rL * vL - rR * vR + S_M * (rL + rR));
+    T dvL = fabs(rR * vR - rL * vL);
+    T alpha = M::div(dvL, M::root(thr * thr + dv2));
+    T S_L = alpha * cL; if (vL > 0) S_L += vL; else S_L -= vL;
+    T S_R = alpha * cR; if (vR > 0) S_R += vR; else S_R -= vR;
+    T S_M = tmax(S_L, S_R);
+    o.S_M = tmax(cL + fabs(vL), cR + fabs(vR));
+    T frl = rL * (vL + S_M);
+    T frr = rR * (vR - S_M);
+    T denom = M::rcp(rL * vL - rR * vR + S_M * (rL + rR));
     o.P_M = (frl * pR - frr * pL) * denom;
     o.Fv[0] = (vRx - vLx) * frl * frr * denom + (o.P_M) * n0;
     o.Fv[1] = (vRy - vLy) * frl * frr * denom + (o.P_M) * n1;
@@ -555,36 +587,219 @@ __device__ __noinline__ ExactOut riemann_exact(double rL, double pL, double rR, 
 
 // Riemann_solver() for MFM hydro (reimann.h:286-388).  Returns a stage mask:
 // bit0 HLLC, bit1 KT, bit2 exact; err gets 1234545 on the exact solver's
-// negative-density exit.
-__device__ __forceinline__ int riemann_solve(const RState &L, const RState &R, const double n[3], double limiter,
-                                             const GammaConsts &G, ROut &o, int &err)
+// negative-density exit.  The exact solver always runs in FP64.
+template <typename T>
+__device__ __forceinline__ int riemann_solve(const RState<T> &L, const RState<T> &R, const T n[3], T limiter,
+                                             const GammaConsts &G, ROut<T> &o, int &err)
 {
+    using M = Real<T>;
     if ((L.p < 0 && R.p < 0) || (L.rho < 0) || (R.rho < 0)) { o.P_M = 0; return 0; }
-    double cL = sqrt(G.G0 * L.p / L.rho);
-    double cR = sqrt(G.G0 * R.p / R.rho);
-    double uL = L.p / (G.G9 * L.rho);
-    double uR = R.p / (G.G9 * R.rho);
-    double hL = L.p / L.rho + uL + 0.5 * (L.v[0] * L.v[0] + L.v[1] * L.v[1] + L.v[2] * L.v[2]);
-    double hR = R.p / R.rho + uR + 0.5 * (R.v[0] * R.v[0] + R.v[1] * R.v[1] + R.v[2] * R.v[2]);
-    double vL = L.v[0] * n[0] + L.v[1] * n[1] + L.v[2] * n[2];
-    double vR = R.v[0] * n[0] + R.v[1] * n[1] + R.v[2] * n[2];
+    T cL = M::root(M::div(T(G.G0) * L.p, L.rho));
+    T cR = M::root(M::div(T(G.G0) * R.p, R.rho));
+    T uL = M::div(L.p, (T(G.G9) * L.rho));
+    T uR = M::div(R.p, (T(G.G9) * R.rho));
+    T hL = M::div(L.p, L.rho) + uL + T(0.5) * (L.v[0] * L.v[0] + L.v[1] * L.v[1] + L.v[2] * L.v[2]);
+    T hR = M::div(R.p, R.rho) + uR + T(0.5) * (R.v[0] * R.v[0] + R.v[1] * R.v[1] + R.v[2] * R.v[2]);
+    T vL = L.v[0] * n[0] + L.v[1] * n[1] + L.v[2] * n[2];
+    T vR = R.v[0] * n[0] + R.v[1] * n[1] + R.v[2] * n[2];
     int mask = 1;
     wavespeeds_pstar(L, R, G, n, vL, vR, cL, cR, hL, hR, limiter, o);
     for (int k = 0; k < 3; k++) o.Fv[k] = o.P_M * n[k];
     o.Fp = o.P_M * o.S_M;
     if ((o.P_M < 0) || isnan(o.P_M)) {
-        o = riemann_kt(L.rho, L.p, L.v[0], L.v[1], L.v[2], R.rho, R.p, R.v[0], R.v[1], R.v[2], n[0], n[1], n[2], vL,
-                       vR, cL, cR, hL, hR);
+        o = riemann_kt<T>(L.rho, L.p, L.v[0], L.v[1], L.v[2], R.rho, R.p, R.v[0], R.v[1], R.v[2], n[0], n[1], n[2],
+                          vL, vR, cL, cR, hL, hR);
         mask |= 2;
     }
     if ((o.P_M < 0) || isnan(o.P_M) || (o.P_M > limiter)) {
         ExactOut e = riemann_exact(L.rho, L.p, R.rho, R.p, vL, vR, cL, cR, o.P_M, o.S_M, G);
-        o.P_M = e.P_M;
-        o.S_M = e.S_M;
+        o.P_M = (T)e.P_M;
+        o.S_M = (T)e.S_M;
         if (e.status) err = e.status;
         mask |= 4;
     }
     return mask;
+}
+
+// ---------------------------------------------------------------------------
+// One (i, j) interaction of the MFM force loop (hydro_evaluate.h:180-516,
+// compute_finitevol_faces.h:14-106, hydro_core_meshless.h:8-385).
+// ---------------------------------------------------------------------------
+// Quantities of one side of a pair.  `e` = P/rho and `cn` stay FP64 in both
+// precisions: the entropic switch compares e_i with e_j (equal-u gas makes
+// that a tie decided by rounding) and the face fallback compares CN^2 with
+// 1e12, so both decisions are taken exactly as in the FP64 pass.
+template <typename T>
+struct PairSide {
+    T v[3], h, hinv, rho, P, cs, V, dhsml, fce;
+    T nvt[6], gr[3], gp[3], gv[9];
+    double cn, e;
+};
+
+// i-side extras: kernel factors and the condition-number threshold
+template <typename T>
+struct PairI : PairSide<T> {
+    T hinv3, hinv4;
+    double cnumcrit2;  // CONDITION_NUMBER_DANGER^2 - CN_i^2 (hydro_evaluate.h:203-210)
+};
+
+// dp = x_i - x_j (nearest image), r2 = |dp|^2.  Returns the status code
+// (0, 1234 or 1234545); Fv, Fp, vsig are the pair's flux terms for i, mask
+// the stages used (bit0 HLLC, bit1 KT, bit2 exact, bit3 retry).
+template <typename T>
+__device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &J, const T dp[3], T r2,
+                                          const GammaConsts &G, ROut<T> &ro, T Fv[3], T &Fp, T &vsig, int &mask)
+{
+    using M = Real<T>;
+    int status = 0;
+    const T r = M::root(r2);
+    const T rinv = M::rcp(r);
+    const T dv[3] = {I.v[0] - J.v[0], I.v[1] - J.v[1], I.v[2] - J.v[2]};
+    vsig = I.cs + J.cs;
+    const T vdotr2 = dp[0] * dv[0] + dp[1] * dv[1] + dp[2] * dv[2];
+    if (vdotr2 < 0) vsig -= T(1.0) * vdotr2 * rinv;  // MFM: fac_mu = 1 (hydro_evaluate.h:235-245)
+    T wk_i = 0, dwk_i = 0, wk_j = 0, dwk_j = 0;
+    if (r < I.h) kernel_wdw(r * I.hinv, I.hinv3, I.hinv4, wk_i, dwk_i);
+    if (r < J.h) {
+        const T hinv3_j = J.hinv * J.hinv * J.hinv;
+        kernel_wdw(r * J.hinv, hinv3_j, hinv3_j * J.hinv, wk_j, dwk_j);
+    }
+    Fv[0] = Fv[1] = Fv[2] = 0;
+    Fp = 0;
+    mask = 0;
+    const double leak = 0.5 * ((double)I.fce + (double)J.fce);
+    // effective face (compute_finitevol_faces.h:14-106)
+    T wt_i = I.V, wt_j = J.V;
+    if (M::div(M::div(fabs(I.V - J.V), tmin(I.V, J.V)), T(3)) > T(1.25)) {
+        wt_i = wt_j = M::div(I.V * J.V * (wk_i + wk_j), (I.V * wk_i + J.V * wk_j));
+    }
+    T A[3];
+    T Anorm = 0, fdp = 0;
+    {
+        const T ti_d[3] = {I.nvt[0] * dp[0] + I.nvt[1] * dp[1] + I.nvt[2] * dp[2],
+                           I.nvt[1] * dp[0] + I.nvt[3] * dp[1] + I.nvt[4] * dp[2],
+                           I.nvt[2] * dp[0] + I.nvt[4] * dp[1] + I.nvt[5] * dp[2]};
+        const T tj_d[3] = {J.nvt[0] * dp[0] + J.nvt[1] * dp[1] + J.nvt[2] * dp[2],
+                           J.nvt[1] * dp[0] + J.nvt[3] * dp[1] + J.nvt[4] * dp[2],
+                           J.nvt[2] * dp[0] + J.nvt[4] * dp[1] + J.nvt[5] * dp[2]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            A[k] = wk_i * wt_i * ti_d[k] + wk_j * wt_j * tj_d[k];
+            Anorm += A[k] * A[k];
+            fdp += A[k] * dp[k];
+        }
+    }
+    if ((J.cn * J.cn > 1.0e12 + I.cnumcrit2) || (fdp < 0)) {
+        T An = M::div(-(wt_i * I.V * dwk_i + wt_j * J.V * dwk_j), r);
+        A[0] = An * dp[0];
+        A[1] = An * dp[1];
+        A[2] = An * dp[2];
+        Anorm = An * An * r2;
+    }
+    Anorm = M::root(Anorm);
+    if (Anorm == 0) return 0;
+
+    const T n[3] = {M::div(A[0], Anorm), M::div(A[1], Anorm), M::div(A[2], Anorm)};
+    // faces at the midpoint: s_i = -r/2, s_j = +r/2 (hydro_core_meshless.h:70-90)
+    const T s_i = T(0) - T(0.5) * r, s_j = T(0) - (T(-0.5) * r);
+    T di[3], dj[3], vf[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const T e = dp[k] * rinv;
+        dj[k] = e * s_j;
+        di[k] = e * s_i;
+        vf[k] = rinv * (-s_i * J.v[k] + s_j * I.v[k]);
+    }
+    T fvi = 0, fvj = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) { fvi += I.v[k] * n[k]; fvj += J.v[k] * n[k]; }
+    const T fadv = rinv * (-s_i * fvj + s_j * fvi);
+    T v2a = 0;
+    T vdotr2_phys = vdotr2;
+    vdotr2_phys *= M::rcp(r * T(1.0));
+    if (vdotr2_phys < 0) v2a = vdotr2_phys * vdotr2_phys;
+    const T vdotf2 = fvi - fvj;
+    if (vdotf2 < 0) v2a = tmax(v2a, vdotf2 * vdotf2);
+    const int recon = (leak > 1) ? 0 : 1;
+    RState<T> SL, SR;
+    reconstruct(I.rho, I.gr, J.rho, J.gr, di, dj, SL.rho, SR.rho, recon);
+    reconstruct(I.P, I.gp, J.P, J.gp, di, dj, SL.p, SR.p, recon);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        reconstruct(I.v[k], I.gv + 3 * k, J.v[k], J.gv + 3 * k, di, dj, SL.v[k], SR.v[k], recon);
+        SL.v[k] -= vf[k];
+        SR.v[k] -= vf[k];
+    }
+    const T pti = I.P + I.rho * v2a;
+    const T ptj = J.P + J.rho * v2a;
+    T lim = T(1.1) * T(1.0) * tmax(pti, ptj);
+    if (recon == 0) lim = tmax(lim, tmax(tmax(I.P, J.P), T(2.) * tmax(I.rho, J.rho) * v2a));
+    // Riemann problem with the reference's two retries (hydro_core_meshless.h:209-265):
+    // attempt 0 reconstructed states, 1 first-order states, 2 first-order with
+    // zero velocities; one solver instance in the code
+    int err = 0;
+    T lim_a = lim;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        mask |= riemann_solve(SL, SR, n, lim_a, G, ro, err);
+        const bool bad = (ro.P_M < 0) || isnan(ro.P_M);
+        if (attempt == 0) {
+            if (!(bad || (ro.P_M > T(1.4) * lim))) break;
+            mask |= 8;
+            lim_a = T(1.4) * lim;
+        } else if (attempt == 1) {
+            if (!bad) break;
+            lim_a = T(2.0) * lim;
+        } else {
+            if (bad) status = 1234;
+            break;
+        }
+        SR.p = I.P; SL.p = J.P;
+        SR.rho = I.rho; SL.rho = J.rho;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            SR.v[k] = (attempt == 0) ? I.v[k] - vf[k] : T(0.0);
+            SL.v[k] = (attempt == 0) ? J.v[k] - vf[k] : T(0.0);
+        }
+    }
+    if (err) status = err;
+    if ((ro.P_M > 0) && !isnan(ro.P_M)) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) ro.Fp += vf[k] * ro.Fv[k];
+        Fp = Anorm * ro.Fp;
+#pragma unroll
+        for (int k = 0; k < 3; k++) Fv[k] = Anorm * ro.Fv[k];
+        // MFM adiabatic face correction (hydro_core_meshless.h:335-374)
+        const T sm_c = M::div(fabs(ro.S_M), tmin(I.cs, J.cs));
+        if ((sm_c < T(0.5)) || (leak > 1)) {
+            int use = 1;
+            const T fpm = ro.P_M * Anorm;
+            const T pdv_fac = ro.P_M * vdotr2_phys / T(1.0);
+            const T pdv_i = dwk_i * I.V * I.V * I.dhsml * pdv_fac;
+            const T pdv_j = dwk_j * J.V * J.V * J.dhsml * pdv_fac;
+            const T du_old = fpm * (ro.S_M + fadv);
+            const T du_new = T(0.5) * (pdv_i - pdv_j + fpm * (fvi + fvj));
+            const double cnum2 = J.cn * J.cn;
+            if (sm_c > T(1.e-3) && cnum2 < I.cnumcrit2) {
+                if (I.e != J.e) {
+                    if (I.e > J.e) {
+                        const T dtoj = -du_old + fpm * fvj;
+                        if (dtoj > 0) use = 0;
+                        else if (dtoj < 0) { if (dtoj > -du_new + fpm * fvj) use = 0; }
+                    } else {
+                        const T dtoi = +du_old - fpm * fvi;
+                        if (dtoi > 0) use = 0;
+                        else if (dtoi < 0) { if (dtoi > +du_new - fpm * fvi) use = 0; }
+                    }
+                }
+            }
+            if (cnum2 >= I.cnumcrit2) use = 1;
+            if (use) Fp += du_new - du_old;
+        }
+    } else {
+        Fp = 0;
+        Fv[0] = Fv[1] = Fv[2] = 0;
+    }
+    return status;
 }
 
 }  // namespace gzm

@@ -66,8 +66,10 @@ typedef struct gzm_params {
     double cell_size;       /* grid cell side; <= 0 picks one from the initial hsml */
     int32_t pair_list_cap;  /* neighbours kept per particle in the pair list that the
                                gradient loop hands to the force loop: 0 = default (64),
-                               < 0 = no list (the force loop scans the grid); particles
-                               with more pairs fall back to the scan */
+                               rounded up to a multiple of 16, at most 4096; < 0 = no
+                               list (the force loop scans the grid); the 64-particle
+                               slices holding a particle with more pairs fall back to
+                               the scan */
     int32_t reserved;
 } gzm_params;
 

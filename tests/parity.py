@@ -13,21 +13,25 @@ import numpy as np
 TOL_DENSITY = 1e-12   # density, hsml, NumNgb, pressure (relative)
 TOL_FLUX = 1e-10      # gradients, dv/dt, du/dt (relative to sum|terms|)
 
+# tolerances of the FP32-bulk pass (precision 1) against the FP64 oracle
+TOL32_DENSITY = 1e-5  # density, hsml, pressure (relative)
+TOL32_FLUX = 1e-4     # dv/dt, du/dt (relative to sum|terms|)
 
-def rel_err(a, b):
+
+def rel_errs(a, b):
+    """element-wise |a - b| / |b| (a zero reference must match exactly)"""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     den = np.abs(b)
     bad = den == 0
     err = np.abs(a - b) / np.where(bad, 1.0, den)
-    err = np.where(bad, np.abs(a - b) * 1e300, err)  # zero reference must match exactly
-    return float(err.max()) if err.size else 0.0
+    return np.where(bad, np.abs(a - b) * 1e300, err)
 
 
-def l1_err(a, b, l1):
-    """max |a - b| / sum|terms|; the scale is floored at 1e-3 of the field's
-    median sum|terms| so that particles whose terms all vanish (a static cold
-    gas) do not turn denormal-level differences into huge ratios."""
+def l1_errs(a, b, l1):
+    """element-wise |a - b| / sum|terms|; the scale is floored at 1e-3 of the
+    field's median sum|terms| so that particles whose terms all vanish (a
+    static cold gas) do not turn denormal-level differences into huge ratios."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     l1 = np.asarray(l1, dtype=np.float64).reshape(a.shape)
@@ -35,8 +39,17 @@ def l1_err(a, b, l1):
     floor = 1e-3 * float(np.median(pos)) if pos.size else 0.0
     den = np.maximum(l1, floor)
     err = np.abs(a - b) / np.where(den > 0, den, 1.0)
-    err = np.where(den > 0, err, np.abs(a - b) * 1e300)  # all-zero field must match exactly
-    return float(err.max()) if err.size else 0.0
+    return np.where(den > 0, err, np.abs(a - b) * 1e300)  # all-zero field must match exactly
+
+
+def rel_err(a, b):
+    e = rel_errs(a, b)
+    return float(e.max()) if e.size else 0.0
+
+
+def l1_err(a, b, l1):
+    e = l1_errs(a, b, l1)
+    return float(e.max()) if e.size else 0.0
 
 
 def compare_pass(out, ref, active=None):
@@ -72,3 +85,30 @@ def check_pass(out, ref, active=None, tol_density=TOL_DENSITY, tol_flux=TOL_FLUX
     limits["n_tied"] = float("inf")
     bad = {k: (v, limits[k]) for k, v in e.items() if not (v <= limits[k])}
     return e, bad
+
+
+def error_stats(out, ref, active=None):
+    """Max and rms of the per-particle error of each output against the
+    oracle, with the normalisations of compare_pass; du/dt split into the
+    particles whose entropic-switch branch is not / is a P/rho tie."""
+    idx = slice(None) if active is None else np.asarray(active)
+    sel = lambda d, k: np.asarray(d[k])[idx]
+    errs = {}
+    for k in ("hsml", "density", "pressure", "num_ngb", "vsig"):
+        errs[k] = rel_errs(sel(out, k), sel(ref, k))
+    g_out = np.concatenate([sel(out, "grad_rho"), sel(out, "grad_p"), sel(out, "grad_v").reshape(-1, 9)], axis=1)
+    g_ref = np.concatenate([sel(ref, "grad_rho"), sel(ref, "grad_p"), sel(ref, "grad_v").reshape(-1, 9)], axis=1)
+    errs["grad_rho"] = l1_errs(g_out[:, 0:3], g_ref[:, 0:3], sel(ref, "grad_l1")[:, 0:3])
+    errs["grad_p"] = l1_errs(g_out[:, 3:6], g_ref[:, 3:6], sel(ref, "grad_l1")[:, 3:6])
+    errs["grad_v"] = l1_errs(g_out[:, 6:], g_ref[:, 6:], sel(ref, "grad_l1")[:, 6:])
+    errs["acc"] = l1_errs(sel(out, "acc"), sel(ref, "acc"), sel(ref, "acc_l1"))
+    tie = sel(ref, "tie_flag").astype(bool) if "tie_flag" in ref else np.zeros(len(sel(ref, "dtu")), bool)
+    errs["dtu"] = l1_errs(sel(out, "dtu")[~tie], sel(ref, "dtu")[~tie], sel(ref, "dtu_l1")[~tie])
+    errs["dtu_tied"] = l1_errs(sel(out, "dtu")[tie], sel(ref, "dtu")[tie], sel(ref, "dtu_l1")[tie])
+    stats = {}
+    for k, e in errs.items():
+        e = np.asarray(e, dtype=np.float64).ravel()
+        stats[k] = {"max": float(e.max()) if e.size else 0.0,
+                    "rms": float(np.sqrt(np.mean(e * e))) if e.size else 0.0}
+    stats["n_tied"] = int(tie.sum())
+    return stats

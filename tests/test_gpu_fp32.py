@@ -1,0 +1,94 @@
+"""FP32-bulk pass (precision 1) of libgzm.so against the FP64 oracle.
+
+The mode's contract: neighbour sets identical to the FP64 pass for the same
+hsml (the FP32 r^2 decides a pair only outside a band around the threshold,
+inside it the FP64 expression does); density, hsml and pressure within 1e-5
+relative, dv/dt and du/dt within 1e-4 of the sum of |pair terms| (the
+normalisation of tests/parity.py).  The max and rms error of every output is
+printed.
+"""
+import numpy as np
+import pytest
+
+import gzm
+import ics
+import oracle as O
+from parity import TOL32_DENSITY, TOL32_FLUX, error_stats
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS32 = {
+    "lattice_gaussian_16": lambda: ics.lattice_gaussian(16),
+    "lattice_gaussian_32": lambda: ics.lattice_gaussian(32),
+    "blast_32": lambda: ics.blast(32),
+    "random_state_20": lambda: ics.random_state(20),
+}
+
+STATE_KEYS = ("hsml", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "cn", "fce",
+              "grad_rho", "grad_p", "grad_v", "acc", "dtu", "vsig")
+
+
+def fp32_params(ic, **kw):
+    return gzm.make_params(box=tuple(ic["box"]), precision=1, **kw)
+
+
+@pytest.mark.parametrize("name", list(CONFIGS32))
+def test_fp32_pass_error_bounds(name):
+    ic = CONFIGS32[name]()
+    ref, st, _ = O.run_pass(ic)
+    out, gst = gzm.run_pass(ic, params=fp32_params(ic))
+    stats = error_stats(out, ref)
+    print(name, "sweeps", gst["density_sweeps"], "vs", st.iterations,
+          {k: (v if isinstance(v, int) else f"max {v['max']:.2e} rms {v['rms']:.2e}") for k, v in stats.items()})
+    for k in ("density", "hsml", "pressure"):
+        assert stats[k]["max"] <= TOL32_DENSITY, (k, stats[k])
+    for k in ("acc", "dtu"):
+        assert stats[k]["max"] <= TOL32_FLUX, (k, stats[k])
+    # conserved mass: F_rho = 0 identically; momentum sums to rounding of the FP32 pair terms
+    m = ic["mass"]
+    p = (m[:, None] * out["acc"]).sum(0)
+    assert np.abs(p).max() <= 1e-5 * np.abs(m[:, None] * out["acc"]).sum()
+
+
+@pytest.mark.parametrize("name", list(CONFIGS32))
+def test_fp32_neighbor_sets_match_fp64_for_same_hsml(name):
+    ic = CONFIGS32[name]()
+    ref, st, _ = O.run_pass(ic)
+    state = dict(ic)
+    for k in STATE_KEYS:
+        state[k] = ref[k]
+    out, gst = gzm.run_pass(state, params=fp32_params(ic), stages=("gradients", "force"))
+    assert gst["gradient_interactions"] == st.grad_interactions
+    assert gst["force_interactions"] == st.force_interactions
+    # density sweep from the converged hsml: a particle accepted at once sees
+    # exactly the set {j : r_ij < h_i} of the FP64 predicate
+    _, gd = gzm.run_pass(state, params=fp32_params(ic), stages=("density",))
+    print(name, "density sweeps from converged hsml:", gd["density_sweeps"])
+    if gd["density_sweeps"] == 1:
+        counts, total = O.neighbor_counts(ic["pos"], ref["hsml"], 0)
+        assert gd["density_interactions"] == total
+
+
+@pytest.mark.parametrize("cap", [-1, 16, 32])
+def test_fp32_pair_list_fallbacks_give_identical_results(cap):
+    ic = ics.random_state(14)
+    base, bst = gzm.run_pass(ic, params=fp32_params(ic))
+    out, st = gzm.run_pass(ic, params=fp32_params(ic, pair_list_cap=cap))
+    assert st["force_interactions"] == bst["force_interactions"]
+    for k in ("acc", "dtu", "vsig", "grad_v"):
+        np.testing.assert_array_equal(out[k], base[k])
+
+
+def test_fp32_repeated_pass_is_bit_identical():
+    ic = ics.blast(16)
+    with gzm.MFMContext(fp32_params(ic)) as ctx:
+        ctx.upload(ic)
+        outs = []
+        for _ in range(2):
+            ctx.restore_initial_state()
+            ctx.density()
+            ctx.gradients()
+            ctx.hydro_force()
+            outs.append(ctx.download())
+    for k in ("hsml", "density", "grad_p", "acc", "dtu", "vsig"):
+        np.testing.assert_array_equal(outs[1][k], outs[0][k])

@@ -66,9 +66,10 @@ def test_pass_matches_oracle(name):
 
 @pytest.mark.parametrize("cap", [-1, 4, 24])
 def test_pair_list_fallbacks_give_identical_results(cap):
-    """The force loop reads the gradient loop's pair list; capacity 4 and 24
-    send all / some slices to the scanning fallback, -1 disables the list.
-    Every variant must equal the default bit for bit (same summation order)."""
+    """The force loop reads the gradient loop's pair list; capacities 4 and 24
+    (rounded up to 16 and 32) send almost all / some slices to the scanning
+    fallback, -1 disables the list.  Every variant must equal the default bit
+    for bit (same summation order)."""
     ic = ics.random_state(14)
     base, bst = gzm.run_pass(ic)
     out, st = gzm.run_pass(ic, params=gzm.make_params(pair_list_cap=cap))
