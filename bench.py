@@ -12,9 +12,11 @@ Interactions per pass I = sum over density sweeps of |{j : r_ij < h_i}| +
 kernels (equal to the CPU oracle's count: the neighbour sets are bit-exact).
 
 Prints one JSON line.  Also reported: the roofline of the dominant kernel
-(algorithmic bytes / its mean HIP-event duration vs 8 TB/s) and the FP64 CPU
-oracle (OpenMP, this host's cores) timed on the same workload, whose outputs
-double as a parity check of the timed pass.
+(algorithmic bytes / its mean HIP-event duration vs 8 TB/s, and the HBM bytes
+the committed counter summary measured for it), the FP32-bulk mode (precision
+1) on the same workload with the max / rms error of each output against the
+FP64 oracle, and the FP64 CPU oracle (OpenMP, this host's cores) timed on the
+same workload, whose outputs double as the parity check of the timed passes.
 """
 from __future__ import annotations
 
@@ -39,6 +41,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 #   force:         Pos, v, h, m, rho, P, CN, FCE, Dhsml, NV_T(6), gradients(15) read (272),
 #                  dv/dt(3), du/dt, v_sig written (40)
 BYTES_PER_PARTICLE = {"density": 64 + 136, "gradients": 128 + 120, "force": 272 + 40}
+# FP32-bulk mode, stored widths: density sweep reads offsets+m (16), velocity (16),
+# hsml (8) and writes the same FP64 outputs (136); gradients read the 48-byte
+# FP32 record of j plus NV_T(6) of i (48) and write 15 FP64 gradients (120);
+# force reads the 160-byte FP32 record and writes dv/dt, du/dt, v_sig (40)
+BYTES_PER_PARTICLE_F32 = {"density": 40 + 136, "gradients": 96 + 120, "force": 160 + 40}
 
 
 def parse():
@@ -49,6 +56,7 @@ def parse():
     ap.add_argument("--n", type=int, default=128, help="lattice side (default 128 -> 2,097,152 particles)")
     ap.add_argument("--cpu-repeats", type=int, default=3, help="oracle passes for the CPU baseline (0 = skip)")
     ap.add_argument("--cell-size", type=float, default=0.0)
+    ap.add_argument("--fp32", type=int, default=1, help="also time the FP32-bulk mode (precision 1) on the workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="instead of the pass, copy a buffer of this many MiB three times and exit: the "
                          "known-size reference that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -80,6 +88,81 @@ def cpu_cores():
     return max(1, n)
 
 
+def run_mode(gzm, ic, args, precision):
+    """Time `args.steps` passes (after `args.warmup`) of one precision mode on
+    `ic`, resident on the GPU.  Returns the measurements and the last outputs."""
+    N = ic["pos"].shape[0]
+    params = gzm.make_params(box=tuple(ic["box"]), cell_size=args.cell_size, precision=precision)
+    ctx = gzm.MFMContext(params)
+    ctx.upload(ic)
+
+    def one_pass():
+        ctx.restore_initial_state()
+        s1 = ctx.density()
+        s2 = ctx.gradients()
+        s3 = ctx.hydro_force()
+        return s1, s2, s3
+
+    for _ in range(args.warmup):
+        one_pass()
+    ctx.timing_reset()
+    interactions, sweeps, updates = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):  # each stage call returns after its kernels finished
+        s1, s2, s3 = one_pass()
+        interactions.append(s1.density_interactions + s2.gradient_interactions + s3.force_interactions)
+        sweeps.append(s1.density_sweeps)
+        updates.append(s1.density_updates)
+    elapsed = time.perf_counter() - t0
+    tm = ctx.timing()
+    out = ctx.download()
+    ctx.close()
+    # roofline of the dominant kernel (largest total device time)
+    kern = {
+        "density": (tm["density_ms"], tm["density_launches"], float(np.mean(updates))),
+        "gradients": (tm["gradients_ms"], tm["gradients_launches"], float(N)),
+        "force": (tm["force_ms"], tm["force_launches"], float(N)),
+    }
+    dom = max(kern, key=lambda k: kern[k][0])
+    tot_ms, launches, units = kern[dom]
+    mean_ms = tot_ms / max(launches, 1)
+    bpp = BYTES_PER_PARTICLE if precision == 0 else BYTES_PER_PARTICLE_F32
+    bytes_per_launch = bpp[dom] * units / max(launches / args.steps, 1)
+    return {
+        "value": float(np.sum(interactions)) / elapsed,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "interactions_per_pass": int(interactions[-1]),
+        "density_sweeps": int(sweeps[-1]),
+        "kernels_ms_per_pass": {
+            "density_sweeps": tm["density_ms"] / args.steps,
+            "gradients": tm["gradients_ms"] / args.steps,
+            "force": tm["force_ms"] / args.steps,
+            "other": tm["other_ms"] / args.steps,
+        },
+        "dominant": dom,
+        "mean_launch_ms": mean_ms,
+        "bytes_per_launch": bytes_per_launch,
+        "achieved_gbs": bytes_per_launch / (mean_ms * 1e-3) / 1e9,
+        "out": out,
+    }
+
+
+def measured_traffic(kernel, precision, n):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known
+    size), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_blast128.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = tab.get("kernels", {}).get(f"{kernel}:{'fp64' if precision == 0 else 'fp32'}")
+    if rec is None or tab.get("n_particles") != n:
+        return None
+    return rec["fetch_bytes"] + rec["write_bytes"]
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -96,60 +179,18 @@ def main():
 
     ic = ics.blast(args.n)
     N = ic["pos"].shape[0]
-    params = gzm.make_params(box=tuple(ic["box"]), cell_size=args.cell_size)
-    ctx = gzm.MFMContext(params)
-    ctx.upload(ic)
+    r64 = run_mode(gzm, ic, args, 0)
+    r32 = run_mode(gzm, ic, args, 1) if args.fp32 else None
 
-    def one_pass():
-        ctx.restore_initial_state()
-        s1 = ctx.density()
-        s2 = ctx.gradients()
-        s3 = ctx.hydro_force()
-        return s1, s2, s3
-
-    for _ in range(args.warmup):
-        one_pass()
-    ctx.timing_reset()
-    interactions = []
-    sweeps = []
-    updates = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):  # each stage call returns after its kernels finished
-        s1, s2, s3 = one_pass()
-        interactions.append(s1.density_interactions + s2.gradient_interactions + s3.force_interactions)
-        sweeps.append(s1.density_sweeps)
-        updates.append(s1.density_updates)
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    tm = ctx.timing()
-    out = ctx.download()
-    last = (s1, s2, s3)
-    ctx.close()
-
-    ms_per_step = 1e3 * elapsed / args.steps
-    i_pass = interactions[-1]
-    value = float(np.sum(interactions)) / elapsed
-
-    # roofline of the dominant kernel (largest total device time)
-    kern = {
-        "density": (tm["density_ms"], tm["density_launches"], float(np.mean(updates))),
-        "gradients": (tm["gradients_ms"], tm["gradients_launches"], float(N)),
-        "force": (tm["force_ms"], tm["force_launches"], float(N)),
-    }
-    dom = max(kern, key=lambda k: kern[k][0])
-    tot_ms, launches, units = kern[dom]
-    mean_ms = tot_ms / max(launches, 1)
-    bytes_per_launch = BYTES_PER_PARTICLE[dom] * units / max(launches / args.steps, 1)
-    achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
-
+    traffic = measured_traffic(r64["dominant"], 0, N)
     result = {
         "metric": "particle-neighbor interactions/s, MFM pass; FP32-batched vs FP64 at stated error",
-        "value": value,
+        "value": r64["value"],
         "unit": "interactions/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": r64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -161,35 +202,43 @@ def main():
             "active": "all",
             "gamma": 5.0 / 3.0,
             "des_num_ngb": 32,
-            "interactions_per_pass": int(i_pass),
-            "density_sweeps": int(sweeps[-1]),
+            "interactions_per_pass": r64["interactions_per_pass"],
+            "density_sweeps": r64["density_sweeps"],
             "parallelism": "single GPU",
         },
-        "kernels_ms_per_pass": {
-            "density_sweeps": tm["density_ms"] / args.steps,
-            "gradients": tm["gradients_ms"] / args.steps,
-            "force": tm["force_ms"] / args.steps,
-            "other": tm["other_ms"] / args.steps,
-        },
+        "kernels_ms_per_pass": r64["kernels_ms_per_pass"],
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
-            "achieved": achieved,
+            "kernel": r64["dominant"],
+            "achieved": r64["achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_launch": bytes_per_launch,
-            "mean_launch_ms": mean_ms,
+            "frac": r64["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": r64["bytes_per_launch"],
+            "mean_launch_ms": r64["mean_launch_ms"],
         },
     }
+    if r32 is not None:
+        result["fp32_bulk"] = {
+            "value": r32["value"],
+            "unit": "interactions/s",
+            "ms_per_step": r32["ms_per_step"],
+            "speedup_vs_fp64": r32["value"] / r64["value"],
+            "interactions_per_pass": r32["interactions_per_pass"],
+            "density_sweeps": r32["density_sweeps"],
+            "kernels_ms_per_pass": r32["kernels_ms_per_pass"],
+            "roofline": {"kernel": r32["dominant"], "achieved": r32["achieved_gbs"], "unit": "GB/s",
+                         "frac": r32["achieved_gbs"] / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(r32["dominant"], 1, N)},
+        }
 
     # CPU baseline: the FP64 oracle (OpenMP) on this host, same workload
     if args.cpu_repeats > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle as O
-        from parity import compare_pass
+        from parity import compare_pass, error_stats
 
         cores = cpu_cores()
         prm = O.make_params(box=tuple(ic["box"]), nthreads=cores)
@@ -209,9 +258,14 @@ def main():
             "sample": f"full {args.n}^3 blast pass, median of {args.cpu_repeats} (s: "
                       + ", ".join(f"{x:.2f}" for x in times) + ")",
         }
-        errs = compare_pass(out, ref)
+        errs = compare_pass(r64["out"], ref)
         result["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
-        result["parity_vs_oracle"]["interactions_equal"] = bool(cpu_int == i_pass)
+        result["parity_vs_oracle"]["interactions_equal"] = bool(cpu_int == r64["interactions_per_pass"])
+        if r32 is not None:
+            es = error_stats(r32["out"], ref)
+            result["fp32_bulk"]["error_vs_fp64_oracle"] = {
+                k: (v if isinstance(v, int) else {"max": float(f"{v['max']:.3e}"), "rms": float(f"{v['rms']:.3e}")})
+                for k, v in es.items()}
     print(json.dumps(result))
 
 
