@@ -116,6 +116,26 @@ __device__ __forceinline__ void wrap_dp(const BoxInfo &box, double dp[3])
     dp[2] = wrap_axis(dp[2], box.L[2], box.half[2]);
 }
 
+// Run f(desc) for this lane's particle with its box descriptor, one
+// wave-uniform box at a time: the box index is taken from the first lane still
+// to run (readfirstlane), so the descriptor's fields are uniform and a
+// wavefront that straddles two boxes runs the body twice with the lanes of
+// each box.
+template <typename F>
+__device__ __forceinline__ void for_box(const BoxDesc *__restrict__ boxes, const int *box_of, int i, bool valid,
+                                        F &&f)
+{
+    bool todo = valid;
+    const int my_b = valid ? box_of[i] : 0;
+    while (todo) {
+        const int b = __builtin_amdgcn_readfirstlane(my_b);
+        if (my_b == b) {
+            f(boxes[b]);
+            todo = false;
+        }
+    }
+}
+
 // wavefront-wide sum, then one atomic per wavefront
 __device__ __forceinline__ void wave_add_u64(unsigned long long *dst, unsigned long long v)
 {

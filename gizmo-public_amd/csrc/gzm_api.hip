@@ -50,21 +50,29 @@ struct TimedLaunch {
 
 }  // namespace
 
+// counter blocks, one per stage (a captured multi-pass graph accumulates each)
+enum CounterSlot { kCtrDensity = 0, kCtrGradients = 1, kCtrForce = 2, kCtrMisc = 3, kCtrSlots = 4 };
+
 struct gzm_ctx {
     gzm_params p;
     hipStream_t stream = nullptr;
     int n = 0;
-    int ncell = 0;
-    Grid grid;
-    BoxInfo box;
+    int ncell = 0;            // cells of all boxes
     GammaConsts G;
-    double hmax = 0;
+    // boxes: one for gzm_upload, many for gzm_upload_batch
+    int n_boxes = 0;
+    std::vector<BoxDesc> boxes_h;
+    BoxDesc *d_boxes = nullptr;
+    int *d_box_of = nullptr;  // box of each sorted slot
+    int2 *d_hmax_tiles = nullptr;
+    int n_hmax_tiles = 0;
+    bool capturing = false;   // enqueueing into a graph: no host synchronisation, no timing events
 
     // caller-order staging
     double *d_stage = nullptr;
     size_t stage_doubles = 0;
     // sorting
-    uint32_t *d_keys = nullptr, *d_keys_sorted = nullptr;
+    uint64_t *d_keys = nullptr, *d_keys_sorted = nullptr;
     int *d_iota = nullptr, *d_perm = nullptr, *d_inv = nullptr;
     void *d_temp = nullptr;
     size_t temp_bytes = 0;
@@ -145,12 +153,15 @@ struct Timed {
     Timed(gzm_ctx *ctx, int kind) : c(ctx)
     {
         t.kind = kind;
+        t.a = t.b = nullptr;
+        if (c->capturing) return;  // a captured graph is timed as a whole
         t.a = take_event(c);
         t.b = take_event(c);
         if (t.a) (void)hipEventRecord(t.a, c->stream);
     }
     ~Timed()
     {
+        if (c->capturing) return;
         if (t.b) (void)hipEventRecord(t.b, c->stream);
         c->pending.push_back(t);
     }
@@ -198,7 +209,7 @@ gzm_status ensure_stage(gzm_ctx *c, size_t doubles)
 void free_particles(gzm_ctx *c)
 {
     dev_free(c->d_keys); dev_free(c->d_keys_sorted); dev_free(c->d_iota); dev_free(c->d_perm); dev_free(c->d_inv);
-    dev_free(c->d_range); dev_free(c->d_posm); dev_free(c->d_velu);
+    dev_free(c->d_posm); dev_free(c->d_velu);
     dev_free(c->d_hsml); dev_free(c->d_hsml0); dev_free(c->d_left); dev_free(c->d_right);
     dev_free(c->d_cn); dev_free(c->d_cn0); dev_free(c->d_vsig); dev_free(c->d_vsig0);
     dev_free(c->d_num_ngb); dev_free(c->d_dhsml); dev_free(c->d_divvel); dev_free(c->d_rho); dev_free(c->d_pres);
@@ -208,9 +219,21 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_flags); dev_free(c->d_grec); dev_free(c->d_frec);
     dev_free(c->d_offm); dev_free(c->d_velf); dev_free(c->d_cellc); dev_free(c->d_grecf); dev_free(c->d_frecf);
     dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
+    dev_free(c->d_box_of);
     c->list_entries = 0;
     c->list_valid = false;
     c->n = 0;
+}
+
+void free_boxes(gzm_ctx *c)
+{
+    dev_free(c->d_boxes);
+    dev_free(c->d_hmax_tiles);
+    dev_free(c->d_range);
+    c->n_hmax_tiles = 0;
+    c->n_boxes = 0;
+    c->ncell = 0;
+    c->boxes_h.clear();
 }
 
 // host array (caller order, `width` doubles per particle) -> sorted device array
@@ -238,17 +261,16 @@ gzm_status download_sorted(gzm_ctx *c, const double *src, double *host, int widt
     return GZM_OK;
 }
 
-gzm_status reset_counters(gzm_ctx *c)
+gzm_status reset_counters(gzm_ctx *c, int slot)
 {
-    Counters z;
-    memset(&z, 0, sizeof(z));
-    GZM_HIP(hipMemcpyAsync(c->d_ctr, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+    launch_clear_counters(c->stream, c->d_ctr + slot);
+    GZM_HIP(hipGetLastError());
     return GZM_OK;
 }
 
-gzm_status read_counters(gzm_ctx *c, Counters *h)
+gzm_status read_counters(gzm_ctx *c, int slot, Counters *h)
 {
-    GZM_HIP(hipMemcpyAsync(h, c->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    GZM_HIP(hipMemcpyAsync(h, c->d_ctr + slot, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
     GZM_HIP(hipStreamSynchronize(c->stream));
     return GZM_OK;
 }
@@ -300,7 +322,6 @@ F32Geom f32_geom(const gzm_ctx *c)
     g.offm = c->d_offm;
     g.velf = c->d_velf;
     g.cellc = c->d_cellc;
-    for (int a = 0; a < 3; a++) g.s[a] = (float)c->grid.s[a];
     return g;
 }
 
@@ -320,30 +341,30 @@ gzm_status ensure_list(gzm_ctx *c)
     return GZM_OK;
 }
 
+// per-box max hsml into the device box descriptors (no host round trip)
 gzm_status compute_hmax(gzm_ctx *c)
 {
-    size_t bytes = 0;
-    GZM_HIP(rocprim::reduce(nullptr, bytes, c->d_hsml, c->d_hmax, 0.0, (size_t)c->n, rocprim::maximum<double>(),
-                            c->stream));
-    gzm_status s = ensure_temp(c, bytes);
-    if (s) return s;
-    GZM_HIP(rocprim::reduce(c->d_temp, bytes, c->d_hsml, c->d_hmax, 0.0, (size_t)c->n, rocprim::maximum<double>(),
-                            c->stream));
-    GZM_HIP(hipMemcpyAsync(&c->hmax, c->d_hmax, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GZM_HIP(hipStreamSynchronize(c->stream));
+    Timed tm(c, kOther);
+    launch_box_hmax(c->stream, c->n_boxes, c->d_boxes, c->d_hsml, c->n_hmax_tiles, c->d_hmax_tiles);
+    GZM_HIP(hipGetLastError());
     return GZM_OK;
 }
 
-// Ordered stream compaction: out = in[flags != 0], count to host.
+// Ordered stream compaction: out = in[flags != 0], count into c->d_count and,
+// when `count` is not NULL, to the host.  While capturing, the temporary
+// storage must already be large enough (sized by the probe pass).
 gzm_status compact(gzm_ctx *c, const int *in, const int *flags, int n, int *out, int *count)
 {
     size_t bytes = 0;
     GZM_HIP(rocprim::select(nullptr, bytes, in, flags, out, c->d_count, (size_t)n, c->stream));
+    if (c->capturing && bytes > c->temp_bytes) { set_error("temporary storage too small while capturing"); return GZM_ERR_STATE; }
     gzm_status s = ensure_temp(c, bytes);
     if (s) return s;
     GZM_HIP(rocprim::select(c->d_temp, bytes, in, flags, out, c->d_count, (size_t)n, c->stream));
-    GZM_HIP(hipMemcpyAsync(count, c->d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    GZM_HIP(hipStreamSynchronize(c->stream));
+    if (count) {
+        GZM_HIP(hipMemcpyAsync(count, c->d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        GZM_HIP(hipStreamSynchronize(c->stream));
+    }
     return GZM_OK;
 }
 
@@ -409,11 +430,7 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
     c->G.G7 = (0.5 * (g - 1.0));
     c->G.G8 = (1.0 / g);
     c->G.G9 = (g - 1.0);
-    for (int a = 0; a < 3; a++) {
-        c->box.L[a] = params->box[a];
-        c->box.half[a] = 0.5 * params->box[a];
-    }
-    if (dev_alloc(&c->d_ctr, 1) || dev_alloc(&c->d_count, 1) || dev_alloc(&c->d_hmax, 1)) {
+    if (dev_alloc(&c->d_ctr, kCtrSlots) || dev_alloc(&c->d_count, 1) || dev_alloc(&c->d_hmax, 1)) {
         gzm_ctx_destroy(c);
         return GZM_ERR_NOMEM;
     }
@@ -428,6 +445,7 @@ gzm_status gzm_ctx_destroy(gzm_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     collect_timings(c);
     free_particles(c);
+    free_boxes(c);
     dev_free(c->d_stage);
     dev_free(c->d_temp);
     dev_free(c->d_ctr);
@@ -439,7 +457,72 @@ gzm_status gzm_ctx_destroy(gzm_ctx *c)
     return GZM_OK;
 }
 
-gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
+}  // extern "C"
+
+namespace {
+
+// Per-box grids, the global cell-range array and the hmax tiles of an upload.
+gzm_status setup_boxes(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const double *box_sizes,
+                       const int64_t *offsets)
+{
+    std::vector<BoxDesc> bx(n_boxes);
+    std::vector<int2> tiles;
+    int64_t ncell = 0;
+    for (int b = 0; b < n_boxes; b++) {
+        BoxDesc &B = bx[b];
+        memset(&B, 0, sizeof(B));
+        const int64_t i0 = offsets[b], i1 = offsets[b + 1];
+        const double *L = box_sizes + 3 * (size_t)b;
+        // cell side from the parameter or ~ the mean initial hsml of the box
+        double s_cell = c->p.cell_size;
+        if (!(s_cell > 0)) {
+            double hsum = 0;
+            int64_t cnt = 0;
+            for (int64_t i = i0; i < i1; i++) {
+                const double h = v->hsml[i];
+                if (h > 0 && isfinite(h)) { hsum += h; cnt++; }
+            }
+            s_cell = cnt ? hsum / cnt : cbrt(L[0] * L[1] * L[2] * 8.0 / (double)std::max<int64_t>(1, i1 - i0));
+        }
+        int64_t nb = 1;
+        for (int ax = 0; ax < 3; ax++) {
+            int nc = (int)(L[ax] / s_cell);
+            nc = std::max(1, std::min(nc, 1024));
+            B.grid.nc[ax] = nc;
+            B.grid.inv_s[ax] = nc / L[ax];
+            B.grid.s[ax] = L[ax] / nc;
+            B.s_f[ax] = (float)B.grid.s[ax];
+            B.box.L[ax] = L[ax];
+            B.box.half[ax] = 0.5 * L[ax];
+            nb *= nc;
+        }
+        B.cell_base = (int)ncell;
+        ncell += nb;
+        if (ncell > ((int64_t)1 << 30)) { set_error("grids of %lld cells too large", (long long)ncell); return GZM_ERR_ARG; }
+        B.first = (int)i0;
+        B.count = (int)(i1 - i0);
+        for (int64_t k = i0; k < i1; k += kHmaxTile) tiles.push_back(make_int2(b, (int)k));
+    }
+    free_boxes(c);
+    gzm_status s = dev_alloc(&c->d_range, (size_t)ncell);
+    s = s ? s : dev_alloc(&c->d_boxes, (size_t)n_boxes);
+    s = s ? s : dev_alloc(&c->d_hmax_tiles, tiles.size());
+    if (s) { free_boxes(c); return s; }
+    for (auto &B : bx) B.grid.range = c->d_range + B.cell_base;
+    GZM_HIP(hipMemcpyAsync(c->d_boxes, bx.data(), sizeof(BoxDesc) * n_boxes, hipMemcpyHostToDevice, c->stream));
+    if (!tiles.empty())
+        GZM_HIP(hipMemcpyAsync(c->d_hmax_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice,
+                               c->stream));
+    GZM_HIP(hipStreamSynchronize(c->stream));
+    c->boxes_h = bx;
+    c->n_boxes = n_boxes;
+    c->ncell = (int)ncell;
+    c->n_hmax_tiles = (int)tiles.size();
+    return GZM_OK;
+}
+
+gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const double *box_sizes,
+                         const int64_t *offsets)
 {
     if (!c || !v) { set_error("NULL argument"); return GZM_ERR_ARG; }
     if (v->n <= 0 || v->n > (int64_t)0x7fffffff / 16) { set_error("n=%lld out of range", (long long)v->n); return GZM_ERR_ARG; }
@@ -447,19 +530,30 @@ gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
         set_error("pos, vel, mass, u and hsml are required");
         return GZM_ERR_ARG;
     }
-    GZM_HIP(hipSetDevice(c->p.device));
-    const int n = (int)v->n;
-    // positions must lie inside the box: the grid maps [0, L) to cells
-    for (int64_t i = 0; i < v->n; i++) {
-        for (int a = 0; a < 3; a++) {
-            double x = v->pos[3 * i + a];
-            if (!(x >= 0.0 && x < c->p.box[a])) {
-                set_error("pos[%lld][%d]=%g outside [0, %g)", (long long)i, a, x, c->p.box[a]);
-                return GZM_ERR_ARG;
+    if (n_boxes < 1 || n_boxes > (1 << 20)) { set_error("n_boxes=%d out of range", n_boxes); return GZM_ERR_ARG; }
+    if (offsets[0] != 0 || offsets[n_boxes] != v->n) {
+        set_error("box offsets must start at 0 and end at n=%lld", (long long)v->n);
+        return GZM_ERR_ARG;
+    }
+    for (int b = 0; b < n_boxes; b++) {
+        if (offsets[b + 1] < offsets[b]) { set_error("box offsets must not decrease (box %d)", b); return GZM_ERR_ARG; }
+        for (int ax = 0; ax < 3; ax++) {
+            const double L = box_sizes[3 * (size_t)b + ax];
+            if (!(L > 0) || !isfinite(L)) { set_error("box %d side %d must be > 0", b, ax); return GZM_ERR_ARG; }
+        }
+        // positions must lie inside their box: the grid maps [0, L) to cells
+        for (int64_t i = offsets[b]; i < offsets[b + 1]; i++) {
+            for (int ax = 0; ax < 3; ax++) {
+                const double x = v->pos[3 * i + ax], L = box_sizes[3 * (size_t)b + ax];
+                if (!(x >= 0.0 && x < L)) {
+                    set_error("pos[%lld][%d]=%g outside [0, %g) of box %d", (long long)i, ax, x, L, b);
+                    return GZM_ERR_ARG;
+                }
             }
         }
-        if (!(v->hsml[i] >= 0) && !isnan(v->hsml[i])) { /* negative hsml is clamped by density() */ }
     }
+    GZM_HIP(hipSetDevice(c->p.device));
+    const int n = (int)v->n;
     if (n != c->n) {
         free_particles(c);
         c->n = n;
@@ -469,6 +563,7 @@ gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
         s = s ? s : dev_alloc(&c->d_iota, n);
         s = s ? s : dev_alloc(&c->d_perm, n);
         s = s ? s : dev_alloc(&c->d_inv, n);
+        s = s ? s : dev_alloc(&c->d_box_of, n);
         s = s ? s : dev_alloc(&c->d_posm, n);
         s = s ? s : dev_alloc(&c->d_velu, n);
         double **scal[] = {&c->d_hsml, &c->d_hsml0, &c->d_left, &c->d_right, &c->d_cn, &c->d_cn0, &c->d_vsig,
@@ -497,39 +592,11 @@ gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
         }
         if (s) { free_particles(c); return s; }
     }
+    gzm_status s = setup_boxes(c, v, n_boxes, box_sizes, offsets);
+    if (s) return s;
 
-    // grid: cell side from the parameter or ~ the mean initial hsml
-    double s_cell = c->p.cell_size;
-    if (!(s_cell > 0)) {
-        double hsum = 0;
-        int64_t cnt = 0;
-        for (int64_t i = 0; i < v->n; i++) {
-            double h = v->hsml[i];
-            if (h > 0 && isfinite(h)) { hsum += h; cnt++; }
-        }
-        double vol = c->p.box[0] * c->p.box[1] * c->p.box[2];
-        s_cell = cnt ? hsum / cnt : cbrt(vol * 8.0 / n);
-    }
-    int64_t ncell = 1;
-    for (int a = 0; a < 3; a++) {
-        int nc = (int)(c->p.box[a] / s_cell);
-        nc = std::max(1, std::min(nc, 1024));
-        c->grid.nc[a] = nc;
-        c->grid.inv_s[a] = nc / c->p.box[a];
-        c->grid.s[a] = c->p.box[a] / nc;
-        ncell *= nc;
-    }
-    if (ncell > (int64_t)1 << 28) { set_error("grid of %lld cells too large", (long long)ncell); return GZM_ERR_ARG; }
-    if (ncell != c->ncell) {
-        dev_free(c->d_range);
-        gzm_status s = dev_alloc(&c->d_range, (size_t)ncell);
-        if (s) return s;
-        c->ncell = (int)ncell;
-    }
-    c->grid.range = c->d_range;
-
-    // inputs to staging (caller order), keys, stable sort by cell key
-    gzm_status s = ensure_stage(c, (size_t)n * 8);
+    // inputs to staging (caller order), keys, stable sort by (box, cell key)
+    s = ensure_stage(c, (size_t)n * 8);
     if (s) return s;
     double *st_pos = c->d_stage, *st_vel = c->d_stage + 3 * (size_t)n, *st_mass = c->d_stage + 6 * (size_t)n,
            *st_u = c->d_stage + 7 * (size_t)n;
@@ -537,20 +604,29 @@ gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
     GZM_HIP(hipMemcpyAsync(st_vel, v->vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(st_mass, v->mass, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(st_u, v->u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    std::vector<int> box_caller(n);
+    for (int b = 0; b < n_boxes; b++)
+        for (int64_t i = offsets[b]; i < offsets[b + 1]; i++) box_caller[i] = b;
+    int *d_box_caller = c->d_flags;  // free until the first stage call
+    GZM_HIP(hipMemcpyAsync(d_box_caller, box_caller.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    int key_bits = 30;
+    while ((1 << (key_bits - 30)) < n_boxes) key_bits++;
     {
         Timed tm(c, kOther);
-        launch_cell_keys(c->stream, n, st_pos, c->grid, c->d_keys, c->d_iota);
+        launch_cell_keys(c->stream, n, st_pos, c->d_boxes, d_box_caller, c->d_keys, c->d_iota);
         size_t bytes = 0;
-        GZM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, c->d_keys, c->d_keys_sorted, c->d_iota, c->d_perm, n, 0, 30,
-                                          c->stream));
+        GZM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, c->d_keys, c->d_keys_sorted, c->d_iota, c->d_perm, n, 0,
+                                          key_bits, c->stream));
         s = ensure_temp(c, bytes);
         if (s) return s;
         GZM_HIP(rocprim::radix_sort_pairs(c->d_temp, bytes, c->d_keys, c->d_keys_sorted, c->d_iota, c->d_perm, n, 0,
-                                          30, c->stream));
-        launch_gather_sorted(c->stream, n, c->d_perm, st_pos, st_vel, st_mass, st_u, c->d_posm, c->d_velu, c->d_inv);
-        launch_cell_ranges(c->stream, n, c->d_posm, c->grid, c->d_range, c->ncell);
+                                          key_bits, c->stream));
+        launch_gather_sorted(c->stream, n, c->d_perm, st_pos, st_vel, st_mass, st_u, d_box_caller, c->d_posm,
+                             c->d_velu, c->d_inv, c->d_box_of);
+        launch_cell_ranges(c->stream, n, c->d_posm, c->d_boxes, c->d_box_of, c->d_range, c->ncell);
         if (c->p.precision == 1)
-            launch_pack_geom_f32(c->stream, n, c->d_posm, c->d_velu, c->grid, c->d_offm, c->d_velf, c->d_cellc);
+            launch_pack_geom_f32(c->stream, n, c->d_posm, c->d_velu, c->d_boxes, c->d_box_of, c->d_offm, c->d_velf,
+                                 c->d_cellc);
         GZM_HIP(hipGetLastError());
     }
     // d_iota was consumed as sort values; rebuild it as the identity for "all active"
@@ -616,23 +692,31 @@ gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
     return GZM_OK;
 }
 
-gzm_status gzm_restore_initial_state(gzm_ctx *c)
+
+gzm_status restore_state(gzm_ctx *c)
 {
-    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
     GZM_HIP(hipMemcpyAsync(c->d_hsml, c->d_hsml0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_cn, c->d_cn0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_vsig, c->d_vsig0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     return GZM_OK;
 }
 
-gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_pass_stats *stats)
+// What a captured pass cannot learn on the host: the number of density
+// sweeps and whether the pair list overflowed, both taken from a probe pass
+// on the same initial state.
+struct Plan {
+    int sweeps;
+    bool overflow;
+};
+
+gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
 {
-    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
-    GZM_HIP(hipSetDevice(c->p.device));
-    gzm_status s = set_active(c, active, n_active);
-    if (s) return s;
-    s = reset_counters(c);
-    if (s) return s;
+    gzm_status s;
+    Counters *ctr = c->d_ctr + kCtrDensity;
+    if (!gp) {
+        s = reset_counters(c, kCtrDensity);
+        if (s) return s;
+    }
     const int na = c->n_active;
     int n_live = 0;
     if (na > 0) {
@@ -641,13 +725,13 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
             launch_density_init(c->stream, na, c->d_active, c->d_posm, c->d_hsml, c->d_left, c->d_right,
                                 c->p.max_hsml, c->d_flags);
         }
-        s = compact(c, c->d_active, c->d_flags, na, c->d_live, &n_live);
+        s = compact(c, c->d_active, c->d_flags, na, c->d_live, gp ? nullptr : &n_live);
         if (s) return s;
     }
     DensitySweepArgs a;
     memset(&a, 0, sizeof(a));
-    a.grid = c->grid;
-    a.box = c->box;
+    a.boxes = c->d_boxes;
+    a.box_of = c->d_box_of;
     a.hp.des_num_ngb = c->p.des_num_ngb;
     a.hp.max_num_ngb_dev = c->p.max_num_ngb_dev;
     a.hp.min_hsml = c->p.min_hsml;
@@ -660,30 +744,49 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
     a.fce = c->d_fce; a.nvt = c->d_nvt;
     a.redo = c->d_flags;
     a.f32 = f32_geom(c);
-    a.ctr = c->d_ctr;
+    a.ctr = ctr;
+    auto sweep = [&]() {
+        Timed tm(c, kDensity);
+        if (c->p.precision == 1) launch_density_sweep_f32(c->stream, a);
+        else launch_density_sweep(c->stream, a);
+    };
     int iter = 0, sweeps = 0;
     int64_t updates = 0;
     int status = GZM_OK;
-    while (n_live > 0) {
-        updates += n_live;
-        a.hp.iter = iter;
-        a.live = c->d_live;
-        a.n_live = n_live;
-        {
-            Timed tm(c, kDensity);
-            if (c->p.precision == 1) launch_density_sweep_f32(c->stream, a);
-            else launch_density_sweep(c->stream, a);
+    if (gp) {
+        // captured: every sweep is sized for all active particles and reads the
+        // live count from the device; the probe pass fixed the number of sweeps
+        if (na > 0) {
+            for (int sw = 0; sw < gp->sweeps; sw++) {
+                a.hp.iter = sw;
+                a.live = c->d_live;
+                a.n_live = na;
+                a.n_live_dev = c->d_count;
+                sweep();
+                s = compact(c, c->d_live, c->d_flags, na, c->d_live2, nullptr);
+                if (s) return s;
+                std::swap(c->d_live, c->d_live2);
+            }
+            launch_flag_unconverged(c->stream, c->d_count, ctr);
         }
-        GZM_HIP(hipGetLastError());
-        sweeps++;
-        int n_next = 0;
-        s = compact(c, c->d_live, c->d_flags, n_live, c->d_live2, &n_next);
-        if (s) return s;
-        std::swap(c->d_live, c->d_live2);
-        n_live = n_next;
-        if (n_live > 0) {
-            iter++;
-            if (iter > c->p.maxiter) { status = GZM_ERR_HSML_NOT_CONVERGED; break; }
+    } else {
+        while (n_live > 0) {
+            updates += n_live;
+            a.hp.iter = iter;
+            a.live = c->d_live;
+            a.n_live = n_live;
+            sweep();
+            GZM_HIP(hipGetLastError());
+            sweeps++;
+            int n_next = 0;
+            s = compact(c, c->d_live, c->d_flags, n_live, c->d_live2, &n_next);
+            if (s) return s;
+            std::swap(c->d_live, c->d_live2);
+            n_live = n_next;
+            if (n_live > 0) {
+                iter++;
+                if (iter > c->p.maxiter) { status = GZM_ERR_HSML_NOT_CONVERGED; break; }
+            }
         }
     }
     if (na > 0) {
@@ -692,8 +795,12 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
                                 c->d_rho, c->d_pres, c->G.G9);
     }
     GZM_HIP(hipGetLastError());
+    c->have_density = true;
+    c->have_gradients = false;
+    c->list_valid = false;
+    if (gp) return GZM_OK;
     Counters h;
-    s = read_counters(c, &h);
+    s = read_counters(c, kCtrDensity, &h);
     if (s) return s;
     collect_timings(c);
     if (h.status == 6 && status == GZM_OK) status = GZM_ERR_BRACKET;
@@ -703,22 +810,17 @@ gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_
         stats->density_updates = updates;
         stats->status = status;
     }
-    c->have_density = true;
-    c->have_gradients = false;
-    c->list_valid = false;
     if (status != GZM_OK) set_error("%s", gzm_status_string(status));
     return (gzm_status)status;
 }
 
-gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_pass_stats *stats)
+gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
 {
-    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
-    if (!c->have_density) { set_error("gradients need density results (run gzm_density or upload them)"); return GZM_ERR_STATE; }
-    GZM_HIP(hipSetDevice(c->p.device));
-    gzm_status s = set_active(c, active, n_active);
-    if (s) return s;
-    s = reset_counters(c);
-    if (s) return s;
+    gzm_status s;
+    if (!gp) {
+        s = reset_counters(c, kCtrGradients);
+        if (s) return s;
+    }
     s = compute_hmax(c);
     if (s) return s;
     {
@@ -730,11 +832,10 @@ gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gz
     }
     GradientArgs a;
     memset(&a, 0, sizeof(a));
-    a.grid = c->grid;
-    a.box = c->box;
+    a.boxes = c->d_boxes;
+    a.box_of = c->d_box_of;
     a.active = c->d_active;
     a.n_active = c->n_active;
-    a.hmax = c->hmax;
     a.rec = c->d_grec;
     a.nvt = c->d_nvt;
     a.cn = c->d_cn;
@@ -745,8 +846,12 @@ gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gz
     a.pres = c->d_pres;
     a.recf = c->d_grecf;
     a.f32 = f32_geom(c);
-    a.ctr = c->d_ctr;
+    a.ctr = c->d_ctr + kCtrGradients;
     if (c->n_active > 0 && c->list_cap > 0) {
+        if (c->capturing && ((size_t)c->n_active + 63) / 64 * (size_t)c->list_cap * 64 > c->list_entries) {
+            set_error("pair list too small while capturing");
+            return GZM_ERR_STATE;
+        }
         s = ensure_list(c);
         if (s) return s;
         a.list.idx = c->d_list;
@@ -760,27 +865,29 @@ gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gz
         else launch_gradients(c->stream, a);
     }
     GZM_HIP(hipGetLastError());
-    Counters h;
-    s = read_counters(c, &h);
-    if (s) return s;
-    collect_timings(c);
-    if (stats) stats->gradient_interactions = (int64_t)h.interactions;
     c->have_gradients = true;
     c->list_valid = (c->n_active > 0 && c->list_cap > 0);
     c->list_version = c->active_version;
+    if (gp) {
+        c->list_overflow_slices = gp->overflow ? 1 : 0;
+        return GZM_OK;
+    }
+    Counters h;
+    s = read_counters(c, kCtrGradients, &h);
+    if (s) return s;
+    collect_timings(c);
+    if (stats) stats->gradient_interactions = (int64_t)h.interactions;
     c->list_overflow_slices = h.overflow_slices;
     return GZM_OK;
 }
 
-gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_pass_stats *stats)
+gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
 {
-    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
-    if (!c->have_density) { set_error("force needs density results"); return GZM_ERR_STATE; }
-    GZM_HIP(hipSetDevice(c->p.device));
-    gzm_status s = set_active(c, active, n_active);
-    if (s) return s;
-    s = reset_counters(c);
-    if (s) return s;
+    gzm_status s;
+    if (!gp) {
+        s = reset_counters(c, kCtrForce);
+        if (s) return s;
+    }
     s = compute_hmax(c);
     if (s) return s;
     {
@@ -800,12 +907,11 @@ gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, 
     }
     ForceArgs a;
     memset(&a, 0, sizeof(a));
-    a.grid = c->grid;
-    a.box = c->box;
+    a.boxes = c->d_boxes;
+    a.box_of = c->d_box_of;
     a.G = c->G;
     a.active = c->d_active;
     a.n_active = c->n_active;
-    a.hmax = c->hmax;
     a.max_hsml = c->p.max_hsml;
     a.rec = c->d_frec;
     a.is_active = c->all_active ? nullptr : c->d_is_active;
@@ -819,11 +925,7 @@ gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, 
     a.hsml = c->d_hsml;
     a.recf = c->d_frecf;
     a.f32 = f32_geom(c);
-    // the FP32 list kernel takes pair displacements from cell differences
-    // unless the search radius spans half the box on some axis
-    for (int ax = 0; ax < 3; ax++)
-        if (!(c->hmax * c->grid.inv_s[ax] < 0.5 * c->grid.nc[ax] - 2.0)) a.exact_geom = 1;
-    a.ctr = c->d_ctr;
+    a.ctr = c->d_ctr + kCtrForce;
     a.fallback_only = 0;
     const bool use_list = c->list_valid && c->list_version == c->active_version;
     if (use_list) {
@@ -842,8 +944,9 @@ gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, 
         }
     }
     GZM_HIP(hipGetLastError());
+    if (gp) return GZM_OK;
     Counters h;
-    s = read_counters(c, &h);
+    s = read_counters(c, kCtrForce, &h);
     if (s) return s;
     collect_timings(c);
     if (stats) {
@@ -855,6 +958,153 @@ gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, 
     if (h.status != 0) {
         set_error("%s", gzm_status_string(h.status));
         return (gzm_status)h.status;
+    }
+    return GZM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
+{
+    if (!c || !v) { set_error("NULL argument"); return GZM_ERR_ARG; }
+    const int64_t offsets[2] = {0, v->n};
+    return upload_common(c, v, 1, c->p.box, offsets);
+}
+
+gzm_status gzm_upload_batch(gzm_ctx *c, const gzm_host_view *v, int32_t n_boxes, const double *box_sizes,
+                            const int64_t *box_offsets)
+{
+    if (!c || !v || !box_sizes || !box_offsets) { set_error("NULL argument"); return GZM_ERR_ARG; }
+    return upload_common(c, v, n_boxes, box_sizes, box_offsets);
+}
+
+gzm_status gzm_restore_initial_state(gzm_ctx *c)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    return restore_state(c);
+}
+
+gzm_status gzm_density(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_pass_stats *stats)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    GZM_HIP(hipSetDevice(c->p.device));
+    gzm_status s = set_active(c, active, n_active);
+    if (s) return s;
+    return density_impl(c, nullptr, stats);
+}
+
+gzm_status gzm_gradients(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_pass_stats *stats)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    if (!c->have_density) { set_error("gradients need density results (run gzm_density or upload them)"); return GZM_ERR_STATE; }
+    GZM_HIP(hipSetDevice(c->p.device));
+    gzm_status s = set_active(c, active, n_active);
+    if (s) return s;
+    return gradients_impl(c, nullptr, stats);
+}
+
+gzm_status gzm_hydro_force(gzm_ctx *c, const int32_t *active, int64_t n_active, gzm_pass_stats *stats)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    if (!c->have_density) { set_error("force needs density results"); return GZM_ERR_STATE; }
+    GZM_HIP(hipSetDevice(c->p.device));
+    gzm_status s = set_active(c, active, n_active);
+    if (s) return s;
+    return force_impl(c, nullptr, stats);
+}
+
+gzm_status gzm_run_passes(gzm_ctx *c, int32_t n_passes, gzm_pass_stats *stats)
+{
+    if (!c || c->n == 0) { set_error("no particles uploaded"); return GZM_ERR_STATE; }
+    if (n_passes < 1 || n_passes > 100000) { set_error("n_passes=%d out of range [1, 100000]", n_passes); return GZM_ERR_ARG; }
+    GZM_HIP(hipSetDevice(c->p.device));
+    // probe pass (eager): sizes every buffer, gives the sweep count and the list overflow
+    gzm_pass_stats probe;
+    memset(&probe, 0, sizeof(probe));
+    gzm_status s = restore_state(c);
+    s = s ? s : set_active(c, nullptr, 0);
+    s = s ? s : density_impl(c, nullptr, &probe);
+    s = s ? s : gradients_impl(c, nullptr, &probe);
+    s = s ? s : force_impl(c, nullptr, &probe);
+    if (s) return s;
+    const Plan gp = {probe.density_sweeps, c->list_overflow_slices > 0};
+    {
+        // the captured compactions run over the whole active list
+        size_t bytes = 0;
+        GZM_HIP(rocprim::select(nullptr, bytes, c->d_active, c->d_flags, c->d_live2, c->d_count, (size_t)c->n_active,
+                                c->stream));
+        s = ensure_temp(c, bytes);
+        if (s) return s;
+    }
+    GZM_HIP(hipStreamSynchronize(c->stream));
+    collect_timings(c);
+
+    hipGraph_t graph = nullptr;
+    GZM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    for (int k = 0; k < kCtrSlots; k++) launch_clear_counters(c->stream, c->d_ctr + k);
+    for (int p = 0; p < n_passes && s == GZM_OK; p++) {
+        s = restore_state(c);
+        s = s ? s : density_impl(c, &gp, nullptr);
+        s = s ? s : gradients_impl(c, &gp, nullptr);
+        s = s ? s : force_impl(c, &gp, nullptr);
+    }
+    c->capturing = false;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (s != GZM_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return s;
+    }
+    if (ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        set_error("hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+        return GZM_ERR_HIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        set_error("hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        return GZM_ERR_HIP;
+    }
+    hipEvent_t e0 = take_event(c), e1 = take_event(c);
+    if (e0) (void)hipEventRecord(e0, c->stream);
+    const hipError_t el = hipGraphLaunch(exec, c->stream);
+    if (e1) (void)hipEventRecord(e1, c->stream);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    (void)hipGraphExecDestroy(exec);
+    float ms = 0;
+    if (e0 && e1 && el == hipSuccess && es == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+        c->timing.graph_ms += ms;
+        c->timing.graph_launches++;
+        c->timing.graph_passes += n_passes;
+    }
+    if (e0) c->event_pool.push_back(e0);
+    if (e1) c->event_pool.push_back(e1);
+    if (el != hipSuccess || es != hipSuccess) {
+        set_error("graph launch failed: %s", hipGetErrorString(el != hipSuccess ? el : es));
+        return GZM_ERR_HIP;
+    }
+    Counters h[kCtrSlots];
+    GZM_HIP(hipMemcpy(h, c->d_ctr, sizeof(h), hipMemcpyDeviceToHost));
+    int status = 0;
+    for (int k = 0; k < kCtrSlots; k++) status = std::max(status, h[k].status);
+    if (status == 6) status = GZM_ERR_BRACKET;
+    if (stats) {
+        stats->density_sweeps = gp.sweeps;
+        stats->density_interactions = (int64_t)h[kCtrDensity].interactions;
+        stats->gradient_interactions = (int64_t)h[kCtrGradients].interactions;
+        stats->force_interactions = (int64_t)h[kCtrForce].interactions;
+        stats->riemann_retries = (int64_t)h[kCtrForce].retries;
+        stats->exact_solver_calls = (int64_t)h[kCtrForce].exact_calls;
+        stats->density_updates = 0;
+        stats->status = status;
+    }
+    if (status != 0) {
+        set_error("%s", gzm_status_string(status));
+        return (gzm_status)status;
     }
     return GZM_OK;
 }
@@ -928,16 +1178,16 @@ gzm_status gzm_neighbor_counts(gzm_ctx *c, int32_t mode, const int32_t *active, 
         GZM_HIP(hipMemcpyAsync(c->d_active_caller, active, sizeof(int) * na, hipMemcpyHostToDevice, c->stream));
         launch_map_active(c->stream, na, c->d_active_caller, c->d_inv, c->d_live);
     }
-    gzm_status s = reset_counters(c);
+    gzm_status s = reset_counters(c, kCtrMisc);
     if (s) return s;
     s = compute_hmax(c);
     if (s) return s;
-    launch_neighbor_counts(c->stream, c->grid, c->box, mode, na, c->d_live, c->d_posm, c->d_hsml, c->hmax,
-                           counts ? c->d_flags : nullptr, c->d_ctr);
+    launch_neighbor_counts(c->stream, c->d_boxes, c->d_box_of, mode, na, c->d_live, c->d_posm, c->d_hsml,
+                           counts ? c->d_flags : nullptr, c->d_ctr + kCtrMisc);
     GZM_HIP(hipGetLastError());
     if (counts) GZM_HIP(hipMemcpyAsync(counts, c->d_flags, sizeof(int) * na, hipMemcpyDeviceToHost, c->stream));
     Counters h;
-    s = read_counters(c, &h);
+    s = read_counters(c, kCtrMisc, &h);
     if (s) return s;
     if (total) *total = (int64_t)h.interactions;
     c->all_active = false;  // d_live was used as scratch; the active list is rebuilt on the next stage call

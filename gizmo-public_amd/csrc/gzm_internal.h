@@ -19,6 +19,20 @@ struct Grid {
     const int2 *range;
 };
 
+// One periodic box of a context.  A context holds one box, or many
+// independent boxes (batched pass) whose particles are sorted box by box; the
+// kernels read the descriptor of a particle's box (box_of[slot]) for its grid,
+// periodic wrap and search radius.
+struct BoxDesc {
+    Grid grid;             // range points at the box's first cell
+    BoxInfo box;           // side lengths and half side lengths
+    double hmax;           // max hsml of the box (the pair loops' search radius)
+    float s_f[3];          // cell side in FP32
+    int cell_base;         // first cell of the box in the global range array
+    int first, count;      // sorted slot range of the box
+    int pad;
+};
+
 // Per-particle record read by the force loop for the neighbour j (and for i).
 // 320 bytes: one aligned block so a gather touches 2.5 cache lines.
 struct alignas(64) ForceRec {
@@ -51,8 +65,7 @@ struct alignas(32) GradRec {
 struct F32Geom {
     const float4 *offm;        // off x, y, z, m (FP32), sorted order
     const float4 *velf;        // v x, y, z, 0 (FP32)
-    const uint32_t *cellc;     // cx | cy << 10 | cz << 20
-    float s[3];                // cell side in FP32
+    const uint32_t *cellc;     // cx | cy << 10 | cz << 20 (cell within the particle's box)
 };
 
 // Per-particle record of the FP32 gradient loop (48 bytes).
@@ -88,11 +101,12 @@ struct Counters {
 };
 
 struct DensitySweepArgs {
-    Grid grid;
-    BoxInfo box;
+    const BoxDesc *boxes;  // per box
+    const int *box_of;     // box of each sorted slot
     HsmlParams hp;
     const int *live;
-    int n_live;
+    int n_live;            // live entries, or the capacity when n_live_dev is set
+    const int *n_live_dev; // device count of live entries (graph-captured passes)
     const double4 *posm;   // x, y, z, m
     const double4 *velu;   // vx, vy, vz, u
     double *hsml, *left, *right, *cn;
@@ -116,11 +130,10 @@ struct PairList {
 };
 
 struct GradientArgs {
-    Grid grid;
-    BoxInfo box;
+    const BoxDesc *boxes;
+    const int *box_of;
     const int *active;
     int n_active;
-    double hmax;
     const GradRec *rec;
     const double *nvt;     // 6 per particle
     const double *cn;
@@ -136,12 +149,11 @@ struct GradientArgs {
 };
 
 struct ForceArgs {
-    Grid grid;
-    BoxInfo box;
+    const BoxDesc *boxes;
+    const int *box_of;
     GammaConsts G;
     const int *active;
     int n_active;
-    double hmax;
     double max_hsml;
     const ForceRec *rec;
     const uint8_t *is_active;  // NULL when every particle is active (no wakeup test)
@@ -157,18 +169,26 @@ struct ForceArgs {
     const double *hsml;    // FP64 hsml (FP32 kernels: exact re-tests)
     const ForceRecF *recf; // FP32 kernels
     F32Geom f32;
-    int exact_geom;        // FP32 list kernel: pair displacements from FP64 positions
-                           // (the search radius spans half the box on some axis)
     Counters *ctr;
 };
 
 // launchers (gzm_kernels.hip); all asynchronous on `stream`
-void launch_cell_keys(hipStream_t s, int n, const double *pos, Grid g, uint32_t *keys, int *iota);
+void launch_cell_keys(hipStream_t s, int n, const double *pos, const BoxDesc *boxes, const int *box_caller,
+                      uint64_t *keys, int *iota);
 void launch_gather_sorted(hipStream_t s, int n, const int *perm, const double *pos, const double *vel,
-                          const double *mass, const double *u, double4 *posm, double4 *velu, int *inv);
+                          const double *mass, const double *u, const int *box_caller, double4 *posm,
+                          double4 *velu, int *inv, int *box_of);
 void launch_gather_scalar(hipStream_t s, int n, const int *perm, const double *src, double *dst, int width);
 void launch_scatter_scalar(hipStream_t s, int n, const int *perm, const double *src, double *dst, int width);
-void launch_cell_ranges(hipStream_t s, int n, const double4 *posm, Grid g, int2 *range, int ncell);
+void launch_cell_ranges(hipStream_t s, int n, const double4 *posm, const BoxDesc *boxes, const int *box_of,
+                        int2 *range, int ncell);
+// per-box max hsml into boxes[b].hmax; tiles = (box, first slot) of each
+// block, each tile at most kHmaxTile slots of one box
+constexpr int kHmaxTile = 4096;
+void launch_box_hmax(hipStream_t s, int n_boxes, BoxDesc *boxes, const double *hsml, int n_tiles, const int2 *tiles);
+// status 1155 when live particles remain after the captured sweeps
+void launch_flag_unconverged(hipStream_t s, const int *n_live_dev, Counters *ctr);
+void launch_clear_counters(hipStream_t s, Counters *ctr);
 void launch_map_active(hipStream_t s, int n_active, const int *active_caller, const int *inv, int *active_sorted);
 void launch_density_init(hipStream_t s, int n_active, const int *active, const double4 *posm, double *hsml,
                          double *left, double *right, double max_hsml, int *live_flag);
@@ -184,8 +204,8 @@ void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 
                        const double *fce, const double *nvt, const double *grad, double gamma, ForceRec *rec);
 void launch_force(hipStream_t s, const ForceArgs &a);  // list-driven if a.list.idx, else scanning
 // FP32-bulk kernels (gzm_kernels_f32.hip)
-void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const double4 *velu, Grid g, float4 *offm,
-                          float4 *velf, uint32_t *cellc);
+void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const double4 *velu, const BoxDesc *boxes,
+                          const int *box_of, float4 *offm, float4 *velf, uint32_t *cellc);
 void launch_density_sweep_f32(hipStream_t s, const DensitySweepArgs &a);
 void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const double *hsml,
                           const double *rho, const double *pres, GradRecF *rec);
@@ -196,8 +216,8 @@ void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float
                            const double *grad, double gamma, ForceRecF *rec);
 void launch_force_f32(hipStream_t s, const ForceArgs &a);
 
-void launch_neighbor_counts(hipStream_t s, Grid g, BoxInfo box, int mode, int n_active, const int *active,
-                            const double4 *posm, const double *hsml, double hmax, int *counts, Counters *ctr);
+void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, int mode, int n_active,
+                            const int *active, const double4 *posm, const double *hsml, int *counts, Counters *ctr);
 void launch_fill_int(hipStream_t s, int *p, int n, int value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value);
 void launch_fill_double(hipStream_t s, double *p, long long n, double value);

@@ -21,19 +21,32 @@ namespace {
 // ---------------------------------------------------------------------------
 // grid build
 // ---------------------------------------------------------------------------
-__global__ void cell_keys_kernel(int n, const double *pos, Grid g, uint32_t *keys, int *iota)
+// global cell index of a position in box B (cell_base + linear cell id)
+__device__ __forceinline__ int global_cell(const BoxDesc &B, double x, double y, double z)
+{
+    const Grid &g = B.grid;
+    return B.cell_base + (cell_coord(x, g.inv_s[0], g.nc[0]) * g.nc[1] + cell_coord(y, g.inv_s[1], g.nc[1])) * g.nc[2] +
+           cell_coord(z, g.inv_s[2], g.nc[2]);
+}
+
+// sort key: box index above the 30-bit Morton code of the cell in its box
+__global__ void cell_keys_kernel(int n, const double *pos, const BoxDesc *boxes, const int *box_caller,
+                                 uint64_t *keys, int *iota)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int b = box_caller ? box_caller[i] : 0;
+    const Grid &g = boxes[b].grid;
     int cx = cell_coord(pos[3 * i + 0], g.inv_s[0], g.nc[0]);
     int cy = cell_coord(pos[3 * i + 1], g.inv_s[1], g.nc[1]);
     int cz = cell_coord(pos[3 * i + 2], g.inv_s[2], g.nc[2]);
-    keys[i] = morton3(cx, cy, cz);
+    keys[i] = ((uint64_t)b << 30) | morton3(cx, cy, cz);
     iota[i] = i;
 }
 
 __global__ void gather_sorted_kernel(int n, const int *perm, const double *pos, const double *vel,
-                                     const double *mass, const double *u, double4 *posm, double4 *velu, int *inv)
+                                     const double *mass, const double *u, const int *box_caller, double4 *posm,
+                                     double4 *velu, int *inv, int *box_of)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -41,6 +54,7 @@ __global__ void gather_sorted_kernel(int n, const int *perm, const double *pos, 
     posm[k] = make_double4(pos[3 * i + 0], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
     velu[k] = make_double4(vel[3 * i + 0], vel[3 * i + 1], vel[3 * i + 2], u[i]);
     inv[i] = k;
+    box_of[k] = box_caller ? box_caller[i] : 0;
 }
 
 __global__ void gather_scalar_kernel(int n, const int *perm, const double *src, double *dst, int width)
@@ -59,26 +73,65 @@ __global__ void scatter_scalar_kernel(int n, const int *perm, const double *src,
     dst[(long long)perm[k] * width + c] = src[t];
 }
 
-__global__ void cell_ranges_kernel(int n, const double4 *posm, Grid g, int2 *range)
+__global__ void cell_ranges_kernel(int n, const double4 *posm, const BoxDesc *boxes, const int *box_of, int2 *range)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double4 p = posm[k];
-    int c = (cell_coord(p.x, g.inv_s[0], g.nc[0]) * g.nc[1] + cell_coord(p.y, g.inv_s[1], g.nc[1])) * g.nc[2] +
-            cell_coord(p.z, g.inv_s[2], g.nc[2]);
+    const int c = global_cell(boxes[box_of[k]], p.x, p.y, p.z);
     int cprev = -1, cnext = -1;
     if (k > 0) {
         double4 q = posm[k - 1];
-        cprev = (cell_coord(q.x, g.inv_s[0], g.nc[0]) * g.nc[1] + cell_coord(q.y, g.inv_s[1], g.nc[1])) * g.nc[2] +
-                cell_coord(q.z, g.inv_s[2], g.nc[2]);
+        cprev = global_cell(boxes[box_of[k - 1]], q.x, q.y, q.z);
     }
     if (k + 1 < n) {
         double4 q = posm[k + 1];
-        cnext = (cell_coord(q.x, g.inv_s[0], g.nc[0]) * g.nc[1] + cell_coord(q.y, g.inv_s[1], g.nc[1])) * g.nc[2] +
-                cell_coord(q.z, g.inv_s[2], g.nc[2]);
+        cnext = global_cell(boxes[box_of[k + 1]], q.x, q.y, q.z);
     }
     if (c != cprev) range[c].x = k;
     if (c != cnext) range[c].y = k + 1;
+}
+
+// Per-box max hsml (the search radius of the pair loops, force_update_hmax):
+// block = one tile of up to kHmaxTile slots of one box; block maximum, then
+// one atomic max on the bit pattern (valid for hsml >= 0).
+__global__ __launch_bounds__(kBlock) void box_hmax_kernel(BoxDesc *boxes, const double *hsml, const int2 *tiles)
+{
+    __shared__ double red[kBlock / 64];
+    const int2 tile = tiles[blockIdx.x];  // (box, first slot)
+    const BoxDesc &B = boxes[tile.x];
+    const int end = min(tile.y + kHmaxTile, B.first + B.count);
+    double m = 0;
+    for (int k = tile.y + threadIdx.x; k < end; k += kBlock) m = fmax(m, hsml[k]);
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; w++) m = fmax(m, red[w]);
+        atomicMax((unsigned long long *)&boxes[tile.x].hmax, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void clear_box_hmax_kernel(int n_boxes, BoxDesc *boxes)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n_boxes) boxes[b].hmax = 0.0;
+}
+
+__global__ void flag_unconverged_kernel(const int *n_live_dev, Counters *ctr)
+{
+    if (threadIdx.x == 0 && *n_live_dev > 0) atomicMax(&ctr->status, 1155);
+}
+
+__global__ void clear_counters_kernel(Counters *ctr)
+{
+    if (threadIdx.x == 0) {
+        ctr->interactions = 0;
+        ctr->retries = 0;
+        ctr->exact_calls = 0;
+        ctr->status = 0;
+        ctr->overflow_slices = 0;
+    }
 }
 
 __global__ void map_active_kernel(int n_active, const int *active_caller, const int *inv, int *active_sorted)
@@ -133,9 +186,11 @@ __global__ void density_init_kernel(int n_active, const int *active, const doubl
 __global__ __launch_bounds__(kBlock) void density_sweep_kernel(DensitySweepArgs a)
 {
     const int t = xcd_block() * kBlock + threadIdx.x;
-    const bool valid = t < a.n_live;
+    const int n_live = a.n_live_dev ? *a.n_live_dev : a.n_live;
+    const bool valid = t < n_live;
+    if (!valid && a.n_live_dev && t < a.n_live) a.redo[t] = 0;  // captured pass: clear up to the capacity
     unsigned long long cnt = 0;
-    if (valid) {
+    for_box(a.boxes, a.box_of, valid ? a.live[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = a.live[t];
         const double4 pi = a.posm[i];
         const double4 vi = a.velu[i];
@@ -145,10 +200,10 @@ __global__ __launch_bounds__(kBlock) void density_sweep_kernel(DensitySweepArgs 
         const double h2 = h * h;
         DensitySums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         const double xs[3] = {pi.x, pi.y, pi.z};
-        for_each_candidate(a.grid, xs, h, [&](int j) {
+        for_each_candidate(G.grid, xs, h, [&](int j) {
             const double4 pj = a.posm[j];
             double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-            wrap_dp(a.box, dp);
+            wrap_dp(G.box, dp);
             const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
             if (r2 < h2 && pj.w > 0) {
                 cnt++;
@@ -191,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void density_sweep_kernel(DensitySweepArgs 
         for (int k = 0; k < 6; k++) a.nvt[6 * (size_t)i + k] = R.nvt[k];
         a.redo[t] = (res == 1) ? 1 : 0;
         if (res < 0) atomicMax(&a.ctr->status, (int)6);
-    }
+    });
     wave_add_u64(&a.ctr->interactions, cnt);
 }
 
@@ -240,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
     const bool valid = t < a.n_active;
     unsigned long long cnt = 0;
     int listed = 0;
-    if (valid) {
+    for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = a.active[t];
         const GradRec ri = a.rec[i];
         double gr0 = 0, gr1 = 0, gr2 = 0, gp0 = 0, gp1 = 0, gp2 = 0;
@@ -254,11 +309,11 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
             kernel_hinv(h_i, hinv, hinv3, hinv4);
             const double h2_i = h_i * h_i;
             const double xs[3] = {ri.xh.x, ri.xh.y, ri.xh.z};
-            const double R = (h_i > a.hmax) ? h_i : a.hmax;
-            for_each_candidate(a.grid, xs, R, [&](int j) {
+            const double R = (h_i > G.hmax) ? h_i : G.hmax;
+            for_each_candidate(G.grid, xs, R, [&](int j) {
                 const double4 xj = a.rec[j].xh;
                 double dp[3] = {xs[0] - xj.x, xs[1] - xj.y, xs[2] - xj.z};
-                wrap_dp(a.box, dp);
+                wrap_dp(G.box, dp);
                 const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
                 const double h_j = xj.w;
                 if (r2 <= 0) return;
@@ -331,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
         o[9] = gv1[0]; o[10] = gv1[1]; o[11] = gv1[2];
         o[12] = gv2[0]; o[13] = gv2[1]; o[14] = gv2[2];
         if (a.list.idx) a.list.count[t] = listed;
-    }
+    });
     if (a.list.idx) {
         // a slice overflows when any of its lanes listed more than cap neighbours
         const bool over = __any(listed > a.list.cap);
@@ -402,7 +457,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
         if (valid && !a.list.overflow[t >> 6]) valid = false;
     }
     unsigned long long cnt = 0, nretry = 0, nexact = 0;
-    if (valid) {
+    for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = a.active[t];
         const ForceRec ri = a.rec[i];
         double acc0 = 0, acc1 = 0, acc2 = 0, dte = 0, vmax = kMinReal;
@@ -421,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
             auto visit = [&](int j, bool check) {
                 const double4 xj = a.rec[j].xh;
                 double dp[3] = {x_i[0] - xj.x, x_i[1] - xj.y, x_i[2] - xj.z};
-                wrap_dp(a.box, dp);
+                wrap_dp(G.box, dp);
                 const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
                 if (check) {
                     const double h_j = xj.w;
@@ -454,8 +509,8 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
                 const int *row = a.list.idx + (size_t)t * a.list.cap;
                 for (int k = 0; k < n; k++) visit(row[k], false);
             } else {
-                const double R = (I.h > a.hmax) ? I.h : a.hmax;
-                for_each_candidate(a.grid, x_i, R, [&](int j) { visit(j, true); });
+                const double R = (I.h > G.hmax) ? I.h : G.hmax;
+                for_each_candidate(G.grid, x_i, R, [&](int j) { visit(j, true); });
             }
             if (vmax < out_vsig) vmax = out_vsig;
             // hydro_final_operations_and_cleanup (hydro_toplevel.c:738-764)
@@ -476,30 +531,30 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
         a.dtu[i] = dte;
         a.vsig[i] = vmax;
         if (status) atomicMax(&a.ctr->status, status);
-    }
+    });
     wave_add_u64(&a.ctr->interactions, cnt);
     wave_add_u64(&a.ctr->retries, nretry);
     wave_add_u64(&a.ctr->exact_calls, nexact);
 }
 
 // neighbour-set sizes for a fixed hsml (same predicates as the stage kernels)
-__global__ __launch_bounds__(kBlock) void neighbor_count_kernel(Grid g, BoxInfo box, int mode, int n_active,
-                                                                const int *active, const double4 *posm,
-                                                                const double *hsml, double hmax, int *counts,
-                                                                Counters *ctr)
+__global__ __launch_bounds__(kBlock) void neighbor_count_kernel(const BoxDesc *boxes, const int *box_of, int mode,
+                                                                int n_active, const int *active, const double4 *posm,
+                                                                const double *hsml, int *counts, Counters *ctr)
 {
     const int t = xcd_block() * kBlock + threadIdx.x;
     unsigned long long cnt = 0;
-    if (t < n_active) {
+    const bool valid = t < n_active;
+    for_box(boxes, box_of, valid ? active[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = active[t];
         const double4 pi = posm[i];
         const double h = hsml[i], h2 = h * h;
         const double xs[3] = {pi.x, pi.y, pi.z};
-        const double R = (mode == 0) ? h : ((h > hmax) ? h : hmax);
-        for_each_candidate(g, xs, R, [&](int j) {
+        const double R = (mode == 0) ? h : ((h > G.hmax) ? h : G.hmax);
+        for_each_candidate(G.grid, xs, R, [&](int j) {
             const double4 pj = posm[j];
             double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-            wrap_dp(box, dp);
+            wrap_dp(G.box, dp);
             const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
             if (mode == 0) {
                 if (r2 < h2) cnt++;
@@ -511,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void neighbor_count_kernel(Grid g, BoxInfo 
             }
         });
         if (counts) counts[t] = (int)cnt;
-    }
+    });
     wave_add_u64(&ctr->interactions, cnt);
 }
 
@@ -523,15 +578,18 @@ inline unsigned nblocks(long long n) { return (unsigned)((n + kBlock - 1) / kBlo
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_cell_keys(hipStream_t s, int n, const double *pos, Grid g, uint32_t *keys, int *iota)
+void launch_cell_keys(hipStream_t s, int n, const double *pos, const BoxDesc *boxes, const int *box_caller,
+                      uint64_t *keys, int *iota)
 {
-    if ((n) > 0) hipLaunchKernelGGL(cell_keys_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pos, g, keys, iota);
+    if (n > 0) hipLaunchKernelGGL(cell_keys_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, pos, boxes, box_caller, keys, iota);
 }
 void launch_gather_sorted(hipStream_t s, int n, const int *perm, const double *pos, const double *vel,
-                          const double *mass, const double *u, double4 *posm, double4 *velu, int *inv)
+                          const double *mass, const double *u, const int *box_caller, double4 *posm,
+                          double4 *velu, int *inv, int *box_of)
 {
-    if ((n) > 0) hipLaunchKernelGGL(gather_sorted_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, perm, pos, vel, mass, u,
-                       posm, velu, inv);
+    if (n > 0)
+        hipLaunchKernelGGL(gather_sorted_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, perm, pos, vel, mass, u, box_caller,
+                           posm, velu, inv, box_of);
 }
 void launch_gather_scalar(hipStream_t s, int n, const int *perm, const double *src, double *dst, int width)
 {
@@ -543,10 +601,24 @@ void launch_scatter_scalar(hipStream_t s, int n, const int *perm, const double *
     if (((long long)n * width) > 0) hipLaunchKernelGGL(scatter_scalar_kernel, dim3(nblocks((long long)n * width)), dim3(kBlock), 0, s, n, perm,
                        src, dst, width);
 }
-void launch_cell_ranges(hipStream_t s, int n, const double4 *posm, Grid g, int2 *range, int ncell)
+void launch_cell_ranges(hipStream_t s, int n, const double4 *posm, const BoxDesc *boxes, const int *box_of,
+                        int2 *range, int ncell)
 {
     if ((2LL * ncell) > 0) hipLaunchKernelGGL(fill_int_kernel, dim3(nblocks(2LL * ncell)), dim3(kBlock), 0, s, (int *)range, 2 * ncell, 0);
-    if ((n) > 0) hipLaunchKernelGGL(cell_ranges_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, posm, g, range);
+    if (n > 0) hipLaunchKernelGGL(cell_ranges_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, posm, boxes, box_of, range);
+}
+void launch_box_hmax(hipStream_t s, int n_boxes, BoxDesc *boxes, const double *hsml, int n_tiles, const int2 *tiles)
+{
+    if (n_boxes > 0) hipLaunchKernelGGL(clear_box_hmax_kernel, dim3(nblocks(n_boxes)), dim3(kBlock), 0, s, n_boxes, boxes);
+    if (n_tiles > 0) hipLaunchKernelGGL(box_hmax_kernel, dim3(n_tiles), dim3(kBlock), 0, s, boxes, hsml, tiles);
+}
+void launch_flag_unconverged(hipStream_t s, const int *n_live_dev, Counters *ctr)
+{
+    hipLaunchKernelGGL(flag_unconverged_kernel, dim3(1), dim3(64), 0, s, n_live_dev, ctr);
+}
+void launch_clear_counters(hipStream_t s, Counters *ctr)
+{
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, s, ctr);
 }
 void launch_map_active(hipStream_t s, int n_active, const int *active_caller, const int *inv, int *active_sorted)
 {
@@ -594,11 +666,12 @@ void launch_force(hipStream_t s, const ForceArgs &a)
     else
         hipLaunchKernelGGL(force_kernel<false>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
 }
-void launch_neighbor_counts(hipStream_t s, Grid g, BoxInfo box, int mode, int n_active, const int *active,
-                            const double4 *posm, const double *hsml, double hmax, int *counts, Counters *ctr)
+void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, int mode, int n_active,
+                            const int *active, const double4 *posm, const double *hsml, int *counts, Counters *ctr)
 {
-    if ((n_active) > 0) hipLaunchKernelGGL(neighbor_count_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, g, box, mode, n_active,
-                       active, posm, hsml, hmax, counts, ctr);
+    if (n_active > 0)
+        hipLaunchKernelGGL(neighbor_count_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, boxes, box_of, mode,
+                           n_active, active, posm, hsml, counts, ctr);
 }
 void launch_fill_int(hipStream_t s, int *p, int n, int value)
 {

@@ -64,12 +64,13 @@ __device__ __forceinline__ int wrap_cell(int d, int nc)
     return (d > half) ? d - nc : ((d < -half) ? d + nc : d);
 }
 
-__global__ void pack_geom_f32_kernel(int n, const double4 *posm, const double4 *velu, Grid g, float4 *offm,
-                                     float4 *velf, uint32_t *cellc)
+__global__ void pack_geom_f32_kernel(int n, const double4 *posm, const double4 *velu, const BoxDesc *boxes,
+                                     const int *box_of, float4 *offm, float4 *velf, uint32_t *cellc)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double4 p = posm[k], v = velu[k];
+    const Grid &g = boxes[box_of[k]].grid;
     const int cx = cell_coord(p.x, g.inv_s[0], g.nc[0]);
     const int cy = cell_coord(p.y, g.inv_s[1], g.nc[1]);
     const int cz = cell_coord(p.z, g.inv_s[2], g.nc[2]);
@@ -85,9 +86,11 @@ __global__ void pack_geom_f32_kernel(int n, const double4 *posm, const double4 *
 __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepArgs a)
 {
     const int t = xcd_block() * kBlock + threadIdx.x;
-    const bool valid = t < a.n_live;
+    const int n_live = a.n_live_dev ? *a.n_live_dev : a.n_live;
+    const bool valid = t < n_live;
+    if (!valid && a.n_live_dev && t < a.n_live) a.redo[t] = 0;  // captured pass: clear up to the capacity
     unsigned long long cnt = 0;
-    if (valid) {
+    for_box(a.boxes, a.box_of, valid ? a.live[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = a.live[t];
         const double4 pi = a.posm[i];
         const float4 oi = a.f32.offm[i];
@@ -103,11 +106,11 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
         float t00 = 0, t01 = 0, t02 = 0, t11 = 0, t12 = 0, t22 = 0, f0 = 0, f1 = 0, f2 = 0;
         const double xs[3] = {pi.x, pi.y, pi.z};
         bool full[3];
-        for_each_cell(a.grid, xs, h, full, [&](int cx, int cy, int cz, int beg, int end) {
+        for_each_cell(G.grid, xs, h, full, [&](int cx, int cy, int cz, int beg, int end) {
             const bool exact_geom = full[0] || full[1] || full[2];
-            const float sh0 = (float)(ci[0] - cx) * a.f32.s[0];
-            const float sh1 = (float)(ci[1] - cy) * a.f32.s[1];
-            const float sh2 = (float)(ci[2] - cz) * a.f32.s[2];
+            const float sh0 = (float)(ci[0] - cx) * G.s_f[0];
+            const float sh1 = (float)(ci[1] - cy) * G.s_f[1];
+            const float sh2 = (float)(ci[2] - cz) * G.s_f[2];
             for (int k = beg; k < end; ++k) {
                 const float4 oj = a.f32.offm[k];
                 float dp[3];
@@ -121,11 +124,11 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
                     in = decide(r2, bh);
                     if (in < 0) {
                         double d64[3];
-                        in = exact_r2(a.box, pi, a.posm[k], d64) < h2;
+                        in = exact_r2(G.box, pi, a.posm[k], d64) < h2;
                     }
                 } else {
                     double d64[3];
-                    const double r2e = exact_r2(a.box, pi, a.posm[k], d64);
+                    const double r2e = exact_r2(G.box, pi, a.posm[k], d64);
                     in = r2e < h2;
                     dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
                     r2 = (float)r2e;
@@ -172,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
         for (int k = 0; k < 6; k++) a.nvt[6 * (size_t)i + k] = R.nvt[k];
         a.redo[t] = (res == 1) ? 1 : 0;
         if (res < 0) atomicMax(&a.ctr->status, (int)6);
-    }
+    });
     wave_add_u64(&a.ctr->interactions, cnt);
 }
 
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void gradients_f32_kernel(GradientArgs a)
     const bool valid = t < a.n_active;
     unsigned long long cnt = 0;
     int listed = 0;
-    if (valid) {
+    for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = a.active[t];
         const double4 pi = a.posm[i];
         const GradRecF ri = a.recf[i];
@@ -220,13 +223,13 @@ __global__ __launch_bounds__(kBlock) void gradients_f32_kernel(GradientArgs a)
             const float pos_min = (float)(h2_i * kBand);  // r2 below this: re-test r2 > 0 in FP64
             const float rho_f = ri.vr.w, p_f = ri.pm.x;
             const double xs[3] = {pi.x, pi.y, pi.z};
-            const double R = (h_i > a.hmax) ? h_i : a.hmax;
+            const double R = (h_i > G.hmax) ? h_i : G.hmax;
             bool full[3];
-            for_each_cell(a.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
+            for_each_cell(G.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
                 const bool exact_geom = full[0] || full[1] || full[2];
-                const float sh0 = (float)(ci[0] - cx) * a.f32.s[0];
-                const float sh1 = (float)(ci[1] - cy) * a.f32.s[1];
-                const float sh2 = (float)(ci[2] - cz) * a.f32.s[2];
+                const float sh0 = (float)(ci[0] - cx) * G.s_f[0];
+                const float sh1 = (float)(ci[1] - cy) * G.s_f[1];
+                const float sh2 = (float)(ci[2] - cz) * G.s_f[2];
                 for (int k = beg; k < end; ++k) {
                     if (k == i) continue;  // r2 = 0 exactly
                     const float4 xj = a.recf[k].xh;
@@ -245,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void gradients_f32_kernel(GradientArgs a)
                         pos = (r2 >= pos_min) ? 1 : -1;
                         if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
                             double d64[3];
-                            const double r2e = exact_r2(a.box, pi, a.posm[k], d64);
+                            const double r2e = exact_r2(G.box, pi, a.posm[k], d64);
                             const double h_j = a.hsml[k];
                             in_i = r2e < h2_i;
                             in_j = r2e < h_j * h_j;
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void gradients_f32_kernel(GradientArgs a)
                         }
                     } else {
                         double d64[3];
-                        const double r2e = exact_r2(a.box, pi, a.posm[k], d64);
+                        const double r2e = exact_r2(G.box, pi, a.posm[k], d64);
                         const double h_j = a.hsml[k];
                         in_i = r2e < h2_i;
                         in_j = r2e < h_j * h_j;
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void gradients_f32_kernel(GradientArgs a)
         o[9] = gv1[0]; o[10] = gv1[1]; o[11] = gv1[2];
         o[12] = gv2[0]; o[13] = gv2[1]; o[14] = gv2[2];
         if (a.list.idx) a.list.count[t] = listed;
-    }
+    });
     if (a.list.idx) {
         const bool over = __any(listed > a.list.cap);
         if ((threadIdx.x & 63) == 0 && t < a.n_active) {
@@ -399,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
         if (valid && !a.list.overflow[t >> 6]) valid = false;
     }
     unsigned long long cnt = 0, nretry = 0, nexact = 0;
-    if (valid) {
+    for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = a.active[t];
         const ForceRecF ri = a.recf[i];
         const double4 pi = a.posm[i];
@@ -434,6 +437,12 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
                 }
             };
             if (USE_LIST) {
+                // pair displacements from cell differences unless the search
+                // radius spans half the box on some axis
+                bool exact_geom = false;
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++)
+                    if (!(G.hmax * G.grid.inv_s[ax] < 0.5 * G.grid.nc[ax] - 2.0)) exact_geom = true;
                 const int n = a.list.count[t];
                 const int *row = a.list.idx + (size_t)t * a.list.cap;
                 for (int k = 0; k < n; k++) {
@@ -443,15 +452,15 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
                     unpack_side_f(Jr, J);
                     const float4 xj = Jr.xh;
                     float dp[3];
-                    if (!a.exact_geom) {
+                    if (!exact_geom) {
                         int cj[3];
                         unpack_cell(__float_as_uint(Jr.g3.w), cj);
-                        dp[0] = (float)wrap_cell(ci[0] - cj[0], a.grid.nc[0]) * a.f32.s[0] + (ri.xh.x - xj.x);
-                        dp[1] = (float)wrap_cell(ci[1] - cj[1], a.grid.nc[1]) * a.f32.s[1] + (ri.xh.y - xj.y);
-                        dp[2] = (float)wrap_cell(ci[2] - cj[2], a.grid.nc[2]) * a.f32.s[2] + (ri.xh.z - xj.z);
+                        dp[0] = (float)wrap_cell(ci[0] - cj[0], G.grid.nc[0]) * G.s_f[0] + (ri.xh.x - xj.x);
+                        dp[1] = (float)wrap_cell(ci[1] - cj[1], G.grid.nc[1]) * G.s_f[1] + (ri.xh.y - xj.y);
+                        dp[2] = (float)wrap_cell(ci[2] - cj[2], G.grid.nc[2]) * G.s_f[2] + (ri.xh.z - xj.z);
                     } else {
                         double d64[3];
-                        exact_r2(a.box, pi, a.posm[j], d64);
+                        exact_r2(G.box, pi, a.posm[j], d64);
                         dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
                     }
                     const float r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
@@ -463,13 +472,13 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
                 const Band bi = band_of(h2_i);
                 const float pos_min = (float)(h2_i * kBand);
                 const double xs[3] = {pi.x, pi.y, pi.z};
-                const double R = (h_i > a.hmax) ? h_i : a.hmax;
+                const double R = (h_i > G.hmax) ? h_i : G.hmax;
                 bool full[3];
-                for_each_cell(a.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
+                for_each_cell(G.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
                     const bool exact_geom = full[0] || full[1] || full[2];
-                    const float sh0 = (float)(ci[0] - cx) * a.f32.s[0];
-                    const float sh1 = (float)(ci[1] - cy) * a.f32.s[1];
-                    const float sh2 = (float)(ci[2] - cz) * a.f32.s[2];
+                    const float sh0 = (float)(ci[0] - cx) * G.s_f[0];
+                    const float sh1 = (float)(ci[1] - cy) * G.s_f[1];
+                    const float sh2 = (float)(ci[2] - cz) * G.s_f[2];
                     for (int j = beg; j < end; ++j) {
                         if (j == i) continue;
                         const float4 xj = a.recf[j].xh;
@@ -488,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
                             pos = (r2 >= pos_min) ? 1 : -1;
                             if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
                                 double d64[3];
-                                const double r2e = exact_r2(a.box, pi, a.posm[j], d64);
+                                const double r2e = exact_r2(G.box, pi, a.posm[j], d64);
                                 const double h_j = a.hsml[j];
                                 in_i = r2e < h2_i;
                                 in_j = r2e < h_j * h_j;
@@ -496,7 +505,7 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
                             }
                         } else {
                             double d64[3];
-                            const double r2e = exact_r2(a.box, pi, a.posm[j], d64);
+                            const double r2e = exact_r2(G.box, pi, a.posm[j], d64);
                             const double h_j = a.hsml[j];
                             in_i = r2e < h2_i;
                             in_j = r2e < h_j * h_j;
@@ -534,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
         a.dtu[i] = dte;
         a.vsig[i] = vmax;
         if (status) atomicMax(&a.ctr->status, status);
-    }
+    });
     wave_add_u64(&a.ctr->interactions, cnt);
     wave_add_u64(&a.ctr->retries, nretry);
     wave_add_u64(&a.ctr->exact_calls, nexact);
@@ -544,10 +553,12 @@ inline unsigned nblocks(long long n) { return (unsigned)((n + kBlock - 1) / kBlo
 
 }  // namespace
 
-void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const double4 *velu, Grid g, float4 *offm,
-                          float4 *velf, uint32_t *cellc)
+void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const double4 *velu, const BoxDesc *boxes,
+                          const int *box_of, float4 *offm, float4 *velf, uint32_t *cellc)
 {
-    if (n > 0) hipLaunchKernelGGL(pack_geom_f32_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, posm, velu, g, offm, velf, cellc);
+    if (n > 0)
+        hipLaunchKernelGGL(pack_geom_f32_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, posm, velu, boxes, box_of, offm,
+                           velf, cellc);
 }
 void launch_density_sweep_f32(hipStream_t s, const DensitySweepArgs &a)
 {

@@ -23,7 +23,7 @@ _u8p = C.POINTER(C.c_uint8)
 EXPORTED_SYMBOLS = (
     "gzm_status_string", "gzm_last_error", "gzm_ctx_create", "gzm_ctx_destroy", "gzm_upload",
     "gzm_density", "gzm_gradients", "gzm_hydro_force", "gzm_download", "gzm_restore_initial_state",
-    "gzm_neighbor_counts", "gzm_timing_get", "gzm_timing_reset",
+    "gzm_neighbor_counts", "gzm_timing_get", "gzm_timing_reset", "gzm_upload_batch", "gzm_run_passes",
 )
 
 
@@ -75,6 +75,8 @@ class Timing(C.Structure):
         ("gradients_ms", C.c_double), ("gradients_launches", C.c_int64),
         ("force_ms", C.c_double), ("force_launches", C.c_int64),
         ("other_ms", C.c_double), ("other_launches", C.c_int64),
+        ("graph_ms", C.c_double), ("graph_launches", C.c_int64),
+        ("graph_passes", C.c_int64),
     ]
 
 
@@ -109,6 +111,9 @@ def load_library():
     L.gzm_neighbor_counts.argtypes = [C.c_void_p, C.c_int32, _ip, C.c_int64, _ip, C.POINTER(C.c_int64)]
     L.gzm_timing_get.argtypes = [C.c_void_p, C.POINTER(Timing)]
     L.gzm_timing_reset.argtypes = [C.c_void_p]
+    L.gzm_upload_batch.argtypes = [C.c_void_p, C.POINTER(HostView), C.c_int32, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int64)]
+    L.gzm_run_passes.argtypes = [C.c_void_p, C.c_int32, C.POINTER(PassStats)]
     for name in EXPORTED_SYMBOLS:
         f = getattr(L, name)
         if name not in ("gzm_status_string", "gzm_last_error"):
@@ -179,6 +184,8 @@ class MFMContext:
             pass
 
     def upload(self, ic: dict):
+        """One periodic box (gzm_upload); with `box_offsets` and `box_sizes` in
+        `ic`, many independent boxes (gzm_upload_batch)."""
         N = ic["pos"].shape[0]
         self.n = N
         keep = {}
@@ -203,7 +210,13 @@ class MFMContext:
         for name in ("cn", "vsig", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "fce",
                      "grad_rho", "grad_p", "grad_v", "acc", "dtu"):
             setattr(v, name, _p(arr(name)))
-        _check(self.L.gzm_upload(self.ctx, C.byref(v)), "gzm_upload")
+        if ic.get("box_offsets") is not None:
+            offs = np.ascontiguousarray(ic["box_offsets"], dtype=np.int64)
+            sizes = np.ascontiguousarray(ic["box_sizes"], dtype=np.float64).reshape(-1, 3)
+            _check(self.L.gzm_upload_batch(self.ctx, C.byref(v), sizes.shape[0], _p(sizes),
+                                           offs.ctypes.data_as(C.POINTER(C.c_int64))), "gzm_upload_batch")
+        else:
+            _check(self.L.gzm_upload(self.ctx, C.byref(v)), "gzm_upload")
 
     @staticmethod
     def _active(active):
@@ -232,6 +245,12 @@ class MFMContext:
 
     def restore_initial_state(self):
         _check(self.L.gzm_restore_initial_state(self.ctx), "gzm_restore_initial_state")
+
+    def run_passes(self, n_passes: int) -> PassStats:
+        """n_passes full passes captured in one HIP graph (stats summed over them)."""
+        st = PassStats()
+        _check(self.L.gzm_run_passes(self.ctx, int(n_passes), C.byref(st)), "gzm_run_passes")
+        return st
 
     def download(self) -> dict:
         N = self.n
@@ -269,7 +288,7 @@ class MFMContext:
 
 def run_pass(ic: dict, active=None, params: Params | None = None, stages=("density", "gradients", "force")):
     """Upload `ic`, run the requested stages, return (outputs, stats dict)."""
-    params = params or make_params(box=tuple(ic["box"]))
+    params = params or make_params(box=tuple(ic["box"]) if "box" in ic else (1.0, 1.0, 1.0))
     with MFMContext(params) as ctx:
         ctx.upload(ic)
         stats = {}

@@ -11,9 +11,19 @@ Configurations (sizes as used by the benchmark and the parity tests):
                     deposited kernel-weighted into the 64 central particles
                     (the 128^3 plain FP64 GPU case at n = 128)
   perfect_lattice   unjittered lattice with a uniform state (invariant tests)
+  shear_layers      two lattices of equal-mass particles, density ~2 inside
+                    |y - 0.5| < 0.25 and 1 outside, vx = +-0.5, P = 2.5
+                    (the 16.8M mixed-precision case at n_out = 224)
+  batch_boxes       many independent jittered-lattice boxes of log-uniform
+                    particle counts (the batched case: 512 boxes, N_b in
+                    [16^3, 64^3])
+  plummer_clusters  1000 Plummer spheres over a 20% uniform background, about
+                    four decades of density (the clustered case at 2^23)
 
 The initial smoothing length is the radius that encloses DesNumNgb particles
-at the mean number density, h0 = (3 N_des / (4 pi n))^(1/3).
+at the local number density, h0 = (3 N_des / (4 pi n))^(1/3): the mean density
+of the box (or of the layer) for the lattice cases, the distance to the 32nd
+nearest neighbour for the clustered case.
 """
 from __future__ import annotations
 
@@ -143,9 +153,144 @@ def random_state(n: int = 12, seed: int = 7, sigma_v: float = 0.3, jitter: float
     return _state(pos, vel, mass, u, hsml, f"random_state_{n}")
 
 
+def _lattice_slab(nx: int, ny: int, nz: int, y0: float, ly: float) -> np.ndarray:
+    """nx * ny * nz lattice filling x, z in [0, 1) and y in [y0, y0 + ly) (wrapped)."""
+    gx = (np.arange(nx, dtype=np.float64) + 0.5) / nx
+    gy = y0 + (np.arange(ny, dtype=np.float64) + 0.5) * (ly / ny)
+    gz = (np.arange(nz, dtype=np.float64) + 0.5) / nz
+    x, y, z = np.meshgrid(gx, gy, gz, indexing="ij")
+    pos = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    pos = np.mod(pos, 1.0)
+    pos[pos >= 1.0] -= 1.0
+    return pos
+
+
+def shear_layers(n_out: int = 224, n_in: int | None = None, pressure: float = 2.5, gamma: float = 5.0 / 3.0,
+                 v_shear: float = 0.5, v_pert: float = 0.01) -> dict:
+    """Dense band |y - 0.5| < 0.25 on an n_in x n_in/2 x n_in lattice, the rest
+    on n_out x n_out/2 x n_out, equal masses with rho_out = 1 (so rho_in =
+    (n_in / n_out)^3 ~ 2), P = 2.5, vx = +0.5 inside and -0.5 outside,
+    vy = 0.01 sin(4 pi x).  n_out = 224, n_in = 282 gives 16,832,596 particles."""
+    if n_in is None:
+        n_in = 2 * int(round(n_out * 2.0 ** (1.0 / 3.0) / 2.0))
+    inner = _lattice_slab(n_in, n_in // 2, n_in, 0.25, 0.5)
+    outer = _lattice_slab(n_out, n_out // 2, n_out, 0.75, 0.5)
+    pos = np.ascontiguousarray(np.concatenate([inner, outer]))
+    n1, n2 = inner.shape[0], outer.shape[0]
+    N = n1 + n2
+    m = 0.5 / n2  # rho_out = n2 m / 0.5 = 1
+    rho = np.concatenate([np.full(n1, n1 * m / 0.5), np.full(n2, 1.0)])
+    vel = np.zeros((N, 3))
+    vel[:n1, 0] = v_shear
+    vel[n1:, 0] = -v_shear
+    vel[:, 1] = v_pert * np.sin(4.0 * np.pi * pos[:, 0])
+    u = pressure / ((gamma - 1.0) * rho)
+    hsml = np.concatenate([np.full(n1, initial_hsml(n1, 0.5)), np.full(n2, initial_hsml(n2, 0.5))])
+    return _state(pos, vel, np.full(N, m), u, hsml, f"shear_layers_{n_out}")
+
+
+def batch_boxes(n_boxes: int = 512, seed: int = 4, n_min: int = 4096, n_max: int = 262144,
+                jitter: float = 0.1) -> dict:
+    """Independent unit boxes, box b a jittered n_b^3 lattice with
+    n_b = round(exp(U(ln n_min, ln n_max))^(1/3)), m = 1 / N_b, Gaussian
+    velocities with sigma ~ U(0.1, 1.0) and one u ~ log-uniform in [0.1, 10]
+    per box.  Rows of box b: [box_offsets[b], box_offsets[b + 1])."""
+    s = SEED_BASE + seed
+    ub = uniform(s, 0, n_boxes)
+    n_side = np.rint(np.exp(np.log(n_min) + ub * (np.log(n_max) - np.log(n_min))) ** (1.0 / 3.0)).astype(np.int64)
+    sig = 0.1 + 0.9 * uniform(s, 1, n_boxes)
+    ubox = np.exp(np.log(0.1) + uniform(s, 2, n_boxes) * (np.log(10.0) - np.log(0.1)))
+    pos, vel, mass, u, hsml = [], [], [], [], []
+    offsets = [0]
+    for b in range(n_boxes):
+        n = int(n_side[b])
+        p = jittered_lattice(n, jitter, s + 1000 + b, 0)
+        Nb = p.shape[0]
+        pos.append(p)
+        vel.append(sig[b] * gaussian(s + 1000 + b, 1, 3 * Nb).reshape(Nb, 3))
+        mass.append(np.full(Nb, 1.0 / Nb))
+        u.append(np.full(Nb, ubox[b]))
+        hsml.append(np.full(Nb, initial_hsml(Nb)))
+        offsets.append(offsets[-1] + Nb)
+    out = _state(np.concatenate(pos), np.concatenate(vel), np.concatenate(mass), np.concatenate(u),
+                 np.concatenate(hsml), f"batch_boxes_{n_boxes}")
+    out["box_offsets"] = np.asarray(offsets, dtype=np.int64)
+    out["box_sizes"] = np.ones((n_boxes, 3))
+    return out
+
+
+def box_slice(ic: dict, b: int) -> dict:
+    """Box b of a batch_boxes set as a single-box particle set."""
+    o = ic["box_offsets"]
+    sl = slice(int(o[b]), int(o[b + 1]))
+    out = {k: v[sl] for k, v in ic.items()
+           if isinstance(v, np.ndarray) and v.shape[:1] == ic["pos"].shape[:1]}
+    out["box"] = np.asarray(ic["box_sizes"][b], dtype=np.float64)
+    out["name"] = f"{ic.get('name', 'batch')}[{b}]"
+    return out
+
+
+def local_hsml(pos: np.ndarray, des_num_ngb: int = 32, workers: int = -1) -> np.ndarray:
+    """Distance to the des_num_ngb-th nearest other particle (periodic unit box)."""
+    from scipy.spatial import cKDTree
+    tree = cKDTree(pos, boxsize=1.0)
+    d, _ = tree.query(pos, k=des_num_ngb + 1, workers=workers)
+    return np.ascontiguousarray(d[:, des_num_ngb])
+
+
+def plummer_clusters(n_total: int = 8388608, n_spheres: int = 1000, seed: int = 5, f_background: float = 0.2,
+                     slope: float = -1.8, m_range=(1.0, 1000.0), a_range=(1e-3, 3e-2), sigma_v: float = 0.2,
+                     u0: float = 1.0) -> dict:
+    """f_background of the particles uniform in the box, the rest in Plummer
+    spheres with centres U(0,1)^3, masses dN/dM ~ M^slope on m_range
+    (particle counts proportional to mass), scale radii log-uniform in
+    a_range, radii truncated at 10 a; equal particle masses 1/N, Gaussian
+    velocities, u = u0.  hsml starts at the 32nd-neighbour distance."""
+    s = SEED_BASE + seed
+    n_bg = int(round(f_background * n_total))
+    n_cl = n_total - n_bg
+    # sphere masses by inverse CDF of the power law
+    q = 1.0 + slope
+    lo, hi = m_range[0] ** q, m_range[1] ** q
+    M = (lo + uniform(s, 0, n_spheres) * (hi - lo)) ** (1.0 / q)
+    counts = np.floor(n_cl * M / M.sum()).astype(np.int64)
+    rest = n_cl - counts.sum()
+    order = np.argsort(-(n_cl * M / M.sum() - counts), kind="stable")
+    counts[order[:rest]] += 1
+    centres = uniform(s, 1, 3 * n_spheres).reshape(n_spheres, 3)
+    a = np.exp(np.log(a_range[0]) + uniform(s, 2, n_spheres) * (np.log(a_range[1]) - np.log(a_range[0])))
+    sph = np.repeat(np.arange(n_spheres), counts)
+    x_max = 1000.0 / 101.0 ** 1.5  # enclosed-mass fraction at r = 10 a
+    X = uniform(s, 3, n_cl) * x_max
+    X = np.where(X <= 0.0, 1e-300, X)
+    r = a[sph] / np.sqrt(X ** (-2.0 / 3.0) - 1.0)
+    cos_t = 2.0 * uniform(s, 4, n_cl) - 1.0
+    phi = 2.0 * np.pi * uniform(s, 5, n_cl)
+    sin_t = np.sqrt(np.maximum(0.0, 1.0 - cos_t * cos_t))
+    d = np.stack([r * sin_t * np.cos(phi), r * sin_t * np.sin(phi), r * cos_t], axis=1)
+    pos = np.concatenate([uniform(s, 6, 3 * n_bg).reshape(n_bg, 3), centres[sph] + d])
+    pos = np.mod(pos, 1.0)
+    pos[pos >= 1.0] -= 1.0
+    pos = np.ascontiguousarray(pos)
+    N = pos.shape[0]
+    vel = sigma_v * gaussian(s, 7, 3 * N).reshape(N, 3)
+    hsml = local_hsml(pos)
+    return _state(pos, vel, np.full(N, 1.0 / N), np.full(N, u0), hsml, f"plummer_clusters_{N}")
+
+
+def random_subset(n: int, fraction: float = 0.1, seed: int = 6) -> np.ndarray:
+    """Sorted random subset of round(fraction * n) indices (the active set)."""
+    k = int(round(fraction * n))
+    keys = uniform(SEED_BASE + seed, 0, n)
+    return np.sort(np.argsort(keys, kind="stable")[:k]).astype(np.int32)
+
+
 CONFIGS = {
     "lattice_gaussian": lattice_gaussian,
     "blast": blast,
     "perfect_lattice": perfect_lattice,
     "random_state": random_state,
+    "shear_layers": shear_layers,
+    "batch_boxes": batch_boxes,
+    "plummer_clusters": plummer_clusters,
 }

@@ -125,6 +125,8 @@ typedef struct gzm_timing {
     double gradients_ms; int64_t gradients_launches;
     double force_ms;     int64_t force_launches;
     double other_ms;     int64_t other_launches;   /* grid build, packing, finalize */
+    double graph_ms;     int64_t graph_launches;   /* gzm_run_passes graphs, whole-graph time */
+    int64_t graph_passes;                          /* passes inside those graphs */
 } gzm_timing;
 
 typedef struct gzm_ctx gzm_ctx;
@@ -140,6 +142,15 @@ gzm_status gzm_ctx_destroy(gzm_ctx *ctx);
  * (the role of domain_Decomposition + force_treebuild, domain.c:106,
  * gravity/forcetree.c:109, for this rank). */
 gzm_status gzm_upload(gzm_ctx *ctx, const gzm_host_view *view);
+
+/* Many independent periodic boxes in one context (batched pass; no
+ * counterpart in the reference, which runs one rank's particle set): box b
+ * holds the caller rows [box_offsets[b], box_offsets[b+1]) of `view` and has
+ * side lengths box_sizes[3b..3b+2]; gzm_params.box is not used.  Every stage
+ * call then runs over all boxes at once, each particle seeing only its own
+ * box, with the results of running each box alone. */
+gzm_status gzm_upload_batch(gzm_ctx *ctx, const gzm_host_view *view, int32_t n_boxes, const double *box_sizes,
+                            const int64_t *box_offsets);
 
 /* density(): hsml iteration to convergence over the active list, then the
  * final density, pressure and NumNgb^(1/3) (density.c:451-1044). */
@@ -159,6 +170,17 @@ gzm_status gzm_download(gzm_ctx *ctx, gzm_host_view *view);
 /* Restore hsml, cn and vsig to the values given at upload (repeated passes
  * over the same input, e.g. for timing). Device-side copy only. */
 gzm_status gzm_restore_initial_state(gzm_ctx *ctx);
+
+/* n_passes full passes over the uploaded state (restore initial state,
+ * density, gradients, force; all particles active) enqueued into one HIP
+ * graph and launched once.  One eager probe pass first fixes what the graph
+ * cannot decide on the host (number of density sweeps, pair-list overflow);
+ * the graph replays the same input, so every pass repeats the probe's
+ * decisions.  stats hold the sums over the n_passes passes (density_sweeps:
+ * per pass); gzm_timing's graph fields hold the graph's device time.  The
+ * outputs left on the device are those of the last pass.  No counterpart in
+ * the reference: for callers that repeat passes without host round trips. */
+gzm_status gzm_run_passes(gzm_ctx *ctx, int32_t n_passes, gzm_pass_stats *stats);
 
 /* Neighbour-set sizes for the current hsml, caller order of `active`:
  * mode 0 density set {j : r2 < h_i^2}, mode 1 pair set

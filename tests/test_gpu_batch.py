@@ -1,0 +1,89 @@
+"""Batched boxes (gzm_upload_batch) and graph-captured passes (gzm_run_passes).
+
+A particle of a batch sees only its own box, so every output must equal, bit
+for bit, the output of running its box alone; a pass replayed from a HIP
+graph must equal the same pass run stage by stage.  The batch is also held
+to the FP64 oracle run box by box.
+"""
+import numpy as np
+import pytest
+
+import gzm
+import ics
+import oracle as O
+from parity import check_pass
+
+pytestmark = pytest.mark.gpu
+
+OUT_KEYS = ("hsml", "density", "pressure", "num_ngb", "dhsml", "divvel", "cn", "fce", "nvt",
+            "grad_rho", "grad_p", "grad_v", "acc", "dtu", "vsig")
+
+
+def small_batch():
+    return ics.batch_boxes(8, n_min=512, n_max=4096)
+
+
+def run_eager(ctx):
+    ctx.restore_initial_state()
+    s1 = ctx.density()
+    s2 = ctx.gradients()
+    s3 = ctx.hydro_force()
+    return ctx.download(), (s1, s2, s3)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_equals_boxes_run_alone(precision):
+    ic = small_batch()
+    out, st = gzm.run_pass(ic, params=gzm.make_params(precision=precision))
+    offs = ic["box_offsets"]
+    tot = {"density_interactions": 0, "gradient_interactions": 0, "force_interactions": 0}
+    sweeps = 0
+    for b in range(len(offs) - 1):
+        one = ics.box_slice(ic, b)
+        o1, s1 = gzm.run_pass(one, params=gzm.make_params(box=tuple(one["box"]), precision=precision))
+        for k in tot:
+            tot[k] += s1[k]
+        sweeps = max(sweeps, s1["density_sweeps"])
+        sl = slice(int(offs[b]), int(offs[b + 1]))
+        for k in OUT_KEYS:
+            np.testing.assert_array_equal(out[k][sl], o1[k], err_msg=f"box {b} {k}")
+    for k in tot:
+        assert st[k] == tot[k], k
+    assert st["density_sweeps"] == sweeps
+
+
+def test_batch_matches_oracle():
+    ic = small_batch()
+    out, st = gzm.run_pass(ic)
+    offs = ic["box_offsets"]
+    for b in range(len(offs) - 1):
+        one = ics.box_slice(ic, b)
+        ref, rst, _ = O.run_pass(one)
+        sl = slice(int(offs[b]), int(offs[b + 1]))
+        sub = {k: out[k][sl] for k in OUT_KEYS}
+        errs, bad = check_pass(sub, ref)
+        assert not bad, (b, bad)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("batched", [False, True])
+def test_graph_passes_equal_eager_pass(precision, batched):
+    ic = small_batch() if batched else ics.random_state(14)
+    params = gzm.make_params(precision=precision, box=(1.0, 1.0, 1.0))
+    with gzm.MFMContext(params) as ctx:
+        ctx.upload(ic)
+        ref, (s1, s2, s3) = run_eager(ctx)
+        for rep in range(3):  # replays with eager work between them
+            st = ctx.run_passes(3)
+            assert st.density_sweeps == s1.density_sweeps
+            assert st.density_interactions == 3 * s1.density_interactions
+            assert st.gradient_interactions == 3 * s2.gradient_interactions
+            assert st.force_interactions == 3 * s3.force_interactions
+            out = ctx.download()
+            for k in OUT_KEYS:
+                np.testing.assert_array_equal(out[k], ref[k], err_msg=f"replay {rep} {k}")
+            again, _ = run_eager(ctx)
+            for k in ("hsml", "acc", "dtu"):
+                np.testing.assert_array_equal(again[k], ref[k])
+        t = ctx.timing()
+        assert t["graph_launches"] == 3 and t["graph_passes"] == 9 and t["graph_ms"] > 0

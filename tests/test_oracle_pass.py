@@ -161,3 +161,45 @@ def test_maxiter_failure_is_reported():
     ic["hsml"] = ic["hsml"] * 3.0
     with pytest.raises(RuntimeError, match="1155"):
         O.run_pass(ic, params=O.make_params(maxiter=1), stages=("density",))
+
+
+def test_generators_of_the_larger_configs():
+    """Shapes, masses and densities of the shear-layer, batched and clustered sets."""
+    c = ics.shear_layers(32)
+    n_in, n_out = 40 * 20 * 40, 32 * 16 * 32
+    assert c["pos"].shape == (n_in + n_out, 3)
+    m = c["mass"][0]
+    assert np.all(c["mass"] == m)
+    assert abs(n_out * m / 0.5 - 1.0) < 1e-12                     # rho_out = 1
+    inside = np.abs(c["pos"][:, 1] - 0.5) < 0.25
+    assert inside.sum() == n_in
+    assert np.all(c["vel"][inside, 0] == 0.5) and np.all(c["vel"][~inside, 0] == -0.5)
+    P = (2.0 / 3.0) * c["u"] * np.where(inside, n_in * m / 0.5, 1.0)
+    assert np.allclose(P, 2.5, rtol=1e-14)
+    assert ics.shear_layers.__defaults__[0] == 224                 # full size 16,832,596
+    assert 282 * 141 * 282 + 224 * 112 * 224 == 16832596
+
+    b = ics.batch_boxes(16, n_min=512, n_max=4096)
+    offs = b["box_offsets"]
+    assert offs[0] == 0 and offs[-1] == b["pos"].shape[0] and np.all(np.diff(offs) > 0)
+    for k in range(16):
+        sl = slice(offs[k], offs[k + 1])
+        nb = offs[k + 1] - offs[k]
+        assert round(nb ** (1 / 3)) ** 3 == nb and 512 <= nb <= 4096
+        assert np.allclose(b["mass"][sl], 1.0 / nb) and np.ptp(b["u"][sl]) == 0
+        one = ics.box_slice(b, k)
+        assert one["pos"].shape[0] == nb and np.all(one["box"] == 1.0)
+    assert np.all((b["pos"] >= 0) & (b["pos"] < 1))
+
+    p = ics.plummer_clusters(4096, 20)
+    assert p["pos"].shape == (4096, 3) and np.all((p["pos"] >= 0) & (p["pos"] < 1))
+    # hsml = distance to the 32nd neighbour: exactly 32 others strictly closer or at it
+    i = 7
+    d = p["pos"] - p["pos"][i]
+    d -= np.rint(d)
+    r = np.sqrt((d * d).sum(1))
+    r = np.sort(r)[1:]
+    assert abs(r[31] - p["hsml"][i]) < 1e-12
+    assert p["hsml"].max() / p["hsml"].min() > 10
+    act = ics.random_subset(1000, 0.1)
+    assert act.size == 100 and np.all(np.diff(act) > 0)
