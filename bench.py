@@ -1,22 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X MFM hydro pass: particle-neighbour interactions/s.
 
-Workload (one GPU): 128^3 = 2,097,152 particles, jittered lattice in a periodic
-unit box, rho = 1, v = 0, u = 1e-6, total energy 1 deposited kernel-weighted
-into the 64 central particles (blast wave); gamma 5/3, 32 neighbours, every
-particle active.  One step = one full plain-FP64 MFM pass on data resident in
-HBM: hsml iteration from the stored initial guess (density), gradients, force.
+Default workload (one GPU), the batched configuration: 512 independent
+periodic unit boxes, box b a jittered n_b^3 lattice with n_b^3 log-uniform in
+[4096, 262144] (31,542,962 particles in all), Gaussian velocities with
+sigma ~ U(0.1, 1) and u ~ log-uniform [0.1, 10] per box, gamma 5/3, 32
+neighbours, every particle active.  One step = one full MFM pass over all
+boxes (hsml iteration from the stored initial guess, gradients, force) on
+data resident in HBM; the K timed steps are K passes captured in one HIP graph
+and launched once (gzm_run_passes), timed with HIP events around the launch.
+`value` is the FP32-bulk pass (precision 1: FP32 pair arithmetic; FP64 hsml
+iteration, NV_T inversion, limiter, exact Riemann solver and flux sums); the
+plain FP64 pass on the same boxes is reported beside it, and the max / rms
+error of every FP32 output against the FP64 oracle run on all boxes.
 
 Interactions per pass I = sum over density sweeps of |{j : r_ij < h_i}| +
 |pair set| of the gradient loop + |pair set| of the force loop, counted by the
-kernels (equal to the CPU oracle's count: the neighbour sets are bit-exact).
+kernels (for FP64 equal to the CPU oracle's count: the neighbour sets are
+bit-exact).
 
-Prints one JSON line.  Also reported: the roofline of the dominant kernel
-(algorithmic bytes / its mean HIP-event duration vs 8 TB/s, and the HBM bytes
-the committed counter summary measured for it), the FP32-bulk mode (precision
-1) on the same workload with the max / rms error of each output against the
-FP64 oracle, and the FP64 CPU oracle (OpenMP, this host's cores) timed on the
-same workload, whose outputs double as the parity check of the timed passes.
+Also reported: the roofline of the dominant kernel (algorithmic bytes over its
+mean HIP-event duration in the eager warm-up passes of the same workload, vs
+8 TB/s, and the HBM bytes the committed counter summary measured for it), the
+FP64 CPU oracle (OpenMP, this host's cores) timed on the same workload, and
+the 128^3 blast-wave case (--workload blast makes it the headline).
 """
 from __future__ import annotations
 
@@ -51,12 +58,15 @@ BYTES_PER_PARTICLE_F32 = {"density": 40 + 136, "gradients": 96 + 120, "force": 1
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None, help="timed passes (default: 100 batched, 10 blast)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=128, help="lattice side (default 128 -> 2,097,152 particles)")
+    ap.add_argument("--workload", choices=("batch", "blast"), default="batch")
+    ap.add_argument("--boxes", type=int, default=512, help="boxes of the batched workload")
+    ap.add_argument("--n", type=int, default=128, help="lattice side of the blast (default 128 -> 2,097,152 particles)")
     ap.add_argument("--cpu-repeats", type=int, default=3, help="oracle passes for the CPU baseline (0 = skip)")
     ap.add_argument("--cell-size", type=float, default=0.0)
-    ap.add_argument("--fp32", type=int, default=1, help="also time the FP32-bulk mode (precision 1) on the workload")
+    ap.add_argument("--fp32", type=int, default=1, help="blast workload: also time the FP32-bulk mode")
+    ap.add_argument("--blast", type=int, default=1, help="batch workload: also time the 128^3 blast case")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="instead of the pass, copy a buffer of this many MiB three times and exit: the "
                          "known-size reference that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -151,7 +161,8 @@ def measured_traffic(kernel, precision, n):
     """HBM bytes per launch of `kernel` from the committed counter summary of
     this workload (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known
     size), or None when no summary covers it."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_blast128.json")
+    name = "hbm_traffic_blast128.json" if n == 2097152 else "hbm_traffic_batch512.json"
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             tab = json.load(f)
@@ -161,6 +172,174 @@ def measured_traffic(kernel, precision, n):
     if rec is None or tab.get("n_particles") != n:
         return None
     return rec["fetch_bytes"] + rec["write_bytes"]
+
+
+def run_graph_mode(gzm, ic, args, precision, steps):
+    """Warm-up eager passes (per-kernel HIP events), then `steps` passes in one
+    HIP graph (timed with events around the launch)."""
+    N = ic["pos"].shape[0]
+    params = gzm.make_params(cell_size=args.cell_size, precision=precision)
+    ctx = gzm.MFMContext(params)
+    ctx.upload(ic)
+    ctx.timing_reset()
+    updates = []
+    for _ in range(max(args.warmup, 1)):
+        ctx.restore_initial_state()
+        s1 = ctx.density()
+        ctx.gradients()
+        ctx.hydro_force()
+        updates.append(s1.density_updates)
+    tw = ctx.timing()
+    w = max(args.warmup, 1)
+    ctx.timing_reset()
+    st = ctx.run_passes(steps)
+    tm = ctx.timing()
+    out = ctx.download()
+    ctx.close()
+    graph_s = tm["graph_ms"] * 1e-3
+    inter = st.density_interactions + st.gradient_interactions + st.force_interactions
+    kern = {
+        "density": (tw["density_ms"], tw["density_launches"], float(np.mean(updates))),
+        "gradients": (tw["gradients_ms"], tw["gradients_launches"], float(N)),
+        "force": (tw["force_ms"], tw["force_launches"], float(N)),
+    }
+    dom = max(kern, key=lambda k: kern[k][0])
+    tot_ms, launches, units = kern[dom]
+    mean_ms = tot_ms / max(launches, 1)
+    bpp = BYTES_PER_PARTICLE if precision == 0 else BYTES_PER_PARTICLE_F32
+    bytes_per_launch = bpp[dom] * units / max(launches / w, 1)
+    return {
+        "value": inter / graph_s,
+        "ms_per_step": 1e3 * graph_s / steps,
+        "interactions_per_pass": inter // steps,
+        "density_sweeps": int(st.density_sweeps),
+        "kernels_ms_per_pass_eager": {
+            "density_sweeps": tw["density_ms"] / w,
+            "gradients": tw["gradients_ms"] / w,
+            "force": tw["force_ms"] / w,
+            "other": tw["other_ms"] / w,
+        },
+        "dominant": dom,
+        "mean_launch_ms": mean_ms,
+        "bytes_per_launch": bytes_per_launch,
+        "achieved_gbs": bytes_per_launch / (mean_ms * 1e-3) / 1e9,
+        "out": out,
+        "density_interactions": st.density_interactions // steps,
+        "grad_interactions": st.gradient_interactions // steps,
+        "force_interactions": st.force_interactions // steps,
+    }
+
+
+def oracle_boxes(O, ic, cores):
+    """FP64 oracle pass box by box; returns concatenated outputs, interactions, seconds."""
+    import ics
+    offs = ic["box_offsets"]
+    outs, inter, t0 = [], 0, time.perf_counter()
+    prm = O.make_params(nthreads=cores)
+    for b in range(len(offs) - 1):
+        one = ics.box_slice(ic, b)
+        ref, st, _ = O.run_pass(one, params=prm)
+        outs.append(ref)
+        inter += st.dens_interactions + st.grad_interactions + st.force_interactions
+    secs = time.perf_counter() - t0
+    ref = {k: np.concatenate([o[k] for o in outs]) for k in outs[0] if isinstance(outs[0][k], np.ndarray)}
+    return ref, inter, secs
+
+
+def main_batch(args, gzm, ics):
+    steps = args.steps or 100
+    ic = ics.batch_boxes(args.boxes)
+    N = ic["pos"].shape[0]
+    nb = len(ic["box_offsets"]) - 1
+    r32 = run_graph_mode(gzm, ic, args, 1, steps)
+    r64 = run_graph_mode(gzm, ic, args, 0, steps)
+    traffic = measured_traffic(r32["dominant"], 1, N)
+    result = {
+        "metric": "particle-neighbor interactions/s, MFM pass; FP32-batched vs FP64 at stated error",
+        "value": r32["value"],
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{nb} periodic boxes, jittered lattices of 16^3..64^3 particles ({N:,} in all), "
+                        f"FP32-bulk MFM pass (density+gradients+force), {steps} passes in one HIP graph",
+            "n_particles": N,
+            "n_boxes": nb,
+            "active": "all",
+            "gamma": 5.0 / 3.0,
+            "des_num_ngb": 32,
+            "passes_per_graph": steps,
+            "interactions_per_pass": int(r32["interactions_per_pass"]),
+            "density_sweeps": r32["density_sweeps"],
+            "parallelism": "single GPU",
+        },
+        "fp64_same_workload": {
+            "value": r64["value"],
+            "unit": "interactions/s",
+            "ms_per_step": r64["ms_per_step"],
+            "interactions_per_pass": int(r64["interactions_per_pass"]),
+            "density_sweeps": r64["density_sweeps"],
+            "kernels_ms_per_pass_eager": r64["kernels_ms_per_pass_eager"],
+        },
+        "fp32_speedup_vs_fp64": r32["value"] / r64["value"],
+        "kernels_ms_per_pass_eager": r32["kernels_ms_per_pass_eager"],
+        "roofline": {
+            "bound": "hbm",
+            "kernel": r32["dominant"],
+            "achieved": r32["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": r32["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": r32["bytes_per_launch"],
+            "mean_launch_ms": r32["mean_launch_ms"],
+        },
+    }
+    if args.cpu_repeats > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle as O
+        from parity import compare_pass, error_stats
+
+        cores = cpu_cores()
+        ref, cpu_int, secs = oracle_boxes(O, ic, cores)
+        result["cpu_baseline"] = {
+            "value": cpu_int / secs,
+            "unit": "interactions/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"one full pass over all {nb} boxes ({N:,} particles), {secs:.1f} s",
+        }
+        es = error_stats(r32["out"], ref)
+        result["error_vs_fp64_oracle"] = {
+            k: (v if isinstance(v, int) else {q: (float(f"{x:.3e}") if isinstance(x, float) else x) for q, x in v.items()})
+            for k, v in es.items()}
+        errs = compare_pass(r64["out"], ref)
+        result["fp64_same_workload"]["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+        result["fp64_same_workload"]["interactions_equal"] = bool(cpu_int == r64["interactions_per_pass"])
+    if args.blast:
+        b_args = argparse.Namespace(**vars(args))
+        b_args.steps = 5
+        b_args.warmup = 1
+        blast_ic = ics.blast(args.n)
+        b64 = run_mode(gzm, blast_ic, b_args, 0)
+        b32 = run_mode(gzm, blast_ic, b_args, 1)
+        result["blast_128"] = {
+            "n_particles": blast_ic["pos"].shape[0],
+            "fp64": {"value": b64["value"], "ms_per_step": b64["ms_per_step"],
+                     "kernels_ms_per_pass": b64["kernels_ms_per_pass"]},
+            "fp32_bulk": {"value": b32["value"], "ms_per_step": b32["ms_per_step"],
+                          "kernels_ms_per_pass": b32["kernels_ms_per_pass"]},
+            "fp32_speedup_vs_fp64": b32["value"] / b64["value"],
+        }
+    print(json.dumps(result))
 
 
 def main():
@@ -177,6 +356,10 @@ def main():
     import gzm
     import ics
 
+    if args.workload == "batch":
+        main_batch(args, gzm, ics)
+        return
+    args.steps = args.steps or 10
     ic = ics.blast(args.n)
     N = ic["pos"].shape[0]
     r64 = run_mode(gzm, ic, args, 0)
@@ -264,7 +447,7 @@ def main():
         if r32 is not None:
             es = error_stats(r32["out"], ref)
             result["fp32_bulk"]["error_vs_fp64_oracle"] = {
-                k: (v if isinstance(v, int) else {"max": float(f"{v['max']:.3e}"), "rms": float(f"{v['rms']:.3e}")})
+                k: (v if isinstance(v, int) else {q: (float(f"{x:.3e}") if isinstance(x, float) else x) for q, x in v.items()})
                 for k, v in es.items()}
     print(json.dumps(result))
 

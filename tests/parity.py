@@ -88,9 +88,11 @@ def check_pass(out, ref, active=None, tol_density=TOL_DENSITY, tol_flux=TOL_FLUX
 
 
 def error_stats(out, ref, active=None):
-    """Max and rms of the per-particle error of each output against the
-    oracle, with the normalisations of compare_pass; du/dt split into the
-    particles whose entropic-switch branch is not / is a P/rho tie."""
+    """Max, rms, 99.9th and 99.99th percentile of the per-particle error of
+    each output against the oracle, with the normalisations of compare_pass,
+    and the number of particles above the FP32-bulk tolerance (n_over); du/dt
+    split into the particles whose entropic-switch branch is not / is a P/rho
+    tie."""
     idx = slice(None) if active is None else np.asarray(active)
     sel = lambda d, k: np.asarray(d[k])[idx]
     errs = {}
@@ -105,10 +107,18 @@ def error_stats(out, ref, active=None):
     tie = sel(ref, "tie_flag").astype(bool) if "tie_flag" in ref else np.zeros(len(sel(ref, "dtu")), bool)
     errs["dtu"] = l1_errs(sel(out, "dtu")[~tie], sel(ref, "dtu")[~tie], sel(ref, "dtu_l1")[~tie])
     errs["dtu_tied"] = l1_errs(sel(out, "dtu")[tie], sel(ref, "dtu")[tie], sel(ref, "dtu_l1")[tie])
+    tol = {"hsml": TOL32_DENSITY, "density": TOL32_DENSITY, "pressure": TOL32_DENSITY,
+           "acc": TOL32_FLUX, "dtu": TOL32_FLUX}
     stats = {}
     for k, e in errs.items():
-        e = np.asarray(e, dtype=np.float64).ravel()
-        stats[k] = {"max": float(e.max()) if e.size else 0.0,
-                    "rms": float(np.sqrt(np.mean(e * e))) if e.size else 0.0}
+        e = np.asarray(e, dtype=np.float64)
+        per = e.reshape(e.shape[0], -1).max(axis=1) if e.size else e  # worst component per particle
+        stats[k] = {"max": float(per.max()) if per.size else 0.0,
+                    "rms": float(np.sqrt(np.mean(per * per))) if per.size else 0.0,
+                    "p99.9": float(np.quantile(per, 0.999)) if per.size else 0.0,
+                    "p99.99": float(np.quantile(per, 0.9999)) if per.size else 0.0}
+        if k in tol:
+            stats[k]["n_over"] = int((per > tol[k]).sum())
     stats["n_tied"] = int(tie.sum())
+    stats["n_particles"] = int(len(sel(ref, "hsml")))
     return stats
