@@ -136,6 +136,40 @@ __device__ __forceinline__ void for_box(const BoxDesc *__restrict__ boxes, const
     }
 }
 
+// hydro_final_operations_and_cleanup (hydro_toplevel.c:738-764) on the raw
+// sums of one particle: dE -= v . a before the mass division
+__device__ __forceinline__ void force_final_ops(double acc[3], double &dte, const double v[3], double m_i,
+                                                double h_i, double max_hsml)
+{
+    dte -= (v[0] / 1.0) * acc[0];
+    acc[0] /= m_i;
+    dte -= (v[1] / 1.0) * acc[1];
+    acc[1] /= m_i;
+    dte -= (v[2] / 1.0) * acc[2];
+    acc[2] /= m_i;
+    dte /= m_i;
+    if (h_i >= 0.99 * max_hsml) dte = 0;
+}
+
+// position of the r-th set bit of m (r < popcount(m))
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r)
+{
+    for (int k = 0; k < r; k++) m &= m - 1;
+    return __builtin_ctzll(m);
+}
+
+// last index t in [0, n) with offset[t] <= q (offset non-decreasing, offset[0] = 0)
+__device__ __forceinline__ int owner_of(const int *offset, int n, int q)
+{
+    int lo = 0, hi = n;  // answer in [lo, hi)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (offset[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // wavefront-wide sum, then one atomic per wavefront
 __device__ __forceinline__ void wave_add_u64(unsigned long long *dst, unsigned long long v)
 {

@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_select.hpp>
 
@@ -98,6 +99,13 @@ struct gzm_ctx {
     int n_active = 0;
     bool all_active = true;
     bool have_density = false, have_gradients = false;
+    // exact-solver pairs deferred out of the force loop
+    uint64_t *d_defer_mask = nullptr;
+    int *d_defer_count = nullptr, *d_defer_off = nullptr, *d_defer_total = nullptr;
+    uint8_t *d_defer_raw = nullptr;
+    double *d_defer_result = nullptr;
+    size_t defer_result_cap = 0;   // deferred pairs the result buffer holds
+    int64_t last_deferred = 0;     // deferred pairs of the last force call
     // pair list written by the gradient kernel, read by the force kernel
     int *d_list = nullptr, *d_list_count = nullptr;
     uint8_t *d_list_over = nullptr;
@@ -220,6 +228,9 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_offm); dev_free(c->d_velf); dev_free(c->d_cellc); dev_free(c->d_grecf); dev_free(c->d_frecf);
     dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
     dev_free(c->d_box_of);
+    dev_free(c->d_defer_mask); dev_free(c->d_defer_count); dev_free(c->d_defer_off); dev_free(c->d_defer_total);
+    dev_free(c->d_defer_raw); dev_free(c->d_defer_result);
+    c->defer_result_cap = 0;
     c->list_entries = 0;
     c->list_valid = false;
     c->n = 0;
@@ -323,6 +334,18 @@ F32Geom f32_geom(const gzm_ctx *c)
     g.velf = c->d_velf;
     g.cellc = c->d_cellc;
     return g;
+}
+
+// per-particle arrays of the deferred exact-solver pairs (sized for all particles)
+gzm_status ensure_defer(gzm_ctx *c)
+{
+    if (c->d_defer_mask) return GZM_OK;
+    gzm_status s = dev_alloc(&c->d_defer_mask, (size_t)c->n);
+    s = s ? s : dev_alloc(&c->d_defer_count, (size_t)c->n);
+    s = s ? s : dev_alloc(&c->d_defer_off, (size_t)c->n);
+    s = s ? s : dev_alloc(&c->d_defer_total, 1);
+    s = s ? s : dev_alloc(&c->d_defer_raw, (size_t)c->n);
+    return s;
 }
 
 // the gradient kernel writes a pair list sized for the active list
@@ -707,6 +730,7 @@ gzm_status restore_state(gzm_ctx *c)
 struct Plan {
     int sweeps;
     bool overflow;
+    int64_t deferred;  // exact-solver pairs deferred out of the force loop
 };
 
 gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
@@ -934,16 +958,75 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         a.list.overflow = c->d_list_over;
         a.list.cap = c->list_cap;
     }
+    // exact-solver pairs leave the list loop for a kernel of their own (lists of <= 64)
+    const bool three_phase = use_list && c->list_cap <= 64 && c->n_active > 0;
+    if (three_phase) {
+        s = ensure_defer(c);
+        if (s) return s;
+        a.defer.mask = c->d_defer_mask;
+        a.defer.count = c->d_defer_count;
+        a.defer.raw = c->d_defer_raw;
+        a.defer.offset = c->d_defer_off;
+        a.defer.total = c->d_defer_total;
+    }
     if (c->n_active > 0) {
         Timed tm(c, kForce);
         auto launch = (c->p.precision == 1) ? launch_force_f32 : launch_force;
         launch(c->stream, a);
         if (use_list && c->list_overflow_slices > 0) {
-            a.fallback_only = 1;  // slices whose list overflowed scan the grid
-            launch(c->stream, a);
+            ForceArgs b = a;
+            b.fallback_only = 1;  // slices whose list overflowed scan the grid
+            launch(c->stream, b);
         }
     }
     GZM_HIP(hipGetLastError());
+    if (three_phase) {
+        size_t bytes = 0;
+        GZM_HIP(rocprim::exclusive_scan(nullptr, bytes, c->d_defer_count, c->d_defer_off, 0, (size_t)c->n_active,
+                                        rocprim::plus<int>(), c->stream));
+        if (c->capturing && bytes > c->temp_bytes) { set_error("temporary storage too small while capturing"); return GZM_ERR_STATE; }
+        s = ensure_temp(c, bytes);
+        if (s) return s;
+        {
+            Timed tm(c, kForce);
+            GZM_HIP(rocprim::exclusive_scan(c->d_temp, bytes, c->d_defer_count, c->d_defer_off, 0, (size_t)c->n_active,
+                                            rocprim::plus<int>(), c->stream));
+            launch_defer_total(c->stream, c->n_active, c->d_defer_off, c->d_defer_count, c->d_defer_total);
+        }
+        int64_t D = 0;
+        if (gp) {
+            D = gp->deferred;
+        } else {
+            int h_total = 0;
+            GZM_HIP(hipMemcpyAsync(&h_total, c->d_defer_total, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            GZM_HIP(hipStreamSynchronize(c->stream));
+            D = h_total;
+            // results of one round: all deferred pairs, or 2 per particle if more
+            const size_t want = (size_t)std::min<int64_t>(D, std::max<int64_t>(1 << 20, 2 * (int64_t)c->n));
+            if (want > c->defer_result_cap) {
+                dev_free(c->d_defer_result);
+                c->defer_result_cap = 0;
+                s = dev_alloc(&c->d_defer_result, want * 5);
+                if (s) return s;
+                c->defer_result_cap = want;
+            }
+        }
+        c->last_deferred = D;
+        if (D > 0 && c->defer_result_cap == 0) { set_error("deferred-pair buffer missing while capturing"); return GZM_ERR_STATE; }
+        const int64_t chunk = D > 0 ? (int64_t)c->defer_result_cap : 1;
+        a.defer.result = c->d_defer_result;
+        for (int64_t q0 = 0; q0 == 0 || q0 < D; q0 += chunk) {
+            const int64_t q1 = std::min<int64_t>(D, q0 + chunk);
+            a.defer.q_begin = (int)q0;
+            a.defer.q_end = (int)q1;
+            Timed tm(c, kForce);
+            if (c->p.precision == 1) launch_force_deferred_f32(c->stream, a, (int)(q1 - q0));
+            else launch_force_deferred(c->stream, a, (int)(q1 - q0));
+            launch_force_finish(c->stream, a);
+            if (D == 0) break;
+        }
+        GZM_HIP(hipGetLastError());
+    }
     if (gp) return GZM_OK;
     Counters h;
     s = read_counters(c, kCtrForce, &h);
@@ -1029,7 +1112,7 @@ gzm_status gzm_run_passes(gzm_ctx *c, int32_t n_passes, gzm_pass_stats *stats)
     s = s ? s : gradients_impl(c, nullptr, &probe);
     s = s ? s : force_impl(c, nullptr, &probe);
     if (s) return s;
-    const Plan gp = {probe.density_sweeps, c->list_overflow_slices > 0};
+    const Plan gp = {probe.density_sweeps, c->list_overflow_slices > 0, c->last_deferred};
     {
         // the captured compactions run over the whole active list
         size_t bytes = 0;

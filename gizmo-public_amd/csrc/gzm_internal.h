@@ -148,6 +148,21 @@ struct GradientArgs {
     Counters *ctr;
 };
 
+// Pairs whose Riemann problem reaches the exact solver are taken out of the
+// force loop (a few percent of pairs in supersonic flow, but they would hold
+// every lane of their wavefront through the iterative solver) and evaluated
+// by a second kernel, one pair per lane; a third kernel adds them to each
+// particle after its other pairs, in list order.
+struct DeferArgs {
+    uint64_t *mask;        // per active entry: list positions k deferred (bit k; lists of <= 64)
+    int *count;            // per active entry: number of deferred pairs
+    uint8_t *raw;          // per active entry: 1 = acc, dtu, vsig hold the raw sums of the loop
+    const int *offset;     // exclusive scan of count
+    const int *total;      // sum of count (device)
+    double *result;        // per deferred pair of the round: dv/dt terms (3), du/dt term, v_sig
+    int q_begin, q_end;    // deferred pairs handled by this round
+};
+
 struct ForceArgs {
     const BoxDesc *boxes;
     const int *box_of;
@@ -164,6 +179,7 @@ struct ForceArgs {
     uint8_t *wakeup;
     PairList list;         // list.idx == NULL: scan the grid for every particle
     int fallback_only;     // scan only the slices flagged in list.overflow
+    DeferArgs defer;       // list kernel: defer.mask != NULL -> exact-solver pairs deferred
     const double4 *posm;   // FP64 positions and masses (FP32 kernels)
     const double4 *velu;   // FP64 velocities (FP32 kernels: final operations)
     const double *hsml;    // FP64 hsml (FP32 kernels: exact re-tests)
@@ -203,6 +219,10 @@ void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 
                        const double *rho, const double *pres, const double *dhsml, const double *cn,
                        const double *fce, const double *nvt, const double *grad, double gamma, ForceRec *rec);
 void launch_force(hipStream_t s, const ForceArgs &a);  // list-driven if a.list.idx, else scanning
+// deferred exact-solver pairs [a.defer.q_begin, a.defer.q_end), then the per-particle finish
+void launch_force_deferred(hipStream_t s, const ForceArgs &a, int n_pairs);
+void launch_force_finish(hipStream_t s, const ForceArgs &a);
+void launch_defer_total(hipStream_t s, int n_active, const int *offset, const int *count, int *total);
 // FP32-bulk kernels (gzm_kernels_f32.hip)
 void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const double4 *velu, const BoxDesc *boxes,
                           const int *box_of, float4 *offm, float4 *velf, uint32_t *cellc);
@@ -215,6 +235,7 @@ void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float
                            const double *dhsml, const double *cn, const double *fce, const double *nvt,
                            const double *grad, double gamma, ForceRecF *rec);
 void launch_force_f32(hipStream_t s, const ForceArgs &a);
+void launch_force_deferred_f32(hipStream_t s, const ForceArgs &a, int n_pairs);
 
 void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, int mode, int n_active,
                             const int *active, const double4 *posm, const double *hsml, int *counts, Counters *ctr);

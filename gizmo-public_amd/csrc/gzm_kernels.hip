@@ -443,16 +443,28 @@ __device__ __forceinline__ void unpack_side(const ForceRec &r, PairSide<double> 
     S.gv[6] = r.g3.x; S.gv[7] = r.g3.y; S.gv[8] = r.g3.z;
 }
 
-// USE_LIST: walk the pair list written by the gradient kernel (slices that
-// overflowed it are skipped and left to the scanning variant); otherwise scan
-// the grid (every slice, or only the overflowed ones when a.fallback_only).
-template <bool USE_LIST>
+// One particle's force loop.  USE_LIST: walk the pair list written by the
+// gradient kernel (slices that overflowed it are skipped and left to the
+// scanning variant); otherwise scan the grid (every slice, or only the
+// overflowed ones when a.fallback_only).  Pairs that reach the exact Riemann
+// solver are summed after all other pairs of the particle: THREE_PHASE (list
+// kernel with defer buffers) leaves them to force_deferred_kernel and
+// force_finish_kernel, otherwise the loop walks the pairs a second time and
+// evaluates them in place -- the same order, so every variant gives the same
+// bits.
+template <bool USE_LIST, bool THREE_PHASE>
 __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
 {
     const int t = xcd_block() * kBlock + threadIdx.x;
     bool valid = t < a.n_active;
     if (USE_LIST) {
-        if (valid && a.list.overflow[t >> 6]) valid = false;
+        if (valid && a.list.overflow[t >> 6]) {
+            valid = false;
+            if (THREE_PHASE) {  // finished by the scanning kernel
+                a.defer.count[t] = 0;
+                a.defer.raw[t] = 0;
+            }
+        }
     } else if (a.fallback_only) {
         if (valid && !a.list.overflow[t >> 6]) valid = false;
     }
@@ -460,9 +472,10 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
     for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = a.active[t];
         const ForceRec ri = a.rec[i];
-        double acc0 = 0, acc1 = 0, acc2 = 0, dte = 0, vmax = kMinReal;
+        double acc[3] = {0, 0, 0}, dte = 0, vmax = kMinReal;
         const double m_i = ri.mc.x, rho_i = ri.th.x;
         int status = 0;
+        uint64_t dmask = 0;
         if ((m_i > 0) && (rho_i > 0)) {
             PairI<double> I;
             unpack_side(ri, I);
@@ -473,7 +486,22 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
             double out_vsig = I.cs;
             ROut<double> ro = {0, 0, {0, 0, 0}, 0};
             const double h2_i = I.h * I.h;
-            auto visit = [&](int j, bool check) {
+            int n_deferred = 0;
+            auto add = [&](int j, const double Fv[3], double Fp, double vsig, int mask, int e) {
+                if (e > status) status = e;
+                nretry += (mask & 8) ? 1 : 0;
+                nexact += (mask & 4) ? 1 : 0;
+                acc[0] += Fv[0];
+                acc[1] += Fv[1];
+                acc[2] += Fv[2];
+                dte += Fp;
+                if (vsig > out_vsig) out_vsig = vsig;
+                if (a.is_active && !a.is_active[j]) {
+                    if (vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;  // WAKEUP (hydro_evaluate.h:485-495)
+                }
+            };
+            // pass 0: every pair but the deferred ones; pass 1: only those
+            auto visit = [&](int j, int k, bool check, int pass) {
                 const double4 xj = a.rec[j].xh;
                 double dp[3] = {x_i[0] - xj.x, x_i[1] - xj.y, x_i[2] - xj.z};
                 wrap_dp(G.box, dp);
@@ -486,48 +514,60 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
                 }
                 PairSide<double> J;
                 unpack_side(a.rec[j], J);
-                cnt++;
                 double Fv[3], Fp, vsig;
                 int mask;
-                const int e = force_pair(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
-                if (e > status) status = e;
-                nretry += (mask & 8) ? 1 : 0;
-                nexact += (mask & 4) ? 1 : 0;
-                acc0 += Fv[0];
-                acc1 += Fv[1];
-                acc2 += Fv[2];
-                dte += Fp;
-                if (vsig > out_vsig) out_vsig = vsig;
-                if (a.is_active && !a.is_active[j]) {
-                    if (vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;  // WAKEUP (hydro_evaluate.h:485-495)
+                const int e = force_pair<double, true>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+                if (pass == 0) {
+                    cnt++;
+                    if (e == kDeferred) {
+                        if (THREE_PHASE) dmask |= 1ull << k;
+                        n_deferred++;
+                        return;
+                    }
+                    add(j, Fv, Fp, vsig, mask, e);
+                } else if (!THREE_PHASE && e == kDeferred) {
+                    const int e2 = force_pair<double, false>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+                    add(j, Fv, Fp, vsig, mask, e2);
                 }
             };
-            if (USE_LIST) {
-                // the list holds the pair set in the gradient kernel's scan order,
-                // which is this kernel's scan order as well (same predicate, same R)
-                const int n = a.list.count[t];
-                const int *row = a.list.idx + (size_t)t * a.list.cap;
-                for (int k = 0; k < n; k++) visit(row[k], false);
-            } else {
-                const double R = (I.h > G.hmax) ? I.h : G.hmax;
-                for_each_candidate(G.grid, x_i, R, [&](int j) { visit(j, true); });
+            const double R = (I.h > G.hmax) ? I.h : G.hmax;
+            for (int pass = 0; pass < 2; pass++) {
+                if (pass == 1 && (THREE_PHASE || n_deferred == 0)) break;
+                if (USE_LIST) {
+                    // the list holds the pair set in the gradient kernel's scan order,
+                    // which is this kernel's scan order as well (same predicate, same R)
+                    const int n = a.list.count[t];
+                    const int *row = a.list.idx + (size_t)t * a.list.cap;
+                    for (int k = 0; k < n; k++) visit(row[k], k, false, pass);
+                } else {
+                    for_each_candidate(G.grid, x_i, R, [&](int j) { visit(j, 0, true, pass); });
+                }
+            }
+            if (THREE_PHASE) {
+                // raw sums: finished by force_finish_kernel after the deferred pairs
+                a.acc[3 * (size_t)i + 0] = acc[0];
+                a.acc[3 * (size_t)i + 1] = acc[1];
+                a.acc[3 * (size_t)i + 2] = acc[2];
+                a.dtu[i] = dte;
+                a.vsig[i] = out_vsig;
+                a.defer.mask[t] = dmask;
+                a.defer.count[t] = n_deferred;
+                a.defer.raw[t] = 1;
+                if (status) atomicMax(&a.ctr->status, status);
+                return;
             }
             if (vmax < out_vsig) vmax = out_vsig;
-            // hydro_final_operations_and_cleanup (hydro_toplevel.c:738-764)
-            dte -= (I.v[0] / 1.0) * acc0;
-            acc0 /= m_i;
-            dte -= (I.v[1] / 1.0) * acc1;
-            acc1 /= m_i;
-            dte -= (I.v[2] / 1.0) * acc2;
-            acc2 /= m_i;
-            dte /= m_i;
-            if (I.h >= 0.99 * a.max_hsml) dte = 0;
+            force_final_ops(acc, dte, I.v, m_i, I.h, a.max_hsml);
         } else if (m_i > 0) {
             dte /= m_i;
         }
-        a.acc[3 * (size_t)i + 0] = acc0;
-        a.acc[3 * (size_t)i + 1] = acc1;
-        a.acc[3 * (size_t)i + 2] = acc2;
+        if (THREE_PHASE) {
+            a.defer.count[t] = 0;
+            a.defer.raw[t] = 0;
+        }
+        a.acc[3 * (size_t)i + 0] = acc[0];
+        a.acc[3 * (size_t)i + 1] = acc[1];
+        a.acc[3 * (size_t)i + 2] = acc[2];
         a.dtu[i] = dte;
         a.vsig[i] = vmax;
         if (status) atomicMax(&a.ctr->status, status);
@@ -535,6 +575,88 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
     wave_add_u64(&a.ctr->interactions, cnt);
     wave_add_u64(&a.ctr->retries, nretry);
     wave_add_u64(&a.ctr->exact_calls, nexact);
+}
+
+// Deferred pairs q in [q_begin, min(total, q_end)): one pair per lane, with
+// the exact Riemann solver.
+__global__ __launch_bounds__(kBlock) void force_deferred_kernel(ForceArgs a)
+{
+    const int q = a.defer.q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int q_end = min(*a.defer.total, a.defer.q_end);
+    const bool valid = q < q_end;
+    const int t = valid ? owner_of(a.defer.offset, a.n_active, q) : 0;
+    unsigned long long nretry = 0, nexact = 0;
+    for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
+        const int i = a.active[t];
+        const int k = nth_set_bit(a.defer.mask[t], q - a.defer.offset[t]);
+        const int j = a.list.idx[(size_t)t * a.list.cap + k];
+        const ForceRec ri = a.rec[i];
+        PairI<double> I;
+        unpack_side(ri, I);
+        I.hinv3 = I.hinv * I.hinv * I.hinv;
+        I.hinv4 = I.hinv3 * I.hinv;
+        I.cnumcrit2 = kCondDanger * kCondDanger - I.cn * I.cn;
+        const double4 xj = a.rec[j].xh;
+        double dp[3] = {ri.xh.x - xj.x, ri.xh.y - xj.y, ri.xh.z - xj.z};
+        wrap_dp(G.box, dp);
+        const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+        PairSide<double> J;
+        unpack_side(a.rec[j], J);
+        ROut<double> ro = {0, 0, {0, 0, 0}, 0};
+        double Fv[3], Fp, vsig;
+        int mask;
+        const int e = force_pair<double, false>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+        nretry += (mask & 8) ? 1 : 0;
+        nexact += (mask & 4) ? 1 : 0;
+        double *o = a.defer.result + 5 * (size_t)(q - a.defer.q_begin);
+        o[0] = Fv[0]; o[1] = Fv[1]; o[2] = Fv[2]; o[3] = Fp; o[4] = vsig;
+        if (a.is_active && !a.is_active[j]) {
+            if (vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;
+        }
+        if (e) atomicMax(&a.ctr->status, e);
+    });
+    wave_add_u64(&a.ctr->retries, nretry);
+    wave_add_u64(&a.ctr->exact_calls, nexact);
+}
+
+// Add the deferred pairs of this round to the raw sums (in list order) and,
+// for particles whose deferred pairs are all in, apply the final operations.
+// `velu` and `hsml` give the FP64 velocity and hsml of i.
+__global__ __launch_bounds__(kBlock) void force_finish_kernel(ForceArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_active || !a.defer.raw[t]) return;
+    const int i = a.active[t];
+    double acc[3] = {a.acc[3 * (size_t)i + 0], a.acc[3 * (size_t)i + 1], a.acc[3 * (size_t)i + 2]};
+    double dte = a.dtu[i], out_vsig = a.vsig[i];
+    const int q0 = a.defer.offset[t], q1 = q0 + a.defer.count[t];
+    for (int q = max(q0, a.defer.q_begin); q < min(q1, a.defer.q_end); q++) {
+        const double *o = a.defer.result + 5 * (size_t)(q - a.defer.q_begin);
+        acc[0] += o[0];
+        acc[1] += o[1];
+        acc[2] += o[2];
+        dte += o[3];
+        if (o[4] > out_vsig) out_vsig = o[4];
+    }
+    if (q1 <= a.defer.q_end) {
+        double vmax = kMinReal;
+        if (vmax < out_vsig) vmax = out_vsig;
+        const double4 v = a.velu[i];
+        const double vi[3] = {v.x, v.y, v.z};
+        force_final_ops(acc, dte, vi, a.posm[i].w, a.hsml[i], a.max_hsml);
+        out_vsig = vmax;
+        a.defer.raw[t] = 0;
+    }
+    a.acc[3 * (size_t)i + 0] = acc[0];
+    a.acc[3 * (size_t)i + 1] = acc[1];
+    a.acc[3 * (size_t)i + 2] = acc[2];
+    a.dtu[i] = dte;
+    a.vsig[i] = out_vsig;
+}
+
+__global__ void defer_total_kernel(int n, const int *offset, const int *count, int *total)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *total = (n > 0) ? offset[n - 1] + count[n - 1] : 0;
 }
 
 // neighbour-set sizes for a fixed hsml (same predicates as the stage kernels)
@@ -661,10 +783,25 @@ void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 
 void launch_force(hipStream_t s, const ForceArgs &a)
 {
     if (a.n_active <= 0) return;
-    if (a.list.idx && !a.fallback_only)
-        hipLaunchKernelGGL(force_kernel<true>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL(force_kernel<false>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    const dim3 grid(nblocks(a.n_active)), block(kBlock);
+    if (a.list.idx && !a.fallback_only) {
+        if (a.defer.mask) hipLaunchKernelGGL((force_kernel<true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((force_kernel<true, false>), grid, block, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((force_kernel<false, false>), grid, block, 0, s, a);
+    }
+}
+void launch_force_deferred(hipStream_t s, const ForceArgs &a, int n_pairs)
+{
+    if (n_pairs > 0) hipLaunchKernelGGL(force_deferred_kernel, dim3(nblocks(n_pairs)), dim3(kBlock), 0, s, a);
+}
+void launch_force_finish(hipStream_t s, const ForceArgs &a)
+{
+    if (a.n_active > 0) hipLaunchKernelGGL(force_finish_kernel, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+}
+void launch_defer_total(hipStream_t s, int n_active, const int *offset, const int *count, int *total)
+{
+    hipLaunchKernelGGL(defer_total_kernel, dim3(1), dim3(64), 0, s, n_active, offset, count, total);
 }
 void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, int mode, int n_active,
                             const int *active, const double4 *posm, const double *hsml, int *counts, Counters *ctr)

@@ -388,16 +388,62 @@ __device__ __forceinline__ void unpack_side_f(const ForceRecF &r, PairSide<float
     S.gv[6] = r.g3.x; S.gv[7] = r.g3.y; S.gv[8] = r.g3.z;
 }
 
+// the FP32 list kernel takes pair displacements from cell differences unless
+// the search radius spans half the box on some axis
+__device__ __forceinline__ bool list_needs_exact_geom(const BoxDesc &G)
+{
+    bool e = false;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++)
+        if (!(G.hmax * G.grid.inv_s[ax] < 0.5 * G.grid.nc[ax] - 2.0)) e = true;
+    return e;
+}
+
+// displacement x_i - x_j of a listed pair in FP32
+__device__ __forceinline__ void list_dp(const BoxDesc &G, bool exact_geom, const ForceRecF &ri, const int ci[3],
+                                        const double4 &pi, const ForceRecF &Jr, const double4 *posm, int j,
+                                        float dp[3])
+{
+    const float4 xj = Jr.xh;
+    if (!exact_geom) {
+        int cj[3];
+        unpack_cell(__float_as_uint(Jr.g3.w), cj);
+        dp[0] = (float)wrap_cell(ci[0] - cj[0], G.grid.nc[0]) * G.s_f[0] + (ri.xh.x - xj.x);
+        dp[1] = (float)wrap_cell(ci[1] - cj[1], G.grid.nc[1]) * G.s_f[1] + (ri.xh.y - xj.y);
+        dp[2] = (float)wrap_cell(ci[2] - cj[2], G.grid.nc[2]) * G.s_f[2] + (ri.xh.z - xj.z);
+    } else {
+        double d64[3];
+        exact_r2(G.box, pi, posm[j], d64);
+        dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+    }
+}
+
+__device__ __forceinline__ void side_i_f(const ForceRecF &ri, PairI<float> &I)
+{
+    unpack_side_f(ri, I);
+    I.hinv3 = I.hinv * I.hinv * I.hinv;
+    I.hinv4 = I.hinv3 * I.hinv;
+    I.cnumcrit2 = kCondDanger * kCondDanger - I.cn * I.cn;
+}
+
 // USE_LIST: walk the pair list of the FP32 gradient kernel (overflowed slices
 // are left to the scanning variant); otherwise scan the grid with the same
-// banded predicate as the gradient kernel.
-template <bool USE_LIST>
+// banded predicate as the gradient kernel.  Exact-solver pairs are summed
+// after the others, by force_deferred_f32_kernel + force_finish_kernel
+// (THREE_PHASE) or by a second walk over the pairs, as in the FP64 kernel.
+template <bool USE_LIST, bool THREE_PHASE>
 __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
 {
     const int t = xcd_block() * kBlock + threadIdx.x;
     bool valid = t < a.n_active;
     if (USE_LIST) {
-        if (valid && a.list.overflow[t >> 6]) valid = false;
+        if (valid && a.list.overflow[t >> 6]) {
+            valid = false;
+            if (THREE_PHASE) {  // finished by the scanning kernel
+                a.defer.count[t] = 0;
+                a.defer.raw[t] = 0;
+            }
+        }
     } else if (a.fallback_only) {
         if (valid && !a.list.overflow[t >> 6]) valid = false;
     }
@@ -406,145 +452,194 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
         const int i = a.active[t];
         const ForceRecF ri = a.recf[i];
         const double4 pi = a.posm[i];
-        double acc0 = 0, acc1 = 0, acc2 = 0, dte = 0, vmax = kMinReal;
+        double acc[3] = {0, 0, 0}, dte = 0, vmax = kMinReal;
         const double m_i = pi.w;
         int status = 0;
+        uint64_t dmask = 0;
         if ((m_i > 0) && (ri.th.x > 0)) {
             PairI<float> I;
-            unpack_side_f(ri, I);
-            I.hinv3 = I.hinv * I.hinv * I.hinv;
-            I.hinv4 = I.hinv3 * I.hinv;
-            I.cnumcrit2 = kCondDanger * kCondDanger - I.cn * I.cn;
+            side_i_f(ri, I);
             int ci[3];
             unpack_cell(__float_as_uint(ri.g3.w), ci);
             double out_vsig = I.cs;
             ROut<float> ro = {0, 0, {0, 0, 0}, 0};
-            auto eval = [&](int j, const PairSide<float> &J, const float dp[3], float r2) {
-                cnt++;
-                float Fv[3], Fp, vsig;
-                int mask;
-                const int e = force_pair(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+            int n_deferred = 0;
+            auto add = [&](int j, const float Fv[3], float Fp, float vsig, int mask, int e) {
                 if (e > status) status = e;
                 nretry += (mask & 8) ? 1 : 0;
                 nexact += (mask & 4) ? 1 : 0;
-                acc0 += (double)Fv[0];
-                acc1 += (double)Fv[1];
-                acc2 += (double)Fv[2];
+                acc[0] += (double)Fv[0];
+                acc[1] += (double)Fv[1];
+                acc[2] += (double)Fv[2];
                 dte += (double)Fp;
                 if ((double)vsig > out_vsig) out_vsig = vsig;
                 if (a.is_active && !a.is_active[j]) {
                     if ((double)vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;  // WAKEUP (hydro_evaluate.h:485-495)
                 }
             };
-            if (USE_LIST) {
-                // pair displacements from cell differences unless the search
-                // radius spans half the box on some axis
-                bool exact_geom = false;
-#pragma unroll
-                for (int ax = 0; ax < 3; ax++)
-                    if (!(G.hmax * G.grid.inv_s[ax] < 0.5 * G.grid.nc[ax] - 2.0)) exact_geom = true;
-                const int n = a.list.count[t];
-                const int *row = a.list.idx + (size_t)t * a.list.cap;
-                for (int k = 0; k < n; k++) {
-                    const int j = row[k];
-                    const ForceRecF &Jr = a.recf[j];
-                    PairSide<float> J;
-                    unpack_side_f(Jr, J);
-                    const float4 xj = Jr.xh;
-                    float dp[3];
-                    if (!exact_geom) {
-                        int cj[3];
-                        unpack_cell(__float_as_uint(Jr.g3.w), cj);
-                        dp[0] = (float)wrap_cell(ci[0] - cj[0], G.grid.nc[0]) * G.s_f[0] + (ri.xh.x - xj.x);
-                        dp[1] = (float)wrap_cell(ci[1] - cj[1], G.grid.nc[1]) * G.s_f[1] + (ri.xh.y - xj.y);
-                        dp[2] = (float)wrap_cell(ci[2] - cj[2], G.grid.nc[2]) * G.s_f[2] + (ri.xh.z - xj.z);
-                    } else {
-                        double d64[3];
-                        exact_r2(G.box, pi, a.posm[j], d64);
-                        dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+            // pass 0: every pair but the deferred ones; pass 1: only those
+            auto eval = [&](int j, int k, const PairSide<float> &J, const float dp[3], float r2, int pass) {
+                float Fv[3], Fp, vsig;
+                int mask;
+                const int e = force_pair<float, true>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+                if (pass == 0) {
+                    cnt++;
+                    if (e == kDeferred) {
+                        if (THREE_PHASE) dmask |= 1ull << k;
+                        n_deferred++;
+                        return;
                     }
-                    const float r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-                    eval(j, J, dp, r2);
+                    add(j, Fv, Fp, vsig, mask, e);
+                } else if (!THREE_PHASE && e == kDeferred) {
+                    const int e2 = force_pair<float, false>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+                    add(j, Fv, Fp, vsig, mask, e2);
                 }
-            } else {
-                const double h_i = a.hsml[i];
-                const double h2_i = h_i * h_i;
-                const Band bi = band_of(h2_i);
-                const float pos_min = (float)(h2_i * kBand);
-                const double xs[3] = {pi.x, pi.y, pi.z};
-                const double R = (h_i > G.hmax) ? h_i : G.hmax;
-                bool full[3];
-                for_each_cell(G.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
-                    const bool exact_geom = full[0] || full[1] || full[2];
-                    const float sh0 = (float)(ci[0] - cx) * G.s_f[0];
-                    const float sh1 = (float)(ci[1] - cy) * G.s_f[1];
-                    const float sh2 = (float)(ci[2] - cz) * G.s_f[2];
-                    for (int j = beg; j < end; ++j) {
-                        if (j == i) continue;
-                        const float4 xj = a.recf[j].xh;
-                        const float hj2 = xj.w * xj.w;
+            };
+            const double h_i = a.hsml[i];
+            const double h2_i = h_i * h_i;
+            const Band bi = band_of(h2_i);
+            const float pos_min = (float)(h2_i * kBand);
+            const double xs[3] = {pi.x, pi.y, pi.z};
+            const double R = (h_i > G.hmax) ? h_i : G.hmax;
+            const bool exact_geom_list = list_needs_exact_geom(G);
+            for (int pass = 0; pass < 2; pass++) {
+                if (pass == 1 && (THREE_PHASE || n_deferred == 0)) break;
+                if (USE_LIST) {
+                    const int n = a.list.count[t];
+                    const int *row = a.list.idx + (size_t)t * a.list.cap;
+                    for (int k = 0; k < n; k++) {
+                        const int j = row[k];
+                        const ForceRecF &Jr = a.recf[j];
+                        PairSide<float> J;
+                        unpack_side_f(Jr, J);
                         float dp[3];
-                        float r2;
-                        int in_i, in_j, pos;
-                        if (!exact_geom) {
-                            dp[0] = sh0 + (ri.xh.x - xj.x);
-                            dp[1] = sh1 + (ri.xh.y - xj.y);
-                            dp[2] = sh2 + (ri.xh.z - xj.z);
-                            r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-                            if ((r2 >= bi.hi) && (r2 >= hj2 * (1 + kBandF))) continue;
-                            in_i = decide(r2, bi);
-                            in_j = decide(r2, Band{hj2 * (1 - kBandF), hj2 * (1 + kBandF)});
-                            pos = (r2 >= pos_min) ? 1 : -1;
-                            if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
+                        list_dp(G, exact_geom_list, ri, ci, pi, Jr, a.posm, j, dp);
+                        const float r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                        eval(j, k, J, dp, r2, pass);
+                    }
+                } else {
+                    bool full[3];
+                    for_each_cell(G.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
+                        const bool exact_geom = full[0] || full[1] || full[2];
+                        const float sh0 = (float)(ci[0] - cx) * G.s_f[0];
+                        const float sh1 = (float)(ci[1] - cy) * G.s_f[1];
+                        const float sh2 = (float)(ci[2] - cz) * G.s_f[2];
+                        for (int j = beg; j < end; ++j) {
+                            if (j == i) continue;
+                            const float4 xj = a.recf[j].xh;
+                            const float hj2 = xj.w * xj.w;
+                            float dp[3];
+                            float r2;
+                            int in_i, in_j, pos;
+                            if (!exact_geom) {
+                                dp[0] = sh0 + (ri.xh.x - xj.x);
+                                dp[1] = sh1 + (ri.xh.y - xj.y);
+                                dp[2] = sh2 + (ri.xh.z - xj.z);
+                                r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                                if ((r2 >= bi.hi) && (r2 >= hj2 * (1 + kBandF))) continue;
+                                in_i = decide(r2, bi);
+                                in_j = decide(r2, Band{hj2 * (1 - kBandF), hj2 * (1 + kBandF)});
+                                pos = (r2 >= pos_min) ? 1 : -1;
+                                if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
+                                    double d64[3];
+                                    const double r2e = exact_r2(G.box, pi, a.posm[j], d64);
+                                    const double h_j = a.hsml[j];
+                                    in_i = r2e < h2_i;
+                                    in_j = r2e < h_j * h_j;
+                                    pos = r2e > 0;
+                                }
+                            } else {
                                 double d64[3];
                                 const double r2e = exact_r2(G.box, pi, a.posm[j], d64);
                                 const double h_j = a.hsml[j];
                                 in_i = r2e < h2_i;
                                 in_j = r2e < h_j * h_j;
                                 pos = r2e > 0;
+                                dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+                                r2 = (float)r2e;
                             }
-                        } else {
-                            double d64[3];
-                            const double r2e = exact_r2(G.box, pi, a.posm[j], d64);
-                            const double h_j = a.hsml[j];
-                            in_i = r2e < h2_i;
-                            in_j = r2e < h_j * h_j;
-                            pos = r2e > 0;
-                            dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
-                            r2 = (float)r2e;
+                            if (!pos || (!in_i && !in_j)) continue;
+                            const ForceRecF &Jr = a.recf[j];
+                            if (!(a.posm[j].w > 0) || !(Jr.th.x > 0)) continue;  // hydro_evaluate.h:149-150
+                            PairSide<float> J;
+                            unpack_side_f(Jr, J);
+                            eval(j, 0, J, dp, r2, pass);
                         }
-                        if (!pos || (!in_i && !in_j)) continue;
-                        const ForceRecF &Jr = a.recf[j];
-                        if (!(a.posm[j].w > 0) || !(Jr.th.x > 0)) continue;  // hydro_evaluate.h:149-150
-                        PairSide<float> J;
-                        unpack_side_f(Jr, J);
-                        eval(j, J, dp, r2);
-                    }
-                });
+                    });
+                }
+            }
+            if (THREE_PHASE) {
+                // raw sums: finished by force_finish_kernel after the deferred pairs
+                a.acc[3 * (size_t)i + 0] = acc[0];
+                a.acc[3 * (size_t)i + 1] = acc[1];
+                a.acc[3 * (size_t)i + 2] = acc[2];
+                a.dtu[i] = dte;
+                a.vsig[i] = out_vsig;
+                a.defer.mask[t] = dmask;
+                a.defer.count[t] = n_deferred;
+                a.defer.raw[t] = 1;
+                if (status) atomicMax(&a.ctr->status, status);
+                return;
             }
             if (vmax < out_vsig) vmax = out_vsig;
-            // hydro_final_operations_and_cleanup (hydro_toplevel.c:738-764), FP64
-            const double4 vi = a.velu[i];
-            const double v0 = vi.x, v1 = vi.y, v2 = vi.z;
-            dte -= (v0 / 1.0) * acc0;
-            acc0 /= m_i;
-            dte -= (v1 / 1.0) * acc1;
-            acc1 /= m_i;
-            dte -= (v2 / 1.0) * acc2;
-            acc2 /= m_i;
-            dte /= m_i;
-            if (a.hsml[i] >= 0.99 * a.max_hsml) dte = 0;
+            const double4 v = a.velu[i];
+            const double vi[3] = {v.x, v.y, v.z};
+            force_final_ops(acc, dte, vi, m_i, h_i, a.max_hsml);
         } else if (m_i > 0) {
             dte /= m_i;
         }
-        a.acc[3 * (size_t)i + 0] = acc0;
-        a.acc[3 * (size_t)i + 1] = acc1;
-        a.acc[3 * (size_t)i + 2] = acc2;
+        if (THREE_PHASE) {
+            a.defer.count[t] = 0;
+            a.defer.raw[t] = 0;
+        }
+        a.acc[3 * (size_t)i + 0] = acc[0];
+        a.acc[3 * (size_t)i + 1] = acc[1];
+        a.acc[3 * (size_t)i + 2] = acc[2];
         a.dtu[i] = dte;
         a.vsig[i] = vmax;
         if (status) atomicMax(&a.ctr->status, status);
     });
     wave_add_u64(&a.ctr->interactions, cnt);
+    wave_add_u64(&a.ctr->retries, nretry);
+    wave_add_u64(&a.ctr->exact_calls, nexact);
+}
+
+// Deferred exact-solver pairs of the FP32 list kernel, one per lane.
+__global__ __launch_bounds__(kBlock) void force_deferred_f32_kernel(ForceArgs a)
+{
+    const int q = a.defer.q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int q_end = min(*a.defer.total, a.defer.q_end);
+    const bool valid = q < q_end;
+    const int t = valid ? owner_of(a.defer.offset, a.n_active, q) : 0;
+    unsigned long long nretry = 0, nexact = 0;
+    for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
+        const int i = a.active[t];
+        const int k = nth_set_bit(a.defer.mask[t], q - a.defer.offset[t]);
+        const int j = a.list.idx[(size_t)t * a.list.cap + k];
+        const ForceRecF ri = a.recf[i];
+        PairI<float> I;
+        side_i_f(ri, I);
+        int ci[3];
+        unpack_cell(__float_as_uint(ri.g3.w), ci);
+        const ForceRecF &Jr = a.recf[j];
+        PairSide<float> J;
+        unpack_side_f(Jr, J);
+        float dp[3];
+        list_dp(G, list_needs_exact_geom(G), ri, ci, a.posm[i], Jr, a.posm, j, dp);
+        const float r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+        ROut<float> ro = {0, 0, {0, 0, 0}, 0};
+        float Fv[3], Fp, vsig;
+        int mask;
+        const int e = force_pair<float, false>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+        nretry += (mask & 8) ? 1 : 0;
+        nexact += (mask & 4) ? 1 : 0;
+        double *o = a.defer.result + 5 * (size_t)(q - a.defer.q_begin);
+        o[0] = Fv[0]; o[1] = Fv[1]; o[2] = Fv[2]; o[3] = Fp; o[4] = vsig;
+        if (a.is_active && !a.is_active[j]) {
+            if ((double)vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;
+        }
+        if (e) atomicMax(&a.ctr->status, e);
+    });
     wave_add_u64(&a.ctr->retries, nretry);
     wave_add_u64(&a.ctr->exact_calls, nexact);
 }
@@ -585,10 +680,17 @@ void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float
 void launch_force_f32(hipStream_t s, const ForceArgs &a)
 {
     if (a.n_active <= 0) return;
-    if (a.list.idx && !a.fallback_only)
-        hipLaunchKernelGGL(force_f32_kernel<true>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL(force_f32_kernel<false>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    const dim3 grid(nblocks(a.n_active)), block(kBlock);
+    if (a.list.idx && !a.fallback_only) {
+        if (a.defer.mask) hipLaunchKernelGGL((force_f32_kernel<true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((force_f32_kernel<true, false>), grid, block, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((force_f32_kernel<false, false>), grid, block, 0, s, a);
+    }
+}
+void launch_force_deferred_f32(hipStream_t s, const ForceArgs &a, int n_pairs)
+{
+    if (n_pairs > 0) hipLaunchKernelGGL(force_deferred_f32_kernel, dim3(nblocks(n_pairs)), dim3(kBlock), 0, s, a);
 }
 
 }  // namespace gzm

@@ -587,8 +587,11 @@ __device__ __noinline__ ExactOut riemann_exact(double rL, double pL, double rR, 
 
 // Riemann_solver() for MFM hydro (reimann.h:286-388).  Returns a stage mask:
 // bit0 HLLC, bit1 KT, bit2 exact; err gets 1234545 on the exact solver's
-// negative-density exit.  The exact solver always runs in FP64.
-template <typename T>
+// negative-density exit.  The exact solver always runs in FP64.  DEFER: stop
+// where the exact solver would be called and return kDeferBit instead.
+constexpr int kDeferBit = 32;
+
+template <typename T, bool DEFER = false>
 __device__ __forceinline__ int riemann_solve(const RState<T> &L, const RState<T> &R, const T n[3], T limiter,
                                              const GammaConsts &G, ROut<T> &o, int &err)
 {
@@ -612,6 +615,7 @@ __device__ __forceinline__ int riemann_solve(const RState<T> &L, const RState<T>
         mask |= 2;
     }
     if ((o.P_M < 0) || isnan(o.P_M) || (o.P_M > limiter)) {
+        if (DEFER) return mask | kDeferBit;
         ExactOut e = riemann_exact(L.rho, L.p, R.rho, R.p, vL, vR, cL, cR, o.P_M, o.S_M, G);
         o.P_M = (T)e.P_M;
         o.S_M = (T)e.S_M;
@@ -645,8 +649,12 @@ struct PairI : PairSide<T> {
 
 // dp = x_i - x_j (nearest image), r2 = |dp|^2.  Returns the status code
 // (0, 1234 or 1234545); Fv, Fp, vsig are the pair's flux terms for i, mask
-// the stages used (bit0 HLLC, bit1 KT, bit2 exact, bit3 retry).
-template <typename T>
+// the stages used (bit0 HLLC, bit1 KT, bit2 exact, bit3 retry).  DEFER:
+// returns kDeferred, with no flux terms, for a pair that would call the exact
+// Riemann solver (the caller evaluates it later with DEFER = false).
+constexpr int kDeferred = -1;
+
+template <typename T, bool DEFER = false>
 __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &J, const T dp[3], T r2,
                                           const GammaConsts &G, ROut<T> &ro, T Fv[3], T &Fp, T &vsig, int &mask)
 {
@@ -740,7 +748,8 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
     int err = 0;
     T lim_a = lim;
     for (int attempt = 0; attempt < 3; attempt++) {
-        mask |= riemann_solve(SL, SR, n, lim_a, G, ro, err);
+        mask |= riemann_solve<T, DEFER>(SL, SR, n, lim_a, G, ro, err);
+        if (DEFER && (mask & kDeferBit)) return kDeferred;
         const bool bad = (ro.P_M < 0) || isnan(ro.P_M);
         if (attempt == 0) {
             if (!(bad || (ro.P_M > T(1.4) * lim))) break;
