@@ -104,7 +104,10 @@ def run_mode(gzm, ic, args, precision):
     N = ic["pos"].shape[0]
     params = gzm.make_params(box=tuple(ic["box"]), cell_size=args.cell_size, precision=precision)
     ctx = gzm.MFMContext(params)
-    ctx.upload(ic)
+    ctx.upload(ic)  # first upload allocates
+    t_up = time.perf_counter()
+    ctx.upload(ic)  # host arrays -> sorted device state (PCIe + sort + grid)
+    t_up = time.perf_counter() - t_up
 
     def one_pass():
         ctx.restore_initial_state()
@@ -125,7 +128,9 @@ def run_mode(gzm, ic, args, precision):
         updates.append(s1.density_updates)
     elapsed = time.perf_counter() - t0
     tm = ctx.timing()
+    t_down = time.perf_counter()
     out = ctx.download()
+    t_down = time.perf_counter() - t_down
     ctx.close()
     # roofline of the dominant kernel (largest total device time)
     kern = {
@@ -147,6 +152,7 @@ def run_mode(gzm, ic, args, precision):
             "density_sweeps": tm["density_ms"] / args.steps,
             "gradients": tm["gradients_ms"] / args.steps,
             "force": tm["force_ms"] / args.steps,
+            "force_deferred_pairs": tm["force_deferred_ms"] / args.steps,
             "other": tm["other_ms"] / args.steps,
         },
         "dominant": dom,
@@ -154,6 +160,10 @@ def run_mode(gzm, ic, args, precision):
         "bytes_per_launch": bytes_per_launch,
         "achieved_gbs": bytes_per_launch / (mean_ms * 1e-3) / 1e9,
         "out": out,
+        "upload_ms": 1e3 * t_up,
+        "download_ms": 1e3 * t_down,
+        # one pass with the host arrays handed over and the results copied back
+        "pcie_inclusive_value": float(interactions[-1]) / (t_up + elapsed / args.steps + t_down),
     }
 
 
@@ -217,6 +227,7 @@ def run_graph_mode(gzm, ic, args, precision, steps):
             "density_sweeps": tw["density_ms"] / w,
             "gradients": tw["gradients_ms"] / w,
             "force": tw["force_ms"] / w,
+            "force_deferred_pairs": tw["force_deferred_ms"] / w,
             "other": tw["other_ms"] / w,
         },
         "dominant": dom,
@@ -334,7 +345,9 @@ def main_batch(args, gzm, ics):
         result["blast_128"] = {
             "n_particles": blast_ic["pos"].shape[0],
             "fp64": {"value": b64["value"], "ms_per_step": b64["ms_per_step"],
-                     "kernels_ms_per_pass": b64["kernels_ms_per_pass"]},
+                     "kernels_ms_per_pass": b64["kernels_ms_per_pass"],
+                     "upload_ms": b64["upload_ms"], "download_ms": b64["download_ms"],
+                     "pcie_inclusive_value": b64["pcie_inclusive_value"]},
             "fp32_bulk": {"value": b32["value"], "ms_per_step": b32["ms_per_step"],
                           "kernels_ms_per_pass": b32["kernels_ms_per_pass"]},
             "fp32_speedup_vs_fp64": b32["value"] / b64["value"],

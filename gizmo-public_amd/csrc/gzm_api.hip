@@ -42,7 +42,7 @@ void set_error(const char *fmt, ...)
         }                                                                                       \
     } while (0)
 
-enum TimerKind { kDensity = 0, kGradients = 1, kForce = 2, kOther = 3 };
+enum TimerKind { kDensity = 0, kGradients = 1, kForce = 2, kOther = 3, kForceDeferred = 4 };
 
 struct TimedLaunch {
     hipEvent_t a, b;
@@ -185,6 +185,7 @@ void collect_timings(gzm_ctx *c)
             case kDensity: c->timing.density_ms += ms; c->timing.density_launches++; break;
             case kGradients: c->timing.gradients_ms += ms; c->timing.gradients_launches++; break;
             case kForce: c->timing.force_ms += ms; c->timing.force_launches++; break;
+            case kForceDeferred: c->timing.force_deferred_ms += ms; c->timing.force_deferred_launches++; break;
             default: c->timing.other_ms += ms; c->timing.other_launches++; break;
             }
         }
@@ -988,7 +989,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         s = ensure_temp(c, bytes);
         if (s) return s;
         {
-            Timed tm(c, kForce);
+            Timed tm(c, kForceDeferred);
             GZM_HIP(rocprim::exclusive_scan(c->d_temp, bytes, c->d_defer_count, c->d_defer_off, 0, (size_t)c->n_active,
                                             rocprim::plus<int>(), c->stream));
             launch_defer_total(c->stream, c->n_active, c->d_defer_off, c->d_defer_count, c->d_defer_total);
@@ -1019,7 +1020,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
             const int64_t q1 = std::min<int64_t>(D, q0 + chunk);
             a.defer.q_begin = (int)q0;
             a.defer.q_end = (int)q1;
-            Timed tm(c, kForce);
+            Timed tm(c, kForceDeferred);
             if (c->p.precision == 1) launch_force_deferred_f32(c->stream, a, (int)(q1 - q0));
             else launch_force_deferred(c->stream, a, (int)(q1 - q0));
             launch_force_finish(c->stream, a);

@@ -77,6 +77,7 @@ class Timing(C.Structure):
         ("other_ms", C.c_double), ("other_launches", C.c_int64),
         ("graph_ms", C.c_double), ("graph_launches", C.c_int64),
         ("graph_passes", C.c_int64),
+        ("force_deferred_ms", C.c_double), ("force_deferred_launches", C.c_int64),
     ]
 
 

@@ -127,6 +127,8 @@ typedef struct gzm_timing {
     double other_ms;     int64_t other_launches;   /* grid build, packing, finalize */
     double graph_ms;     int64_t graph_launches;   /* gzm_run_passes graphs, whole-graph time */
     int64_t graph_passes;                          /* passes inside those graphs */
+    double force_deferred_ms; int64_t force_deferred_launches;  /* exact-solver pairs deferred
+                                                   out of the force loop, their scan and finish */
 } gzm_timing;
 
 typedef struct gzm_ctx gzm_ctx;
