@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--cell-size", type=float, default=0.0)
     ap.add_argument("--fp32", type=int, default=1, help="blast workload: also time the FP32-bulk mode")
     ap.add_argument("--blast", type=int, default=1, help="batch workload: also time the 128^3 blast case")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="batch workload: pick cell size and pair-list capacity per mode on a sample of boxes")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="instead of the pass, copy a buffer of this many MiB three times and exit: the "
                          "known-size reference that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -184,11 +186,15 @@ def measured_traffic(kernel, precision, n):
     return rec["fetch_bytes"] + rec["write_bytes"]
 
 
-def run_graph_mode(gzm, ic, args, precision, steps):
+def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
     """Warm-up eager passes (per-kernel HIP events), then `steps` passes in one
     HIP graph (timed with events around the launch)."""
     N = ic["pos"].shape[0]
-    params = gzm.make_params(cell_size=args.cell_size, precision=precision)
+    if tuned:
+        params = gzm.make_params(cell_size=-tuned["cell_factor"], pair_list_cap=tuned["pair_list_cap"],
+                                 precision=precision)
+    else:
+        params = gzm.make_params(cell_size=args.cell_size, precision=precision)
     ctx = gzm.MFMContext(params)
     ctx.upload(ic)
     ctx.timing_reset()
@@ -262,8 +268,15 @@ def main_batch(args, gzm, ics):
     ic = ics.batch_boxes(args.boxes)
     N = ic["pos"].shape[0]
     nb = len(ic["box_offsets"]) - 1
-    r32 = run_graph_mode(gzm, ic, args, 1, steps)
-    r64 = run_graph_mode(gzm, ic, args, 0, steps)
+    tuned = {}
+    if args.autotune:
+        import autotune
+        t0 = time.perf_counter()
+        for prec in (1, 0):
+            tuned[prec] = autotune.tune(ic, prec)
+        tune_s = time.perf_counter() - t0
+    r32 = run_graph_mode(gzm, ic, args, 1, steps, tuned.get(1))
+    r64 = run_graph_mode(gzm, ic, args, 0, steps, tuned.get(0))
     traffic = measured_traffic(r32["dominant"], 1, N)
     result = {
         "metric": "particle-neighbor interactions/s, MFM pass; FP32-batched vs FP64 at stated error",
@@ -300,6 +313,7 @@ def main_batch(args, gzm, ics):
             "kernels_ms_per_pass_eager": r64["kernels_ms_per_pass_eager"],
         },
         "fp32_speedup_vs_fp64": r32["value"] / r64["value"],
+        "autotune": ({"fp32": tuned[1], "fp64": tuned[0], "seconds": round(tune_s, 1)} if tuned else None),
         "kernels_ms_per_pass_eager": r32["kernels_ms_per_pass_eager"],
         "roofline": {
             "bound": "hbm",

@@ -497,9 +497,11 @@ gzm_status setup_boxes(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const do
         memset(&B, 0, sizeof(B));
         const int64_t i0 = offsets[b], i1 = offsets[b + 1];
         const double *L = box_sizes + 3 * (size_t)b;
-        // cell side from the parameter or ~ the mean initial hsml of the box
+        // cell side from the parameter, or a factor (-cell_size, default 1) of
+        // the mean initial hsml of the box
         double s_cell = c->p.cell_size;
         if (!(s_cell > 0)) {
+            const double factor = (c->p.cell_size < 0) ? -c->p.cell_size : 1.0;
             double hsum = 0;
             int64_t cnt = 0;
             for (int64_t i = i0; i < i1; i++) {
@@ -507,6 +509,7 @@ gzm_status setup_boxes(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const do
                 if (h > 0 && isfinite(h)) { hsum += h; cnt++; }
             }
             s_cell = cnt ? hsum / cnt : cbrt(L[0] * L[1] * L[2] * 8.0 / (double)std::max<int64_t>(1, i1 - i0));
+            s_cell *= factor;
         }
         int64_t nb = 1;
         for (int ax = 0; ax < 3; ax++) {

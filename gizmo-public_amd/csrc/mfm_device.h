@@ -676,6 +676,11 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
     Fp = 0;
     mask = 0;
     const double leak = 0.5 * ((double)I.fce + (double)J.fce);
+    // FP32 only: a listed pair (r^2 < h^2 decided in FP64) whose FP32 r / h
+    // rounds to 1 on both sides has no kernel weight; in FP64 its weights are
+    // tiny but positive and the face vanishes, so give it no face here instead
+    // of the 0/0 of the volume-weighted switch below.
+    if (sizeof(T) == sizeof(float) && !(wk_i + wk_j > 0)) return 0;
     // effective face (compute_finitevol_faces.h:14-106)
     T wt_i = I.V, wt_j = J.V;
     if (M::div(M::div(fabs(I.V - J.V), tmin(I.V, J.V)), T(3)) > T(1.25)) {

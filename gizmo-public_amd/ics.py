@@ -22,8 +22,8 @@ Configurations (sizes as used by the benchmark and the parity tests):
 
 The initial smoothing length is the radius that encloses DesNumNgb particles
 at the local number density, h0 = (3 N_des / (4 pi n))^(1/3): the mean density
-of the box (or of the layer) for the lattice cases, the distance to the 32nd
-nearest neighbour for the clustered case.
+of the box (or of the layer) for the lattice cases, the distances to the 32nd and
+33rd nearest neighbours for the clustered case.
 """
 from __future__ import annotations
 
@@ -231,11 +231,15 @@ def box_slice(ic: dict, b: int) -> dict:
 
 
 def local_hsml(pos: np.ndarray, des_num_ngb: int = 32, workers: int = -1) -> np.ndarray:
-    """Distance to the des_num_ngb-th nearest other particle (periodic unit box)."""
+    """Midpoint of the distances to the des_num_ngb-th and the next nearest
+    other particle (periodic unit box): about des_num_ngb neighbours, with no
+    particle exactly at distance h.  (At r == h the pair test r^2 < h^2 can
+    pass while r / h rounds to 1, so the kernel weights of both sides vanish
+    and the volume-weighted face of compute_finitevol_faces.h:21 divides 0/0.)"""
     from scipy.spatial import cKDTree
     tree = cKDTree(pos, boxsize=1.0)
-    d, _ = tree.query(pos, k=des_num_ngb + 1, workers=workers)
-    return np.ascontiguousarray(d[:, des_num_ngb])
+    d, _ = tree.query(pos, k=des_num_ngb + 2, workers=workers)
+    return np.ascontiguousarray(0.5 * (d[:, des_num_ngb] + d[:, des_num_ngb + 1]))
 
 
 def plummer_clusters(n_total: int = 8388608, n_spheres: int = 1000, seed: int = 5, f_background: float = 0.2,
@@ -245,7 +249,7 @@ def plummer_clusters(n_total: int = 8388608, n_spheres: int = 1000, seed: int = 
     spheres with centres U(0,1)^3, masses dN/dM ~ M^slope on m_range
     (particle counts proportional to mass), scale radii log-uniform in
     a_range, radii truncated at 10 a; equal particle masses 1/N, Gaussian
-    velocities, u = u0.  hsml starts at the 32nd-neighbour distance."""
+    velocities, u = u0.  hsml starts between the 32nd and 33rd neighbour distances."""
     s = SEED_BASE + seed
     n_bg = int(round(f_background * n_total))
     n_cl = n_total - n_bg

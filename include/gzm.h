@@ -63,7 +63,8 @@ typedef struct gzm_params {
     int32_t maxiter;        /* MAXITER, 150 (allvars.h:1294) */
     int32_t device;         /* HIP device ordinal */
     int32_t precision;      /* 0: plain FP64 pass (the only mode of this release) */
-    double cell_size;       /* grid cell side; <= 0 picks one from the initial hsml */
+    double cell_size;       /* grid cell side; 0: the mean initial hsml of each box;
+                               < 0: -cell_size times that mean */
     int32_t pair_list_cap;  /* neighbours kept per particle in the pair list that the
                                gradient loop hands to the force loop: 0 = default (64),
                                rounded up to a multiple of 16, at most 4096; < 0 = no

@@ -685,7 +685,7 @@ int orc_gradients(const orc_params *p, orc_state *s, const int *active, int n_ac
             int i = active[a];
             double gr[3] = {0, 0, 0}, gp[3] = {0, 0, 0}, gv[3][3] = {{0}};
             double maxd = 0, mx_rho = 0, mn_rho = 0, mx_p = 0, mn_p = 0, mx_v[3] = {0, 0, 0}, mn_v[3] = {0, 0, 0};
-            double l1[15] = {0};
+            double l1[15] = {0}, in1[15] = {0};
             if (gasgrad_isactive(s, i)) {
                 double rho_i = s->density[i], p_i = s->pressure[i];
                 const double *v_i = s->vel + 3 * i;
@@ -717,6 +717,10 @@ int orc_gradients(const orc_params *p, orc_state *s, const int *active, int n_ac
                             l1[k] += fabs(wk_xyz_i * d_rho);
                             l1[3 + k] += fabs(wk_xyz_i * dpp);
                             for (int k2 = 0; k2 < 3; k2++) l1[6 + 3 * k2 + k] += fabs(wk_xyz_i * dv[k2]);
+                            in1[k] += fabs(wk_xyz_i) * (fabs(s->density[j]) + fabs(rho_i));
+                            in1[3 + k] += fabs(wk_xyz_i) * (fabs(s->pressure[j]) + fabs(p_i));
+                            for (int k2 = 0; k2 < 3; k2++)
+                                in1[6 + 3 * k2 + k] += fabs(wk_xyz_i) * (fabs(s->vel[3 * j + k2]) + fabs(v_i[k2]));
                         }
                     }
                 }
@@ -729,6 +733,13 @@ int orc_gradients(const orc_params *p, orc_state *s, const int *active, int n_ac
                     for (int k = 0; k < 3; k++)
                         o[3 * q + k] = fabs(nvt[3 * k + 0]) * l1[3 * q + 0] + fabs(nvt[3 * k + 1]) * l1[3 * q + 1] +
                                        fabs(nvt[3 * k + 2]) * l1[3 * q + 2];
+            }
+            if (s->grad_in) {
+                double *o = s->grad_in + 15 * (size_t)i;
+                for (int q = 0; q < 5; q++)
+                    for (int k = 0; k < 3; k++)
+                        o[3 * q + k] = fabs(nvt[3 * k + 0]) * in1[3 * q + 0] + fabs(nvt[3 * k + 1]) * in1[3 * q + 1] +
+                                       fabs(nvt[3 * k + 2]) * in1[3 * q + 2];
             }
             construct_gradient(gr, nvt);
             construct_gradient(gp, nvt);

@@ -76,6 +76,13 @@ typedef struct orc_state {
      * Such a branch is decided by rounding, so any change of summation order
      * (in the reference as well) can flip it. */
     unsigned char *tie_flag;  /* [n] */
+    /* optional: sum over the gradient pairs of |w_ij dp_ij| (|Q_i| + |Q_j|),
+     * through |NV_T| like grad_l1: the scale of the error that rounding of
+     * the inputs rho, P, v themselves (not of this stage's sums) puts into
+     * the differences Q_j - Q_i.  Where Q is uniform up to rounding (an
+     * unjittered lattice) the differences are pure rounding and grad_l1 is
+     * itself rounding-sized. */
+    double *grad_in;          /* [n][15] */
 } orc_state;
 
 typedef struct orc_stats {

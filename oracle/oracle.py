@@ -45,7 +45,7 @@ class State(C.Structure):
         ("acc", _dp), ("dtu", _dp), ("vsig", _dp),
         ("wakeup", _ub), ("is_active", _ub),
         ("divvel_l1", _dp), ("fce_l1", _dp), ("grad_l1", _dp), ("acc_l1", _dp), ("dtu_l1", _dp),
-        ("tie_flag", _ub),
+        ("tie_flag", _ub), ("grad_in", _dp),
     ]
 
 
@@ -189,6 +189,7 @@ class OracleState:
         self.divvel_l1 = z(N)
         self.fce_l1 = z(N)
         self.grad_l1 = z(N, 15)
+        self.grad_in = z(N, 15)
         self.acc_l1 = z(N, 3)
         self.dtu_l1 = z(N)
         self.tie_flag = np.zeros(N, dtype=np.uint8)
@@ -206,7 +207,7 @@ class OracleState:
             setattr(s, name, ptr(getattr(self, name)))
         s.wakeup = ptr(self.wakeup, _ub)
         s.is_active = ptr(self.is_active, _ub)
-        for name in ("divvel_l1", "fce_l1", "grad_l1", "acc_l1", "dtu_l1"):
+        for name in ("divvel_l1", "fce_l1", "grad_l1", "acc_l1", "dtu_l1", "grad_in"):
             setattr(s, name, ptr(getattr(self, name)))
         s.tie_flag = ptr(self.tie_flag, _ub)
         return s
@@ -214,7 +215,7 @@ class OracleState:
     def outputs(self) -> dict:
         names = ("hsml", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "cn", "fce",
                  "grad_rho", "grad_p", "grad_v", "acc", "dtu", "vsig", "wakeup",
-                 "divvel_l1", "fce_l1", "grad_l1", "acc_l1", "dtu_l1", "tie_flag")
+                 "divvel_l1", "fce_l1", "grad_l1", "acc_l1", "dtu_l1", "tie_flag", "grad_in")
         return {k: getattr(self, k).copy() for k in names}
 
 

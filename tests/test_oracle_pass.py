@@ -193,13 +193,12 @@ def test_generators_of_the_larger_configs():
 
     p = ics.plummer_clusters(4096, 20)
     assert p["pos"].shape == (4096, 3) and np.all((p["pos"] >= 0) & (p["pos"] < 1))
-    # hsml = distance to the 32nd neighbour: exactly 32 others strictly closer or at it
-    i = 7
-    d = p["pos"] - p["pos"][i]
-    d -= np.rint(d)
-    r = np.sqrt((d * d).sum(1))
-    r = np.sort(r)[1:]
-    assert abs(r[31] - p["hsml"][i]) < 1e-12
+    # hsml = midpoint of the 32nd and 33rd neighbour distances (brute force)
+    for i in (7, 1000, 4095):
+        d = p["pos"] - p["pos"][i]
+        d -= np.rint(d)
+        r = np.sort(np.sqrt((d * d).sum(1)))[1:]
+        assert abs(0.5 * (r[31] + r[32]) - p["hsml"][i]) < 1e-12
     assert p["hsml"].max() / p["hsml"].min() > 10
     act = ics.random_subset(1000, 0.1)
     assert act.size == 100 and np.all(np.diff(act) > 0)
