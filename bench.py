@@ -60,7 +60,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed passes (default: 100 batched, 10 blast)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("batch", "blast"), default="batch")
+    ap.add_argument("--workload", choices=("batch", "blast", "shear", "clustered"), default="batch",
+                    help="batch: 512 boxes (default); blast: 128^3 blast wave; shear: 16.8M-particle shear "
+                         "layers; clustered: Plummer spheres in a uniform background, tree search")
+    ap.add_argument("--cluster-n", type=int, default=8388608, help="particles of the clustered workload")
+    ap.add_argument("--spheres", type=int, default=1000, help="Plummer spheres of the clustered workload")
     ap.add_argument("--boxes", type=int, default=512, help="boxes of the batched workload")
     ap.add_argument("--n", type=int, default=128, help="lattice side of the blast (default 128 -> 2,097,152 particles)")
     ap.add_argument("--cpu-repeats", type=int, default=3, help="oracle passes for the CPU baseline (0 = skip)")
@@ -100,11 +104,13 @@ def cpu_cores():
     return max(1, n)
 
 
-def run_mode(gzm, ic, args, precision):
+def run_mode(gzm, ic, args, precision, search=0, active=None):
     """Time `args.steps` passes (after `args.warmup`) of one precision mode on
-    `ic`, resident on the GPU.  Returns the measurements and the last outputs."""
+    `ic`, resident on the GPU, over all particles or the `active` subset.
+    Returns the measurements and the last outputs."""
     N = ic["pos"].shape[0]
-    params = gzm.make_params(box=tuple(ic["box"]), cell_size=args.cell_size, precision=precision)
+    n_act = N if active is None else len(active)
+    params = gzm.make_params(box=tuple(ic["box"]), cell_size=args.cell_size, precision=precision, search=search)
     ctx = gzm.MFMContext(params)
     ctx.upload(ic)  # first upload allocates
     t_up = time.perf_counter()
@@ -113,9 +119,9 @@ def run_mode(gzm, ic, args, precision):
 
     def one_pass():
         ctx.restore_initial_state()
-        s1 = ctx.density()
-        s2 = ctx.gradients()
-        s3 = ctx.hydro_force()
+        s1 = ctx.density(active)
+        s2 = ctx.gradients(active)
+        s3 = ctx.hydro_force(active)
         return s1, s2, s3
 
     for _ in range(args.warmup):
@@ -137,8 +143,8 @@ def run_mode(gzm, ic, args, precision):
     # roofline of the dominant kernel (largest total device time)
     kern = {
         "density": (tm["density_ms"], tm["density_launches"], float(np.mean(updates))),
-        "gradients": (tm["gradients_ms"], tm["gradients_launches"], float(N)),
-        "force": (tm["force_ms"], tm["force_launches"], float(N)),
+        "gradients": (tm["gradients_ms"], tm["gradients_launches"], float(n_act)),
+        "force": (tm["force_ms"], tm["force_launches"], float(n_act)),
     }
     dom = max(kern, key=lambda k: kern[k][0])
     tot_ms, launches, units = kern[dom]
@@ -244,12 +250,17 @@ def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
         "density_interactions": st.density_interactions // steps,
         "grad_interactions": st.gradient_interactions // steps,
         "force_interactions": st.force_interactions // steps,
+        "exact_solver_calls": st.exact_solver_calls // steps,
     }
 
 
 def oracle_boxes(O, ic, cores):
     """FP64 oracle pass box by box; returns concatenated outputs, interactions, seconds."""
     import ics
+    if "box_offsets" not in ic:
+        t0 = time.perf_counter()
+        ref, st, _ = O.run_pass(ic, params=O.make_params(box=tuple(ic["box"]), nthreads=cores))
+        return ref, st.dens_interactions + st.grad_interactions + st.force_interactions, time.perf_counter() - t0
     offs = ic["box_offsets"]
     outs, inter, t0 = [], 0, time.perf_counter()
     prm = O.make_params(nthreads=cores)
@@ -265,9 +276,16 @@ def oracle_boxes(O, ic, cores):
 
 def main_batch(args, gzm, ics):
     steps = args.steps or 100
-    ic = ics.batch_boxes(args.boxes)
+    if args.workload == "shear":
+        ic = ics.shear_layers(224)
+        nb = 1
+        what = (f"two shear layers, density 2:1 by lattice spacing (282^2x141 + 224^2x112 lattices, "
+                f"{ic['pos'].shape[0]:,} particles), vx = +-0.5, P = 2.5")
+    else:
+        ic = ics.batch_boxes(args.boxes)
+        nb = len(ic["box_offsets"]) - 1
+        what = f"{nb} periodic boxes, jittered lattices of 16^3..64^3 particles ({ic['pos'].shape[0]:,} in all)"
     N = ic["pos"].shape[0]
-    nb = len(ic["box_offsets"]) - 1
     tuned = {}
     if args.autotune:
         import autotune
@@ -292,8 +310,7 @@ def main_batch(args, gzm, ics):
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{nb} periodic boxes, jittered lattices of 16^3..64^3 particles ({N:,} in all), "
-                        f"FP32-bulk MFM pass (density+gradients+force), {steps} passes in one HIP graph",
+            "workload": f"{what}, FP32-bulk MFM pass (density+gradients+force), {steps} passes in one HIP graph",
             "n_particles": N,
             "n_boxes": nb,
             "active": "all",
@@ -313,6 +330,8 @@ def main_batch(args, gzm, ics):
             "kernels_ms_per_pass_eager": r64["kernels_ms_per_pass_eager"],
         },
         "fp32_speedup_vs_fp64": r32["value"] / r64["value"],
+        "fp32_interactions_equal_fp64": bool(r32["interactions_per_pass"] == r64["interactions_per_pass"]),
+        "exact_solver_calls_per_pass": r64["exact_solver_calls"],
         "autotune": ({"fp32": tuned[1], "fp64": tuned[0], "seconds": round(tune_s, 1)} if tuned else None),
         "kernels_ms_per_pass_eager": r32["kernels_ms_per_pass_eager"],
         "roofline": {
@@ -340,7 +359,7 @@ def main_batch(args, gzm, ics):
             "unit": "interactions/s",
             "cores": cores,
             "kind": "port",
-            "sample": f"one full pass over all {nb} boxes ({N:,} particles), {secs:.1f} s",
+            "sample": f"one full pass over all {nb} box(es) ({N:,} particles), {secs:.1f} s",
         }
         es = error_stats(r32["out"], ref)
         result["error_vs_fp64_oracle"] = {
@@ -349,7 +368,7 @@ def main_batch(args, gzm, ics):
         errs = compare_pass(r64["out"], ref)
         result["fp64_same_workload"]["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
         result["fp64_same_workload"]["interactions_equal"] = bool(cpu_int == r64["interactions_per_pass"])
-    if args.blast:
+    if args.blast and args.workload == "batch":
         b_args = argparse.Namespace(**vars(args))
         b_args.steps = 5
         b_args.warmup = 1
@@ -369,6 +388,97 @@ def main_batch(args, gzm, ics):
     print(json.dumps(result))
 
 
+STATE_KEYS = ("hsml", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "cn", "fce",
+              "grad_rho", "grad_p", "grad_v", "acc", "dtu", "vsig")
+
+
+def main_clustered(args, gzm, ics):
+    """Plummer spheres in a uniform background, FP64 pass with the tree search:
+    every particle active, then 10 % active on the state the full pass left
+    (the other 90 % keep their hsml, density, gradients)."""
+    args.steps = steps = args.steps or 5
+    t0 = time.perf_counter()
+    ic = ics.plummer_clusters(args.cluster_n, args.spheres)
+    ic_s = time.perf_counter() - t0
+    N = ic["pos"].shape[0]
+    full = run_mode(gzm, ic, args, 0, search=1)
+    state = dict(ic)
+    for k in STATE_KEYS:
+        state[k] = full["out"][k]
+    active = ics.random_subset(N, 0.1)
+    part = run_mode(gzm, state, args, 0, search=1, active=active)
+    h = ic["hsml"]
+    result = {
+        "metric": "particle-neighbor interactions/s, MFM pass; FP32-batched vs FP64 at stated error",
+        "value": full["value"],
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": full["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.spheres} Plummer spheres + 20% uniform background ({N:,} particles), "
+                        "plain FP64 MFM pass (density+gradients+force), tree neighbour search",
+            "n_particles": N,
+            "active": "all",
+            "hsml_range": [float(h.min()), float(h.max())],
+            "interactions_per_pass": full["interactions_per_pass"],
+            "density_sweeps": full["density_sweeps"],
+            "ic_seconds": round(ic_s, 1),
+            "parallelism": "single GPU",
+        },
+        "kernels_ms_per_pass": full["kernels_ms_per_pass"],
+        "roofline": {
+            "bound": "hbm",
+            "kernel": full["dominant"],
+            "achieved": full["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": full["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": full["bytes_per_launch"],
+            "mean_launch_ms": full["mean_launch_ms"],
+        },
+        "active_10pct": {
+            "n_active": int(len(active)),
+            "value": part["value"],
+            "unit": "interactions/s",
+            "ms_per_step": part["ms_per_step"],
+            "interactions_per_pass": part["interactions_per_pass"],
+            "density_sweeps": part["density_sweeps"],
+            "kernels_ms_per_pass": part["kernels_ms_per_pass"],
+        },
+    }
+    if args.cpu_repeats > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle as O
+        from parity import compare_pass
+
+        cores = cpu_cores()
+        prm = O.make_params(box=tuple(ic["box"]), nthreads=cores)
+        t = time.perf_counter()
+        ref, st, _ = O.run_pass(ic, params=prm)
+        t = time.perf_counter() - t
+        cpu_int = st.dens_interactions + st.grad_interactions + st.force_interactions
+        result["cpu_baseline"] = {
+            "value": cpu_int / t,
+            "unit": "interactions/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"one full pass of the same {N:,}-particle clustered set ({t:.1f} s)",
+        }
+        errs = compare_pass(full["out"], ref)
+        result["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+        result["parity_vs_oracle"]["interactions_equal"] = bool(cpu_int == full["interactions_per_pass"])
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -383,8 +493,11 @@ def main():
     import gzm
     import ics
 
-    if args.workload == "batch":
+    if args.workload in ("batch", "shear"):
         main_batch(args, gzm, ics)
+        return
+    if args.workload == "clustered":
+        main_clustered(args, gzm, ics)
         return
     args.steps = args.steps or 10
     ic = ics.blast(args.n)

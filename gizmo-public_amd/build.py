@@ -30,7 +30,10 @@ HIP_FLAGS = [
     "-Wall", f"-I{INC}",
 ]
 
-HIP_SOURCES = ["csrc/gzm_api.hip", "csrc/gzm_kernels.hip", "csrc/gzm_kernels_f32.hip"]
+HIP_SOURCES = ["csrc/gzm_api.hip", "csrc/gzm_kernels.hip", "csrc/gzm_kernels_f32.hip", "csrc/gzm_tree.hip"]
+# per-file flags after HIP_FLAGS: the FP32-bulk kernels may fuse multiply-adds
+# (the FP64 files keep -ffp-contract=off to round as the CPU pass does)
+FILE_FLAGS = {"csrc/gzm_kernels_f32.hip": ["-ffp-contract=fast-honor-pragmas"]}
 HOST_SOURCES = ["host/gizmo_host.c"]
 
 
@@ -50,7 +53,7 @@ def build_gzm():
     procs = []
     for src in HIP_SOURCES:
         obj = os.path.join(LIB, os.path.basename(src).replace(".hip", ".o"))
-        cmd = [HIPCC, *HIP_FLAGS, "-c", src, "-o", obj]
+        cmd = [HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src, []), "-c", src, "-o", obj]
         print("+", " ".join(cmd), flush=True)
         procs.append((cmd, subprocess.Popen(cmd, cwd=HERE, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)

@@ -116,6 +116,69 @@ __device__ __forceinline__ void wrap_dp(const BoxInfo &box, double dp[3])
     dp[2] = wrap_axis(dp[2], box.L[2], box.half[2]);
 }
 
+// periodic distance from x to the interval [lo, hi] (both inside [0, L))
+__device__ __forceinline__ double periodic_gap(double x, double lo, double hi, double L)
+{
+    double g = lo - x;
+    const double g2 = x - hi;
+    g = (g > g2) ? g : g2;
+    g = (g > 0) ? g : 0.0;
+    const double a = x + L - hi, b = lo + L - x;  // the images x + L, x - L
+    g = (a < g) ? a : g;
+    g = (b < g) ? b : g;
+    return g;
+}
+
+// Visit (f(slot)) every particle of box G whose position may lie within R of
+// x -- and, for the pair search, within the hsml of a particle below the
+// node -- in depth-first Morton order.  Conservative by the same 1e-9
+// relative margin as the cell walk; the caller applies the exact predicate.
+template <typename F>
+__device__ __forceinline__ void tree_walk(const TreeArgs &T, const BoxDesc &G, const double x[3], double R,
+                                          bool pairs, F &&f)
+{
+    int node = G.root;
+    // a well-formed tree visits each of its 2 count - 1 nodes at most once
+    for (int steps = 2 * G.count; node != kTreeEnd && steps > 0; steps--) {
+        if (node < 0) {
+            const int j = -node - 1;
+            f(j);
+            node = T.leaf_esc[j];
+            continue;
+        }
+        const double4 lo = T.lo[node], hi = T.hi[node];
+        const double Rn = (pairs && lo.w > R) ? lo.w : R;
+        const double gx = periodic_gap(x[0], lo.x, hi.x, G.box.L[0]);
+        const double gy = periodic_gap(x[1], lo.y, hi.y, G.box.L[1]);
+        const double gz = periodic_gap(x[2], lo.z, hi.z, G.box.L[2]);
+        if (!(gx * gx + gy * gy + gz * gz <= Rn * Rn * (1.0 + 1e-9))) {
+            node = T.esc[node];
+            continue;
+        }
+        const int2 r = T.range[node];
+        if (r.y - r.x < kTreeBucket) {
+            for (int j = r.x; j <= r.y; j++) f(j);
+            node = T.esc[node];
+        } else {
+            node = T.child[node].x;
+        }
+    }
+}
+
+// Every sorted slot that may hold a neighbour of x: the cell grid within R
+// (pair search: R = max(h_i, box hmax)), or the tree (R = h_i, node hmax).
+template <typename F>
+__device__ __forceinline__ void for_each_neighbour_candidate(const TreeArgs &T, const BoxDesc &G,
+                                                             const double x[3], double h_i, bool pairs, F &&f)
+{
+    if (T.child) {
+        tree_walk(T, G, x, h_i, pairs, f);
+    } else {
+        const double R = (pairs && G.hmax > h_i) ? G.hmax : h_i;
+        for_each_candidate(G.grid, x, R, f);
+    }
+}
+
 // Run f(desc) for this lane's particle with its box descriptor, one
 // wave-uniform box at a time: the box index is taken from the first lane still
 // to run (readfirstlane), so the descriptor's fields are uniform and a

@@ -98,6 +98,7 @@ struct gzm_ctx {
     Counters *d_ctr = nullptr;
     int n_active = 0;
     bool all_active = true;
+    std::vector<int32_t> active_h;  // caller active list now in d_active (when !all_active)
     bool have_density = false, have_gradients = false;
     // exact-solver pairs deferred out of the force loop
     uint64_t *d_defer_mask = nullptr;
@@ -114,6 +115,13 @@ struct gzm_ctx {
     bool list_valid = false;
     int list_overflow_slices = 0;
     uint64_t active_version = 0, list_version = 0;
+    // neighbour-search tree (search = 1)
+    int2 *d_t_child = nullptr, *d_t_range = nullptr;
+    int *d_t_parent = nullptr, *d_t_leaf_parent = nullptr, *d_t_split = nullptr, *d_t_esc = nullptr,
+        *d_t_leaf_esc = nullptr, *d_t_flag = nullptr;
+    double4 *d_t_lo = nullptr, *d_t_hi = nullptr;
+    TreeArgs tree;      // all null in grid mode
+    TreeBuild tbuild;
 
     gzm_timing timing;
     std::vector<TimedLaunch> pending;
@@ -231,6 +239,10 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_box_of);
     dev_free(c->d_defer_mask); dev_free(c->d_defer_count); dev_free(c->d_defer_off); dev_free(c->d_defer_total);
     dev_free(c->d_defer_raw); dev_free(c->d_defer_result);
+    dev_free(c->d_t_child); dev_free(c->d_t_range); dev_free(c->d_t_parent); dev_free(c->d_t_leaf_parent);
+    dev_free(c->d_t_split); dev_free(c->d_t_esc); dev_free(c->d_t_leaf_esc); dev_free(c->d_t_flag);
+    dev_free(c->d_t_lo); dev_free(c->d_t_hi);
+    c->tree = TreeArgs{};
     c->defer_result_cap = 0;
     c->list_entries = 0;
     c->list_valid = false;
@@ -304,6 +316,11 @@ gzm_status set_active(gzm_ctx *c, const int32_t *active, int64_t n_active)
         set_error("n_active=%lld out of range [0, %d]", (long long)n_active, c->n);
         return GZM_ERR_ARG;
     }
+    // the same list as the previous stage call: d_active already holds it, and
+    // the pair list the gradient stage wrote for it stays valid for the force
+    if (!c->all_active && n_active == c->n_active && (size_t)n_active == c->active_h.size() &&
+        (n_active == 0 || memcmp(active, c->active_h.data(), sizeof(int32_t) * n_active) == 0))
+        return GZM_OK;
     // validate indices on the host before any kernel uses them
     for (int64_t k = 0; k < n_active; k++) {
         if (active[k] < 0 || active[k] >= c->n) {
@@ -324,6 +341,7 @@ gzm_status set_active(gzm_ctx *c, const int32_t *active, int64_t n_active)
     }
     c->n_active = (int)n_active;
     c->all_active = false;
+    c->active_h.assign(active, active + n_active);
     c->active_version++;
     return GZM_OK;
 }
@@ -370,6 +388,7 @@ gzm_status compute_hmax(gzm_ctx *c)
 {
     Timed tm(c, kOther);
     launch_box_hmax(c->stream, c->n_boxes, c->d_boxes, c->d_hsml, c->n_hmax_tiles, c->d_hmax_tiles);
+    if (c->tree.child) launch_tree_hmax(c->stream, c->tbuild, c->d_hsml);
     GZM_HIP(hipGetLastError());
     return GZM_OK;
 }
@@ -425,6 +444,10 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
     if (!(params->gamma > 1)) { set_error("gamma must be > 1"); return GZM_ERR_ARG; }
     if (params->precision != 0 && params->precision != 1) {
         set_error("precision must be 0 (plain FP64) or 1 (FP32 bulk)");
+        return GZM_ERR_ARG;
+    }
+    if (params->search != 0 && params->search != 1) {
+        set_error("search must be 0 (cell grid) or 1 (tree)");
         return GZM_ERR_ARG;
     }
     if (params->pair_list_cap > 4096) { set_error("pair_list_cap %d exceeds 4096", params->pair_list_cap); return GZM_ERR_ARG; }
@@ -492,6 +515,7 @@ gzm_status setup_boxes(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const do
     std::vector<BoxDesc> bx(n_boxes);
     std::vector<int2> tiles;
     int64_t ncell = 0;
+    int inode = 0;
     for (int b = 0; b < n_boxes; b++) {
         BoxDesc &B = bx[b];
         memset(&B, 0, sizeof(B));
@@ -528,6 +552,9 @@ gzm_status setup_boxes(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const do
         if (ncell > ((int64_t)1 << 30)) { set_error("grids of %lld cells too large", (long long)ncell); return GZM_ERR_ARG; }
         B.first = (int)i0;
         B.count = (int)(i1 - i0);
+        // internal tree nodes: count - 1 per box, numbered box after box
+        B.root = B.count >= 2 ? inode : (B.count == 1 ? -B.first - 1 : kTreeEnd);
+        inode += std::max(0, B.count - 1);
         for (int64_t k = i0; k < i1; k += kHmaxTile) tiles.push_back(make_int2(b, (int)k));
     }
     free_boxes(c);
@@ -617,6 +644,18 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
             s = s ? s : dev_alloc(&c->d_grecf, n);
             s = s ? s : dev_alloc(&c->d_frecf, n);
         }
+        if (c->p.search == 1) {
+            s = s ? s : dev_alloc(&c->d_t_child, n);
+            s = s ? s : dev_alloc(&c->d_t_range, n);
+            s = s ? s : dev_alloc(&c->d_t_parent, n);
+            s = s ? s : dev_alloc(&c->d_t_leaf_parent, n);
+            s = s ? s : dev_alloc(&c->d_t_split, n);
+            s = s ? s : dev_alloc(&c->d_t_esc, n);
+            s = s ? s : dev_alloc(&c->d_t_leaf_esc, n);
+            s = s ? s : dev_alloc(&c->d_t_flag, n);
+            s = s ? s : dev_alloc(&c->d_t_lo, n);
+            s = s ? s : dev_alloc(&c->d_t_hi, n);
+        }
         if (s) { free_particles(c); return s; }
     }
     gzm_status s = setup_boxes(c, v, n_boxes, box_sizes, offsets);
@@ -636,11 +675,18 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
         for (int64_t i = offsets[b]; i < offsets[b + 1]; i++) box_caller[i] = b;
     int *d_box_caller = c->d_flags;  // free until the first stage call
     GZM_HIP(hipMemcpyAsync(d_box_caller, box_caller.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
-    int key_bits = 30;
-    while ((1 << (key_bits - 30)) < n_boxes) key_bits++;
+    int box_bits = 0;
+    while ((1 << box_bits) < n_boxes) box_bits++;
+    const bool tree = c->p.search == 1;
+    // tree: Morton bits per axis that fit beside the box index in 64 bits
+    const int tree_bits = std::min(21, (64 - box_bits) / 3);
+    const int key_bits = box_bits + (tree ? 3 * tree_bits : 30);
     {
         Timed tm(c, kOther);
-        launch_cell_keys(c->stream, n, st_pos, c->d_boxes, d_box_caller, c->d_keys, c->d_iota);
+        if (tree)
+            launch_tree_keys(c->stream, n, st_pos, c->d_boxes, d_box_caller, tree_bits, c->d_keys, c->d_iota);
+        else
+            launch_cell_keys(c->stream, n, st_pos, c->d_boxes, d_box_caller, c->d_keys, c->d_iota);
         size_t bytes = 0;
         GZM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, c->d_keys, c->d_keys_sorted, c->d_iota, c->d_perm, n, 0,
                                           key_bits, c->stream));
@@ -650,7 +696,7 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
                                           key_bits, c->stream));
         launch_gather_sorted(c->stream, n, c->d_perm, st_pos, st_vel, st_mass, st_u, d_box_caller, c->d_posm,
                              c->d_velu, c->d_inv, c->d_box_of);
-        launch_cell_ranges(c->stream, n, c->d_posm, c->d_boxes, c->d_box_of, c->d_range, c->ncell);
+        if (!tree) launch_cell_ranges(c->stream, n, c->d_posm, c->d_boxes, c->d_box_of, c->d_range, c->ncell);
         if (c->p.precision == 1)
             launch_pack_geom_f32(c->stream, n, c->d_posm, c->d_velu, c->d_boxes, c->d_box_of, c->d_offm, c->d_velf,
                                  c->d_cellc);
@@ -672,6 +718,24 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
     GZM_HIP(hipMemcpyAsync(c->d_hsml0, c->d_hsml, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_cn0, c->d_cn, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_vsig0, c->d_vsig, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    if (tree) {
+        TreeBuild &t = c->tbuild;
+        t.n = n;
+        t.box_of = c->d_box_of;
+        t.boxes = c->d_boxes;
+        t.keys = c->d_keys_sorted;
+        t.posm = c->d_posm;
+        t.child = c->d_t_child; t.range = c->d_t_range; t.parent = c->d_t_parent;
+        t.leaf_parent = c->d_t_leaf_parent; t.split_right = c->d_t_split;
+        t.esc = c->d_t_esc; t.leaf_esc = c->d_t_leaf_esc; t.lo = c->d_t_lo; t.hi = c->d_t_hi; t.flag = c->d_t_flag;
+        {
+            Timed tm(c, kOther);
+            launch_tree_build(c->stream, t, c->d_hsml);
+        }
+        GZM_HIP(hipGetLastError());
+        c->tree.child = c->d_t_child; c->tree.esc = c->d_t_esc; c->tree.leaf_esc = c->d_t_leaf_esc;
+        c->tree.range = c->d_t_range; c->tree.lo = c->d_t_lo; c->tree.hi = c->d_t_hi;
+    }
 
     // optional state of particles that may stay inactive
     struct Opt { const double *h; double *d; int w; } opts[] = {
@@ -715,6 +779,7 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
     c->have_gradients = (v->grad_rho != nullptr);
     c->all_active = false;
     c->n_active = 0;
+    c->active_h.clear();
     c->list_valid = false;
     return GZM_OK;
 }
@@ -760,6 +825,7 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     memset(&a, 0, sizeof(a));
     a.boxes = c->d_boxes;
     a.box_of = c->d_box_of;
+    a.tree = c->tree;
     a.hp.des_num_ngb = c->p.des_num_ngb;
     a.hp.max_num_ngb_dev = c->p.max_num_ngb_dev;
     a.hp.min_hsml = c->p.min_hsml;
@@ -862,6 +928,7 @@ gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     memset(&a, 0, sizeof(a));
     a.boxes = c->d_boxes;
     a.box_of = c->d_box_of;
+    a.tree = c->tree;
     a.active = c->d_active;
     a.n_active = c->n_active;
     a.rec = c->d_grec;
@@ -937,6 +1004,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     memset(&a, 0, sizeof(a));
     a.boxes = c->d_boxes;
     a.box_of = c->d_box_of;
+    a.tree = c->tree;
     a.G = c->G;
     a.active = c->d_active;
     a.n_active = c->n_active;
@@ -1269,7 +1337,7 @@ gzm_status gzm_neighbor_counts(gzm_ctx *c, int32_t mode, const int32_t *active, 
     if (s) return s;
     s = compute_hmax(c);
     if (s) return s;
-    launch_neighbor_counts(c->stream, c->d_boxes, c->d_box_of, mode, na, c->d_live, c->d_posm, c->d_hsml,
+    launch_neighbor_counts(c->stream, c->d_boxes, c->d_box_of, c->tree, mode, na, c->d_live, c->d_posm, c->d_hsml,
                            counts ? c->d_flags : nullptr, c->d_ctr + kCtrMisc);
     GZM_HIP(hipGetLastError());
     if (counts) GZM_HIP(hipMemcpyAsync(counts, c->d_flags, sizeof(int) * na, hipMemcpyDeviceToHost, c->stream));
@@ -1278,6 +1346,7 @@ gzm_status gzm_neighbor_counts(gzm_ctx *c, int32_t mode, const int32_t *active, 
     if (s) return s;
     if (total) *total = (int64_t)h.interactions;
     c->all_active = false;  // d_live was used as scratch; the active list is rebuilt on the next stage call
+    c->active_h.clear();
     c->n_active = 0;
     return GZM_OK;
 }

@@ -30,7 +30,28 @@ struct BoxDesc {
     float s_f[3];          // cell side in FP32
     int cell_base;         // first cell of the box in the global range array
     int first, count;      // sorted slot range of the box
-    int pad;
+    int root;              // tree search: root node of the box (kTreeEnd if empty); with two or
+                           // more particles an internal node, the first of the box's count - 1
+};
+
+// Tree search (gzm_params.search = 1): a bounding-volume hierarchy per box
+// over the particles sorted by fine Morton key (binary radix tree from
+// adjacent key prefixes), walked without a stack through "escape" links:
+// from a node whose box is too far, continue at the node that follows its
+// subtree in depth-first order.  Node ids: internal nodes >= 0, leaf (sorted
+// slot) s as -(s + 1), end of walk kTreeEnd.  A node of at most kTreeBucket
+// particles is not descended: its sorted slot range is scanned directly.
+// The per-node hmax is the radius test of the pair search, as the octree
+// node hmax of force_update_hmax (forcetree_update.c:348-476).
+constexpr int kTreeEnd = 0x7fffffff;
+constexpr int kTreeBucket = 8;
+struct TreeArgs {
+    const int2 *child = nullptr;     // internal node: (left, right) children; null: grid search
+    const int *esc = nullptr;        // internal node: next node after its subtree
+    const int *leaf_esc = nullptr;   // sorted slot: next node after the leaf
+    const int2 *range = nullptr;     // internal node: first and last sorted slot below it
+    const double4 *lo = nullptr;     // internal node: box minimum (x, y, z), max hsml below it (w)
+    const double4 *hi = nullptr;     // internal node: box maximum (x, y, z), unused (w)
 };
 
 // Per-particle record read by the force loop for the neighbour j (and for i).
@@ -103,6 +124,7 @@ struct Counters {
 struct DensitySweepArgs {
     const BoxDesc *boxes;  // per box
     const int *box_of;     // box of each sorted slot
+    TreeArgs tree;         // tree.child != NULL: walk the tree instead of the cell grid
     HsmlParams hp;
     const int *live;
     int n_live;            // live entries, or the capacity when n_live_dev is set
@@ -132,6 +154,7 @@ struct PairList {
 struct GradientArgs {
     const BoxDesc *boxes;
     const int *box_of;
+    TreeArgs tree;
     const int *active;
     int n_active;
     const GradRec *rec;
@@ -166,6 +189,7 @@ struct DeferArgs {
 struct ForceArgs {
     const BoxDesc *boxes;
     const int *box_of;
+    TreeArgs tree;
     GammaConsts G;
     const int *active;
     int n_active;
@@ -237,8 +261,33 @@ void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float
 void launch_force_f32(hipStream_t s, const ForceArgs &a);
 void launch_force_deferred_f32(hipStream_t s, const ForceArgs &a, int n_pairs);
 
-void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, int mode, int n_active,
-                            const int *active, const double4 *posm, const double *hsml, int *counts, Counters *ctr);
+void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, const TreeArgs &tree, int mode,
+                            int n_active, const int *active, const double4 *posm, const double *hsml, int *counts,
+                            Counters *ctr);
+
+// tree search (gzm_tree.hip)
+struct TreeBuild {
+    int n;
+    const int *box_of;
+    const BoxDesc *boxes;
+    const uint64_t *keys;      // sorted fine keys
+    const double4 *posm;
+    int2 *child;               // internal nodes (n entries: at most n - 1 used)
+    int2 *range;
+    int *parent;               // internal nodes: parent node, -1 at a root
+    int *leaf_parent;          // sorted slots: parent node, -1 for a box of one particle
+    int *split_right;          // sorted slot s: right child of the node that splits after s
+    int *esc, *leaf_esc;
+    double4 *lo, *hi;
+    int *flag;                 // internal nodes: bottom-up arrival counters (left at 0)
+};
+// fine keys: box index above a 3 * bits_per_axis-bit Morton code of the position
+void launch_tree_keys(hipStream_t s, int n, const double *pos, const BoxDesc *boxes, const int *box_caller,
+                      int bits_per_axis, uint64_t *keys, int *iota);
+// radix tree, escape links, node boxes and hmax from the sorted keys and hsml
+void launch_tree_build(hipStream_t s, const TreeBuild &b, const double *hsml);
+// node hmax from the current hsml (bottom-up; node boxes unchanged)
+void launch_tree_hmax(hipStream_t s, const TreeBuild &b, const double *hsml);
 void launch_fill_int(hipStream_t s, int *p, int n, int value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value);
 void launch_fill_double(hipStream_t s, double *p, long long n, double value);

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void density_sweep_kernel(DensitySweepArgs 
         const double h2 = h * h;
         DensitySums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         const double xs[3] = {pi.x, pi.y, pi.z};
-        for_each_candidate(G.grid, xs, h, [&](int j) {
+        for_each_neighbour_candidate(a.tree, G, xs, h, false, [&](int j) {
             const double4 pj = a.posm[j];
             double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
             wrap_dp(G.box, dp);
@@ -309,8 +309,7 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
             kernel_hinv(h_i, hinv, hinv3, hinv4);
             const double h2_i = h_i * h_i;
             const double xs[3] = {ri.xh.x, ri.xh.y, ri.xh.z};
-            const double R = (h_i > G.hmax) ? h_i : G.hmax;
-            for_each_candidate(G.grid, xs, R, [&](int j) {
+            for_each_neighbour_candidate(a.tree, G, xs, h_i, true, [&](int j) {
                 const double4 xj = a.rec[j].xh;
                 double dp[3] = {xs[0] - xj.x, xs[1] - xj.y, xs[2] - xj.z};
                 wrap_dp(G.box, dp);
@@ -530,7 +529,6 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
                     add(j, Fv, Fp, vsig, mask, e2);
                 }
             };
-            const double R = (I.h > G.hmax) ? I.h : G.hmax;
             for (int pass = 0; pass < 2; pass++) {
                 if (pass == 1 && (THREE_PHASE || n_deferred == 0)) break;
                 if (USE_LIST) {
@@ -540,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
                     const int *row = a.list.idx + (size_t)t * a.list.cap;
                     for (int k = 0; k < n; k++) visit(row[k], k, false, pass);
                 } else {
-                    for_each_candidate(G.grid, x_i, R, [&](int j) { visit(j, 0, true, pass); });
+                    for_each_neighbour_candidate(a.tree, G, x_i, I.h, true, [&](int j) { visit(j, 0, true, pass); });
                 }
             }
             if (THREE_PHASE) {
@@ -660,8 +658,9 @@ __global__ void defer_total_kernel(int n, const int *offset, const int *count, i
 }
 
 // neighbour-set sizes for a fixed hsml (same predicates as the stage kernels)
-__global__ __launch_bounds__(kBlock) void neighbor_count_kernel(const BoxDesc *boxes, const int *box_of, int mode,
-                                                                int n_active, const int *active, const double4 *posm,
+__global__ __launch_bounds__(kBlock) void neighbor_count_kernel(const BoxDesc *boxes, const int *box_of,
+                                                                TreeArgs tree, int mode, int n_active,
+                                                                const int *active, const double4 *posm,
                                                                 const double *hsml, int *counts, Counters *ctr)
 {
     const int t = xcd_block() * kBlock + threadIdx.x;
@@ -672,8 +671,7 @@ __global__ __launch_bounds__(kBlock) void neighbor_count_kernel(const BoxDesc *b
         const double4 pi = posm[i];
         const double h = hsml[i], h2 = h * h;
         const double xs[3] = {pi.x, pi.y, pi.z};
-        const double R = (mode == 0) ? h : ((h > G.hmax) ? h : G.hmax);
-        for_each_candidate(G.grid, xs, R, [&](int j) {
+        for_each_neighbour_candidate(tree, G, xs, h, mode != 0, [&](int j) {
             const double4 pj = posm[j];
             double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
             wrap_dp(G.box, dp);
@@ -803,12 +801,13 @@ void launch_defer_total(hipStream_t s, int n_active, const int *offset, const in
 {
     hipLaunchKernelGGL(defer_total_kernel, dim3(1), dim3(64), 0, s, n_active, offset, count, total);
 }
-void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, int mode, int n_active,
-                            const int *active, const double4 *posm, const double *hsml, int *counts, Counters *ctr)
+void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, const TreeArgs &tree, int mode,
+                            int n_active, const int *active, const double4 *posm, const double *hsml, int *counts,
+                            Counters *ctr)
 {
     if (n_active > 0)
-        hipLaunchKernelGGL(neighbor_count_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, boxes, box_of, mode,
-                           n_active, active, posm, hsml, counts, ctr);
+        hipLaunchKernelGGL(neighbor_count_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, boxes, box_of, tree,
+                           mode, n_active, active, posm, hsml, counts, ctr);
 }
 void launch_fill_int(hipStream_t s, int *p, int n, int value)
 {

@@ -12,6 +12,11 @@
 //   * FP64 kept for the hsml iteration (hsml_update), the NV_T inversion, the
 //     slope limiter, the exact Riemann solver, the entropic-switch comparison
 //     of P/rho, and the accumulation of the conserved fluxes (dv/dt, du/dt).
+// Unlike the FP64 kernels, this file is compiled with floating-point
+// contraction (fused multiply-adds): the FP32 pass is not held to the FP64
+// oracle's rounding, only the neighbour predicate is (exact_r2), and the
+// grid walk's cell pruning carries relative margins (1e-9, 1e-6) far above a
+// one-ulp change.
 #include "grid_device.h"
 #include "gzm_internal.h"
 
@@ -22,10 +27,22 @@ namespace {
 // Bound on |r2_fp32 - r2_fp64| relative to the threshold h^2.  The offsets
 // carry 2^-24 s rounding, a cell shift dc * s_f carries 2 * 2^-24 |dc| s, and
 // the component sums, squares and their sum add 2^-24 each; for the
-// neighbourhoods walked here (|dc| s ~ r ~ h) that is below 2e-6 h^2, so 2e-5
-// leaves a factor of ten.  Pairs inside the band are decided in FP64.
+// neighbourhoods walked here (|dc| s ~ r ~ h, s <= kFineCells h) that is
+// below 2e-6 h^2, so 2e-5 leaves a factor of ten.  Pairs inside the band are
+// decided in FP64.
 constexpr double kBand = 2e-5;
 constexpr float kBandF = 2e-5f;
+
+// A pair whose smaller hsml is below 1/kFineCells of the cell side (a dense
+// clump in a grid sized for the mean hsml; every pair of the tree search)
+// takes its displacement from the FP64 positions: the offsets' absolute
+// rounding, 2^-24 s, would otherwise exceed ~1e-6 of the separation.
+constexpr float kFineCells = 16.0f;
+
+__device__ __forceinline__ float smax_of(const BoxDesc &G)
+{
+    return fmaxf(G.s_f[0], fmaxf(G.s_f[1], G.s_f[2]));
+}
 
 struct Band {
     float lo, hi;
@@ -39,14 +56,27 @@ __device__ __forceinline__ Band band_of(double t2)
 // FP32 decision of r2 < t2: 1 inside, 0 outside, -1 undecided
 __device__ __forceinline__ int decide(float r2, Band b) { return (r2 < b.lo) ? 1 : ((r2 >= b.hi) ? 0 : -1); }
 
-// squared distance exactly as the FP64 kernels compute it
+// squared distance exactly as the FP64 kernels compute it (this file is
+// compiled with contraction on, honouring pragmas; the neighbour predicate
+// must round as in the FP64 kernels, so no fused multiply-add here)
 __device__ __forceinline__ double exact_r2(const BoxInfo &box, const double4 &pi, const double4 &pj, double dp[3])
 {
+#pragma clang fp contract(off)
     dp[0] = pi.x - pj.x;
     dp[1] = pi.y - pj.y;
     dp[2] = pi.z - pj.z;
     wrap_dp(box, dp);
     return dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+}
+
+// FP32 displacement along one axis of cell-anchored positions: the cell shift
+// dc * s rounded on its own, then added to the offset difference.  No fused
+// multiply-add, so that the grid scan (shift hoisted per cell) and the pair
+// list (shift per pair) give the same bits.
+__device__ __forceinline__ float cell_shift(int dc, float s)
+{
+#pragma clang fp contract(off)
+    return (float)dc * s;
 }
 
 __device__ __forceinline__ void unpack_cell(uint32_t p, int c[3])
@@ -83,6 +113,7 @@ __global__ void pack_geom_f32_kernel(int n, const double4 *posm, const double4 *
 // ---------------------------------------------------------------------------
 // density (density.c:252-347 pair sums in FP32 + 505-850 iteration in FP64)
 // ---------------------------------------------------------------------------
+template <bool TREE>
 __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepArgs a)
 {
     const int t = xcd_block() * kBlock + threadIdx.x;
@@ -105,59 +136,65 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
         float s_ngb = 0, s_rho = 0, s_dh = 0, s_div = 0;
         float t00 = 0, t01 = 0, t02 = 0, t11 = 0, t12 = 0, t22 = 0, f0 = 0, f1 = 0, f2 = 0;
         const double xs[3] = {pi.x, pi.y, pi.z};
-        bool full[3];
-        for_each_cell(G.grid, xs, h, full, [&](int cx, int cy, int cz, int beg, int end) {
-            const bool exact_geom = full[0] || full[1] || full[2];
-            const float sh0 = (float)(ci[0] - cx) * G.s_f[0];
-            const float sh1 = (float)(ci[1] - cy) * G.s_f[1];
-            const float sh2 = (float)(ci[2] - cz) * G.s_f[2];
-            for (int k = beg; k < end; ++k) {
-                const float4 oj = a.f32.offm[k];
-                float dp[3];
-                float r2;
-                int in;
-                if (!exact_geom) {
-                    dp[0] = sh0 + (oi.x - oj.x);
-                    dp[1] = sh1 + (oi.y - oj.y);
-                    dp[2] = sh2 + (oi.z - oj.z);
-                    r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-                    in = decide(r2, bh);
-                    if (in < 0) {
-                        double d64[3];
-                        in = exact_r2(G.box, pi, a.posm[k], d64) < h2;
-                    }
-                } else {
+        auto visit = [&](int k, float sh0, float sh1, float sh2, bool exact_geom) {
+            const float4 oj = a.f32.offm[k];
+            float dp[3];
+            float r2;
+            int in;
+            if (!exact_geom) {
+                dp[0] = sh0 + (oi.x - oj.x);
+                dp[1] = sh1 + (oi.y - oj.y);
+                dp[2] = sh2 + (oi.z - oj.z);
+                r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                in = decide(r2, bh);
+                if (in < 0) {
                     double d64[3];
-                    const double r2e = exact_r2(G.box, pi, a.posm[k], d64);
-                    in = r2e < h2;
-                    dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
-                    r2 = (float)r2e;
+                    in = exact_r2(G.box, pi, a.posm[k], d64) < h2;
                 }
-                if (!in || !(oj.w > 0)) continue;
-                cnt++;
-                const float r = Real<float>::root(r2);
-                const float u = r * hinv;
-                float wk, dwk;
-                kernel_wdw(u, hinv3, hinv4, wk, dwk);
-                s_ngb += wk;
-                s_rho += oj.w * wk;
-                s_dh += -(3 * hinv * wk + u * dwk);
-                if (r > 0) {
-                    t00 += wk * dp[0] * dp[0];
-                    t01 += wk * dp[0] * dp[1];
-                    t02 += wk * dp[0] * dp[2];
-                    t11 += wk * dp[1] * dp[1];
-                    t12 += wk * dp[1] * dp[2];
-                    t22 += wk * dp[2] * dp[2];
-                    f0 += wk * dp[0];
-                    f1 += wk * dp[1];
-                    f2 += wk * dp[2];
-                    const float4 vj = a.f32.velf[k];
-                    const float dv0 = vi.x - vj.x, dv1 = vi.y - vj.y, dv2 = vi.z - vj.z;
-                    s_div -= Real<float>::div(dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2), r);
-                }
+            } else {
+                double d64[3];
+                const double r2e = exact_r2(G.box, pi, a.posm[k], d64);
+                in = r2e < h2;
+                dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+                r2 = (float)r2e;
             }
-        });
+            if (!in || !(oj.w > 0)) return;
+            cnt++;
+            const float r = Real<float>::root(r2);
+            const float u = r * hinv;
+            float wk, dwk;
+            kernel_wdw(u, hinv3, hinv4, wk, dwk);
+            s_ngb += wk;
+            s_rho += oj.w * wk;
+            s_dh += -(3 * hinv * wk + u * dwk);
+            if (r > 0) {
+                t00 += wk * dp[0] * dp[0];
+                t01 += wk * dp[0] * dp[1];
+                t02 += wk * dp[0] * dp[2];
+                t11 += wk * dp[1] * dp[1];
+                t12 += wk * dp[1] * dp[2];
+                t22 += wk * dp[2] * dp[2];
+                f0 += wk * dp[0];
+                f1 += wk * dp[1];
+                f2 += wk * dp[2];
+                const float4 vj = a.f32.velf[k];
+                const float dv0 = vi.x - vj.x, dv1 = vi.y - vj.y, dv2 = vi.z - vj.z;
+                s_div -= Real<float>::div(dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2), r);
+            }
+        };
+        if (TREE) {
+            tree_walk(a.tree, G, xs, h, false, [&](int k) { visit(k, 0.0f, 0.0f, 0.0f, true); });
+        } else {
+            const bool fine_i = (float)h * kFineCells < smax_of(G);
+            bool full[3];
+            for_each_cell(G.grid, xs, h, full, [&](int cx, int cy, int cz, int beg, int end) {
+                const bool exact_geom = full[0] || full[1] || full[2] || fine_i;
+                const float sh0 = cell_shift(ci[0] - cx, G.s_f[0]);
+                const float sh1 = cell_shift(ci[1] - cy, G.s_f[1]);
+                const float sh2 = cell_shift(ci[2] - cz, G.s_f[2]);
+                for (int k = beg; k < end; ++k) visit(k, sh0, sh1, sh2, exact_geom);
+            });
+        }
         DensitySums S = {s_ngb, s_rho, s_dh, s_div, t00, t01, t02, t11, t12, t22, f0, f1, f2};
         double left = a.left[i], right = a.right[i];
         DensityResult R;
@@ -195,7 +232,8 @@ __global__ void pack_grad_f32_kernel(int n, const float4 *offm, const float4 *ve
     rec[k] = r;
 }
 
-__global__ __launch_bounds__(kBlock) void gradients_f32_kernel(GradientArgs a)
+template <bool TREE>
+__global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a)
 {
     __shared__ int stage[kListChunk * kBlock];
     const int t = xcd_block() * kBlock + threadIdx.x;
@@ -224,83 +262,90 @@ __global__ __launch_bounds__(kBlock) void gradients_f32_kernel(GradientArgs a)
             const float rho_f = ri.vr.w, p_f = ri.pm.x;
             const double xs[3] = {pi.x, pi.y, pi.z};
             const double R = (h_i > G.hmax) ? h_i : G.hmax;
-            bool full[3];
-            for_each_cell(G.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
-                const bool exact_geom = full[0] || full[1] || full[2];
-                const float sh0 = (float)(ci[0] - cx) * G.s_f[0];
-                const float sh1 = (float)(ci[1] - cy) * G.s_f[1];
-                const float sh2 = (float)(ci[2] - cz) * G.s_f[2];
-                for (int k = beg; k < end; ++k) {
-                    if (k == i) continue;  // r2 = 0 exactly
-                    const float4 xj = a.recf[k].xh;
-                    const float hj2 = xj.w * xj.w;
-                    float dp[3];
-                    float r2;
-                    int in_i, in_j, pos;
-                    if (!exact_geom) {
-                        dp[0] = sh0 + (ri.xh.x - xj.x);
-                        dp[1] = sh1 + (ri.xh.y - xj.y);
-                        dp[2] = sh2 + (ri.xh.z - xj.z);
-                        r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-                        if ((r2 >= bi.hi) && (r2 >= hj2 * (1 + kBandF))) continue;  // clearly outside both
-                        in_i = decide(r2, bi);
-                        in_j = decide(r2, Band{hj2 * (1 - kBandF), hj2 * (1 + kBandF)});
-                        pos = (r2 >= pos_min) ? 1 : -1;
-                        if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
-                            double d64[3];
-                            const double r2e = exact_r2(G.box, pi, a.posm[k], d64);
-                            const double h_j = a.hsml[k];
-                            in_i = r2e < h2_i;
-                            in_j = r2e < h_j * h_j;
-                            pos = r2e > 0;
-                        }
-                    } else {
+            const float h_fine = smax_of(G) / kFineCells;  // pairs with a smaller hsml: FP64 geometry
+            const bool fine_i = ri.xh.w < h_fine;
+            auto visit = [&](int k, float sh0, float sh1, float sh2, bool exact_geom) {
+                if (k == i) return;  // r2 = 0 exactly
+                const float4 xj = a.recf[k].xh;
+                const float hj2 = xj.w * xj.w;
+                float dp[3];
+                float r2;
+                int in_i, in_j, pos;
+                if (!exact_geom && !(xj.w < h_fine)) {
+                    dp[0] = sh0 + (ri.xh.x - xj.x);
+                    dp[1] = sh1 + (ri.xh.y - xj.y);
+                    dp[2] = sh2 + (ri.xh.z - xj.z);
+                    r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                    if ((r2 >= bi.hi) && (r2 >= hj2 * (1 + kBandF))) return;  // clearly outside both
+                    in_i = decide(r2, bi);
+                    in_j = decide(r2, Band{hj2 * (1 - kBandF), hj2 * (1 + kBandF)});
+                    pos = (r2 >= pos_min) ? 1 : -1;
+                    if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
                         double d64[3];
                         const double r2e = exact_r2(G.box, pi, a.posm[k], d64);
                         const double h_j = a.hsml[k];
                         in_i = r2e < h2_i;
                         in_j = r2e < h_j * h_j;
                         pos = r2e > 0;
-                        dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
-                        r2 = (float)r2e;
                     }
-                    if (!pos || (!in_i && !in_j)) continue;
-                    const float4 vr = a.recf[k].vr;
-                    const float4 pm = a.recf[k].pm;
-                    if (!((pm.y > 0) && (vr.w > 0))) continue;
-                    // the list holds the force loop's pair set (m_j > 0, rho_j > 0,
-                    // hydro_evaluate.h:149-150); gradients also need h_j > 0
-                    if (list_row) {
-                        if (listed < a.list.cap) {
-                            const int c = listed & (kListChunk - 1);
-                            stage[c * kBlock + threadIdx.x] = k;
-                            if (c == kListChunk - 1) flush_list_chunk(stage, list_row + (listed - c));
-                        }
-                        listed++;
-                    }
-                    if (!(xj.w > 0)) continue;
-                    cnt++;
-                    const float r = Real<float>::root(r2);
-                    if (r > maxd) maxd = r;
-                    const float d_rho = vr.w - rho_f;
-                    minmax_check(d_rho, mn_r, mx_r);
-                    const float dpp = pm.x - p_f;
-                    minmax_check(dpp, mn_p, mx_p);
-                    const float dv0 = vr.x - ri.vr.x, dv1 = vr.y - ri.vr.y, dv2 = vr.z - ri.vr.z;
-                    minmax_check(dv0, mn_v0, mx_v0);
-                    minmax_check(dv1, mn_v1, mx_v1);
-                    minmax_check(dv2, mn_v2, mx_v2);
-                    if (in_i) {  // r < h_i (the weight vanishes at the boundary)
-                        const float wk_i = kernel_w(r * hinv, hinv3);
-                        const float w0 = -wk_i * dp[0], w1 = -wk_i * dp[1], w2 = -wk_i * dp[2];
-                        gr0 += w0 * d_rho; gr1 += w1 * d_rho; gr2 += w2 * d_rho;
-                        gp0 += w0 * dpp; gp1 += w1 * dpp; gp2 += w2 * dpp;
-                        gv00 += w0 * dv0; gv01 += w1 * dv0; gv02 += w2 * dv0;
-                        gv10 += w0 * dv1; gv11 += w1 * dv1; gv12 += w2 * dv1;
-                        gv20 += w0 * dv2; gv21 += w1 * dv2; gv22 += w2 * dv2;
-                    }
+                } else {
+                    double d64[3];
+                    const double r2e = exact_r2(G.box, pi, a.posm[k], d64);
+                    const double h_j = a.hsml[k];
+                    in_i = r2e < h2_i;
+                    in_j = r2e < h_j * h_j;
+                    pos = r2e > 0;
+                    dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+                    r2 = (float)r2e;
                 }
-            });
+                if (!pos || (!in_i && !in_j)) return;
+                const float4 vr = a.recf[k].vr;
+                const float4 pm = a.recf[k].pm;
+                if (!((pm.y > 0) && (vr.w > 0))) return;
+                // the list holds the force loop's pair set (m_j > 0, rho_j > 0,
+                // hydro_evaluate.h:149-150); gradients also need h_j > 0
+                if (list_row) {
+                    if (listed < a.list.cap) {
+                        const int c = listed & (kListChunk - 1);
+                        stage[c * kBlock + threadIdx.x] = k;
+                        if (c == kListChunk - 1) flush_list_chunk(stage, list_row + (listed - c));
+                    }
+                    listed++;
+                }
+                if (!(xj.w > 0)) return;
+                cnt++;
+                const float r = Real<float>::root(r2);
+                if (r > maxd) maxd = r;
+                const float d_rho = vr.w - rho_f;
+                minmax_check(d_rho, mn_r, mx_r);
+                const float dpp = pm.x - p_f;
+                minmax_check(dpp, mn_p, mx_p);
+                const float dv0 = vr.x - ri.vr.x, dv1 = vr.y - ri.vr.y, dv2 = vr.z - ri.vr.z;
+                minmax_check(dv0, mn_v0, mx_v0);
+                minmax_check(dv1, mn_v1, mx_v1);
+                minmax_check(dv2, mn_v2, mx_v2);
+                if (in_i) {  // r < h_i (the weight vanishes at the boundary)
+                    const float wk_i = kernel_w(r * hinv, hinv3);
+                    const float w0 = -wk_i * dp[0], w1 = -wk_i * dp[1], w2 = -wk_i * dp[2];
+                    gr0 += w0 * d_rho; gr1 += w1 * d_rho; gr2 += w2 * d_rho;
+                    gp0 += w0 * dpp; gp1 += w1 * dpp; gp2 += w2 * dpp;
+                    gv00 += w0 * dv0; gv01 += w1 * dv0; gv02 += w2 * dv0;
+                    gv10 += w0 * dv1; gv11 += w1 * dv1; gv12 += w2 * dv1;
+                    gv20 += w0 * dv2; gv21 += w1 * dv2; gv22 += w2 * dv2;
+                }
+            };
+            if (TREE) {
+                tree_walk(a.tree, G, xs, h_i, true, [&](int k) { visit(k, 0.0f, 0.0f, 0.0f, true); });
+            } else {
+                bool full[3];
+                for_each_cell(G.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
+                    const bool exact_geom = full[0] || full[1] || full[2] || fine_i;
+                    const float sh0 = cell_shift(ci[0] - cx, G.s_f[0]);
+                    const float sh1 = cell_shift(ci[1] - cy, G.s_f[1]);
+                    const float sh2 = cell_shift(ci[2] - cz, G.s_f[2]);
+                    for (int k = beg; k < end; ++k) visit(k, sh0, sh1, sh2, exact_geom);
+                });
+            }
             if (list_row && listed < a.list.cap && (listed & (kListChunk - 1)))
                 flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)));
         } else if ((m_i > 0) && (rho_i > 0)) {
@@ -399,18 +444,20 @@ __device__ __forceinline__ bool list_needs_exact_geom(const BoxDesc &G)
     return e;
 }
 
-// displacement x_i - x_j of a listed pair in FP32
+// displacement x_i - x_j of a listed pair in FP32 (the scan's choice of
+// geometry: cell offsets unless the walk spans half the box, the search is the
+// tree, or the smaller hsml is below the cell side / kFineCells)
 __device__ __forceinline__ void list_dp(const BoxDesc &G, bool exact_geom, const ForceRecF &ri, const int ci[3],
                                         const double4 &pi, const ForceRecF &Jr, const double4 *posm, int j,
                                         float dp[3])
 {
     const float4 xj = Jr.xh;
-    if (!exact_geom) {
+    if (!exact_geom && !(fminf(ri.xh.w, xj.w) * kFineCells < smax_of(G))) {
         int cj[3];
         unpack_cell(__float_as_uint(Jr.g3.w), cj);
-        dp[0] = (float)wrap_cell(ci[0] - cj[0], G.grid.nc[0]) * G.s_f[0] + (ri.xh.x - xj.x);
-        dp[1] = (float)wrap_cell(ci[1] - cj[1], G.grid.nc[1]) * G.s_f[1] + (ri.xh.y - xj.y);
-        dp[2] = (float)wrap_cell(ci[2] - cj[2], G.grid.nc[2]) * G.s_f[2] + (ri.xh.z - xj.z);
+        dp[0] = cell_shift(wrap_cell(ci[0] - cj[0], G.grid.nc[0]), G.s_f[0]) + (ri.xh.x - xj.x);
+        dp[1] = cell_shift(wrap_cell(ci[1] - cj[1], G.grid.nc[1]), G.s_f[1]) + (ri.xh.y - xj.y);
+        dp[2] = cell_shift(wrap_cell(ci[2] - cj[2], G.grid.nc[2]), G.s_f[2]) + (ri.xh.z - xj.z);
     } else {
         double d64[3];
         exact_r2(G.box, pi, posm[j], d64);
@@ -431,7 +478,7 @@ __device__ __forceinline__ void side_i_f(const ForceRecF &ri, PairI<float> &I)
 // banded predicate as the gradient kernel.  Exact-solver pairs are summed
 // after the others, by force_deferred_f32_kernel + force_finish_kernel
 // (THREE_PHASE) or by a second walk over the pairs, as in the FP64 kernel.
-template <bool USE_LIST, bool THREE_PHASE>
+template <bool USE_LIST, bool THREE_PHASE, bool TREE = false>
 __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
 {
     const int t = xcd_block() * kBlock + threadIdx.x;
@@ -501,7 +548,8 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
             const float pos_min = (float)(h2_i * kBand);
             const double xs[3] = {pi.x, pi.y, pi.z};
             const double R = (h_i > G.hmax) ? h_i : G.hmax;
-            const bool exact_geom_list = list_needs_exact_geom(G);
+            const bool exact_geom_list = a.tree.child != nullptr || list_needs_exact_geom(G);
+            const float smax = smax_of(G);
             for (int pass = 0; pass < 2; pass++) {
                 if (pass == 1 && (THREE_PHASE || n_deferred == 0)) break;
                 if (USE_LIST) {
@@ -518,54 +566,59 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
                         eval(j, k, J, dp, r2, pass);
                     }
                 } else {
-                    bool full[3];
-                    for_each_cell(G.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
-                        const bool exact_geom = full[0] || full[1] || full[2];
-                        const float sh0 = (float)(ci[0] - cx) * G.s_f[0];
-                        const float sh1 = (float)(ci[1] - cy) * G.s_f[1];
-                        const float sh2 = (float)(ci[2] - cz) * G.s_f[2];
-                        for (int j = beg; j < end; ++j) {
-                            if (j == i) continue;
-                            const float4 xj = a.recf[j].xh;
-                            const float hj2 = xj.w * xj.w;
-                            float dp[3];
-                            float r2;
-                            int in_i, in_j, pos;
-                            if (!exact_geom) {
-                                dp[0] = sh0 + (ri.xh.x - xj.x);
-                                dp[1] = sh1 + (ri.xh.y - xj.y);
-                                dp[2] = sh2 + (ri.xh.z - xj.z);
-                                r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-                                if ((r2 >= bi.hi) && (r2 >= hj2 * (1 + kBandF))) continue;
-                                in_i = decide(r2, bi);
-                                in_j = decide(r2, Band{hj2 * (1 - kBandF), hj2 * (1 + kBandF)});
-                                pos = (r2 >= pos_min) ? 1 : -1;
-                                if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
-                                    double d64[3];
-                                    const double r2e = exact_r2(G.box, pi, a.posm[j], d64);
-                                    const double h_j = a.hsml[j];
-                                    in_i = r2e < h2_i;
-                                    in_j = r2e < h_j * h_j;
-                                    pos = r2e > 0;
-                                }
-                            } else {
+                    auto visit = [&](int j, float sh0, float sh1, float sh2, bool exact_geom) {
+                        if (j == i) return;
+                        const float4 xj = a.recf[j].xh;
+                        const float hj2 = xj.w * xj.w;
+                        float dp[3];
+                        float r2;
+                        int in_i, in_j, pos;
+                        if (!exact_geom && !(fminf(ri.xh.w, xj.w) * kFineCells < smax)) {
+                            dp[0] = sh0 + (ri.xh.x - xj.x);
+                            dp[1] = sh1 + (ri.xh.y - xj.y);
+                            dp[2] = sh2 + (ri.xh.z - xj.z);
+                            r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                            if ((r2 >= bi.hi) && (r2 >= hj2 * (1 + kBandF))) return;
+                            in_i = decide(r2, bi);
+                            in_j = decide(r2, Band{hj2 * (1 - kBandF), hj2 * (1 + kBandF)});
+                            pos = (r2 >= pos_min) ? 1 : -1;
+                            if ((in_i < 0) || (in_j < 0) || (pos < 0)) {
                                 double d64[3];
                                 const double r2e = exact_r2(G.box, pi, a.posm[j], d64);
                                 const double h_j = a.hsml[j];
                                 in_i = r2e < h2_i;
                                 in_j = r2e < h_j * h_j;
                                 pos = r2e > 0;
-                                dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
-                                r2 = (float)r2e;
                             }
-                            if (!pos || (!in_i && !in_j)) continue;
-                            const ForceRecF &Jr = a.recf[j];
-                            if (!(a.posm[j].w > 0) || !(Jr.th.x > 0)) continue;  // hydro_evaluate.h:149-150
-                            PairSide<float> J;
-                            unpack_side_f(Jr, J);
-                            eval(j, 0, J, dp, r2, pass);
+                        } else {
+                            double d64[3];
+                            const double r2e = exact_r2(G.box, pi, a.posm[j], d64);
+                            const double h_j = a.hsml[j];
+                            in_i = r2e < h2_i;
+                            in_j = r2e < h_j * h_j;
+                            pos = r2e > 0;
+                            dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
+                            r2 = (float)r2e;
                         }
-                    });
+                        if (!pos || (!in_i && !in_j)) return;
+                        const ForceRecF &Jr = a.recf[j];
+                        if (!(a.posm[j].w > 0) || !(Jr.th.x > 0)) return;  // hydro_evaluate.h:149-150
+                        PairSide<float> J;
+                        unpack_side_f(Jr, J);
+                        eval(j, 0, J, dp, r2, pass);
+                    };
+                    if (TREE) {
+                        tree_walk(a.tree, G, xs, h_i, true, [&](int j) { visit(j, 0.0f, 0.0f, 0.0f, true); });
+                    } else {
+                        bool full[3];
+                        for_each_cell(G.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {
+                            const bool exact_geom = full[0] || full[1] || full[2];
+                            const float sh0 = cell_shift(ci[0] - cx, G.s_f[0]);
+                            const float sh1 = cell_shift(ci[1] - cy, G.s_f[1]);
+                            const float sh2 = cell_shift(ci[2] - cz, G.s_f[2]);
+                            for (int j = beg; j < end; ++j) visit(j, sh0, sh1, sh2, exact_geom);
+                        });
+                    }
                 }
             }
             if (THREE_PHASE) {
@@ -625,7 +678,7 @@ __global__ __launch_bounds__(kBlock) void force_deferred_f32_kernel(ForceArgs a)
         PairSide<float> J;
         unpack_side_f(Jr, J);
         float dp[3];
-        list_dp(G, list_needs_exact_geom(G), ri, ci, a.posm[i], Jr, a.posm, j, dp);
+        list_dp(G, a.tree.child != nullptr || list_needs_exact_geom(G), ri, ci, a.posm[i], Jr, a.posm, j, dp);
         const float r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
         ROut<float> ro = {0, 0, {0, 0, 0}, 0};
         float Fv[3], Fp, vsig;
@@ -657,7 +710,9 @@ void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const doubl
 }
 void launch_density_sweep_f32(hipStream_t s, const DensitySweepArgs &a)
 {
-    if (a.n_live > 0) hipLaunchKernelGGL(density_sweep_f32_kernel, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+    if (a.n_live <= 0) return;
+    if (a.tree.child) hipLaunchKernelGGL(density_sweep_f32_kernel<true>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(density_sweep_f32_kernel<false>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
 }
 void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const double *hsml,
                           const double *rho, const double *pres, GradRecF *rec)
@@ -666,7 +721,9 @@ void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4
 }
 void launch_gradients_f32(hipStream_t s, const GradientArgs &a)
 {
-    if (a.n_active > 0) hipLaunchKernelGGL(gradients_f32_kernel, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    if (a.n_active <= 0) return;
+    if (a.tree.child) hipLaunchKernelGGL(gradients_f32_kernel<true>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(gradients_f32_kernel<false>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
 }
 void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const uint32_t *cellc,
                            const double4 *posm, const double *hsml, const double *rho, const double *pres,
@@ -684,6 +741,8 @@ void launch_force_f32(hipStream_t s, const ForceArgs &a)
     if (a.list.idx && !a.fallback_only) {
         if (a.defer.mask) hipLaunchKernelGGL((force_f32_kernel<true, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((force_f32_kernel<true, false>), grid, block, 0, s, a);
+    } else if (a.tree.child) {
+        hipLaunchKernelGGL((force_f32_kernel<false, false, true>), grid, block, 0, s, a);
     } else {
         hipLaunchKernelGGL((force_f32_kernel<false, false>), grid, block, 0, s, a);
     }

@@ -1,0 +1,208 @@
+// gzm_tree.hip -- neighbour-search tree for strongly clustered particle sets.
+//
+// The cell grid sizes its cells for the mean smoothing length of a box; where
+// hsml spans orders of magnitude (a Plummer-sphere cluster: dense cores in a
+// sparse halo) the cells of the core hold thousands of particles each and
+// every core particle tests them all.  The tree adapts to the density the way
+// the reference's octree does (forcetree.c, walked by ngb.c:102-130), built
+// for the GPU from the Morton-sorted particles in four data-parallel passes:
+//
+//   1. keys:   box index above a 63-bit Morton code of the position
+//              (21 bits per axis for a single box);
+//   2. build:  one thread per internal node of the binary radix tree over the
+//              sorted keys, each finding its key range and split from the
+//              common-prefix lengths of its neighbours (Karras, "Maximizing
+//              Parallelism in the Construction of BVHs, Octrees, and k-d
+//              Trees", HPG 2012, section 4); equal keys are split by slot;
+//   3. links:  the stackless walk's escape link of every node: the node that
+//              follows a subtree in depth-first order is the right child of
+//              the node whose split lies at the subtree's last slot;
+//   4. bounds: bottom-up union of the children's boxes and hsml maxima, one
+//              thread per leaf; the second thread to reach a node (atomic
+//              arrival counter, agent-scope fences) continues upwards.
+//
+// Every pass is O(n) threads with no host synchronisation, so a rebuild of
+// the node hmax after each density stage can sit in a captured graph.
+#include "grid_device.h"
+#include "gzm_internal.h"
+
+namespace gzm {
+
+namespace {
+
+__device__ __forceinline__ uint64_t spread21(uint64_t v)
+{
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__device__ __forceinline__ uint64_t quantise(double x, double L, int bits)
+{
+    const double q = floor(x / L * (double)(1ull << bits));
+    const double top = (double)((1ull << bits) - 1);
+    return (uint64_t)(q < 0 ? 0.0 : (q > top ? top : q));
+}
+
+__global__ void tree_keys_kernel(int n, const double *pos, const BoxDesc *boxes, const int *box_caller, int bits,
+                                 uint64_t *keys, int *iota)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = box_caller ? box_caller[i] : 0;
+    const BoxInfo &B = boxes[b].box;
+    const uint64_t m = (spread21(quantise(pos[3 * i + 0], B.L[0], bits)) << 2) |
+                       (spread21(quantise(pos[3 * i + 1], B.L[1], bits)) << 1) |
+                       spread21(quantise(pos[3 * i + 2], B.L[2], bits));
+    keys[i] = ((uint64_t)b << (3 * bits)) | m;
+    iota[i] = i;
+}
+
+// common-prefix length of the keys at local slots i and j of a box of m
+// particles starting at slot `first`; -1 outside the box; equal keys compare
+// their slot numbers
+__device__ __forceinline__ int prefix(const uint64_t *keys, int first, int m, int i, int j)
+{
+    if (j < 0 || j >= m) return -1;
+    const uint64_t a = keys[first + i], b = keys[first + j];
+    if (a != b) return __clzll((long long)(a ^ b));
+    return 64 + __clz(i ^ j);
+}
+
+__device__ __forceinline__ int leaf_id(int slot) { return -slot - 1; }
+
+__global__ void tree_build_kernel(TreeBuild t)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= t.n) return;
+    const BoxDesc &B = t.boxes[t.box_of[s]];
+    const int first = B.first, m = B.count, i = s - first;
+    if (m == 1) {
+        t.leaf_parent[s] = -1;
+        return;
+    }
+    if (i >= m - 1) return;  // m - 1 internal nodes per box
+    const int g0 = B.root;
+    const uint64_t *k = t.keys;
+    // direction of the node's range and the prefix length it must exceed
+    const int d = (prefix(k, first, m, i, i + 1) > prefix(k, first, m, i, i - 1)) ? 1 : -1;
+    const int dmin = prefix(k, first, m, i, i - d);
+    int lmax = 2;
+    while (prefix(k, first, m, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int step = lmax >> 1; step >= 1; step >>= 1)
+        if (prefix(k, first, m, i, i + (l + step) * d) > dmin) l += step;
+    const int j = i + l * d;
+    // split: the last slot sharing more than the node's prefix with slot i
+    const int dnode = prefix(k, first, m, i, j);
+    int sp = 0, step = l;
+    do {
+        step = (step + 1) >> 1;
+        if (prefix(k, first, m, i, i + (sp + step) * d) > dnode) sp += step;
+    } while (step > 1);
+    const int gamma = i + sp * d + (d < 0 ? -1 : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int g = g0 + i;
+    const int left = (lo == gamma) ? leaf_id(first + gamma) : g0 + gamma;
+    const int right = (hi == gamma + 1) ? leaf_id(first + gamma + 1) : g0 + gamma + 1;
+    t.child[g] = make_int2(left, right);
+    t.range[g] = make_int2(first + lo, first + hi);
+    if (left < 0) t.leaf_parent[-left - 1] = g; else t.parent[left] = g;
+    if (right < 0) t.leaf_parent[-right - 1] = g; else t.parent[right] = g;
+    if (i == 0) t.parent[g] = -1;
+    t.split_right[first + gamma] = right;
+}
+
+__global__ void tree_links_kernel(TreeBuild t)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= t.n) return;
+    const BoxDesc &B = t.boxes[t.box_of[s]];
+    const int first = B.first, m = B.count, i = s - first;
+    const int last_slot = first + m - 1;
+    t.leaf_esc[s] = (s == last_slot) ? kTreeEnd : t.split_right[s];
+    if (i < m - 1) {
+        const int g = B.root + i;
+        const int last = t.range[g].y;
+        t.esc[g] = (last == last_slot) ? kTreeEnd : t.split_right[last];
+        t.flag[g] = 0;
+    }
+}
+
+// Bottom-up: node boxes (BOUNDS) and the max hsml below each node.  The
+// second thread to arrive at a node has both children final; it resets the
+// node's counter for the next launch and carries on to the parent.
+template <bool BOUNDS>
+__global__ void tree_bounds_kernel(TreeBuild t, const double *hsml)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= t.n) return;
+    int node = t.leaf_parent[s];
+    // depth <= 64 key bits + 32 slot bits
+    for (int depth = 0; node >= 0 && depth < 128; depth++) {
+        __threadfence();
+        if (atomicAdd(&t.flag[node], 1) == 0) return;
+        __threadfence();
+        t.flag[node] = 0;
+        const int2 c = t.child[node];
+        double4 lo[2], hi[2];
+        const int ch[2] = {c.x, c.y};
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            if (ch[q] < 0) {
+                const int j = -ch[q] - 1;
+                if (BOUNDS) {
+                    const double4 p = t.posm[j];
+                    lo[q] = make_double4(p.x, p.y, p.z, hsml[j]);
+                    hi[q] = make_double4(p.x, p.y, p.z, 0.0);
+                } else {
+                    lo[q].w = hsml[j];
+                }
+            } else if (BOUNDS) {
+                lo[q] = t.lo[ch[q]];
+                hi[q] = t.hi[ch[q]];
+            } else {
+                lo[q].w = reinterpret_cast<const double *>(t.lo + ch[q])[3];
+            }
+        }
+        const double hmax = lo[0].w > lo[1].w ? lo[0].w : lo[1].w;
+        if (BOUNDS) {
+            t.lo[node] = make_double4(fmin(lo[0].x, lo[1].x), fmin(lo[0].y, lo[1].y), fmin(lo[0].z, lo[1].z), hmax);
+            t.hi[node] = make_double4(fmax(hi[0].x, hi[1].x), fmax(hi[0].y, hi[1].y), fmax(hi[0].z, hi[1].z), 0.0);
+        } else {
+            reinterpret_cast<double *>(t.lo + node)[3] = hmax;
+        }
+        node = t.parent[node];
+    }
+}
+
+inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+void launch_tree_keys(hipStream_t s, int n, const double *pos, const BoxDesc *boxes, const int *box_caller,
+                      int bits_per_axis, uint64_t *keys, int *iota)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(tree_keys_kernel, dim3(blocks(n)), dim3(kBlock), 0, s, n, pos, boxes, box_caller,
+                           bits_per_axis, keys, iota);
+}
+
+void launch_tree_build(hipStream_t s, const TreeBuild &b, const double *hsml)
+{
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(tree_build_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(tree_links_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(tree_bounds_kernel<true>, dim3(blocks(b.n)), dim3(kBlock), 0, s, b, hsml);
+}
+
+void launch_tree_hmax(hipStream_t s, const TreeBuild &b, const double *hsml)
+{
+    if (b.n > 0) hipLaunchKernelGGL(tree_bounds_kernel<false>, dim3(blocks(b.n)), dim3(kBlock), 0, s, b, hsml);
+}
+
+}  // namespace gzm

@@ -41,7 +41,7 @@ class Params(C.Structure):
         ("precision", C.c_int32),
         ("cell_size", C.c_double),
         ("pair_list_cap", C.c_int32),
-        ("reserved", C.c_int32),
+        ("search", C.c_int32),
     ]
 
 
@@ -135,9 +135,10 @@ def _p(a, t=_dp):
 
 def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_ngb_dev=0.05,
                 min_hsml=0.0, max_hsml=1e56, first_step=1, maxiter=150, device=0, cell_size=0.0,
-                pair_list_cap=0, precision=0) -> Params:
+                pair_list_cap=0, precision=0, search=0) -> Params:
     """precision 0: plain FP64 pass; 1: FP32 pair arithmetic with FP64 hsml
-    iteration, matrix inversion, limiter and flux accumulation."""
+    iteration, matrix inversion, limiter and flux accumulation.  search 0:
+    cell grid; 1: tree with per-node hsml maxima (clustered sets)."""
     p = Params()
     p.box[:] = list(box)
     p.gamma = gamma
@@ -151,6 +152,7 @@ def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_
     p.precision = precision
     p.cell_size = cell_size
     p.pair_list_cap = pair_list_cap
+    p.search = search
     return p
 
 

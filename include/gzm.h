@@ -15,7 +15,8 @@
  * (system/code_block_xchange_perform_ops.h) over the active list
  * FirstActiveParticle/NextActiveParticle[] (allvars.h:1652-1654) with an
  * octree search (ngb.c:102-130).  Here the search is a cell grid over
- * particles sorted by cell key, and every stage is a HIP kernel for gfx950.
+ * particles sorted by cell key, or (gzm_params.search = 1) a radix tree over
+ * particles sorted by Morton key, and every stage is a HIP kernel for gfx950.
  *
  * Conventions
  *  - Plain pointers and sizes only.  Host arrays are in the caller's particle
@@ -62,7 +63,9 @@ typedef struct gzm_params {
     int32_t first_step;     /* 1 when All.Time == All.TimeBegin (density.c:509) */
     int32_t maxiter;        /* MAXITER, 150 (allvars.h:1294) */
     int32_t device;         /* HIP device ordinal */
-    int32_t precision;      /* 0: plain FP64 pass (the only mode of this release) */
+    int32_t precision;      /* 0: plain FP64 pass; 1: FP32 bulk (pair geometry, kernel
+                               weights and fluxes in FP32 against FP64 anchors, FP64
+                               hsml iteration, limiter and sums; neighbour sets exact) */
     double cell_size;       /* grid cell side; 0: the mean initial hsml of each box;
                                < 0: -cell_size times that mean */
     int32_t pair_list_cap;  /* neighbours kept per particle in the pair list that the
@@ -71,7 +74,9 @@ typedef struct gzm_params {
                                list (the force loop scans the grid); the 64-particle
                                slices holding a particle with more pairs fall back to
                                the scan */
-    int32_t reserved;
+    int32_t search;         /* neighbour search: 0 = cell grid (default); 1 = tree over
+                               Morton-sorted particles with per-node hsml maxima, for
+                               strongly clustered sets */
 } gzm_params;
 
 /* Host arrays of one particle set, caller order.  Inputs are read by

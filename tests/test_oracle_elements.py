@@ -69,6 +69,27 @@ def test_condition_number_closed_forms():
     assert cn == pytest.approx(math.sqrt((a * a + b * b + c * c) * (a**-2 + b**-2 + c**-2)) / 3, rel=1e-14)
 
 
+def test_condition_number_sensitivity_within_parity_bound():
+    """The parity bar of ConditionNumber, COND_ROUNDING (1 + 3 cn), bounds the
+    change of cn when every entry of NV_T moves by up to 5e-15 relative (the
+    rounding of a ~50-term sum in another order), from cn ~ 1 to ~1e4."""
+    from parity import COND_ROUNDING
+
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for scale in (1.0, 1e-1, 1e-2, 1e-3, 1e-4):
+        for _ in range(200):
+            Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+            T = Q @ np.diag([1.0, rng.uniform(scale, 1.0), scale]) @ Q.T
+            T = 0.5 * (T + T.T)
+            _, cn = O.matrix_invert(T)
+            Tp = T * (1.0 + 5e-15 * rng.uniform(-1.0, 1.0, size=(3, 3)))
+            _, cnp = O.matrix_invert(0.5 * (Tp + Tp.T))
+            ratio = abs(cnp - cn) / cn / (COND_ROUNDING * (1.0 + 3.0 * cn))
+            worst = max(worst, ratio)
+    assert worst <= 1.0, worst
+
+
 def test_singular_matrix_gives_zero_inverse():
     T = np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 6.0], [1.0, 1.0, 1.0]])
     Ti, cn = O.matrix_invert(T)
