@@ -65,11 +65,18 @@ def parse():
                          "layers; clustered: Plummer spheres in a uniform background, tree search")
     ap.add_argument("--cluster-n", type=int, default=8388608, help="particles of the clustered workload")
     ap.add_argument("--spheres", type=int, default=1000, help="Plummer spheres of the clustered workload")
+    ap.add_argument("--cpu-sample", type=int, default=262144,
+                    help="clustered workload: particles of the CPU-baseline sample")
+    ap.add_argument("--list-cap", type=int, default=None,
+                    help="pair-list capacity (default: 64; 256 for the clustered workload, whose particles "
+                         "near large-hsml neighbours have more than 64 force pairs)")
     ap.add_argument("--boxes", type=int, default=512, help="boxes of the batched workload")
     ap.add_argument("--n", type=int, default=128, help="lattice side of the blast (default 128 -> 2,097,152 particles)")
     ap.add_argument("--cpu-repeats", type=int, default=3, help="oracle passes for the CPU baseline (0 = skip)")
     ap.add_argument("--cell-size", type=float, default=0.0)
     ap.add_argument("--fp32", type=int, default=1, help="blast workload: also time the FP32-bulk mode")
+    ap.add_argument("--fp32-density", type=int, default=1,
+                    help="batch workload: also time precision 2 (FP32 density pair sums)")
     ap.add_argument("--blast", type=int, default=1, help="batch workload: also time the 128^3 blast case")
     ap.add_argument("--autotune", type=int, default=1,
                     help="batch workload: pick cell size and pair-list capacity per mode on a sample of boxes")
@@ -110,7 +117,8 @@ def run_mode(gzm, ic, args, precision, search=0, active=None):
     Returns the measurements and the last outputs."""
     N = ic["pos"].shape[0]
     n_act = N if active is None else len(active)
-    params = gzm.make_params(box=tuple(ic["box"]), cell_size=args.cell_size, precision=precision, search=search)
+    params = gzm.make_params(box=tuple(ic["box"]), cell_size=args.cell_size, precision=precision, search=search,
+                             pair_list_cap=getattr(args, "list_cap", None) or 0)
     ctx = gzm.MFMContext(params)
     ctx.upload(ic)  # first upload allocates
     t_up = time.perf_counter()
@@ -294,6 +302,7 @@ def main_batch(args, gzm, ics):
             tuned[prec] = autotune.tune(ic, prec)
         tune_s = time.perf_counter() - t0
     r32 = run_graph_mode(gzm, ic, args, 1, steps, tuned.get(1))
+    r32d = run_graph_mode(gzm, ic, args, 2, steps, tuned.get(1)) if args.fp32_density else None
     r64 = run_graph_mode(gzm, ic, args, 0, steps, tuned.get(0))
     traffic = measured_traffic(r32["dominant"], 1, N)
     result = {
@@ -331,6 +340,15 @@ def main_batch(args, gzm, ics):
         },
         "fp32_speedup_vs_fp64": r32["value"] / r64["value"],
         "fp32_interactions_equal_fp64": bool(r32["interactions_per_pass"] == r64["interactions_per_pass"]),
+        "fp32_density_too": (None if r32d is None else {
+            "precision": 2,
+            "value": r32d["value"],
+            "ms_per_step": r32d["ms_per_step"],
+            "speedup_vs_fp64": r32d["value"] / r64["value"],
+            "interactions_per_pass": int(r32d["interactions_per_pass"]),
+            "density_sweeps": r32d["density_sweeps"],
+            "kernels_ms_per_pass_eager": r32d["kernels_ms_per_pass_eager"],
+        }),
         "exact_solver_calls_per_pass": r64["exact_solver_calls"],
         "autotune": ({"fp32": tuned[1], "fp64": tuned[0], "seconds": round(tune_s, 1)} if tuned else None),
         "kernels_ms_per_pass_eager": r32["kernels_ms_per_pass_eager"],
@@ -361,10 +379,13 @@ def main_batch(args, gzm, ics):
             "kind": "port",
             "sample": f"one full pass over all {nb} box(es) ({N:,} particles), {secs:.1f} s",
         }
-        es = error_stats(r32["out"], ref)
-        result["error_vs_fp64_oracle"] = {
-            k: (v if isinstance(v, int) else {q: (float(f"{x:.3e}") if isinstance(x, float) else x) for q, x in v.items()})
-            for k, v in es.items()}
+        def fmt(es):
+            return {k: (v if isinstance(v, int) else {q: (float(f"{x:.3e}") if isinstance(x, float) else x)
+                                                      for q, x in v.items()})
+                    for k, v in es.items()}
+        result["error_vs_fp64_oracle"] = fmt(error_stats(r32["out"], ref))
+        if r32d is not None:
+            result["fp32_density_too"]["error_vs_fp64_oracle"] = fmt(error_stats(r32d["out"], ref))
         errs = compare_pass(r64["out"], ref)
         result["fp64_same_workload"]["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
         result["fp64_same_workload"]["interactions_equal"] = bool(cpu_int == r64["interactions_per_pass"])
@@ -397,6 +418,8 @@ def main_clustered(args, gzm, ics):
     every particle active, then 10 % active on the state the full pass left
     (the other 90 % keep their hsml, density, gradients)."""
     args.steps = steps = args.steps or 5
+    if args.list_cap is None:
+        args.list_cap = 256
     t0 = time.perf_counter()
     ic = ics.plummer_clusters(args.cluster_n, args.spheres)
     ic_s = time.perf_counter() - t0
@@ -430,6 +453,7 @@ def main_clustered(args, gzm, ics):
             "interactions_per_pass": full["interactions_per_pass"],
             "density_sweeps": full["density_sweeps"],
             "ic_seconds": round(ic_s, 1),
+            "pair_list_cap": args.list_cap,
             "parallelism": "single GPU",
         },
         "kernels_ms_per_pass": full["kernels_ms_per_pass"],
@@ -455,15 +479,22 @@ def main_clustered(args, gzm, ics):
         },
     }
     if args.cpu_repeats > 0:
+        # CPU baseline on a bounded sample: a clustered set of --cpu-sample
+        # particles with proportionally fewer spheres (same generator), FP64
+        # oracle on the host cores; the GPU pass of the same sample is held
+        # to the FP64 parity bar against it
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle as O
         from parity import compare_pass
 
+        n_s = min(args.cpu_sample, N)
+        sph_s = max(1, round(args.spheres * n_s / N))
+        sample = ics.plummer_clusters(n_s, sph_s) if n_s < N else ic
         cores = cpu_cores()
-        prm = O.make_params(box=tuple(ic["box"]), nthreads=cores)
+        prm = O.make_params(box=tuple(sample["box"]), nthreads=cores)
         t = time.perf_counter()
-        ref, st, _ = O.run_pass(ic, params=prm)
+        ref, st, _ = O.run_pass(sample, params=prm)
         t = time.perf_counter() - t
         cpu_int = st.dens_interactions + st.grad_interactions + st.force_interactions
         result["cpu_baseline"] = {
@@ -471,11 +502,15 @@ def main_clustered(args, gzm, ics):
             "unit": "interactions/s",
             "cores": cores,
             "kind": "port",
-            "sample": f"one full pass of the same {N:,}-particle clustered set ({t:.1f} s)",
+            "sample": f"one full pass of a {n_s:,}-particle clustered set ({sph_s} spheres, same generator), "
+                      f"{t:.1f} s",
         }
-        errs = compare_pass(full["out"], ref)
-        result["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
-        result["parity_vs_oracle"]["interactions_equal"] = bool(cpu_int == full["interactions_per_pass"])
+        out_s, st_s = gzm.run_pass(sample, params=gzm.make_params(box=tuple(sample["box"]), search=1,
+                                                                   pair_list_cap=args.list_cap))
+        errs = compare_pass(out_s, ref)
+        result["parity_vs_oracle_on_sample"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+        result["parity_vs_oracle_on_sample"]["interactions_equal"] = bool(
+            cpu_int == st_s["density_interactions"] + st_s["gradient_interactions"] + st_s["force_interactions"])
     print(json.dumps(result))
 
 

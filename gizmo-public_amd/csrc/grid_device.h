@@ -2,9 +2,23 @@
 // geometry, the cell-grid walk, the periodic wrap and wavefront reductions.
 #pragma once
 
+#include <type_traits>
+
 #include "gzm_internal.h"
 
 namespace gzm {
+
+// Call a walk's visitor; a visitor returning bool stops the walk with false.
+template <typename F, typename... A>
+__device__ __forceinline__ bool visit_more(F &f, A... args)
+{
+    if constexpr (std::is_void_v<decltype(f(args...))>) {
+        f(args...);
+        return true;
+    } else {
+        return f(args...);
+    }
+}
 
 constexpr int kBlock = 256;
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, each with its own L2
@@ -58,7 +72,8 @@ __device__ __forceinline__ double axis_gap(double x, double c0, double s)
 // begin, end) receives the cell's unwrapped coordinates (outside [0, nc) when
 // the walk crosses the periodic boundary; the image of a particle of that cell
 // lies at x + (c - c_wrapped) * s) and its sorted slot range.  `full` reports
-// the axes walked whole because R spans half the box.
+// the axes walked whole because R spans half the box.  An f returning bool
+// ends the walk by returning false.
 template <typename F>
 __device__ __forceinline__ void for_each_cell(const Grid &g, const double xs[3], double R, bool full[3], F &&f)
 {
@@ -92,7 +107,7 @@ __device__ __forceinline__ void for_each_cell(const Grid &g, const double xs[3],
                 const double gz = full[2] ? 0.0 : axis_gap(xs[2], cz * g.s[2], g.s[2]);
                 if (gxy2 + gz * gz > R2) continue;
                 const int2 r = g.range[rowbase + wz];
-                f(cx, cy, cz, r.x, r.y);
+                if (!visit_more(f, cx, cy, cz, r.x, r.y)) return;
             }
         }
     }
@@ -105,7 +120,9 @@ __device__ __forceinline__ void for_each_candidate(const Grid &g, const double x
 {
     bool full[3];
     for_each_cell(g, xs, R, full, [&](int, int, int, int begin, int end) {
-        for (int k = begin; k < end; ++k) f(k);
+        for (int k = begin; k < end; ++k)
+            if (!visit_more(f, k)) return false;
+        return true;
     });
 }
 
@@ -133,6 +150,7 @@ __device__ __forceinline__ double periodic_gap(double x, double lo, double hi, d
 // x -- and, for the pair search, within the hsml of a particle below the
 // node -- in depth-first Morton order.  Conservative by the same 1e-9
 // relative margin as the cell walk; the caller applies the exact predicate.
+// An f returning bool ends the walk by returning false.
 template <typename F>
 __device__ __forceinline__ void tree_walk(const TreeArgs &T, const BoxDesc &G, const double x[3], double R,
                                           bool pairs, F &&f)
@@ -142,25 +160,27 @@ __device__ __forceinline__ void tree_walk(const TreeArgs &T, const BoxDesc &G, c
     for (int steps = 2 * G.count; node != kTreeEnd && steps > 0; steps--) {
         if (node < 0) {
             const int j = -node - 1;
-            f(j);
+            if (!visit_more(f, j)) return;
             node = T.leaf_esc[j];
             continue;
         }
-        const double4 lo = T.lo[node], hi = T.hi[node];
-        const double Rn = (pairs && lo.w > R) ? lo.w : R;
-        const double gx = periodic_gap(x[0], lo.x, hi.x, G.box.L[0]);
-        const double gy = periodic_gap(x[1], lo.y, hi.y, G.box.L[1]);
-        const double gz = periodic_gap(x[2], lo.z, hi.z, G.box.L[2]);
+        const TreeNode nd = T.nodes[node];
+        const double hm = nd.lo.w;
+        const double Rn = (pairs && hm > R) ? hm : R;
+        const double gx = periodic_gap(x[0], nd.lo.x, nd.hi.x, G.box.L[0]);
+        const double gy = periodic_gap(x[1], nd.lo.y, nd.hi.y, G.box.L[1]);
+        const double gz = periodic_gap(x[2], nd.lo.z, nd.hi.z, G.box.L[2]);
+        const int esc = __float_as_int(nd.hi.w);
         if (!(gx * gx + gy * gy + gz * gz <= Rn * Rn * (1.0 + 1e-9))) {
-            node = T.esc[node];
+            node = esc;
             continue;
         }
-        const int2 r = T.range[node];
-        if (r.y - r.x < kTreeBucket) {
-            for (int j = r.x; j <= r.y; j++) f(j);
-            node = T.esc[node];
+        if (nd.link.z - nd.link.y < kTreeBucket) {
+            for (int j = nd.link.y; j <= nd.link.z; j++)
+                if (!visit_more(f, j)) return;
+            node = esc;
         } else {
-            node = T.child[node].x;
+            node = nd.link.x;
         }
     }
 }
@@ -171,12 +191,59 @@ template <typename F>
 __device__ __forceinline__ void for_each_neighbour_candidate(const TreeArgs &T, const BoxDesc &G,
                                                              const double x[3], double h_i, bool pairs, F &&f)
 {
-    if (T.child) {
+    if (T.nodes) {
         tree_walk(T, G, x, h_i, pairs, f);
     } else {
         const double R = (pairs && G.hmax > h_i) ? G.hmax : h_i;
         for_each_candidate(G.grid, x, R, f);
     }
+}
+
+// Every candidate of one particle (search radius R around x, no hsml of j),
+// spread over the 64 lanes of a wavefront that walks the tree or the grid in
+// step: each lane receives its share, f(k).  For the few particles whose
+// search covers far more candidates than a lane can walk alone.
+template <typename F>
+__device__ __forceinline__ void wave_candidates(const TreeArgs &T, const BoxDesc &G, const double x[3], double R,
+                                                int lane, F &&f)
+{
+    if (T.nodes) {
+        int node = G.root;
+        for (int steps = 2 * G.count; node != kTreeEnd && steps > 0; steps--) {
+            if (node < 0) {
+                const int j = -node - 1;
+                if (lane == 0) f(j);
+                node = T.leaf_esc[j];
+                continue;
+            }
+            const TreeNode nd = T.nodes[node];
+            const double gx = periodic_gap(x[0], nd.lo.x, nd.hi.x, G.box.L[0]);
+            const double gy = periodic_gap(x[1], nd.lo.y, nd.hi.y, G.box.L[1]);
+            const double gz = periodic_gap(x[2], nd.lo.z, nd.hi.z, G.box.L[2]);
+            const int esc = __float_as_int(nd.hi.w);
+            if (!(gx * gx + gy * gy + gz * gz <= R * R * (1.0 + 1e-9))) {
+                node = esc;
+                continue;
+            }
+            if (nd.link.z - nd.link.y < 64) {
+                if (nd.link.y + lane <= nd.link.z) f(nd.link.y + lane);
+                node = esc;
+            } else {
+                node = nd.link.x;
+            }
+        }
+    } else {
+        bool full[3];
+        for_each_cell(G.grid, x, R, full, [&](int, int, int, int beg, int end) {
+            for (int k = beg + lane; k < end; k += 64) f(k);
+        });
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
 
 // Run f(desc) for this lane's particle with its box descriptor, one

@@ -86,11 +86,12 @@ struct gzm_ctx {
     double *d_nvt = nullptr, *d_fce = nullptr, *d_grad = nullptr, *d_acc = nullptr, *d_dtu = nullptr;
     uint8_t *d_wakeup = nullptr, *d_is_active = nullptr;
     int *d_active_caller = nullptr, *d_active = nullptr, *d_live = nullptr, *d_live2 = nullptr, *d_flags = nullptr;
-    int *d_count = nullptr;
+    int *d_count = nullptr, *d_count_heavy = nullptr;
+    int *d_heavy_list = nullptr;  // density sweeps: live positions past the candidate budget
     double *d_hmax = nullptr;
     GradRec *d_grec = nullptr;
     ForceRec *d_frec = nullptr;
-    // FP32-bulk records (precision 1)
+    // FP32-bulk records (precision 1 and 2)
     float4 *d_offm = nullptr, *d_velf = nullptr;
     uint32_t *d_cellc = nullptr;
     GradRecF *d_grecf = nullptr;
@@ -100,6 +101,7 @@ struct gzm_ctx {
     bool all_active = true;
     std::vector<int32_t> active_h;  // caller active list now in d_active (when !all_active)
     bool have_density = false, have_gradients = false;
+    bool hmax_valid = false;  // box (and node) hmax reflect the current hsml
     // exact-solver pairs deferred out of the force loop
     uint64_t *d_defer_mask = nullptr;
     int *d_defer_count = nullptr, *d_defer_off = nullptr, *d_defer_total = nullptr;
@@ -115,11 +117,14 @@ struct gzm_ctx {
     bool list_valid = false;
     int list_overflow_slices = 0;
     uint64_t active_version = 0, list_version = 0;
+    std::vector<int> last_sweep_live;  // particles of each density sweep of the last eager call
     // neighbour-search tree (search = 1)
     int2 *d_t_child = nullptr, *d_t_range = nullptr;
     int *d_t_parent = nullptr, *d_t_leaf_parent = nullptr, *d_t_split = nullptr, *d_t_esc = nullptr,
         *d_t_leaf_esc = nullptr, *d_t_flag = nullptr;
     double4 *d_t_lo = nullptr, *d_t_hi = nullptr;
+    float *d_t_wmax = nullptr;
+    TreeNode *d_t_nodes = nullptr;
     TreeArgs tree;      // all null in grid mode
     TreeBuild tbuild;
 
@@ -234,6 +239,7 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_wakeup); dev_free(c->d_is_active);
     dev_free(c->d_active_caller); dev_free(c->d_active); dev_free(c->d_live); dev_free(c->d_live2);
     dev_free(c->d_flags); dev_free(c->d_grec); dev_free(c->d_frec);
+    dev_free(c->d_heavy_list);
     dev_free(c->d_offm); dev_free(c->d_velf); dev_free(c->d_cellc); dev_free(c->d_grecf); dev_free(c->d_frecf);
     dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
     dev_free(c->d_box_of);
@@ -241,7 +247,7 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_defer_raw); dev_free(c->d_defer_result);
     dev_free(c->d_t_child); dev_free(c->d_t_range); dev_free(c->d_t_parent); dev_free(c->d_t_leaf_parent);
     dev_free(c->d_t_split); dev_free(c->d_t_esc); dev_free(c->d_t_leaf_esc); dev_free(c->d_t_flag);
-    dev_free(c->d_t_lo); dev_free(c->d_t_hi);
+    dev_free(c->d_t_lo); dev_free(c->d_t_hi); dev_free(c->d_t_wmax); dev_free(c->d_t_nodes);
     c->tree = TreeArgs{};
     c->defer_result_cap = 0;
     c->list_entries = 0;
@@ -386,9 +392,11 @@ gzm_status ensure_list(gzm_ctx *c)
 // per-box max hsml into the device box descriptors (no host round trip)
 gzm_status compute_hmax(gzm_ctx *c)
 {
+    if (c->hmax_valid) return GZM_OK;
+    c->hmax_valid = true;
     Timed tm(c, kOther);
     launch_box_hmax(c->stream, c->n_boxes, c->d_boxes, c->d_hsml, c->n_hmax_tiles, c->d_hmax_tiles);
-    if (c->tree.child) launch_tree_hmax(c->stream, c->tbuild, c->d_hsml);
+    if (c->tree.nodes) launch_tree_hmax(c->stream, c->tbuild, c->d_hsml);
     GZM_HIP(hipGetLastError());
     return GZM_OK;
 }
@@ -396,16 +404,17 @@ gzm_status compute_hmax(gzm_ctx *c)
 // Ordered stream compaction: out = in[flags != 0], count into c->d_count and,
 // when `count` is not NULL, to the host.  While capturing, the temporary
 // storage must already be large enough (sized by the probe pass).
-gzm_status compact(gzm_ctx *c, const int *in, const int *flags, int n, int *out, int *count)
+gzm_status compact(gzm_ctx *c, const int *in, const int *flags, int n, int *out, int *count, int *d_count = nullptr)
 {
+    if (!d_count) d_count = c->d_count;
     size_t bytes = 0;
-    GZM_HIP(rocprim::select(nullptr, bytes, in, flags, out, c->d_count, (size_t)n, c->stream));
+    GZM_HIP(rocprim::select(nullptr, bytes, in, flags, out, d_count, (size_t)n, c->stream));
     if (c->capturing && bytes > c->temp_bytes) { set_error("temporary storage too small while capturing"); return GZM_ERR_STATE; }
     gzm_status s = ensure_temp(c, bytes);
     if (s) return s;
-    GZM_HIP(rocprim::select(c->d_temp, bytes, in, flags, out, c->d_count, (size_t)n, c->stream));
+    GZM_HIP(rocprim::select(c->d_temp, bytes, in, flags, out, d_count, (size_t)n, c->stream));
     if (count) {
-        GZM_HIP(hipMemcpyAsync(count, c->d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        GZM_HIP(hipMemcpyAsync(count, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         GZM_HIP(hipStreamSynchronize(c->stream));
     }
     return GZM_OK;
@@ -442,8 +451,8 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
         if (!(params->box[a] > 0)) { set_error("box[%d] must be > 0", a); return GZM_ERR_ARG; }
     }
     if (!(params->gamma > 1)) { set_error("gamma must be > 1"); return GZM_ERR_ARG; }
-    if (params->precision != 0 && params->precision != 1) {
-        set_error("precision must be 0 (plain FP64) or 1 (FP32 bulk)");
+    if (params->precision < 0 || params->precision > 2) {
+        set_error("precision must be 0 (plain FP64), 1 (FP32 bulk) or 2 (FP32 bulk with FP32 density sums)");
         return GZM_ERR_ARG;
     }
     if (params->search != 0 && params->search != 1) {
@@ -477,7 +486,8 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
     c->G.G7 = (0.5 * (g - 1.0));
     c->G.G8 = (1.0 / g);
     c->G.G9 = (g - 1.0);
-    if (dev_alloc(&c->d_ctr, kCtrSlots) || dev_alloc(&c->d_count, 1) || dev_alloc(&c->d_hmax, 1)) {
+    if (dev_alloc(&c->d_ctr, kCtrSlots) || dev_alloc(&c->d_count, 1) || dev_alloc(&c->d_count_heavy, 1) ||
+        dev_alloc(&c->d_hmax, 1)) {
         gzm_ctx_destroy(c);
         return GZM_ERR_NOMEM;
     }
@@ -497,6 +507,7 @@ gzm_status gzm_ctx_destroy(gzm_ctx *c)
     dev_free(c->d_temp);
     dev_free(c->d_ctr);
     dev_free(c->d_count);
+    dev_free(c->d_count_heavy);
     dev_free(c->d_hmax);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -634,6 +645,7 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
         s = s ? s : dev_alloc(&c->d_live, n);
         s = s ? s : dev_alloc(&c->d_live2, n);
         s = s ? s : dev_alloc(&c->d_flags, n);
+        s = s ? s : dev_alloc(&c->d_heavy_list, n);
         if (c->p.precision == 0) {
             s = s ? s : dev_alloc(&c->d_grec, n);
             s = s ? s : dev_alloc(&c->d_frec, n);
@@ -655,6 +667,7 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
             s = s ? s : dev_alloc(&c->d_t_flag, n);
             s = s ? s : dev_alloc(&c->d_t_lo, n);
             s = s ? s : dev_alloc(&c->d_t_hi, n);
+            s = s ? s : dev_alloc(&c->d_t_nodes, n);
         }
         if (s) { free_particles(c); return s; }
     }
@@ -697,7 +710,7 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
         launch_gather_sorted(c->stream, n, c->d_perm, st_pos, st_vel, st_mass, st_u, d_box_caller, c->d_posm,
                              c->d_velu, c->d_inv, c->d_box_of);
         if (!tree) launch_cell_ranges(c->stream, n, c->d_posm, c->d_boxes, c->d_box_of, c->d_range, c->ncell);
-        if (c->p.precision == 1)
+        if (c->p.precision != 0)
             launch_pack_geom_f32(c->stream, n, c->d_posm, c->d_velu, c->d_boxes, c->d_box_of, c->d_offm, c->d_velf,
                                  c->d_cellc);
         GZM_HIP(hipGetLastError());
@@ -728,13 +741,25 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
         t.child = c->d_t_child; t.range = c->d_t_range; t.parent = c->d_t_parent;
         t.leaf_parent = c->d_t_leaf_parent; t.split_right = c->d_t_split;
         t.esc = c->d_t_esc; t.leaf_esc = c->d_t_leaf_esc; t.lo = c->d_t_lo; t.hi = c->d_t_hi; t.flag = c->d_t_flag;
+        t.nodes = c->d_t_nodes;
+        int largest = 1;
+        for (const BoxDesc &B : c->boxes_h) largest = std::max(largest, B.count);
+        int levels = 1;
+        while ((2 << (levels - 1)) <= largest) levels++;  // 1 + floor(log2(largest))
+        if (levels != t.levels || !c->d_t_wmax) {
+            dev_free(c->d_t_wmax);
+            s = dev_alloc(&c->d_t_wmax, (size_t)levels * n);
+            if (s) return s;
+        }
+        t.levels = levels;
+        t.window_max = c->d_t_wmax;
         {
             Timed tm(c, kOther);
             launch_tree_build(c->stream, t, c->d_hsml);
         }
         GZM_HIP(hipGetLastError());
-        c->tree.child = c->d_t_child; c->tree.esc = c->d_t_esc; c->tree.leaf_esc = c->d_t_leaf_esc;
-        c->tree.range = c->d_t_range; c->tree.lo = c->d_t_lo; c->tree.hi = c->d_t_hi;
+        c->tree.nodes = c->d_t_nodes;
+        c->tree.leaf_esc = c->d_t_leaf_esc;
     }
 
     // optional state of particles that may stay inactive
@@ -781,12 +806,14 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
     c->n_active = 0;
     c->active_h.clear();
     c->list_valid = false;
+    c->hmax_valid = false;
     return GZM_OK;
 }
 
 
 gzm_status restore_state(gzm_ctx *c)
 {
+    c->hmax_valid = false;
     GZM_HIP(hipMemcpyAsync(c->d_hsml, c->d_hsml0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_cn, c->d_cn0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_vsig, c->d_vsig0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
@@ -799,13 +826,15 @@ gzm_status restore_state(gzm_ctx *c)
 struct Plan {
     int sweeps;
     bool overflow;
-    int64_t deferred;  // exact-solver pairs deferred out of the force loop
+    int64_t deferred;             // exact-solver pairs deferred out of the force loop
+    std::vector<int> sweep_live;  // particles of each density sweep
 };
 
 gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
 {
     gzm_status s;
     Counters *ctr = c->d_ctr + kCtrDensity;
+    c->hmax_valid = false;
     if (!gp) {
         s = reset_counters(c, kCtrDensity);
         if (s) return s;
@@ -837,14 +866,31 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     a.num_ngb = c->d_num_ngb; a.dhsml = c->d_dhsml; a.divvel = c->d_divvel; a.rho = c->d_rho;
     a.fce = c->d_fce; a.nvt = c->d_nvt;
     a.redo = c->d_flags;
+    a.budget = c->p.density_budget == 0 ? kDensityBudget : (c->p.density_budget < 0 ? 0 : c->p.density_budget);
+    a.heavy_list = c->d_heavy_list;
+    a.n_heavy = c->d_count_heavy;
     a.f32 = f32_geom(c);
     a.ctr = ctr;
-    auto sweep = [&]() {
+    // one sweep over up to `cap` live entries: the budgeted launch (entries
+    // past the budget are listed on the device), then one wavefront per listed
+    // entry
+    // (a sweep over at most kWaveSweepMax entries gives every entry a wavefront)
+    auto sweep = [&](int cap, int expected) -> gzm_status {
         Timed tm(c, kDensity);
-        if (c->p.precision == 1) launch_density_sweep_f32(c->stream, a);
+        if (expected <= kWaveSweepMax) {
+            DensitySweepArgs w = a;
+            w.wave_all = 1;
+            launch_density_heavy(c->stream, w, expected);
+            return GZM_OK;
+        }
+        if (a.budget) launch_fill_int(c->stream, c->d_count_heavy, 1, 0);
+        if (c->p.precision == 2) launch_density_sweep_f32(c->stream, a);
         else launch_density_sweep(c->stream, a);
+        if (a.budget) launch_density_heavy(c->stream, a, cap);
+        return GZM_OK;
     };
     int iter = 0, sweeps = 0;
+    if (!gp) c->last_sweep_live.clear();
     int64_t updates = 0;
     int status = GZM_OK;
     if (gp) {
@@ -856,7 +902,8 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
                 a.live = c->d_live;
                 a.n_live = na;
                 a.n_live_dev = c->d_count;
-                sweep();
+                s = sweep(na, sw < (int)gp->sweep_live.size() ? gp->sweep_live[sw] : na);
+                if (s) return s;
                 s = compact(c, c->d_live, c->d_flags, na, c->d_live2, nullptr);
                 if (s) return s;
                 std::swap(c->d_live, c->d_live2);
@@ -869,7 +916,9 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
             a.hp.iter = iter;
             a.live = c->d_live;
             a.n_live = n_live;
-            sweep();
+            c->last_sweep_live.push_back(n_live);
+            s = sweep(n_live, n_live);
+            if (s) return s;
             GZM_HIP(hipGetLastError());
             sweeps++;
             int n_next = 0;
@@ -919,7 +968,7 @@ gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     if (s) return s;
     {
         Timed tm(c, kOther);
-        if (c->p.precision == 1)
+        if (c->p.precision != 0)
             launch_pack_grad_f32(c->stream, c->n, c->d_offm, c->d_velf, c->d_hsml, c->d_rho, c->d_pres, c->d_grecf);
         else
             launch_pack_grad(c->stream, c->n, c->d_posm, c->d_velu, c->d_hsml, c->d_rho, c->d_pres, c->d_grec);
@@ -956,7 +1005,7 @@ gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     }
     if (c->n_active > 0) {
         Timed tm(c, kGradients);
-        if (c->p.precision == 1) launch_gradients_f32(c->stream, a);
+        if (c->p.precision != 0) launch_gradients_f32(c->stream, a);
         else launch_gradients(c->stream, a);
     }
     GZM_HIP(hipGetLastError());
@@ -987,7 +1036,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     if (s) return s;
     {
         Timed tm(c, kOther);
-        if (c->p.precision == 1)
+        if (c->p.precision != 0)
             launch_pack_force_f32(c->stream, c->n, c->d_offm, c->d_velf, c->d_cellc, c->d_posm, c->d_hsml, c->d_rho,
                                   c->d_pres, c->d_dhsml, c->d_cn, c->d_fce, c->d_nvt, c->d_grad, c->p.gamma,
                                   c->d_frecf);
@@ -1043,7 +1092,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     }
     if (c->n_active > 0) {
         Timed tm(c, kForce);
-        auto launch = (c->p.precision == 1) ? launch_force_f32 : launch_force;
+        auto launch = (c->p.precision != 0) ? launch_force_f32 : launch_force;
         launch(c->stream, a);
         if (use_list && c->list_overflow_slices > 0) {
             ForceArgs b = a;
@@ -1092,7 +1141,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
             a.defer.q_begin = (int)q0;
             a.defer.q_end = (int)q1;
             Timed tm(c, kForceDeferred);
-            if (c->p.precision == 1) launch_force_deferred_f32(c->stream, a, (int)(q1 - q0));
+            if (c->p.precision != 0) launch_force_deferred_f32(c->stream, a, (int)(q1 - q0));
             else launch_force_deferred(c->stream, a, (int)(q1 - q0));
             launch_force_finish(c->stream, a);
             if (D == 0) break;
@@ -1184,7 +1233,7 @@ gzm_status gzm_run_passes(gzm_ctx *c, int32_t n_passes, gzm_pass_stats *stats)
     s = s ? s : gradients_impl(c, nullptr, &probe);
     s = s ? s : force_impl(c, nullptr, &probe);
     if (s) return s;
-    const Plan gp = {probe.density_sweeps, c->list_overflow_slices > 0, c->last_deferred};
+    const Plan gp = {probe.density_sweeps, c->list_overflow_slices > 0, c->last_deferred, c->last_sweep_live};
     {
         // the captured compactions run over the whole active list
         size_t bytes = 0;

@@ -45,13 +45,21 @@ struct BoxDesc {
 // node hmax of force_update_hmax (forcetree_update.c:348-476).
 constexpr int kTreeEnd = 0x7fffffff;
 constexpr int kTreeBucket = 8;
+
+// What the walk reads of an internal node, in one 48-byte record (one
+// dependent load per visited node): its box rounded outwards to FP32, its
+// hsml maximum rounded up, the escape link, the left child and the slot
+// range.  Rounding only enlarges the box, so the opening test stays
+// conservative; the exact predicates decide every pair.
+struct alignas(16) TreeNode {
+    float4 lo;             // x, y, z rounded down; w = max hsml below the node, rounded up
+    float4 hi;             // x, y, z rounded up; w = escape link (int bits)
+    int4 link;             // left child, first slot, last slot, unused
+};
+
 struct TreeArgs {
-    const int2 *child = nullptr;     // internal node: (left, right) children; null: grid search
-    const int *esc = nullptr;        // internal node: next node after its subtree
-    const int *leaf_esc = nullptr;   // sorted slot: next node after the leaf
-    const int2 *range = nullptr;     // internal node: first and last sorted slot below it
-    const double4 *lo = nullptr;     // internal node: box minimum (x, y, z), max hsml below it (w)
-    const double4 *hi = nullptr;     // internal node: box maximum (x, y, z), unused (w)
+    const TreeNode *nodes = nullptr;  // internal nodes; null: grid search
+    const int *leaf_esc = nullptr;    // sorted slot: next node after the leaf
 };
 
 // Per-particle record read by the force loop for the neighbour j (and for i).
@@ -121,10 +129,17 @@ struct Counters {
     int overflow_slices;
 };
 
+// candidates a density-sweep lane walks before it defers its particle to the
+// second, load-balanced launch (gzm_params.density_budget = 0)
+constexpr int kDensityBudget = 512;
+// a sweep over at most this many particles runs one wavefront per particle:
+// they all fit on the GPU at once, and a walk shared by 64 lanes is shorter
+constexpr int kWaveSweepMax = 4096;
+
 struct DensitySweepArgs {
     const BoxDesc *boxes;  // per box
     const int *box_of;     // box of each sorted slot
-    TreeArgs tree;         // tree.child != NULL: walk the tree instead of the cell grid
+    TreeArgs tree;         // tree.nodes != NULL: walk the tree instead of the cell grid
     HsmlParams hp;
     const int *live;
     int n_live;            // live entries, or the capacity when n_live_dev is set
@@ -134,16 +149,26 @@ struct DensitySweepArgs {
     double *hsml, *left, *right, *cn;
     double *num_ngb, *dhsml, *divvel, *rho, *fce, *nvt;  // nvt: 6 per particle
     int *redo;             // per live entry: 1 = sweep again
+    // Load balance: a particle whose walk passes `budget` candidates (0: no
+    // limit) stops and appends its live position to heavy_list (count
+    // n_heavy); a second launch gives each such particle a whole wavefront,
+    // so one expensive search neither holds a wavefront of cheap ones nor
+    // runs on a single lane.
+    int budget;
+    int *heavy_list;
+    int *n_heavy;
+    int wave_all;          // wavefront-per-particle launch over all live entries (a small sweep)
     F32Geom f32;           // FP32-bulk kernels only
     Counters *ctr;
 };
 
 // Pair list: row t (entry t of the active list) holds the force-loop pair
 // set of that particle in scan order, idx[t * cap + k]; cap is a multiple of
-// 16 so each row is a whole number of 64-byte chunks.  The 64 consecutive
-// entries of the active list that one wavefront handles form a slice; a
-// slice in which any particle has more than `cap` pairs is flagged and
-// handled by the scanning fallback.
+// 16 so each row is a whole number of 64-byte chunks.  A particle with more
+// than `cap` pairs (count > cap) is left to the scanning fallback; the 64
+// consecutive entries of the active list that one wavefront handles form a
+// slice, flagged when it holds such a particle (the fallback kernel runs
+// only when some slice is flagged).
 struct PairList {
     int *idx;              // (n_slices * 64 * cap)
     int *count;            // per active entry
@@ -261,6 +286,10 @@ void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float
 void launch_force_f32(hipStream_t s, const ForceArgs &a);
 void launch_force_deferred_f32(hipStream_t s, const ForceArgs &a, int n_pairs);
 
+// density sweep of the entries in a.heavy_list (*a.n_heavy of them), one
+// wavefront each, FP64 sums (also for the FP32-bulk sweeps)
+void launch_density_heavy(hipStream_t s, const DensitySweepArgs &a, int max_entries);
+
 void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, const TreeArgs &tree, int mode,
                             int n_active, const int *active, const double4 *posm, const double *hsml, int *counts,
                             Counters *ctr);
@@ -278,15 +307,21 @@ struct TreeBuild {
     int *leaf_parent;          // sorted slots: parent node, -1 for a box of one particle
     int *split_right;          // sorted slot s: right child of the node that splits after s
     int *esc, *leaf_esc;
-    double4 *lo, *hi;
+    double4 *lo, *hi;          // FP64 node boxes (build only)
     int *flag;                 // internal nodes: bottom-up arrival counters (left at 0)
+    TreeNode *nodes;           // the walk's records
+    // node hmax refresh: max of hsml over a node's slot range from a table of
+    // window maxima, level k holding max(hsml[s .. s + 2^k - 1]) rounded up to
+    // FP32 (a bound, never below the FP64 value)
+    float *window_max;         // levels * n
+    int levels;                // 1 + floor(log2(largest box))
 };
 // fine keys: box index above a 3 * bits_per_axis-bit Morton code of the position
 void launch_tree_keys(hipStream_t s, int n, const double *pos, const BoxDesc *boxes, const int *box_caller,
                       int bits_per_axis, uint64_t *keys, int *iota);
 // radix tree, escape links, node boxes and hmax from the sorted keys and hsml
 void launch_tree_build(hipStream_t s, const TreeBuild &b, const double *hsml);
-// node hmax from the current hsml (bottom-up; node boxes unchanged)
+// node hmax from the current hsml (window-maximum table; node boxes unchanged)
 void launch_tree_hmax(hipStream_t s, const TreeBuild &b, const double *hsml);
 void launch_fill_int(hipStream_t s, int *p, int n, int value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value);

@@ -183,6 +183,62 @@ __global__ void density_init_kernel(int n_active, const int *active, const doubl
     if ((h < 0) || !isfinite(h) || (h > 0.99 * max_hsml)) hsml[i] = 0.99 * max_hsml;  // density.c:484
 }
 
+// one density pair (density.c:252-347): j's terms into S when r_ij < h_i
+__device__ __forceinline__ bool density_pair(const BoxDesc &G, const double4 &pi, const double4 &vi,
+                                             const DensitySweepArgs &a, int j, double h2, double hinv,
+                                             double hinv3, double hinv4, DensitySums &S)
+{
+    const double4 pj = a.posm[j];
+    double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+    wrap_dp(G.box, dp);
+    const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+    if (!(r2 < h2 && pj.w > 0)) return false;
+    const double r = sqrt(r2);
+    const double u = r * hinv;
+    double wk, dwk;
+    kernel_wdw(u, hinv3, hinv4, wk, dwk);
+    S.ngb += wk;
+    S.rho += pj.w * wk;
+    S.dhsml += -(3 * hinv * wk + u * dwk);
+    if (r > 0) {
+        S.t00 += wk * dp[0] * dp[0];
+        S.t01 += wk * dp[0] * dp[1];
+        S.t02 += wk * dp[0] * dp[2];
+        S.t11 += wk * dp[1] * dp[1];
+        S.t12 += wk * dp[1] * dp[2];
+        S.t22 += wk * dp[2] * dp[2];
+        S.f0 += wk * dp[0];
+        S.f1 += wk * dp[1];
+        S.f2 += wk * dp[2];
+        const double4 vj = a.velu[j];
+        const double dv0 = vi.x - vj.x, dv1 = vi.y - vj.y, dv2 = vi.z - vj.z;
+        S.divvel -= dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2) / r;
+    }
+    return true;
+}
+
+// the hsml update of one particle from its sums, and its outputs
+__device__ __forceinline__ void density_store(const DensitySweepArgs &a, int t, int i, double h, const DensitySums &S)
+{
+    double left = a.left[i], right = a.right[i];
+    DensityResult R;
+    const int res = hsml_update(a.hp, S, h, left, right, a.cn[i], R);
+    a.hsml[i] = h;
+    a.left[i] = left;
+    a.right[i] = right;
+    a.cn[i] = R.cn_stored;
+    a.num_ngb[i] = R.num_ngb;
+    a.dhsml[i] = R.dhsml;
+    a.divvel[i] = R.divvel;
+    a.rho[i] = R.rho_sum;
+    a.fce[i] = R.fce;
+#pragma unroll
+    for (int k = 0; k < 6; k++) a.nvt[6 * (size_t)i + k] = R.nvt[k];
+    a.redo[t] = (res == 1) ? 1 : 0;
+    if (res < 0) atomicMax(&a.ctr->status, (int)6);
+}
+
+template <bool TREE>
 __global__ __launch_bounds__(kBlock) void density_sweep_kernel(DensitySweepArgs a)
 {
     const int t = xcd_block() * kBlock + threadIdx.x;
@@ -194,60 +250,89 @@ __global__ __launch_bounds__(kBlock) void density_sweep_kernel(DensitySweepArgs 
         const int i = a.live[t];
         const double4 pi = a.posm[i];
         const double4 vi = a.velu[i];
-        double h = a.hsml[i];
+        const double h = a.hsml[i];
         double hinv, hinv3, hinv4;
         kernel_hinv(h, hinv, hinv3, hinv4);
         const double h2 = h * h;
         DensitySums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         const double xs[3] = {pi.x, pi.y, pi.z};
-        for_each_neighbour_candidate(a.tree, G, xs, h, false, [&](int j) {
-            const double4 pj = a.posm[j];
-            double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-            wrap_dp(G.box, dp);
-            const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-            if (r2 < h2 && pj.w > 0) {
-                cnt++;
-                const double r = sqrt(r2);
-                const double u = r * hinv;
-                double wk, dwk;
-                kernel_wdw(u, hinv3, hinv4, wk, dwk);
-                S.ngb += wk;
-                S.rho += pj.w * wk;
-                S.dhsml += -(3 * hinv * wk + u * dwk);
-                if (r > 0) {
-                    S.t00 += wk * dp[0] * dp[0];
-                    S.t01 += wk * dp[0] * dp[1];
-                    S.t02 += wk * dp[0] * dp[2];
-                    S.t11 += wk * dp[1] * dp[1];
-                    S.t12 += wk * dp[1] * dp[2];
-                    S.t22 += wk * dp[2] * dp[2];
-                    S.f0 += wk * dp[0];
-                    S.f1 += wk * dp[1];
-                    S.f2 += wk * dp[2];
-                    const double4 vj = a.velu[j];
-                    const double dv0 = vi.x - vj.x, dv1 = vi.y - vj.y, dv2 = vi.z - vj.z;
-                    S.divvel -= dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2) / r;
+        const int budget = a.budget;
+        int visited = 0;
+        bool heavy = false;
+        unsigned long long lane_cnt = 0;
+        auto pair = [&](int j) {
+            if (density_pair(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S)) lane_cnt++;
+        };
+        if (TREE) {
+            tree_walk(a.tree, G, xs, h, false, [&](int j) -> bool {
+                if (budget && ++visited > budget) {
+                    heavy = true;
+                    return false;
                 }
-            }
-        });
-        double left = a.left[i], right = a.right[i];
-        DensityResult R;
-        const int res = hsml_update(a.hp, S, h, left, right, a.cn[i], R);
-        a.hsml[i] = h;
-        a.left[i] = left;
-        a.right[i] = right;
-        a.cn[i] = R.cn_stored;
-        a.num_ngb[i] = R.num_ngb;
-        a.dhsml[i] = R.dhsml;
-        a.divvel[i] = R.divvel;
-        a.rho[i] = R.rho_sum;
-        a.fce[i] = R.fce;
-#pragma unroll
-        for (int k = 0; k < 6; k++) a.nvt[6 * (size_t)i + k] = R.nvt[k];
-        a.redo[t] = (res == 1) ? 1 : 0;
-        if (res < 0) atomicMax(&a.ctr->status, (int)6);
+                pair(j);
+                return true;
+            });
+        } else {
+            // the budget is charged a cell at a time, outside the candidate loop
+            bool full[3];
+            for_each_cell(G.grid, xs, h, full, [&](int, int, int, int beg, int end) -> bool {
+                visited += end - beg;
+                if (budget && visited > budget) {
+                    heavy = true;
+                    return false;
+                }
+                for (int j = beg; j < end; ++j) pair(j);
+                return true;
+            });
+        }
+        if (heavy) {  // left to density_heavy_kernel
+            a.heavy_list[atomicAdd(a.n_heavy, 1)] = t;
+            return;
+        }
+        cnt += lane_cnt;
+        density_store(a, t, i, h, S);
     });
     wave_add_u64(&a.ctr->interactions, cnt);
+}
+
+// The particles that passed the candidate budget: one wavefront each, its
+// lanes sharing the walk (wave_candidates); the sums are reduced across the
+// wavefront (another summation order than a single lane's, FP64 throughout).
+// A fixed grid strides over the list, so the launch is sized independently of
+// the (device-side) count.
+__global__ __launch_bounds__(kBlock) void density_heavy_kernel(DensitySweepArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int n_waves = (int)(gridDim.x * (kBlock / 64));
+    const int n = a.wave_all ? (a.n_live_dev ? *a.n_live_dev : a.n_live) : *a.n_heavy;
+    if (a.wave_all && a.n_live_dev) {  // captured pass: clear the flags up to the capacity
+        for (int t = n + (int)(blockIdx.x * kBlock + threadIdx.x); t < a.n_live; t += (int)(gridDim.x * kBlock))
+            a.redo[t] = 0;
+    }
+    for (int u = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6); u < n; u += n_waves) {
+        const int t = a.wave_all ? u : a.heavy_list[u];
+        const int i = a.live[t];
+        const BoxDesc &G = a.boxes[a.box_of[i]];
+        const double4 pi = a.posm[i];
+        const double4 vi = a.velu[i];
+        const double h = a.hsml[i];
+        double hinv, hinv3, hinv4;
+        kernel_hinv(h, hinv, hinv3, hinv4);
+        const double h2 = h * h;
+        DensitySums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const double xs[3] = {pi.x, pi.y, pi.z};
+        unsigned long long c = 0;
+        wave_candidates(a.tree, G, xs, h, lane, [&](int j) {
+            if (density_pair(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S)) c++;
+        });
+        S.ngb = wave_sum(S.ngb); S.rho = wave_sum(S.rho); S.dhsml = wave_sum(S.dhsml);
+        S.divvel = wave_sum(S.divvel);
+        S.t00 = wave_sum(S.t00); S.t01 = wave_sum(S.t01); S.t02 = wave_sum(S.t02);
+        S.t11 = wave_sum(S.t11); S.t12 = wave_sum(S.t12); S.t22 = wave_sum(S.t22);
+        S.f0 = wave_sum(S.f0); S.f1 = wave_sum(S.f1); S.f2 = wave_sum(S.f2);
+        wave_add_u64(&a.ctr->interactions, c);
+        if (lane == 0) density_store(a, t, i, h, S);
+    }
 }
 
 __global__ void density_finalize_kernel(int n_active, const int *active, const double4 *posm, const double4 *velu,
@@ -356,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
             if (list_row && listed < a.list.cap && (listed & (kListChunk - 1)))
                 flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)));
         } else if ((m_i > 0) && (rho_i > 0)) {
-            listed = a.list.cap + 1;  // the force loop still runs for i: send its slice to the scan
+            listed = a.list.cap + 1;  // the force loop still runs for i: send it to the scan
         }
         double nvt[6];
 #pragma unroll
@@ -443,9 +528,9 @@ __device__ __forceinline__ void unpack_side(const ForceRec &r, PairSide<double> 
 }
 
 // One particle's force loop.  USE_LIST: walk the pair list written by the
-// gradient kernel (slices that overflowed it are skipped and left to the
-// scanning variant); otherwise scan the grid (every slice, or only the
-// overflowed ones when a.fallback_only).  Pairs that reach the exact Riemann
+// gradient kernel (particles that overflowed it are skipped and left to the
+// scanning variant); otherwise scan the grid or tree (every particle, or only
+// the overflowed ones when a.fallback_only).  Pairs that reach the exact Riemann
 // solver are summed after all other pairs of the particle: THREE_PHASE (list
 // kernel with defer buffers) leaves them to force_deferred_kernel and
 // force_finish_kernel, otherwise the loop walks the pairs a second time and
@@ -457,7 +542,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
     const int t = xcd_block() * kBlock + threadIdx.x;
     bool valid = t < a.n_active;
     if (USE_LIST) {
-        if (valid && a.list.overflow[t >> 6]) {
+        if (valid && a.list.count[t] > a.list.cap) {
             valid = false;
             if (THREE_PHASE) {  // finished by the scanning kernel
                 a.defer.count[t] = 0;
@@ -465,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
             }
         }
     } else if (a.fallback_only) {
-        if (valid && !a.list.overflow[t >> 6]) valid = false;
+        if (valid && !(a.list.count[t] > a.list.cap)) valid = false;
     }
     unsigned long long cnt = 0, nretry = 0, nexact = 0;
     for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
@@ -753,7 +838,9 @@ void launch_density_init(hipStream_t s, int n_active, const int *active, const d
 }
 void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a)
 {
-    if ((a.n_live) > 0) hipLaunchKernelGGL(density_sweep_kernel, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+    if (a.n_live <= 0) return;
+    if (a.tree.nodes) hipLaunchKernelGGL(density_sweep_kernel<true>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(density_sweep_kernel<false>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
 }
 void launch_density_finalize(hipStream_t s, int n_active, const int *active, const double4 *posm,
                              const double4 *velu, const double *hsml, double *num_ngb, double *rho, double *pres,
@@ -801,6 +888,14 @@ void launch_defer_total(hipStream_t s, int n_active, const int *offset, const in
 {
     hipLaunchKernelGGL(defer_total_kernel, dim3(1), dim3(64), 0, s, n_active, offset, count, total);
 }
+void launch_density_heavy(hipStream_t s, const DensitySweepArgs &a, int max_entries)
+{
+    if (max_entries <= 0) return;
+    // up to 2048 workgroups of 4 wavefronts stride over the entries
+    const long long blocks = ((long long)max_entries + 3) / 4;
+    hipLaunchKernelGGL(density_heavy_kernel, dim3((unsigned)(blocks < 2048 ? blocks : 2048)), dim3(kBlock), 0, s, a);
+}
+
 void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, const TreeArgs &tree, int mode,
                             int n_active, const int *active, const double4 *posm, const double *hsml, int *counts,
                             Counters *ctr)

@@ -12,11 +12,10 @@
 //   * FP64 kept for the hsml iteration (hsml_update), the NV_T inversion, the
 //     slope limiter, the exact Riemann solver, the entropic-switch comparison
 //     of P/rho, and the accumulation of the conserved fluxes (dv/dt, du/dt).
-// Unlike the FP64 kernels, this file is compiled with floating-point
-// contraction (fused multiply-adds): the FP32 pass is not held to the FP64
-// oracle's rounding, only the neighbour predicate is (exact_r2), and the
-// grid walk's cell pruning carries relative margins (1e-9, 1e-6) far above a
-// one-ulp change.
+// Like the FP64 kernels, this file is compiled without floating-point
+// contraction: the pair-list and scanning force loops (and the exact FP64
+// neighbour predicate, exact_r2) must round identically wherever the same
+// expression is inlined, and fused multiply-adds measured no faster here.
 #include "grid_device.h"
 #include "gzm_internal.h"
 
@@ -56,9 +55,9 @@ __device__ __forceinline__ Band band_of(double t2)
 // FP32 decision of r2 < t2: 1 inside, 0 outside, -1 undecided
 __device__ __forceinline__ int decide(float r2, Band b) { return (r2 < b.lo) ? 1 : ((r2 >= b.hi) ? 0 : -1); }
 
-// squared distance exactly as the FP64 kernels compute it (this file is
-// compiled with contraction on, honouring pragmas; the neighbour predicate
-// must round as in the FP64 kernels, so no fused multiply-add here)
+// squared distance exactly as the FP64 kernels compute it (the neighbour
+// predicate must round as in the FP64 kernels: no fused multiply-add, also
+// should this file be built with contraction)
 __device__ __forceinline__ double exact_r2(const BoxInfo &box, const double4 &pi, const double4 &pj, double dp[3])
 {
 #pragma clang fp contract(off)
@@ -70,9 +69,9 @@ __device__ __forceinline__ double exact_r2(const BoxInfo &box, const double4 &pi
 }
 
 // FP32 displacement along one axis of cell-anchored positions: the cell shift
-// dc * s rounded on its own, then added to the offset difference.  No fused
-// multiply-add, so that the grid scan (shift hoisted per cell) and the pair
-// list (shift per pair) give the same bits.
+// dc * s rounded on its own, then added to the offset difference, so that the
+// grid scan (shift hoisted per cell) and the pair list (shift per pair) give
+// the same bits.
 __device__ __forceinline__ float cell_shift(int dc, float s)
 {
 #pragma clang fp contract(off)
@@ -136,6 +135,10 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
         float s_ngb = 0, s_rho = 0, s_dh = 0, s_div = 0;
         float t00 = 0, t01 = 0, t02 = 0, t11 = 0, t12 = 0, t22 = 0, f0 = 0, f1 = 0, f2 = 0;
         const double xs[3] = {pi.x, pi.y, pi.z};
+        const int budget = a.budget;
+        int visited = 0;
+        bool heavy = false;
+        unsigned long long lane_cnt = 0;
         auto visit = [&](int k, float sh0, float sh1, float sh2, bool exact_geom) {
             const float4 oj = a.f32.offm[k];
             float dp[3];
@@ -159,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
                 r2 = (float)r2e;
             }
             if (!in || !(oj.w > 0)) return;
-            cnt++;
+            lane_cnt++;
             const float r = Real<float>::root(r2);
             const float u = r * hinv;
             float wk, dwk;
@@ -183,18 +186,37 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
             }
         };
         if (TREE) {
-            tree_walk(a.tree, G, xs, h, false, [&](int k) { visit(k, 0.0f, 0.0f, 0.0f, true); });
+            tree_walk(a.tree, G, xs, h, false, [&](int k) -> bool {
+                if (budget && ++visited > budget) {
+                    heavy = true;
+                    return false;
+                }
+                visit(k, 0.0f, 0.0f, 0.0f, true);
+                return true;
+            });
         } else {
             const bool fine_i = (float)h * kFineCells < smax_of(G);
             bool full[3];
-            for_each_cell(G.grid, xs, h, full, [&](int cx, int cy, int cz, int beg, int end) {
+            // the budget is charged a cell at a time, outside the candidate loop
+            for_each_cell(G.grid, xs, h, full, [&](int cx, int cy, int cz, int beg, int end) -> bool {
+                visited += end - beg;
+                if (budget && visited > budget) {
+                    heavy = true;
+                    return false;
+                }
                 const bool exact_geom = full[0] || full[1] || full[2] || fine_i;
                 const float sh0 = cell_shift(ci[0] - cx, G.s_f[0]);
                 const float sh1 = cell_shift(ci[1] - cy, G.s_f[1]);
                 const float sh2 = cell_shift(ci[2] - cz, G.s_f[2]);
                 for (int k = beg; k < end; ++k) visit(k, sh0, sh1, sh2, exact_geom);
+                return true;
             });
         }
+        if (heavy) {  // left to the FP64 wavefront-per-particle kernel
+            a.heavy_list[atomicAdd(a.n_heavy, 1)] = t;
+            return;
+        }
+        cnt += lane_cnt;
         DensitySums S = {s_ngb, s_rho, s_dh, s_div, t00, t01, t02, t11, t12, t22, f0, f1, f2};
         double left = a.left[i], right = a.right[i];
         DensityResult R;
@@ -349,7 +371,7 @@ __global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a
             if (list_row && listed < a.list.cap && (listed & (kListChunk - 1)))
                 flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)));
         } else if ((m_i > 0) && (rho_i > 0)) {
-            listed = a.list.cap + 1;  // the force loop still runs for i: send its slice to the scan
+            listed = a.list.cap + 1;  // the force loop still runs for i: send it to the scan
         }
         double nvt[6];
 #pragma unroll
@@ -473,9 +495,9 @@ __device__ __forceinline__ void side_i_f(const ForceRecF &ri, PairI<float> &I)
     I.cnumcrit2 = kCondDanger * kCondDanger - I.cn * I.cn;
 }
 
-// USE_LIST: walk the pair list of the FP32 gradient kernel (overflowed slices
-// are left to the scanning variant); otherwise scan the grid with the same
-// banded predicate as the gradient kernel.  Exact-solver pairs are summed
+// USE_LIST: walk the pair list of the FP32 gradient kernel (particles that
+// overflowed it are left to the scanning variant); otherwise scan the grid or
+// tree with the same banded predicate as the gradient kernel.  Exact-solver pairs are summed
 // after the others, by force_deferred_f32_kernel + force_finish_kernel
 // (THREE_PHASE) or by a second walk over the pairs, as in the FP64 kernel.
 template <bool USE_LIST, bool THREE_PHASE, bool TREE = false>
@@ -484,7 +506,7 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
     const int t = xcd_block() * kBlock + threadIdx.x;
     bool valid = t < a.n_active;
     if (USE_LIST) {
-        if (valid && a.list.overflow[t >> 6]) {
+        if (valid && a.list.count[t] > a.list.cap) {
             valid = false;
             if (THREE_PHASE) {  // finished by the scanning kernel
                 a.defer.count[t] = 0;
@@ -492,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
             }
         }
     } else if (a.fallback_only) {
-        if (valid && !a.list.overflow[t >> 6]) valid = false;
+        if (valid && !(a.list.count[t] > a.list.cap)) valid = false;
     }
     unsigned long long cnt = 0, nretry = 0, nexact = 0;
     for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
@@ -548,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
             const float pos_min = (float)(h2_i * kBand);
             const double xs[3] = {pi.x, pi.y, pi.z};
             const double R = (h_i > G.hmax) ? h_i : G.hmax;
-            const bool exact_geom_list = a.tree.child != nullptr || list_needs_exact_geom(G);
+            const bool exact_geom_list = a.tree.nodes != nullptr || list_needs_exact_geom(G);
             const float smax = smax_of(G);
             for (int pass = 0; pass < 2; pass++) {
                 if (pass == 1 && (THREE_PHASE || n_deferred == 0)) break;
@@ -678,7 +700,7 @@ __global__ __launch_bounds__(kBlock) void force_deferred_f32_kernel(ForceArgs a)
         PairSide<float> J;
         unpack_side_f(Jr, J);
         float dp[3];
-        list_dp(G, a.tree.child != nullptr || list_needs_exact_geom(G), ri, ci, a.posm[i], Jr, a.posm, j, dp);
+        list_dp(G, a.tree.nodes != nullptr || list_needs_exact_geom(G), ri, ci, a.posm[i], Jr, a.posm, j, dp);
         const float r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
         ROut<float> ro = {0, 0, {0, 0, 0}, 0};
         float Fv[3], Fp, vsig;
@@ -711,7 +733,7 @@ void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const doubl
 void launch_density_sweep_f32(hipStream_t s, const DensitySweepArgs &a)
 {
     if (a.n_live <= 0) return;
-    if (a.tree.child) hipLaunchKernelGGL(density_sweep_f32_kernel<true>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+    if (a.tree.nodes) hipLaunchKernelGGL(density_sweep_f32_kernel<true>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
     else hipLaunchKernelGGL(density_sweep_f32_kernel<false>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
 }
 void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const double *hsml,
@@ -722,7 +744,7 @@ void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4
 void launch_gradients_f32(hipStream_t s, const GradientArgs &a)
 {
     if (a.n_active <= 0) return;
-    if (a.tree.child) hipLaunchKernelGGL(gradients_f32_kernel<true>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    if (a.tree.nodes) hipLaunchKernelGGL(gradients_f32_kernel<true>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
     else hipLaunchKernelGGL(gradients_f32_kernel<false>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
 }
 void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const uint32_t *cellc,
@@ -741,7 +763,7 @@ void launch_force_f32(hipStream_t s, const ForceArgs &a)
     if (a.list.idx && !a.fallback_only) {
         if (a.defer.mask) hipLaunchKernelGGL((force_f32_kernel<true, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((force_f32_kernel<true, false>), grid, block, 0, s, a);
-    } else if (a.tree.child) {
+    } else if (a.tree.nodes) {
         hipLaunchKernelGGL((force_f32_kernel<false, false, true>), grid, block, 0, s, a);
     } else {
         hipLaunchKernelGGL((force_f32_kernel<false, false>), grid, block, 0, s, a);

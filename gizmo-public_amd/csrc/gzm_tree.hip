@@ -21,8 +21,12 @@
 //              thread per leaf; the second thread to reach a node (atomic
 //              arrival counter, agent-scope fences) continues upwards.
 //
-// Every pass is O(n) threads with no host synchronisation, so a rebuild of
-// the node hmax after each density stage can sit in a captured graph.
+// The node hmax changes with every density stage; it is refreshed without
+// the bottom-up chain (whose fences cost a few ms per million particles):
+// every node covers a contiguous slot range, whose maximum hsml is the larger
+// of two overlapping power-of-two windows of a window-maximum table built in
+// log2(n) data-parallel levels.  No pass needs a host synchronisation, so
+// the refresh can sit in a captured graph.
 #include "grid_device.h"
 #include "gzm_internal.h"
 
@@ -133,10 +137,9 @@ __global__ void tree_links_kernel(TreeBuild t)
     }
 }
 
-// Bottom-up: node boxes (BOUNDS) and the max hsml below each node.  The
-// second thread to arrive at a node has both children final; it resets the
-// node's counter for the next launch and carries on to the parent.
-template <bool BOUNDS>
+// Bottom-up: node boxes and the max hsml below each node.  The second thread
+// to arrive at a node has both children final; it resets the node's counter
+// for the next build and carries on to the parent.
 __global__ void tree_bounds_kernel(TreeBuild t, const double *hsml)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -155,29 +158,72 @@ __global__ void tree_bounds_kernel(TreeBuild t, const double *hsml)
         for (int q = 0; q < 2; q++) {
             if (ch[q] < 0) {
                 const int j = -ch[q] - 1;
-                if (BOUNDS) {
-                    const double4 p = t.posm[j];
-                    lo[q] = make_double4(p.x, p.y, p.z, hsml[j]);
-                    hi[q] = make_double4(p.x, p.y, p.z, 0.0);
-                } else {
-                    lo[q].w = hsml[j];
-                }
-            } else if (BOUNDS) {
+                const double4 p = t.posm[j];
+                lo[q] = make_double4(p.x, p.y, p.z, hsml[j]);
+                hi[q] = make_double4(p.x, p.y, p.z, 0.0);
+            } else {
                 lo[q] = t.lo[ch[q]];
                 hi[q] = t.hi[ch[q]];
-            } else {
-                lo[q].w = reinterpret_cast<const double *>(t.lo + ch[q])[3];
             }
         }
         const double hmax = lo[0].w > lo[1].w ? lo[0].w : lo[1].w;
-        if (BOUNDS) {
-            t.lo[node] = make_double4(fmin(lo[0].x, lo[1].x), fmin(lo[0].y, lo[1].y), fmin(lo[0].z, lo[1].z), hmax);
-            t.hi[node] = make_double4(fmax(hi[0].x, hi[1].x), fmax(hi[0].y, hi[1].y), fmax(hi[0].z, hi[1].z), 0.0);
-        } else {
-            reinterpret_cast<double *>(t.lo + node)[3] = hmax;
-        }
+        t.lo[node] = make_double4(fmin(lo[0].x, lo[1].x), fmin(lo[0].y, lo[1].y), fmin(lo[0].z, lo[1].z), hmax);
+        t.hi[node] = make_double4(fmax(hi[0].x, hi[1].x), fmax(hi[0].y, hi[1].y), fmax(hi[0].z, hi[1].z), 0.0);
         node = t.parent[node];
     }
+}
+
+// level 0 of the window-maximum table: hsml rounded up to FP32
+__global__ void window_max_base_kernel(int n, const double *hsml, float *m0)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) m0[s] = __double2float_ru(hsml[s]);
+}
+
+// level k from level k - 1: windows of 2^k slots (cut at n)
+__global__ void window_max_level_kernel(int n, int half, const float *prev, float *next)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float a = prev[s];
+    const float b = (s + half < n) ? prev[s + half] : a;
+    next[s] = fmaxf(a, b);
+}
+
+// hmax of every internal node: two overlapping windows cover its slot range
+__global__ void node_hmax_kernel(TreeBuild t)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= t.n) return;
+    const BoxDesc &B = t.boxes[t.box_of[s]];
+    const int i = s - B.first;
+    if (i >= B.count - 1) return;
+    const int g = B.root + i;
+    const int2 r = t.range[g];
+    const int len = r.y - r.x + 1;
+    const int k = 31 - __clz(len);
+    const float *m = t.window_max + (size_t)k * t.n;
+    t.nodes[g].lo.w = fmaxf(m[r.x], m[r.y - (1 << k) + 1]);
+}
+
+// the walk's 48-byte records from the build's arrays (box rounded outwards)
+__global__ void pack_nodes_kernel(TreeBuild t)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= t.n) return;
+    const BoxDesc &B = t.boxes[t.box_of[s]];
+    const int i = s - B.first;
+    if (i >= B.count - 1) return;
+    const int g = B.root + i;
+    const double4 lo = t.lo[g], hi = t.hi[g];
+    const int2 r = t.range[g];
+    TreeNode nd;
+    nd.lo = make_float4(__double2float_rd(lo.x), __double2float_rd(lo.y), __double2float_rd(lo.z),
+                        __double2float_ru(lo.w));
+    nd.hi = make_float4(__double2float_ru(hi.x), __double2float_ru(hi.y), __double2float_ru(hi.z),
+                        __int_as_float(t.esc[g]));
+    nd.link = make_int4(t.child[g].x, r.x, r.y, 0);
+    t.nodes[g] = nd;
 }
 
 inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
@@ -197,12 +243,19 @@ void launch_tree_build(hipStream_t s, const TreeBuild &b, const double *hsml)
     if (b.n <= 0) return;
     hipLaunchKernelGGL(tree_build_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b);
     hipLaunchKernelGGL(tree_links_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b);
-    hipLaunchKernelGGL(tree_bounds_kernel<true>, dim3(blocks(b.n)), dim3(kBlock), 0, s, b, hsml);
+    hipLaunchKernelGGL(tree_bounds_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b, hsml);
+    hipLaunchKernelGGL(pack_nodes_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b);
 }
 
 void launch_tree_hmax(hipStream_t s, const TreeBuild &b, const double *hsml)
 {
-    if (b.n > 0) hipLaunchKernelGGL(tree_bounds_kernel<false>, dim3(blocks(b.n)), dim3(kBlock), 0, s, b, hsml);
+    if (b.n <= 0) return;
+    const dim3 grid(blocks(b.n)), block(kBlock);
+    hipLaunchKernelGGL(window_max_base_kernel, grid, block, 0, s, b.n, hsml, b.window_max);
+    for (int k = 1; k < b.levels; k++)
+        hipLaunchKernelGGL(window_max_level_kernel, grid, block, 0, s, b.n, 1 << (k - 1),
+                           b.window_max + (size_t)(k - 1) * b.n, b.window_max + (size_t)k * b.n);
+    hipLaunchKernelGGL(node_hmax_kernel, grid, block, 0, s, b);
 }
 
 }  // namespace gzm

@@ -13,7 +13,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libgzm.so")
+# GZM_LIBRARY selects another build of the library (e.g. to compare two builds)
+LIB_PATH = os.environ.get("GZM_LIBRARY") or os.path.join(LIB_DIR, "libgzm.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -42,6 +43,7 @@ class Params(C.Structure):
         ("cell_size", C.c_double),
         ("pair_list_cap", C.c_int32),
         ("search", C.c_int32),
+        ("density_budget", C.c_int32),
     ]
 
 
@@ -135,10 +137,12 @@ def _p(a, t=_dp):
 
 def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_ngb_dev=0.05,
                 min_hsml=0.0, max_hsml=1e56, first_step=1, maxiter=150, device=0, cell_size=0.0,
-                pair_list_cap=0, precision=0, search=0) -> Params:
-    """precision 0: plain FP64 pass; 1: FP32 pair arithmetic with FP64 hsml
-    iteration, matrix inversion, limiter and flux accumulation.  search 0:
-    cell grid; 1: tree with per-node hsml maxima (clustered sets)."""
+                pair_list_cap=0, precision=0, search=0, density_budget=0) -> Params:
+    """precision 0: plain FP64 pass; 1: FP64 density stage, FP32 pair
+    arithmetic in gradients and force with FP64 matrix inversion, limiter,
+    exact Riemann solver and flux accumulation; 2: as 1 with FP32 density
+    pair sums.  search 0: cell grid; 1: tree with per-node hsml maxima
+    (clustered sets)."""
     p = Params()
     p.box[:] = list(box)
     p.gamma = gamma
@@ -153,6 +157,7 @@ def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_
     p.cell_size = cell_size
     p.pair_list_cap = pair_list_cap
     p.search = search
+    p.density_budget = density_budget
     return p
 
 

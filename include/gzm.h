@@ -63,9 +63,13 @@ typedef struct gzm_params {
     int32_t first_step;     /* 1 when All.Time == All.TimeBegin (density.c:509) */
     int32_t maxiter;        /* MAXITER, 150 (allvars.h:1294) */
     int32_t device;         /* HIP device ordinal */
-    int32_t precision;      /* 0: plain FP64 pass; 1: FP32 bulk (pair geometry, kernel
-                               weights and fluxes in FP32 against FP64 anchors, FP64
-                               hsml iteration, limiter and sums; neighbour sets exact) */
+    int32_t precision;      /* 0: plain FP64 pass; 1: FP32 bulk: density stage as in
+                               the FP64 pass (same hsml, densities and neighbour sets,
+                               bit for bit), gradient and force pairs in FP32 against
+                               FP64 anchors with FP64 limiter, exact Riemann solver and
+                               flux sums; 2: as 1 with the density pair sums in FP32 too
+                               (hsml within ~1e-7 of the FP64 pass, so a few pairs per
+                               million sit on the other side of the kernel edge) */
     double cell_size;       /* grid cell side; 0: the mean initial hsml of each box;
                                < 0: -cell_size times that mean */
     int32_t pair_list_cap;  /* neighbours kept per particle in the pair list that the
@@ -77,6 +81,11 @@ typedef struct gzm_params {
     int32_t search;         /* neighbour search: 0 = cell grid (default); 1 = tree over
                                Morton-sorted particles with per-node hsml maxima, for
                                strongly clustered sets */
+    int32_t density_budget; /* candidates a density-sweep thread tests before its particle
+                               is deferred to a second launch over such particles only
+                               (load balance where a few hsml jump far in one iteration):
+                               0 = default (512), < 0 = no limit; results do not depend
+                               on it */
 } gzm_params;
 
 /* Host arrays of one particle set, caller order.  Inputs are read by
