@@ -55,6 +55,23 @@ BYTES_PER_PARTICLE = {"density": 64 + 136, "gradients": 128 + 120, "force": 272 
 BYTES_PER_PARTICLE_F32 = {"density": 40 + 136, "gradients": 96 + 120, "force": 160 + 40}
 
 
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
+def fp64_kernel(precision, kernel):
+    """The stage runs the FP64 kernel: precision 0, and precision 1's density stage."""
+    return precision == 0 or (precision == 1 and kernel == "density")
+
+
+def bytes_per_particle(precision, kernel):
+    return (BYTES_PER_PARTICLE if fp64_kernel(precision, kernel) else BYTES_PER_PARTICLE_F32)[kernel]
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,8 +174,7 @@ def run_mode(gzm, ic, args, precision, search=0, active=None):
     dom = max(kern, key=lambda k: kern[k][0])
     tot_ms, launches, units = kern[dom]
     mean_ms = tot_ms / max(launches, 1)
-    bpp = BYTES_PER_PARTICLE if precision == 0 else BYTES_PER_PARTICLE_F32
-    bytes_per_launch = bpp[dom] * units / max(launches / args.steps, 1)
+    bytes_per_launch = bytes_per_particle(precision, dom) * units / max(launches / args.steps, 1)
     return {
         "value": float(np.sum(interactions)) / elapsed,
         "ms_per_step": 1e3 * elapsed / args.steps,
@@ -166,6 +182,7 @@ def run_mode(gzm, ic, args, precision, search=0, active=None):
         "density_sweeps": int(sweeps[-1]),
         "kernels_ms_per_pass": {
             "density_sweeps": tm["density_ms"] / args.steps,
+            "density_wave_per_particle": tm["density_wave_ms"] / args.steps,
             "gradients": tm["gradients_ms"] / args.steps,
             "force": tm["force_ms"] / args.steps,
             "force_deferred_pairs": tm["force_deferred_ms"] / args.steps,
@@ -194,7 +211,7 @@ def measured_traffic(kernel, precision, n):
             tab = json.load(f)
     except (OSError, ValueError):
         return None
-    rec = tab.get("kernels", {}).get(f"{kernel}:{'fp64' if precision == 0 else 'fp32'}")
+    rec = tab.get("kernels", {}).get(f"{kernel}:{'fp64' if fp64_kernel(precision, kernel) else 'fp32'}")
     if rec is None or tab.get("n_particles") != n:
         return None
     return rec["fetch_bytes"] + rec["write_bytes"]
@@ -236,8 +253,7 @@ def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
     dom = max(kern, key=lambda k: kern[k][0])
     tot_ms, launches, units = kern[dom]
     mean_ms = tot_ms / max(launches, 1)
-    bpp = BYTES_PER_PARTICLE if precision == 0 else BYTES_PER_PARTICLE_F32
-    bytes_per_launch = bpp[dom] * units / max(launches / w, 1)
+    bytes_per_launch = bytes_per_particle(precision, dom) * units / max(launches / w, 1)
     return {
         "value": inter / graph_s,
         "ms_per_step": 1e3 * graph_s / steps,
@@ -245,6 +261,7 @@ def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
         "density_sweeps": int(st.density_sweeps),
         "kernels_ms_per_pass_eager": {
             "density_sweeps": tw["density_ms"] / w,
+            "density_wave_per_particle": tw["density_wave_ms"] / w,
             "gradients": tw["gradients_ms"] / w,
             "force": tw["force_ms"] / w,
             "force_deferred_pairs": tw["force_deferred_ms"] / w,
@@ -294,16 +311,22 @@ def main_batch(args, gzm, ics):
         nb = len(ic["box_offsets"]) - 1
         what = f"{nb} periodic boxes, jittered lattices of 16^3..64^3 particles ({ic['pos'].shape[0]:,} in all)"
     N = ic["pos"].shape[0]
+    log(f"workload built: {N:,} particles")
     tuned = {}
     if args.autotune:
         import autotune
         t0 = time.perf_counter()
         for prec in (1, 0):
             tuned[prec] = autotune.tune(ic, prec)
+            log(f"autotune precision {prec}: {tuned[prec]['cell_factor']}, cap {tuned[prec]['pair_list_cap']}")
         tune_s = time.perf_counter() - t0
     r32 = run_graph_mode(gzm, ic, args, 1, steps, tuned.get(1))
+    log(f"precision 1: {r32['ms_per_step']:.2f} ms/pass")
     r32d = run_graph_mode(gzm, ic, args, 2, steps, tuned.get(1)) if args.fp32_density else None
+    if r32d is not None:
+        log(f"precision 2: {r32d['ms_per_step']:.2f} ms/pass")
     r64 = run_graph_mode(gzm, ic, args, 0, steps, tuned.get(0))
+    log(f"precision 0: {r64['ms_per_step']:.2f} ms/pass")
     traffic = measured_traffic(r32["dominant"], 1, N)
     result = {
         "metric": "particle-neighbor interactions/s, MFM pass; FP32-batched vs FP64 at stated error",
@@ -371,7 +394,9 @@ def main_batch(args, gzm, ics):
         from parity import compare_pass, error_stats
 
         cores = cpu_cores()
+        log(f"CPU oracle on {cores} cores")
         ref, cpu_int, secs = oracle_boxes(O, ic, cores)
+        log(f"CPU oracle: {secs:.1f} s")
         result["cpu_baseline"] = {
             "value": cpu_int / secs,
             "unit": "interactions/s",
@@ -396,6 +421,7 @@ def main_batch(args, gzm, ics):
         blast_ic = ics.blast(args.n)
         b64 = run_mode(gzm, blast_ic, b_args, 0)
         b32 = run_mode(gzm, blast_ic, b_args, 1)
+        log("blast done")
         result["blast_128"] = {
             "n_particles": blast_ic["pos"].shape[0],
             "fp64": {"value": b64["value"], "ms_per_step": b64["ms_per_step"],
@@ -424,12 +450,15 @@ def main_clustered(args, gzm, ics):
     ic = ics.plummer_clusters(args.cluster_n, args.spheres)
     ic_s = time.perf_counter() - t0
     N = ic["pos"].shape[0]
+    log(f"clustered set built: {N:,} particles, {ic_s:.1f} s")
     full = run_mode(gzm, ic, args, 0, search=1)
+    log(f"full pass: {full['ms_per_step']:.1f} ms")
     state = dict(ic)
     for k in STATE_KEYS:
         state[k] = full["out"][k]
     active = ics.random_subset(N, 0.1)
     part = run_mode(gzm, state, args, 0, search=1, active=active)
+    log(f"10% active pass: {part['ms_per_step']:.1f} ms")
     h = ic["hsml"]
     result = {
         "metric": "particle-neighbor interactions/s, MFM pass; FP32-batched vs FP64 at stated error",
@@ -493,9 +522,11 @@ def main_clustered(args, gzm, ics):
         sample = ics.plummer_clusters(n_s, sph_s) if n_s < N else ic
         cores = cpu_cores()
         prm = O.make_params(box=tuple(sample["box"]), nthreads=cores)
+        log(f"CPU oracle on a {n_s:,}-particle sample, {cores} cores")
         t = time.perf_counter()
         ref, st, _ = O.run_pass(sample, params=prm)
         t = time.perf_counter() - t
+        log(f"CPU oracle: {t:.1f} s")
         cpu_int = st.dens_interactions + st.grad_interactions + st.force_interactions
         result["cpu_baseline"] = {
             "value": cpu_int / t,

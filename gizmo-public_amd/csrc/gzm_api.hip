@@ -42,7 +42,7 @@ void set_error(const char *fmt, ...)
         }                                                                                       \
     } while (0)
 
-enum TimerKind { kDensity = 0, kGradients = 1, kForce = 2, kOther = 3, kForceDeferred = 4 };
+enum TimerKind { kDensity = 0, kGradients = 1, kForce = 2, kOther = 3, kForceDeferred = 4, kDensityWave = 5 };
 
 struct TimedLaunch {
     hipEvent_t a, b;
@@ -199,6 +199,7 @@ void collect_timings(gzm_ctx *c)
             case kGradients: c->timing.gradients_ms += ms; c->timing.gradients_launches++; break;
             case kForce: c->timing.force_ms += ms; c->timing.force_launches++; break;
             case kForceDeferred: c->timing.force_deferred_ms += ms; c->timing.force_deferred_launches++; break;
+            case kDensityWave: c->timing.density_wave_ms += ms; c->timing.density_wave_launches++; break;
             default: c->timing.other_ms += ms; c->timing.other_launches++; break;
             }
         }
@@ -876,17 +877,23 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     // entry
     // (a sweep over at most kWaveSweepMax entries gives every entry a wavefront)
     auto sweep = [&](int cap, int expected) -> gzm_status {
-        Timed tm(c, kDensity);
         if (expected <= kWaveSweepMax) {
+            Timed tm(c, kDensityWave);
             DensitySweepArgs w = a;
             w.wave_all = 1;
             launch_density_heavy(c->stream, w, expected);
             return GZM_OK;
         }
         if (a.budget) launch_fill_int(c->stream, c->d_count_heavy, 1, 0);
-        if (c->p.precision == 2) launch_density_sweep_f32(c->stream, a);
-        else launch_density_sweep(c->stream, a);
-        if (a.budget) launch_density_heavy(c->stream, a, cap);
+        {
+            Timed tm(c, kDensity);
+            if (c->p.precision == 2) launch_density_sweep_f32(c->stream, a);
+            else launch_density_sweep(c->stream, a);
+        }
+        if (a.budget) {
+            Timed tm(c, kDensityWave);
+            launch_density_heavy(c->stream, a, cap);
+        }
         return GZM_OK;
     };
     int iter = 0, sweeps = 0;

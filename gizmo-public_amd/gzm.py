@@ -80,6 +80,7 @@ class Timing(C.Structure):
         ("graph_ms", C.c_double), ("graph_launches", C.c_int64),
         ("graph_passes", C.c_int64),
         ("force_deferred_ms", C.c_double), ("force_deferred_launches", C.c_int64),
+        ("density_wave_ms", C.c_double), ("density_wave_launches", C.c_int64),
     ]
 
 

@@ -136,7 +136,7 @@ typedef struct gzm_pass_stats {
 /* Per-kernel device time (HIP events on the context stream), accumulated
  * since the last gzm_timing_reset. */
 typedef struct gzm_timing {
-    double density_ms;   int64_t density_launches;
+    double density_ms;   int64_t density_launches;  /* density sweeps, one thread per particle */
     double gradients_ms; int64_t gradients_launches;
     double force_ms;     int64_t force_launches;
     double other_ms;     int64_t other_launches;   /* grid build, packing, finalize */
@@ -144,6 +144,9 @@ typedef struct gzm_timing {
     int64_t graph_passes;                          /* passes inside those graphs */
     double force_deferred_ms; int64_t force_deferred_launches;  /* exact-solver pairs deferred
                                                    out of the force loop, their scan and finish */
+    double density_wave_ms; int64_t density_wave_launches;  /* density sweeps run one wavefront
+                                                   per particle: the particles past the candidate
+                                                   budget, and sweeps of few particles */
 } gzm_timing;
 
 typedef struct gzm_ctx gzm_ctx;
