@@ -73,9 +73,12 @@ __device__ __forceinline__ double axis_gap(double x, double c0, double s)
 // the walk crosses the periodic boundary; the image of a particle of that cell
 // lies at x + (c - c_wrapped) * s) and its sorted slot range.  `full` reports
 // the axes walked whole because R spans half the box.  An f returning bool
-// ends the walk by returning false.
+// ends the walk by returning false.  With h_pair >= 0 (pair searches: R is
+// the box's max hsml) a cell farther than h_pair is visited only when its own
+// max hsml reaches x: no particle of it can pair with x otherwise.
 template <typename F>
-__device__ __forceinline__ void for_each_cell(const Grid &g, const double xs[3], double R, bool full[3], F &&f)
+__device__ __forceinline__ void for_each_cell(const Grid &g, const double xs[3], double R, bool full[3], F &&f,
+                                              double h_pair = -1.0)
 {
     int lo[3], hi[3];
 #pragma unroll
@@ -91,6 +94,7 @@ __device__ __forceinline__ void for_each_cell(const Grid &g, const double xs[3],
         }
     }
     const double R2 = R * R * (1.0 + 1e-9);
+    const double hp2 = h_pair * h_pair * (1.0 + 1e-9);
     for (int cx = lo[0]; cx <= hi[0]; ++cx) {
         const int wx = (cx < 0) ? cx + g.nc[0] : ((cx >= g.nc[0]) ? cx - g.nc[0] : cx);
         const double gx = full[0] ? 0.0 : axis_gap(xs[0], cx * g.s[0], g.s[0]);
@@ -105,7 +109,12 @@ __device__ __forceinline__ void for_each_cell(const Grid &g, const double xs[3],
             for (int cz = lo[2]; cz <= hi[2]; ++cz) {
                 const int wz = (cz < 0) ? cz + g.nc[2] : ((cz >= g.nc[2]) ? cz - g.nc[2] : cz);
                 const double gz = full[2] ? 0.0 : axis_gap(xs[2], cz * g.s[2], g.s[2]);
-                if (gxy2 + gz * gz > R2) continue;
+                const double d2 = gxy2 + gz * gz;
+                if (d2 > R2) continue;
+                if (h_pair >= 0 && d2 > hp2) {
+                    const double hm = g.hmax[rowbase + wz];
+                    if (d2 > hm * hm * (1.0 + 1e-9)) continue;
+                }
                 const int2 r = g.range[rowbase + wz];
                 if (!visit_more(f, cx, cy, cz, r.x, r.y)) return;
             }
@@ -116,14 +125,15 @@ __device__ __forceinline__ void for_each_cell(const Grid &g, const double xs[3],
 // Visit every sorted slot k whose cell may hold a particle within distance R
 // of (x, y, z), periodic.
 template <typename F>
-__device__ __forceinline__ void for_each_candidate(const Grid &g, const double xs[3], double R, F &&f)
+__device__ __forceinline__ void for_each_candidate(const Grid &g, const double xs[3], double R, F &&f,
+                                                   double h_pair = -1.0)
 {
     bool full[3];
     for_each_cell(g, xs, R, full, [&](int, int, int, int begin, int end) {
         for (int k = begin; k < end; ++k)
             if (!visit_more(f, k)) return false;
         return true;
-    });
+    }, h_pair);
 }
 
 __device__ __forceinline__ void wrap_dp(const BoxInfo &box, double dp[3])
@@ -195,7 +205,7 @@ __device__ __forceinline__ void for_each_neighbour_candidate(const TreeArgs &T, 
         tree_walk(T, G, x, h_i, pairs, f);
     } else {
         const double R = (pairs && G.hmax > h_i) ? G.hmax : h_i;
-        for_each_candidate(G.grid, x, R, f);
+        for_each_candidate(G.grid, x, R, f, pairs ? h_i : -1.0);
     }
 }
 

@@ -78,6 +78,7 @@ struct gzm_ctx {
     void *d_temp = nullptr;
     size_t temp_bytes = 0;
     int2 *d_range = nullptr;
+    float *d_cell_hmax = nullptr;  // per cell max hsml (pair-search culling)
     // sorted particle state
     double4 *d_posm = nullptr, *d_velu = nullptr;
     double *d_hsml = nullptr, *d_hsml0 = nullptr, *d_left = nullptr, *d_right = nullptr;
@@ -261,6 +262,7 @@ void free_boxes(gzm_ctx *c)
     dev_free(c->d_boxes);
     dev_free(c->d_hmax_tiles);
     dev_free(c->d_range);
+    dev_free(c->d_cell_hmax);
     c->n_hmax_tiles = 0;
     c->n_boxes = 0;
     c->ncell = 0;
@@ -398,6 +400,7 @@ gzm_status compute_hmax(gzm_ctx *c)
     Timed tm(c, kOther);
     launch_box_hmax(c->stream, c->n_boxes, c->d_boxes, c->d_hsml, c->n_hmax_tiles, c->d_hmax_tiles);
     if (c->tree.nodes) launch_tree_hmax(c->stream, c->tbuild, c->d_hsml);
+    else launch_cell_hmax(c->stream, c->ncell, c->d_range, c->d_hsml, c->d_cell_hmax);
     GZM_HIP(hipGetLastError());
     return GZM_OK;
 }
@@ -571,10 +574,14 @@ gzm_status setup_boxes(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const do
     }
     free_boxes(c);
     gzm_status s = dev_alloc(&c->d_range, (size_t)ncell);
+    s = s ? s : dev_alloc(&c->d_cell_hmax, (size_t)ncell);
     s = s ? s : dev_alloc(&c->d_boxes, (size_t)n_boxes);
     s = s ? s : dev_alloc(&c->d_hmax_tiles, tiles.size());
     if (s) { free_boxes(c); return s; }
-    for (auto &B : bx) B.grid.range = c->d_range + B.cell_base;
+    for (auto &B : bx) {
+        B.grid.range = c->d_range + B.cell_base;
+        B.grid.hmax = c->d_cell_hmax + B.cell_base;
+    }
     GZM_HIP(hipMemcpyAsync(c->d_boxes, bx.data(), sizeof(BoxDesc) * n_boxes, hipMemcpyHostToDevice, c->stream));
     if (!tiles.empty())
         GZM_HIP(hipMemcpyAsync(c->d_hmax_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice,

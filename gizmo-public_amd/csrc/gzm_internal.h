@@ -17,6 +17,7 @@ struct Grid {
     double inv_s[3];  // cells per unit length
     double s[3];      // cell side
     const int2 *range;
+    const float *hmax;  // per cell: max hsml of its particles, rounded up (pair searches)
 };
 
 // One periodic box of a context.  A context holds one box, or many
@@ -323,6 +324,8 @@ void launch_tree_keys(hipStream_t s, int n, const double *pos, const BoxDesc *bo
 void launch_tree_build(hipStream_t s, const TreeBuild &b, const double *hsml);
 // node hmax from the current hsml (window-maximum table; node boxes unchanged)
 void launch_tree_hmax(hipStream_t s, const TreeBuild &b, const double *hsml);
+// per-cell max hsml (rounded up to FP32) of the cells' slot ranges
+void launch_cell_hmax(hipStream_t s, int ncell, const int2 *range, const double *hsml, float *hmax);
 void launch_fill_int(hipStream_t s, int *p, int n, int value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value);
 void launch_fill_double(hipStream_t s, double *p, long long n, double value);

@@ -688,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void force_deferred_kernel(ForceArgs a)
         ROut<double> ro = {0, 0, {0, 0, 0}, 0};
         double Fv[3], Fp, vsig;
         int mask;
-        const int e = force_pair<double, false>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+        const int e = force_pair<double, false, true>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
         nretry += (mask & 8) ? 1 : 0;
         nexact += (mask & 4) ? 1 : 0;
         double *o = a.defer.result + 5 * (size_t)(q - a.defer.q_begin);
@@ -894,6 +894,24 @@ void launch_density_heavy(hipStream_t s, const DensitySweepArgs &a, int max_entr
     // up to 2048 workgroups of 4 wavefronts stride over the entries
     const long long blocks = ((long long)max_entries + 3) / 4;
     hipLaunchKernelGGL(density_heavy_kernel, dim3((unsigned)(blocks < 2048 ? blocks : 2048)), dim3(kBlock), 0, s, a);
+}
+
+namespace {
+__global__ void cell_hmax_kernel(int ncell, const int2 *range, const double *hsml, float *hmax)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int2 r = range[c];
+    double m = 0.0;
+    for (int k = r.x; k < r.y; ++k) m = (hsml[k] > m) ? hsml[k] : m;
+    hmax[c] = __double2float_ru(m);
+}
+}  // namespace
+
+void launch_cell_hmax(hipStream_t s, int ncell, const int2 *range, const double *hsml, float *hmax)
+{
+    if (ncell > 0)
+        hipLaunchKernelGGL(cell_hmax_kernel, dim3(nblocks(ncell)), dim3(kBlock), 0, s, ncell, range, hsml, hmax);
 }
 
 void launch_neighbor_counts(hipStream_t s, const BoxDesc *boxes, const int *box_of, const TreeArgs &tree, int mode,

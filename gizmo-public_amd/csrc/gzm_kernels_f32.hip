@@ -366,7 +366,7 @@ __global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a
                     const float sh1 = cell_shift(ci[1] - cy, G.s_f[1]);
                     const float sh2 = cell_shift(ci[2] - cz, G.s_f[2]);
                     for (int k = beg; k < end; ++k) visit(k, sh0, sh1, sh2, exact_geom);
-                });
+                }, h_i);
             }
             if (list_row && listed < a.list.cap && (listed & (kListChunk - 1)))
                 flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)));
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
                             const float sh1 = cell_shift(ci[1] - cy, G.s_f[1]);
                             const float sh2 = cell_shift(ci[2] - cz, G.s_f[2]);
                             for (int j = beg; j < end; ++j) visit(j, sh0, sh1, sh2, exact_geom);
-                        });
+                        }, h_i);
                     }
                 }
             }
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(kBlock) void force_deferred_f32_kernel(ForceArgs a)
         ROut<float> ro = {0, 0, {0, 0, 0}, 0};
         float Fv[3], Fp, vsig;
         int mask;
-        const int e = force_pair<float, false>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+        const int e = force_pair<float, false, true>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
         nretry += (mask & 8) ? 1 : 0;
         nexact += (mask & 4) ? 1 : 0;
         double *o = a.defer.result + 5 * (size_t)(q - a.defer.q_begin);

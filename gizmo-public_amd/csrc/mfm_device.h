@@ -463,10 +463,12 @@ __device__ inline void wavespeeds_pstar(const RState<T> &L, const RState<T> &R, 
 }
 
 // Kurganov-Tadmor/PWK flux, MFM branch (reimann.h:484-519).  Reached only on
-// a NaN/negative HLLC pressure, so it lives out of line with scalar arguments.
+// a NaN/negative HLLC pressure, so the force loop calls it out of line
+// (riemann_kt); the deferred-pair kernels, whose every lane may take it,
+// inline it (riemann_kt_body).
 template <typename T>
-__device__ __noinline__ ROut<T> riemann_kt(T rL, T pL, T vLx, T vLy, T vLz, T rR, T pR, T vRx, T vRy, T vRz, T n0,
-                                           T n1, T n2, T vL, T vR, T cL, T cR, T hL, T hR)
+__device__ __forceinline__ ROut<T> riemann_kt_body(T rL, T pL, T vLx, T vLy, T vLz, T rR, T pR, T vRx, T vRy, T vRz,
+                                                   T n0, T n1, T n2, T vL, T vR, T cL, T cR, T hL, T hR)
 {
     using M = Real<T>;
     ROut<T> o;
@@ -497,14 +499,21 @@ __device__ __noinline__ ROut<T> riemann_kt(T rL, T pL, T vLx, T vLy, T vLz, T rR
 // S_M change (sample_reimann_standard returns at once, reimann.h:873-877).
 // Rare (HLLC pressure above the limiter), so out of line with scalar
 // arguments; returns (P_M, S_M, status) with status 0 or 1234545.
+template <typename T>
+__device__ __noinline__ ROut<T> riemann_kt(T rL, T pL, T vLx, T vLy, T vLz, T rR, T pR, T vRx, T vRy, T vRz, T n0,
+                                           T n1, T n2, T vL, T vR, T cL, T cR, T hL, T hR)
+{
+    return riemann_kt_body<T>(rL, pL, vLx, vLy, vLz, rR, pR, vRx, vRy, vRz, n0, n1, n2, vL, vR, cL, cR, hL, hR);
+}
+
 struct ExactOut {
     double P_M, S_M;
     int status;
 };
 
-__device__ __noinline__ ExactOut riemann_exact(double rL, double pL, double rR, double pR, double vL, double vR,
-                                               double cL, double cR, double P_M_in, double S_M_in,
-                                               GammaConsts G)
+__device__ __forceinline__ ExactOut riemann_exact_body(double rL, double pL, double rR, double pR, double vL,
+                                                      double vR, double cL, double cR, double P_M_in, double S_M_in,
+                                                      const GammaConsts &G)
 {
     ExactOut o = {P_M_in, S_M_in, 0};
     if ((pL == 0 && pR == 0) || (rL == 0 && rR == 0)) { o.P_M = o.S_M = 0; return o; }
@@ -585,13 +594,20 @@ __device__ __noinline__ ExactOut riemann_exact(double rL, double pL, double rR, 
     return o;
 }
 
+// out of line for the force loop (rarely taken), inline for the deferred-pair kernels
+__device__ __noinline__ ExactOut riemann_exact(double rL, double pL, double rR, double pR, double vL, double vR,
+                                               double cL, double cR, double P_M_in, double S_M_in, GammaConsts G)
+{
+    return riemann_exact_body(rL, pL, rR, pR, vL, vR, cL, cR, P_M_in, S_M_in, G);
+}
+
 // Riemann_solver() for MFM hydro (reimann.h:286-388).  Returns a stage mask:
 // bit0 HLLC, bit1 KT, bit2 exact; err gets 1234545 on the exact solver's
 // negative-density exit.  The exact solver always runs in FP64.  DEFER: stop
 // where the exact solver would be called and return kDeferBit instead.
 constexpr int kDeferBit = 32;
 
-template <typename T, bool DEFER = false>
+template <typename T, bool DEFER = false, bool INLINE = false>
 __device__ __forceinline__ int riemann_solve(const RState<T> &L, const RState<T> &R, const T n[3], T limiter,
                                              const GammaConsts &G, ROut<T> &o, int &err)
 {
@@ -610,13 +626,18 @@ __device__ __forceinline__ int riemann_solve(const RState<T> &L, const RState<T>
     for (int k = 0; k < 3; k++) o.Fv[k] = o.P_M * n[k];
     o.Fp = o.P_M * o.S_M;
     if ((o.P_M < 0) || isnan(o.P_M)) {
-        o = riemann_kt<T>(L.rho, L.p, L.v[0], L.v[1], L.v[2], R.rho, R.p, R.v[0], R.v[1], R.v[2], n[0], n[1], n[2],
-                          vL, vR, cL, cR, hL, hR);
+        if (INLINE)
+            o = riemann_kt_body<T>(L.rho, L.p, L.v[0], L.v[1], L.v[2], R.rho, R.p, R.v[0], R.v[1], R.v[2], n[0], n[1],
+                                   n[2], vL, vR, cL, cR, hL, hR);
+        else
+            o = riemann_kt<T>(L.rho, L.p, L.v[0], L.v[1], L.v[2], R.rho, R.p, R.v[0], R.v[1], R.v[2], n[0], n[1], n[2],
+                              vL, vR, cL, cR, hL, hR);
         mask |= 2;
     }
     if ((o.P_M < 0) || isnan(o.P_M) || (o.P_M > limiter)) {
         if (DEFER) return mask | kDeferBit;
-        ExactOut e = riemann_exact(L.rho, L.p, R.rho, R.p, vL, vR, cL, cR, o.P_M, o.S_M, G);
+        const ExactOut e = INLINE ? riemann_exact_body(L.rho, L.p, R.rho, R.p, vL, vR, cL, cR, o.P_M, o.S_M, G)
+                                  : riemann_exact(L.rho, L.p, R.rho, R.p, vL, vR, cL, cR, o.P_M, o.S_M, G);
         o.P_M = (T)e.P_M;
         o.S_M = (T)e.S_M;
         if (e.status) err = e.status;
@@ -654,7 +675,7 @@ struct PairI : PairSide<T> {
 // Riemann solver (the caller evaluates it later with DEFER = false).
 constexpr int kDeferred = -1;
 
-template <typename T, bool DEFER = false>
+template <typename T, bool DEFER = false, bool INLINE = false>
 __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &J, const T dp[3], T r2,
                                           const GammaConsts &G, ROut<T> &ro, T Fv[3], T &Fp, T &vsig, int &mask)
 {
@@ -753,7 +774,7 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
     int err = 0;
     T lim_a = lim;
     for (int attempt = 0; attempt < 3; attempt++) {
-        mask |= riemann_solve<T, DEFER>(SL, SR, n, lim_a, G, ro, err);
+        mask |= riemann_solve<T, DEFER, INLINE>(SL, SR, n, lim_a, G, ro, err);
         if (DEFER && (mask & kDeferBit)) return kDeferred;
         const bool bad = (ro.P_M < 0) || isnan(ro.P_M);
         if (attempt == 0) {
