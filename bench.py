@@ -50,8 +50,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 BYTES_PER_PARTICLE = {"density": 64 + 136, "gradients": 128 + 120, "force": 272 + 40}
 # FP32-bulk mode, stored widths: density sweep reads offsets+m (16), velocity (16),
 # hsml (8) and writes the same FP64 outputs (136); gradients read the 48-byte
-# FP32 record of j plus NV_T(6) of i (48) and write 15 FP64 gradients (120);
-# force reads the 160-byte FP32 record and writes dv/dt, du/dt, v_sig (40)
+# FP32 record of j (rho and P as high and low words) plus NV_T(6) of i (48) and
+# write 15 FP64 gradients (120); force reads the 160-byte FP32 record and writes
+# dv/dt, du/dt, v_sig (40)
 BYTES_PER_PARTICLE_F32 = {"density": 40 + 136, "gradients": 96 + 120, "force": 160 + 40}
 
 
@@ -276,6 +277,7 @@ def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
         "grad_interactions": st.gradient_interactions // steps,
         "force_interactions": st.force_interactions // steps,
         "exact_solver_calls": st.exact_solver_calls // steps,
+        "fp64_force_particles": st.fp64_force_particles // steps,
     }
 
 
@@ -373,6 +375,8 @@ def main_batch(args, gzm, ics):
             "kernels_ms_per_pass_eager": r32d["kernels_ms_per_pass_eager"],
         }),
         "exact_solver_calls_per_pass": r64["exact_solver_calls"],
+        # FP32 force loops redone in FP64: a pair's wavespeed or retry branch within FP32 rounding of its threshold
+        "fp64_force_particles_per_pass": r32["fp64_force_particles"],
         "autotune": ({"fp32": tuned[1], "fp64": tuned[0], "seconds": round(tune_s, 1)} if tuned else None),
         "kernels_ms_per_pass_eager": r32["kernels_ms_per_pass_eager"],
         "roofline": {

@@ -209,13 +209,15 @@ __device__ __forceinline__ void for_each_neighbour_candidate(const TreeArgs &T, 
     }
 }
 
-// Every candidate of one particle (search radius R around x, no hsml of j),
-// spread over the 64 lanes of a wavefront that walks the tree or the grid in
-// step: each lane receives its share, f(k).  For the few particles whose
-// search covers far more candidates than a lane can walk alone.
+// Every candidate of one particle (search radius R around x), spread over the
+// 64 lanes of a wavefront that walks the tree or the grid in step: each lane
+// receives its share, f(k).  For the few particles whose search covers far
+// more candidates than a lane can walk alone.  h_pair >= 0 (the pair search,
+// R = max(h_i, box hsml max), h_pair = h_i): cells and nodes are also culled
+// by the largest hsml they hold, as in the pair walks.
 template <typename F>
 __device__ __forceinline__ void wave_candidates(const TreeArgs &T, const BoxDesc &G, const double x[3], double R,
-                                                int lane, F &&f)
+                                                int lane, F &&f, double h_pair = -1.0)
 {
     if (T.nodes) {
         int node = G.root;
@@ -227,11 +229,13 @@ __device__ __forceinline__ void wave_candidates(const TreeArgs &T, const BoxDesc
                 continue;
             }
             const TreeNode nd = T.nodes[node];
+            const double hm = nd.lo.w;
+            const double Rn = (h_pair < 0) ? R : ((hm > h_pair) ? hm : h_pair);
             const double gx = periodic_gap(x[0], nd.lo.x, nd.hi.x, G.box.L[0]);
             const double gy = periodic_gap(x[1], nd.lo.y, nd.hi.y, G.box.L[1]);
             const double gz = periodic_gap(x[2], nd.lo.z, nd.hi.z, G.box.L[2]);
             const int esc = __float_as_int(nd.hi.w);
-            if (!(gx * gx + gy * gy + gz * gz <= R * R * (1.0 + 1e-9))) {
+            if (!(gx * gx + gy * gy + gz * gz <= Rn * Rn * (1.0 + 1e-9))) {
                 node = esc;
                 continue;
             }
@@ -246,13 +250,84 @@ __device__ __forceinline__ void wave_candidates(const TreeArgs &T, const BoxDesc
         bool full[3];
         for_each_cell(G.grid, x, R, full, [&](int, int, int, int beg, int end) {
             for (int k = beg + lane; k < end; k += 64) f(k);
-        });
+        }, h_pair);
+    }
+}
+
+// The pair-search candidates of one particle (R = max(h_i, box hsml max),
+// cells and nodes culled by their hsml max against h_pair = h_i, as in the pair
+// walks) spread over the 64 lanes of a wavefront: in the grid each lane takes
+// whole cells of the search range in turn (at the usual cell sizes a cell holds
+// a particle or two, so lanes sharing one cell would mostly idle), in the tree
+// the lanes share each bucket.
+template <typename F>
+__device__ __forceinline__ void wave_pair_candidates(const TreeArgs &T, const BoxDesc &G, const double x[3],
+                                                     double R, double h_pair, int lane, F &&f)
+{
+    if (T.nodes) {
+        wave_candidates(T, G, x, R, lane, f, h_pair);
+        return;
+    }
+    const Grid &g = G.grid;
+    int lo[3], n[3];
+    bool full[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        full[a] = !(R * g.inv_s[a] < 0.5 * g.nc[a] - 2.0);
+        if (!full[a]) {
+            lo[a] = (int)floor((x[a] - R) * g.inv_s[a] - 1e-6);
+            n[a] = (int)floor((x[a] + R) * g.inv_s[a] + 1e-6) - lo[a] + 1;
+        } else {
+            lo[a] = 0;
+            n[a] = g.nc[a];
+        }
+    }
+    const double R2 = R * R * (1.0 + 1e-9);
+    const double hp2 = h_pair * h_pair * (1.0 + 1e-9);
+    const int total = n[0] * n[1] * n[2];
+    for (int c = lane; c < total; c += 64) {
+        const int cz = lo[2] + c % n[2];
+        const int cy = lo[1] + (c / n[2]) % n[1];
+        const int cx = lo[0] + c / (n[2] * n[1]);
+        const int wx = (cx < 0) ? cx + g.nc[0] : ((cx >= g.nc[0]) ? cx - g.nc[0] : cx);
+        const int wy = (cy < 0) ? cy + g.nc[1] : ((cy >= g.nc[1]) ? cy - g.nc[1] : cy);
+        const int wz = (cz < 0) ? cz + g.nc[2] : ((cz >= g.nc[2]) ? cz - g.nc[2] : cz);
+        const double gx = full[0] ? 0.0 : axis_gap(x[0], cx * g.s[0], g.s[0]);
+        const double gy = full[1] ? 0.0 : axis_gap(x[1], cy * g.s[1], g.s[1]);
+        const double gz = full[2] ? 0.0 : axis_gap(x[2], cz * g.s[2], g.s[2]);
+        const double d2 = gx * gx + gy * gy + gz * gz;
+        if (d2 > R2) continue;
+        const int cell = (wx * g.nc[1] + wy) * g.nc[2] + wz;
+        if (d2 > hp2) {
+            const double hm = g.hmax[cell];
+            if (d2 > hm * hm * (1.0 + 1e-9)) continue;
+        }
+        const int2 r = g.range[cell];
+        for (int k = r.x; k < r.y; ++k) f(k);
     }
 }
 
 __device__ __forceinline__ double wave_sum(double v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o < v) ? o : v;
+    }
     return v;
 }
 

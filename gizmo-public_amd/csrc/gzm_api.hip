@@ -97,6 +97,7 @@ struct gzm_ctx {
     uint32_t *d_cellc = nullptr;
     GradRecF *d_grecf = nullptr;
     ForceRecF *d_frecf = nullptr;
+    int *d_redo_flag = nullptr, *d_redo_list = nullptr, *d_redo_count = nullptr;  // FP32 force: FP64 redo
     Counters *d_ctr = nullptr;
     int n_active = 0;
     bool all_active = true;
@@ -243,6 +244,7 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_flags); dev_free(c->d_grec); dev_free(c->d_frec);
     dev_free(c->d_heavy_list);
     dev_free(c->d_offm); dev_free(c->d_velf); dev_free(c->d_cellc); dev_free(c->d_grecf); dev_free(c->d_frecf);
+    dev_free(c->d_redo_flag); dev_free(c->d_redo_list); dev_free(c->d_redo_count);
     dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
     dev_free(c->d_box_of);
     dev_free(c->d_defer_mask); dev_free(c->d_defer_count); dev_free(c->d_defer_off); dev_free(c->d_defer_total);
@@ -663,6 +665,9 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
             s = s ? s : dev_alloc(&c->d_cellc, n);
             s = s ? s : dev_alloc(&c->d_grecf, n);
             s = s ? s : dev_alloc(&c->d_frecf, n);
+            s = s ? s : dev_alloc(&c->d_redo_flag, n);
+            s = s ? s : dev_alloc(&c->d_redo_list, 2 * (size_t)n);
+            s = s ? s : dev_alloc(&c->d_redo_count, 2);
         }
         if (c->p.search == 1) {
             s = s ? s : dev_alloc(&c->d_t_child, n);
@@ -679,6 +684,7 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
         }
         if (s) { free_particles(c); return s; }
     }
+    if (c->d_redo_flag) launch_fill_int(c->stream, c->d_redo_flag, n, 0);
     gzm_status s = setup_boxes(c, v, n_boxes, box_sizes, offsets);
     if (s) return s;
 
@@ -1083,6 +1089,17 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     a.velu = c->d_velu;
     a.hsml = c->d_hsml;
     a.recf = c->d_frecf;
+    a.rho = c->d_rho;
+    a.pres = c->d_pres;
+    a.dhsml = c->d_dhsml;
+    a.cn = c->d_cn;
+    a.fce = c->d_fce;
+    a.nvt = c->d_nvt;
+    a.grad = c->d_grad;
+    a.redo_flag = c->d_redo_flag;
+    a.redo_list = c->d_redo_list;
+    a.redo_count = c->d_redo_count;
+    a.n = c->n;
     a.f32 = f32_geom(c);
     a.ctr = c->d_ctr + kCtrForce;
     a.fallback_only = 0;
@@ -1106,6 +1123,8 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     }
     if (c->n_active > 0) {
         Timed tm(c, kForce);
+        // a kernel, not a memset: this stage may be captured into a graph
+        if (c->p.precision != 0) launch_fill_int(c->stream, c->d_redo_count, 2, 0);
         auto launch = (c->p.precision != 0) ? launch_force_f32 : launch_force;
         launch(c->stream, a);
         if (use_list && c->list_overflow_slices > 0) {
@@ -1162,6 +1181,11 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         }
         GZM_HIP(hipGetLastError());
     }
+    if (c->p.precision != 0 && c->n_active > 0) {
+        Timed tm(c, kForceDeferred);
+        launch_force_redo_fp64(c->stream, a);
+        GZM_HIP(hipGetLastError());
+    }
     if (gp) return GZM_OK;
     Counters h;
     s = read_counters(c, kCtrForce, &h);
@@ -1171,6 +1195,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         stats->force_interactions = (int64_t)h.interactions;
         stats->riemann_retries = (int64_t)h.retries;
         stats->exact_solver_calls = (int64_t)h.exact_calls;
+        stats->fp64_force_particles = (int64_t)h.redo_n;
         stats->status = h.status;
     }
     if (h.status != 0) {
@@ -1317,6 +1342,7 @@ gzm_status gzm_run_passes(gzm_ctx *c, int32_t n_passes, gzm_pass_stats *stats)
         stats->force_interactions = (int64_t)h[kCtrForce].interactions;
         stats->riemann_retries = (int64_t)h[kCtrForce].retries;
         stats->exact_solver_calls = (int64_t)h[kCtrForce].exact_calls;
+        stats->fp64_force_particles = (int64_t)h[kCtrForce].redo_n;
         stats->density_updates = 0;
         stats->status = status;
     }

@@ -98,11 +98,15 @@ struct F32Geom {
     const uint32_t *cellc;     // cx | cy << 10 | cz << 20 (cell within the particle's box)
 };
 
-// Per-particle record of the FP32 gradient loop (48 bytes).
+// Per-particle record of the FP32 gradient loop (48 bytes).  rho and P are
+// split into a high FP32 word and the FP32 rounding of the remainder, so that
+// a pair difference (hi_j - hi_i) + (lo_j - lo_i) is the FP64 difference to
+// FP32 rounding of the difference itself, not of the values: neighbour
+// densities and pressures agree to many digits in smooth flow.
 struct alignas(16) GradRecF {
     float4 xh;     // off x, y, z, hsml
-    float4 vr;     // vx, vy, vz, rho
-    float4 pm;     // P, m, 0, 0
+    float4 vr;     // vx, vy, vz, rho (high word)
+    float4 pm;     // P (high word), m, rho and P low words
 };
 
 // Per-particle record of the FP32 force loop (160 bytes); P / rho and the
@@ -127,6 +131,7 @@ struct Counters {
     unsigned long long retries;
     unsigned long long exact_calls;
     int status;
+    int redo_n;  // FP32 force: particles redone in FP64 (summed over stages)
     int overflow_slices;
 };
 
@@ -234,6 +239,17 @@ struct ForceArgs {
     const double4 *velu;   // FP64 velocities (FP32 kernels: final operations)
     const double *hsml;    // FP64 hsml (FP32 kernels: exact re-tests)
     const ForceRecF *recf; // FP32 kernels
+    // FP32 kernels: a pair whose branch decision lies within the FP32 rounding
+    // margin of its threshold sends both its particles to an FP64 gradient
+    // redo (gradients of the FP64 pass, into grad) and the particle whose loop
+    // met it to an FP64 force redo.  redo_flag per sorted slot (bit 0 force,
+    // bit 1 gradients; zero between stages); redo_list[0, n) force and
+    // redo_list[n, 2n) gradient entries, counted by redo_count[0] and [1]
+    // (zeroed before each stage).
+    const double *rho, *pres, *dhsml, *cn, *fce, *nvt;
+    double *grad;
+    int *redo_flag, *redo_list, *redo_count;
+    int n;
     F32Geom f32;
     Counters *ctr;
 };
@@ -285,6 +301,7 @@ void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float
                            const double *dhsml, const double *cn, const double *fce, const double *nvt,
                            const double *grad, double gamma, ForceRecF *rec);
 void launch_force_f32(hipStream_t s, const ForceArgs &a);
+void launch_force_redo_fp64(hipStream_t s, const ForceArgs &a);  // gradients, then force loops
 void launch_force_deferred_f32(hipStream_t s, const ForceArgs &a, int n_pairs);
 
 // density sweep of the entries in a.heavy_list (*a.n_heavy of them), one

@@ -129,6 +129,7 @@ __global__ void clear_counters_kernel(Counters *ctr)
         ctr->interactions = 0;
         ctr->retries = 0;
         ctr->exact_calls = 0;
+        ctr->redo_n = 0;
         ctr->status = 0;
         ctr->overflow_slices = 0;
     }

@@ -241,16 +241,21 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
 // ---------------------------------------------------------------------------
 // gradients (gradients.c:1477-2089 pair loop in FP32, 1052-1253 finalize in FP64)
 // ---------------------------------------------------------------------------
+// x = hi + lo with hi = (float)x, lo = (float)(x - hi)
+__device__ __forceinline__ float low_word(double x, float hi) { return (float)(x - (double)hi); }
+
 __global__ void pack_grad_f32_kernel(int n, const float4 *offm, const float4 *velf, const double *hsml,
                                      const double *rho, const double *pres, GradRecF *rec)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float4 o = offm[k], v = velf[k];
+    const double d = rho[k], P = pres[k];
+    const float df = (float)d, Pf = (float)P;
     GradRecF r;
     r.xh = make_float4(o.x, o.y, o.z, (float)hsml[k]);
-    r.vr = make_float4(v.x, v.y, v.z, (float)rho[k]);
-    r.pm = make_float4((float)pres[k], o.w, 0.0f, 0.0f);
+    r.vr = make_float4(v.x, v.y, v.z, df);
+    r.pm = make_float4(Pf, o.w, low_word(d, df), low_word(P, Pf));
     rec[k] = r;
 }
 
@@ -281,7 +286,7 @@ __global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a
             const double h2_i = h_i * h_i;
             const Band bi = band_of(h2_i);
             const float pos_min = (float)(h2_i * kBand);  // r2 below this: re-test r2 > 0 in FP64
-            const float rho_f = ri.vr.w, p_f = ri.pm.x;
+            const float rho_f = ri.vr.w, p_f = ri.pm.x, rho_lo = ri.pm.z, p_lo = ri.pm.w;
             const double xs[3] = {pi.x, pi.y, pi.z};
             const double R = (h_i > G.hmax) ? h_i : G.hmax;
             const float h_fine = smax_of(G) / kFineCells;  // pairs with a smaller hsml: FP64 geometry
@@ -338,9 +343,10 @@ __global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a
                 cnt++;
                 const float r = Real<float>::root(r2);
                 if (r > maxd) maxd = r;
-                const float d_rho = vr.w - rho_f;
+                // differences of the split values (GradRecF)
+                const float d_rho = (vr.w - rho_f) + (pm.z - rho_lo);
                 minmax_check(d_rho, mn_r, mx_r);
-                const float dpp = pm.x - p_f;
+                const float dpp = (pm.x - p_f) + (pm.w - p_lo);
                 minmax_check(dpp, mn_p, mx_p);
                 const float dv0 = vr.x - ri.vr.x, dv1 = vr.y - ri.vr.y, dv2 = vr.z - ri.vr.z;
                 minmax_check(dv0, mn_v0, mx_v0);
@@ -487,6 +493,38 @@ __device__ __forceinline__ void list_dp(const BoxDesc &G, bool exact_geom, const
     }
 }
 
+// FP64 pair side of particle k from the FP64 state, with the expressions of
+// the FP64 force records (pack_force_kernel: cs, V, e, 1/h)
+__device__ __forceinline__ void side_fp64(const ForceArgs &a, int k, PairSide<double> &S)
+{
+    const double4 p = a.posm[k], v = a.velu[k];
+    const double d = a.rho[k], P = a.pres[k], h = a.hsml[k];
+    S.v[0] = v.x; S.v[1] = v.y; S.v[2] = v.z;
+    S.cs = sqrt(a.G.G0 * P / d);  // eos.c:245-254
+    S.h = h; S.hinv = 1.0 / h;
+    S.rho = d; S.P = P; S.V = p.w / d; S.dhsml = a.dhsml[k];
+    S.cn = a.cn[k]; S.fce = a.fce[k]; S.e = P / d;
+    const double *t = a.nvt + 6 * (size_t)k;
+#pragma unroll
+    for (int q = 0; q < 6; q++) S.nvt[q] = t[q];
+    const double *g = a.grad + 15 * (size_t)k;
+#pragma unroll
+    for (int q = 0; q < 3; q++) { S.gr[q] = g[q]; S.gp[q] = g[3 + q]; }
+#pragma unroll
+    for (int q = 0; q < 9; q++) S.gv[q] = g[6 + q];
+}
+
+// The pair (i, j) of particle i's loop returned kNearThreshold: list i for the
+// FP64 force redo and i and j for the FP64 gradient redo, each once.
+__device__ __forceinline__ void mark_redo(const ForceArgs &a, int i, int j)
+{
+    const int fi = atomicOr(&a.redo_flag[i], 3);
+    if (!(fi & 1)) a.redo_list[atomicAdd(&a.redo_count[0], 1)] = i;
+    if (!(fi & 2)) a.redo_list[a.n + atomicAdd(&a.redo_count[1], 1)] = i;
+    const int fj = atomicOr(&a.redo_flag[j], 2);
+    if (!(fj & 2)) a.redo_list[a.n + atomicAdd(&a.redo_count[1], 1)] = j;
+}
+
 __device__ __forceinline__ void side_i_f(const ForceRecF &ri, PairI<float> &I)
 {
     unpack_side_f(ri, I);
@@ -561,7 +599,8 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
                     add(j, Fv, Fp, vsig, mask, e);
                 } else if (!THREE_PHASE && e == kDeferred) {
                     const int e2 = force_pair<float, false>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
-                    add(j, Fv, Fp, vsig, mask, e2);
+                    if (e2 == kNearThreshold) mark_redo(a, i, j);
+                    else add(j, Fv, Fp, vsig, mask, e2);
                 }
             };
             const double h_i = a.hsml[i];
@@ -679,7 +718,9 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
     wave_add_u64(&a.ctr->exact_calls, nexact);
 }
 
-// Deferred exact-solver pairs of the FP32 list kernel, one per lane.
+// Deferred pairs of the FP32 list kernel, one per lane: exact-solver pairs
+// (FP32 with the FP64 exact solver) and near-threshold pairs, whose particle
+// this lists for the FP64 redo (its result slot stays zero).
 __global__ __launch_bounds__(kBlock) void force_deferred_f32_kernel(ForceArgs a)
 {
     const int q = a.defer.q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -705,18 +746,206 @@ __global__ __launch_bounds__(kBlock) void force_deferred_f32_kernel(ForceArgs a)
         ROut<float> ro = {0, 0, {0, 0, 0}, 0};
         float Fv[3], Fp, vsig;
         int mask;
-        const int e = force_pair<float, false, true>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
-        nretry += (mask & 8) ? 1 : 0;
-        nexact += (mask & 4) ? 1 : 0;
+        int e = force_pair<float, false, true>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
         double *o = a.defer.result + 5 * (size_t)(q - a.defer.q_begin);
-        o[0] = Fv[0]; o[1] = Fv[1]; o[2] = Fv[2]; o[3] = Fp; o[4] = vsig;
-        if (a.is_active && !a.is_active[j]) {
-            if ((double)vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;
+        if (e == kNearThreshold) {
+            mark_redo(a, i, j);
+            o[0] = 0; o[1] = 0; o[2] = 0; o[3] = 0; o[4] = 0;
+            e = 0;
+        } else {
+            o[0] = Fv[0]; o[1] = Fv[1]; o[2] = Fv[2]; o[3] = Fp; o[4] = vsig;
+            if (a.is_active && !a.is_active[j]) {
+                if ((double)vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;
+            }
+            nretry += (mask & 8) ? 1 : 0;
+            nexact += (mask & 4) ? 1 : 0;
         }
         if (e) atomicMax(&a.ctr->status, e);
     });
     wave_add_u64(&a.ctr->retries, nretry);
     wave_add_u64(&a.ctr->exact_calls, nexact);
+}
+
+// The gradients of each particle listed by mark_redo, again, in FP64 from the
+// FP64 state with the arithmetic of gradients_kernel (gradients.c:66-75,
+// 1219-1232), written over the FP32 gradients.  One wavefront per listed
+// particle: the lanes share its candidates (wave_pair_candidates) and reduce
+// their partial sums, maxima and minima; lane 0 applies NV_T and the limiter.  Sums in another order than the FP64 pass: equal to ~1e-16.
+__global__ __launch_bounds__(kBlock) void gradients_redo_fp64_kernel(ForceArgs a)
+{
+    const int n_redo = a.redo_count[1];
+    const int lane = threadIdx.x & 63;
+    const int n_waves = gridDim.x * (kBlock / 64);
+    for (int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < n_redo; w += n_waves) {
+        const int i = a.redo_list[a.n + w];
+        const BoxDesc &G = a.boxes[a.box_of[i]];
+        const double4 pi = a.posm[i], vi = a.velu[i];
+        double g[15];
+#pragma unroll
+        for (int q = 0; q < 15; q++) g[q] = 0;
+        double maxd = 0, mn_r = 0, mx_r = 0, mn_p = 0, mx_p = 0;
+        double mn_v0 = 0, mx_v0 = 0, mn_v1 = 0, mx_v1 = 0, mn_v2 = 0, mx_v2 = 0;
+        const double h_i = a.hsml[i], rho_i = a.rho[i], p_i = a.pres[i], m_i = pi.w;
+        if ((m_i > 0) && (rho_i > 0) && (h_i > 0)) {
+            double hinv, hinv3, hinv4;
+            kernel_hinv(h_i, hinv, hinv3, hinv4);
+            const double h2_i = h_i * h_i;
+            const double xs[3] = {pi.x, pi.y, pi.z};
+            const double R = (h_i > G.hmax) ? h_i : G.hmax;
+            wave_pair_candidates(a.tree, G, xs, R, h_i, lane, [&](int j) {
+                const double4 xj = a.posm[j];
+                double dp[3] = {xs[0] - xj.x, xs[1] - xj.y, xs[2] - xj.z};
+                wrap_dp(G.box, dp);
+                const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                const double h_j = a.hsml[j];
+                if (r2 <= 0) return;
+                if ((r2 >= h2_i) && (r2 >= h_j * h_j)) return;
+                const double rho_j = a.rho[j];
+                if (!((xj.w > 0) && (rho_j > 0))) return;
+                if (!(h_j > 0)) return;
+                const double4 vj = a.velu[j];
+                const double r = sqrt(r2);
+                const double wk_i = (r < h_i) ? kernel_w(r * hinv, hinv3) : 0.0;
+                if (r > maxd) maxd = r;
+                const double d_rho = rho_j - rho_i;
+                minmax_check(d_rho, mn_r, mx_r);
+                const double dpp = a.pres[j] - p_i;
+                minmax_check(dpp, mn_p, mx_p);
+                const double dv0 = vj.x - vi.x, dv1 = vj.y - vi.y, dv2 = vj.z - vi.z;
+                minmax_check(dv0, mn_v0, mx_v0);
+                minmax_check(dv1, mn_v1, mx_v1);
+                minmax_check(dv2, mn_v2, mx_v2);
+                if (r < h_i) {
+                    const double w0 = -wk_i * dp[0], w1 = -wk_i * dp[1], w2 = -wk_i * dp[2];
+                    g[0] += w0 * d_rho; g[1] += w1 * d_rho; g[2] += w2 * d_rho;
+                    g[3] += w0 * dpp; g[4] += w1 * dpp; g[5] += w2 * dpp;
+                    g[6] += w0 * dv0; g[7] += w1 * dv0; g[8] += w2 * dv0;
+                    g[9] += w0 * dv1; g[10] += w1 * dv1; g[11] += w2 * dv1;
+                    g[12] += w0 * dv2; g[13] += w1 * dv2; g[14] += w2 * dv2;
+                }
+            });
+        }
+#pragma unroll
+        for (int q = 0; q < 15; q++) g[q] = wave_sum(g[q]);
+        maxd = wave_max(maxd);
+        mx_r = wave_max(mx_r); mn_r = wave_min(mn_r);
+        mx_p = wave_max(mx_p); mn_p = wave_min(mn_p);
+        mx_v0 = wave_max(mx_v0); mn_v0 = wave_min(mn_v0);
+        mx_v1 = wave_max(mx_v1); mn_v1 = wave_min(mn_v1);
+        mx_v2 = wave_max(mx_v2); mn_v2 = wave_min(mn_v2);
+        if (lane == 0) {
+            double nvt[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) nvt[q] = a.nvt[6 * (size_t)i + q];
+            double gr[3] = {g[0], g[1], g[2]}, gp[3] = {g[3], g[4], g[5]};
+            double gv0[3] = {g[6], g[7], g[8]}, gv1[3] = {g[9], g[10], g[11]}, gv2[3] = {g[12], g[13], g[14]};
+            apply_nvt(nvt, gr);
+            apply_nvt(nvt, gp);
+            apply_nvt(nvt, gv0);
+            apply_nvt(nvt, gv1);
+            apply_nvt(nvt, gv2);
+            const double cn = a.cn[i];
+            const double d_max = (h_i > maxd) ? h_i : maxd;
+            double a_lim = 0.25;
+            if (cn > 100) a_lim = dmin(0.5, 0.25 + 0.25 * (cn - 100) / 100);
+            slope_limit(gr, mx_r, mn_r, a_lim, h_i, 0, 1, d_max, rho_i);
+            slope_limit(gp, mx_p, mn_p, a_lim, h_i, 0.1, 1, d_max, p_i);
+            slope_limit(gv0, mx_v0, mn_v0, a_lim, h_i, 0.1, 0, 0, 0);
+            slope_limit(gv1, mx_v1, mn_v1, a_lim, h_i, 0.1, 0, 0, 0);
+            slope_limit(gv2, mx_v2, mn_v2, a_lim, h_i, 0.1, 0, 0, 0);
+            double *o = a.grad + 15 * (size_t)i;
+            o[0] = gr[0]; o[1] = gr[1]; o[2] = gr[2];
+            o[3] = gp[0]; o[4] = gp[1]; o[5] = gp[2];
+            o[6] = gv0[0]; o[7] = gv0[1]; o[8] = gv0[2];
+            o[9] = gv1[0]; o[10] = gv1[1]; o[11] = gv1[2];
+            o[12] = gv2[0]; o[13] = gv2[1]; o[14] = gv2[2];
+            a.redo_flag[i] &= 1;
+        }
+    }
+}
+
+// The force loop of each particle listed by mark_redo, again, in FP64 from the
+// FP64 state (with the redone gradients): the FP64 pass's neighbour predicate,
+// pair arithmetic and inline exact solver (hydro_evaluate.h:22-516),
+// overwriting the FP32 result, and clearing the particle's flag.  One
+// wavefront per listed particle: the lanes share its candidates
+// (wave_pair_candidates) and reduce their flux sums; lane 0 applies the final
+// operations.
+__global__ __launch_bounds__(kBlock) void force_redo_fp64_kernel(ForceArgs a)
+{
+    const int n_redo = a.redo_count[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n_redo > 0) atomicAdd(&a.ctr->redo_n, n_redo);
+    const int lane = threadIdx.x & 63;
+    const int n_waves = gridDim.x * (kBlock / 64);
+    for (int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < n_redo; w += n_waves) {
+        const int i = a.redo_list[w];
+        const BoxDesc &G = a.boxes[a.box_of[i]];
+        const double4 pi = a.posm[i];
+        const double m_i = pi.w;
+        double acc[3] = {0, 0, 0}, dte = 0, out_vsig = kMinReal;
+        int status = 0;
+        const bool live = (m_i > 0) && (a.rho[i] > 0);
+        const double h_i = a.hsml[i];
+        if (live) {
+            PairI<double> I;
+            side_fp64(a, i, I);
+            I.hinv3 = I.hinv * I.hinv * I.hinv;
+            I.hinv4 = I.hinv3 * I.hinv;
+            I.cnumcrit2 = kCondDanger * kCondDanger - I.cn * I.cn;
+            out_vsig = I.cs;
+            const double h2_i = h_i * h_i;
+            const double xs[3] = {pi.x, pi.y, pi.z};
+            const double R = (h_i > G.hmax) ? h_i : G.hmax;
+            ROut<double> ro = {0, 0, {0, 0, 0}, 0};
+            wave_pair_candidates(a.tree, G, xs, R, h_i, lane, [&](int j) {
+                if (j == i) return;
+                double dp[3];
+                const double r2 = exact_r2(G.box, pi, a.posm[j], dp);
+                const double h_j = a.hsml[j];
+                if (!(r2 > 0) || (!(r2 < h2_i) && !(r2 < h_j * h_j))) return;
+                if (!(a.posm[j].w > 0) || !(a.rho[j] > 0)) return;  // hydro_evaluate.h:149-150
+                PairSide<double> J;
+                side_fp64(a, j, J);
+                double Fv[3], Fp, vsig;
+                int mask;
+                const int e = force_pair<double, false, true>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
+                if (e > status) status = e;
+                acc[0] += Fv[0];
+                acc[1] += Fv[1];
+                acc[2] += Fv[2];
+                dte += Fp;
+                if (vsig > out_vsig) out_vsig = vsig;
+                if (a.is_active && !a.is_active[j]) {
+                    if (vsig > kWakeup * a.vsig_old[j]) a.wakeup[j] = 1;
+                }
+            });
+        }
+        acc[0] = wave_sum(acc[0]);
+        acc[1] = wave_sum(acc[1]);
+        acc[2] = wave_sum(acc[2]);
+        dte = wave_sum(dte);
+        out_vsig = wave_max(out_vsig);
+        for (int off = 32; off > 0; off >>= 1) {
+            const int o = __shfl_xor(status, off, 64);
+            status = (o > status) ? o : status;
+        }
+        if (lane == 0) {
+            double vmax = kMinReal;
+            if (live) {
+                if (vmax < out_vsig) vmax = out_vsig;
+                const double4 v = a.velu[i];
+                const double vi[3] = {v.x, v.y, v.z};
+                force_final_ops(acc, dte, vi, m_i, h_i, a.max_hsml);
+            }
+            a.acc[3 * (size_t)i + 0] = acc[0];
+            a.acc[3 * (size_t)i + 1] = acc[1];
+            a.acc[3 * (size_t)i + 2] = acc[2];
+            a.dtu[i] = dte;
+            a.vsig[i] = vmax;
+            a.redo_flag[i] = 0;
+            if (status) atomicMax(&a.ctr->status, status);
+        }
+    }
 }
 
 inline unsigned nblocks(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -768,6 +997,12 @@ void launch_force_f32(hipStream_t s, const ForceArgs &a)
     } else {
         hipLaunchKernelGGL((force_f32_kernel<false, false>), grid, block, 0, s, a);
     }
+}
+void launch_force_redo_fp64(hipStream_t s, const ForceArgs &a)
+{
+    // fixed grids: the counts are known on the device only (graph capture)
+    hipLaunchKernelGGL(gradients_redo_fp64_kernel, dim3(1024), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(force_redo_fp64_kernel, dim3(1024), dim3(kBlock), 0, s, a);
 }
 void launch_force_deferred_f32(hipStream_t s, const ForceArgs &a, int n_pairs)
 {

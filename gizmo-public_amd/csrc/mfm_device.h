@@ -416,16 +416,33 @@ struct ROut {
     T P_M, S_M, Fv[3], Fp;
 };
 
-// get_wavespeeds_and_pressure_star: Gaburov HLLC estimate, Roe, then PVRS
-// (reimann.h:527-593).  Sets P_M and S_M.
+// FP32 only: a branch of the wavespeed estimate or of the Riemann retry whose
+// FP32 operand lies within this relative margin of its threshold may go the
+// other way than in the FP64 pass (the reconstructed FP32 states carry
+// rounding of ~1e-7 to 1e-6 relative); the pair is then evaluated in FP64
+// instead (kNearThreshold).
+constexpr float kDecideMargin = 1e-5f;
+
 template <typename T>
-__device__ inline void wavespeeds_pstar(const RState<T> &L, const RState<T> &R, const GammaConsts &G, const T n[3],
+__device__ __forceinline__ bool near_threshold(T x, T threshold, T scale)
+{
+    return sizeof(T) == sizeof(float) && !(fabs(x - threshold) > T(kDecideMargin) * fabs(scale));
+}
+
+// get_wavespeeds_and_pressure_star: Gaburov HLLC estimate, Roe, then PVRS
+// (reimann.h:527-593).  Sets P_M and S_M.  Returns true (FP32 only, with
+// P_M and S_M unset) when the vacuum test or a P_M > limiter test is within
+// kDecideMargin of its threshold.
+template <typename T>
+__device__ inline bool wavespeeds_pstar(const RState<T> &L, const RState<T> &R, const GammaConsts &G, const T n[3],
                                         T vL, T vR, T cL, T cR, T hL, T hR, T limiter, ROut<T> &o)
 {
     using M = Real<T>;
-    if ((vR - vL) > tmax(cL, cR)) {
+    const T cmax = tmax(cL, cR);
+    if (near_threshold(vR - vL, cmax, cmax)) return true;
+    if ((vR - vL) > cmax) {
         o.P_M = M::tiny; o.S_M = 0;
-        return;
+        return false;
     }
     T PT_L = L.p, PT_R = R.p;
     T S_L = tmin(vL, vR) - tmax(cL, cR);
@@ -434,6 +451,7 @@ __device__ inline void wavespeeds_pstar(const RState<T> &L, const RState<T> &R, 
     T rwR = R.rho * (S_R - vR);
     o.S_M = M::div(((PT_R - PT_L) + rwL * vL - rwR * vR), (rwL - rwR));
     o.P_M = M::div((PT_L * rwR - PT_R * rwL + rwL * rwR * (vR - vL)), (rwR - rwL));
+    if (near_threshold(o.P_M, limiter, limiter)) return true;
     if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
     if ((o.P_M <= 0) || isnan(o.P_M) || (o.P_M > limiter)) {
         T sL = M::root(L.rho), sR = M::root(R.rho);
@@ -450,6 +468,7 @@ __device__ inline void wavespeeds_pstar(const RState<T> &L, const RState<T> &R, 
         rwL = -L.rho * (S_L - vL);
         o.S_M = M::div(((rwR * vR + rwL * vL) + (PT_L - PT_R)), (rwR + rwL));
         o.P_M = L.rho * (vL - S_L) * (vL - o.S_M) + PT_L;
+        if (near_threshold(o.P_M, limiter, limiter)) return true;
         if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
         if ((o.P_M <= 0) || isnan(o.P_M) || (o.P_M > limiter)) {
             o.P_M = T(0.5) * ((PT_L + PT_R) + (vL - vR) * T(0.25) * (L.rho + R.rho) * (cL + cR));
@@ -460,6 +479,7 @@ __device__ inline void wavespeeds_pstar(const RState<T> &L, const RState<T> &R, 
             if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
         }
     }
+    return false;
 }
 
 // Kurganov-Tadmor/PWK flux, MFM branch (reimann.h:484-519).  Reached only on
@@ -605,7 +625,9 @@ __device__ __noinline__ ExactOut riemann_exact(double rL, double pL, double rR, 
 // bit0 HLLC, bit1 KT, bit2 exact; err gets 1234545 on the exact solver's
 // negative-density exit.  The exact solver always runs in FP64.  DEFER: stop
 // where the exact solver would be called and return kDeferBit instead.
+// FP32: kNearBit when the wavespeed estimate stopped at a near-threshold test.
 constexpr int kDeferBit = 32;
+constexpr int kNearBit = 64;
 
 template <typename T, bool DEFER = false, bool INLINE = false>
 __device__ __forceinline__ int riemann_solve(const RState<T> &L, const RState<T> &R, const T n[3], T limiter,
@@ -622,7 +644,7 @@ __device__ __forceinline__ int riemann_solve(const RState<T> &L, const RState<T>
     T vL = L.v[0] * n[0] + L.v[1] * n[1] + L.v[2] * n[2];
     T vR = R.v[0] * n[0] + R.v[1] * n[1] + R.v[2] * n[2];
     int mask = 1;
-    wavespeeds_pstar(L, R, G, n, vL, vR, cL, cR, hL, hR, limiter, o);
+    if (wavespeeds_pstar(L, R, G, n, vL, vR, cL, cR, hL, hR, limiter, o)) return mask | kNearBit;
     for (int k = 0; k < 3; k++) o.Fv[k] = o.P_M * n[k];
     o.Fp = o.P_M * o.S_M;
     if ((o.P_M < 0) || isnan(o.P_M)) {
@@ -672,8 +694,12 @@ struct PairI : PairSide<T> {
 // (0, 1234 or 1234545); Fv, Fp, vsig are the pair's flux terms for i, mask
 // the stages used (bit0 HLLC, bit1 KT, bit2 exact, bit3 retry).  DEFER:
 // returns kDeferred, with no flux terms, for a pair that would call the exact
-// Riemann solver (the caller evaluates it later with DEFER = false).
+// Riemann solver (the caller evaluates it later with DEFER = false).  FP32:
+// a pair whose wavespeed estimate or first-order-retry test lies within
+// kDecideMargin of its threshold returns kDeferred (DEFER) or kNearThreshold,
+// with no flux terms; the caller evaluates it in FP64.
 constexpr int kDeferred = -1;
+constexpr int kNearThreshold = -2;
 
 template <typename T, bool DEFER = false, bool INLINE = false>
 __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &J, const T dp[3], T r2,
@@ -775,8 +801,10 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
     T lim_a = lim;
     for (int attempt = 0; attempt < 3; attempt++) {
         mask |= riemann_solve<T, DEFER, INLINE>(SL, SR, n, lim_a, G, ro, err);
+        if (sizeof(T) == sizeof(float) && (mask & kNearBit)) return DEFER ? kDeferred : kNearThreshold;
         if (DEFER && (mask & kDeferBit)) return kDeferred;
         const bool bad = (ro.P_M < 0) || isnan(ro.P_M);
+        if (attempt == 0 && near_threshold(ro.P_M, T(1.4) * lim, lim)) return DEFER ? kDeferred : kNearThreshold;
         if (attempt == 0) {
             if (!(bad || (ro.P_M > T(1.4) * lim))) break;
             mask |= 8;

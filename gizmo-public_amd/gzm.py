@@ -68,6 +68,7 @@ class PassStats(C.Structure):
         ("riemann_retries", C.c_int64),
         ("exact_solver_calls", C.c_int64),
         ("density_updates", C.c_int64),
+        ("fp64_force_particles", C.c_int64),
     ]
 
 
@@ -311,7 +312,7 @@ def run_pass(ic: dict, active=None, params: Params | None = None, stages=("densi
         if "force" in stages:
             s = ctx.hydro_force(active)
             stats.update(force_interactions=s.force_interactions, riemann_retries=s.riemann_retries,
-                         exact_solver_calls=s.exact_solver_calls)
+                         exact_solver_calls=s.exact_solver_calls, fp64_force_particles=s.fp64_force_particles)
         out = ctx.download()
         stats["timing"] = ctx.timing()
     return out, stats

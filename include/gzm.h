@@ -131,6 +131,9 @@ typedef struct gzm_pass_stats {
     int64_t riemann_retries;         /* pairs that needed the first-order retry */
     int64_t exact_solver_calls;      /* pairs that reached the exact solver */
     int64_t density_updates;         /* sum over sweeps of the particles swept */
+    int64_t fp64_force_particles;    /* FP32-bulk force loop: particles whose loop was redone
+                                        in FP64 because a pair's wavespeed-estimate or Riemann
+                                        retry branch lay within FP32 rounding of its threshold */
 } gzm_pass_stats;
 
 /* Per-kernel device time (HIP events on the context stream), accumulated
