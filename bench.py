@@ -218,6 +218,24 @@ def measured_traffic(kernel, precision, n):
     return rec["fetch_bytes"] + rec["write_bytes"]
 
 
+def measured_valu(kernel, precision, n):
+    """VALU busy share and FLOP rates of `kernel` from the committed rocprof
+    counter summary of the 512-box workload, or None when it does not cover it."""
+    if n != 31542962:
+        return None
+    try:
+        with open(os.path.join(ROOT, "profiles", "valu_batch512.json")) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = tab.get("kernels", {}).get(f"{kernel}:{'fp64' if fp64_kernel(precision, kernel) else 'fp32'}")
+    if rec is None:
+        return None
+    return {"kernel": kernel, "valu_busy": rec["valu_busy"], "tflops_fp64": rec["tflops_fp64"],
+            "tflops_fp32": rec["tflops_fp32"], "peak_tflops": tab["peaks_tflops"],
+            "source": "profiles/valu_batch512.json"}
+
+
 def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
     """Warm-up eager passes (per-kernel HIP events), then `steps` passes in one
     HIP graph (timed with events around the launch)."""
@@ -390,6 +408,8 @@ def main_batch(args, gzm, ics):
             "bytes_per_launch": r32["bytes_per_launch"],
             "mean_launch_ms": r32["mean_launch_ms"],
         },
+        # rocprof VALU counters of the dominant kernel on this workload (committed summary)
+        "valu": measured_valu(r32["dominant"], 1, N),
     }
     if args.cpu_repeats > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
