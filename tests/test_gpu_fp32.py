@@ -120,3 +120,24 @@ def test_fp32_near_threshold_pairs_redone_in_fp64():
     rel = np.abs(out32["acc"] - out64["acc"]).sum(1) / (a64 + np.median(a64))
     print("fp64 redo particles", st["fp64_force_particles"], "max rel dv/dt", rel.max())
     assert rel.max() < 1e-4
+
+
+def test_fp32_redo_in_graph_passes_equals_eager():
+    """The redo lists are counted on the device and their flags cleared by the
+    redo kernels: passes captured in one graph (several per graph, replayed)
+    redo the same particles and give the eager pass's bits."""
+    ic = ics.batch_boxes(64)
+    with gzm.MFMContext(gzm.make_params(precision=1)) as ctx:
+        ctx.upload(ic)
+        ctx.restore_initial_state()
+        ctx.density()
+        ctx.gradients()
+        s3 = ctx.hydro_force()
+        assert s3.fp64_force_particles > 0
+        eager = ctx.download()
+        for rep in range(2):
+            st = ctx.run_passes(3)
+            assert st.fp64_force_particles == 3 * s3.fp64_force_particles
+            out = ctx.download()
+            for k in ("acc", "dtu", "vsig", "grad_rho", "grad_p", "grad_v"):
+                np.testing.assert_array_equal(out[k], eager[k], err_msg=f"replay {rep} {k}")
