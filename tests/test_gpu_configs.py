@@ -1,7 +1,7 @@
 """The shear-layer (C3) and clustered (C5) configurations at reduced size:
 FP64 pass against the oracle to the parity bar, FP32-bulk pass within its
 error bounds, and the clustered case's 10 %-active pass that reuses the state
-of the other 90 % (SURVEY row d, C5)."""
+of the other 90 %."""
 import numpy as np
 import pytest
 
