@@ -93,6 +93,8 @@ def parse():
     ap.add_argument("--cpu-repeats", type=int, default=3, help="oracle passes for the CPU baseline (0 = skip)")
     ap.add_argument("--cell-size", type=float, default=0.0)
     ap.add_argument("--fp32", type=int, default=1, help="blast workload: also time the FP32-bulk mode")
+    ap.add_argument("--plain", type=int, default=1,
+                    help="batch workload: also time the plain form (FP64, one box per pass, eager API)")
     ap.add_argument("--fp32-density", type=int, default=1,
                     help="batch workload: also time precision 2 (FP32 density pair sums)")
     ap.add_argument("--blast", type=int, default=1, help="batch workload: also time the 128^3 blast case")
@@ -236,6 +238,37 @@ def measured_valu(kernel, precision, n):
             "source": "profiles/valu_batch512.json"}
 
 
+def run_plain_per_box(gzm, ic, tuned=None):
+    """The plain form: FP64, one box per pass through the eager API (density,
+    gradients, hydro_force, each returning when its kernels finished), as the
+    drop-in runs one particle set per call.  Each box gets its own context,
+    upload and one untimed warm-up pass; the timed pass starts from the
+    restored initial state with the inputs resident in HBM."""
+    import ics
+    params = dict(precision=0)
+    if tuned:
+        params.update(cell_size=-tuned["cell_factor"], pair_list_cap=tuned["pair_list_cap"])
+    nb = len(ic["box_offsets"]) - 1
+    total_s, inter = 0.0, 0
+    for b in range(nb):
+        sub = ics.box_slice(ic, b)
+        with gzm.MFMContext(gzm.make_params(box=tuple(sub["box"]), **params)) as ctx:
+            ctx.upload(sub)
+            for timed in (False, True):
+                ctx.restore_initial_state()
+                t0 = time.perf_counter()
+                s1 = ctx.density()
+                s2 = ctx.gradients()
+                s3 = ctx.hydro_force()
+                dt = time.perf_counter() - t0
+            total_s += dt
+            inter += s1.density_interactions + s2.gradient_interactions + s3.force_interactions
+        if (b + 1) % 128 == 0:
+            log(f"plain FP64 per box: {b + 1}/{nb} boxes, {1e3 * total_s:.1f} ms so far")
+    return {"value": inter / total_s, "ms_per_pass": 1e3 * total_s, "interactions_per_pass": int(inter),
+            "boxes": nb}
+
+
 def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
     """Warm-up eager passes (per-kernel HIP events), then `steps` passes in one
     HIP graph (timed with events around the launch)."""
@@ -347,6 +380,9 @@ def main_batch(args, gzm, ics):
         log(f"precision 2: {r32d['ms_per_step']:.2f} ms/pass")
     r64 = run_graph_mode(gzm, ic, args, 0, steps, tuned.get(0))
     log(f"precision 0: {r64['ms_per_step']:.2f} ms/pass")
+    plain = run_plain_per_box(gzm, ic, tuned.get(0)) if args.plain else None
+    if plain is not None:
+        log(f"plain FP64, one box per pass: {plain['ms_per_pass']:.1f} ms for all boxes")
     traffic = measured_traffic(r32["dominant"], 1, N)
     result = {
         "metric": "particle-neighbor interactions/s, MFM pass; FP32-batched vs FP64 at stated error",
@@ -381,7 +417,14 @@ def main_batch(args, gzm, ics):
             "density_sweeps": r64["density_sweeps"],
             "kernels_ms_per_pass_eager": r64["kernels_ms_per_pass_eager"],
         },
+        # precision alone: FP32-bulk vs FP64, both batched in one graph
         "fp32_speedup_vs_fp64": r32["value"] / r64["value"],
+        # the plain form (FP64, one box per pass through the eager API) and the
+        # chosen form's speed-up over it (FP32-bulk and batching together)
+        "fp64_plain_per_box": plain,
+        "fp32_batched_speedup_vs_plain_fp64": (None if plain is None else r32["value"] / plain["value"]),
+        "plain_interactions_equal_batched": (None if plain is None else
+                                             bool(plain["interactions_per_pass"] == r64["interactions_per_pass"])),
         "fp32_interactions_equal_fp64": bool(r32["interactions_per_pass"] == r64["interactions_per_pass"]),
         "fp32_density_too": (None if r32d is None else {
             "precision": 2,
