@@ -112,6 +112,32 @@ __global__ void pack_geom_f32_kernel(int n, const double4 *posm, const double4 *
 // ---------------------------------------------------------------------------
 // density (density.c:252-347 pair sums in FP32 + 505-850 iteration in FP64)
 // ---------------------------------------------------------------------------
+// FP32 density sums carry rounding of up to ~1e-6 relative.  The hsml step's
+// discrete decisions (the redo test against the neighbour-number tolerance,
+// the bracket test, the Newton / bisection branches, the sign of a clamped
+// jump) may then go the other way than with FP64 sums, moving hsml by a whole
+// step.  A particle whose next hsml jumps when its neighbour number is moved
+// by +-kSumMargin (relative) is close to such a decision: its sums are redone
+// in FP64 (density_heavy_kernel) instead.
+constexpr double kSumMargin = 2e-5;
+
+__device__ inline bool hsml_step_near(const HsmlParams &P, const HsmlStep &X, double h, double left, double right)
+{
+    double hs[2];
+    int rs[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        HsmlStep Y = X;
+        Y.NumNgb = X.NumNgb * (k ? 1 + kSumMargin : 1 - kSumMargin);
+        double hh = h, l = left, r = right, cn;
+        rs[k] = hsml_step(P, Y, hh, l, r, cn);
+        hs[k] = hh;
+    }
+    // away from a decision the step moves hsml by ~Dhsml/3 of the relative
+    // change, 7e-6 here; a decision moves it by ~1e-2 or more
+    return rs[0] != rs[1] || !(fabs(hs[1] - hs[0]) <= 1e-4 * h);
+}
+
 template <bool TREE>
 __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepArgs a)
 {
@@ -216,11 +242,17 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
             a.heavy_list[atomicAdd(a.n_heavy, 1)] = t;
             return;
         }
-        cnt += lane_cnt;
         DensitySums S = {s_ngb, s_rho, s_dh, s_div, t00, t01, t02, t11, t12, t22, f0, f1, f2};
         double left = a.left[i], right = a.right[i];
         DensityResult R;
-        const int res = hsml_update(a.hp, S, h, left, right, a.cn[i], R);
+        HsmlStep X;
+        hsml_prepare(a.hp, S, h, a.cn[i], R, X);
+        if (hsml_step_near(a.hp, X, h, left, right)) {  // decided in FP64 by density_heavy_kernel
+            a.heavy_list[atomicAdd(a.n_heavy, 1)] = t;
+            return;
+        }
+        cnt += lane_cnt;
+        const int res = hsml_step(a.hp, X, h, left, right, R.cn_stored);
         a.hsml[i] = h;
         a.left[i] = left;
         a.right[i] = right;

@@ -176,8 +176,17 @@ __device__ __forceinline__ double ncorr_from_cn(double cn)
     return nc;
 }
 
-__device__ inline int hsml_update(const HsmlParams &P, const DensitySums &S, double &h, double &Left,
-                                  double &Right, double cn_stored_in, DensityResult &R)
+// What the hsml step (density.c:594-850) decides on: the normalised
+// neighbour number, its target and tolerance, and the condition numbers.
+struct HsmlStep {
+    double NumNgb, Dhsml, desnumngb, desnumngbdev, ConditionNumber, cn_stored;
+};
+
+// density.c:505-595 on one particle's sums: normalisation, NV_T inversion
+// with preconditioning, face-closure error and the neighbour-number target;
+// fills R but R.cn_stored (set by hsml_step).
+__device__ inline void hsml_prepare(const HsmlParams &P, const DensitySums &S, double h, double cn_stored_in,
+                                    DensityResult &R, HsmlStep &X)
 {
     const int iter = P.iter;
     double desnumngb = P.des_num_ngb, desnumngbdev = P.max_num_ngb_dev;
@@ -245,7 +254,28 @@ __device__ inline int hsml_update(const HsmlParams &P, const DensitySums &S, dou
     desnumngbdev = desnumngbdev_0 * ncorr_ngb;
     if (iter > 1)
         desnumngbdev = dmin(0.25 * desnumngb, desnumngbdev * exp(0.1 * log(desnumngb / (16. * desnumngbdev)) * (double)iter));
+    R.num_ngb = NumNgb;
+    R.dhsml = Dhsml;
+    R.divvel = DivVel;
+    R.rho_sum = S.rho;
+    X.NumNgb = NumNgb;
+    X.Dhsml = Dhsml;
+    X.desnumngb = desnumngb;
+    X.desnumngbdev = desnumngbdev;
+    X.ConditionNumber = ConditionNumber;
+    X.cn_stored = cn_stored;
+}
 
+// density.c:659-850: the redo test, the bracket update and the next hsml.
+// Returns 1 (iterate again), 0 (done) or -1 (no bracket); cn_out is the
+// condition number to store.
+__device__ inline int hsml_step(const HsmlParams &P, const HsmlStep &X, double &h, double &Left, double &Right,
+                                double &cn_out)
+{
+    const int iter = P.iter;
+    const double NumNgb = X.NumNgb, Dhsml = X.Dhsml, desnumngb = X.desnumngb, desnumngbdev = X.desnumngbdev;
+    const double ConditionNumber = X.ConditionNumber;
+    double cn_stored = X.cn_stored;
     const double maxsoft = P.max_hsml, minsoft = P.min_hsml;
     int redo = 0, maxflag = 0, minflag = 0;
     if ((NumNgb < (desnumngb - desnumngbdev) && h < 0.999 * maxsoft) ||
@@ -260,14 +290,10 @@ __device__ inline int hsml_update(const HsmlParams &P, const DensitySums &S, dou
     }
     if (redo == 0) cn_stored = ConditionNumber;
 
-    R.num_ngb = NumNgb;
-    R.dhsml = Dhsml;
-    R.divvel = DivVel;
-    R.rho_sum = S.rho;
     int result = redo;
     if (redo) {
         if (Left > 0 && Right > 0 && (Right - Left) < 1.0e-3 * Left) {
-            R.cn_stored = ConditionNumber;  // bracket narrow enough: accept (density.c:732-740)
+            cn_out = ConditionNumber;  // bracket narrow enough: accept (density.c:732-740)
             return 0;
         }
         if ((maxflag == 0) && (minflag == 0)) {
@@ -298,7 +324,7 @@ __device__ inline int hsml_update(const HsmlParams &P, const DensitySums &S, dou
                     h = pow(h * Left * Right, 1.0 / 3.0);
                 }
             } else {
-                if (Right == 0 && Left == 0) { R.cn_stored = cn_stored; return -1; }
+                if (Right == 0 && Left == 0) { cn_out = cn_stored; return -1; }
                 if (Right == 0 && Left > 0) {
                     double fac_lim = (NumNgb > 1) ? log(desnumngb / NumNgb) / 3 : 1.4;
                     if ((NumNgb < 2 * desnumngb) && (NumNgb > 0.1 * desnumngb)) {
@@ -329,8 +355,16 @@ __device__ inline int hsml_update(const HsmlParams &P, const DensitySums &S, dou
         if (h > maxsoft) h = maxsoft;
         if (maxflag == 1) h = maxsoft;
     }
-    R.cn_stored = cn_stored;
+    cn_out = cn_stored;
     return result;
+}
+
+__device__ inline int hsml_update(const HsmlParams &P, const DensitySums &S, double &h, double &Left,
+                                  double &Right, double cn_stored_in, DensityResult &R)
+{
+    HsmlStep X;
+    hsml_prepare(P, S, h, cn_stored_in, R, X);
+    return hsml_step(P, X, h, Left, Right, R.cn_stored);
 }
 
 // ---------------------------------------------------------------------------
