@@ -86,8 +86,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=262144,
                     help="clustered workload: particles of the CPU-baseline sample")
     ap.add_argument("--list-cap", type=int, default=None,
-                    help="pair-list capacity (default: 64; 256 for the clustered workload, whose particles "
-                         "near large-hsml neighbours have more than 64 force pairs)")
+                    help="pair-list capacity (default 64: a particle with more force pairs is scanned on its "
+                         "own, the lists keep the exact-solver pairs out of the loop)")
     ap.add_argument("--boxes", type=int, default=512, help="boxes of the batched workload")
     ap.add_argument("--n", type=int, default=128, help="lattice side of the blast (default 128 -> 2,097,152 particles)")
     ap.add_argument("--cpu-repeats", type=int, default=3, help="oracle passes for the CPU baseline (0 = skip)")
@@ -512,7 +512,7 @@ def main_clustered(args, gzm, ics):
     (the other 90 % keep their hsml, density, gradients)."""
     args.steps = steps = args.steps or 5
     if args.list_cap is None:
-        args.list_cap = 256
+        args.list_cap = 64
     t0 = time.perf_counter()
     ic = ics.plummer_clusters(args.cluster_n, args.spheres)
     ic_s = time.perf_counter() - t0
