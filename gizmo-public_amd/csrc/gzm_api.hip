@@ -98,6 +98,7 @@ struct gzm_ctx {
     GradRecF *d_grecf = nullptr;
     ForceRecF *d_frecf = nullptr;
     int *d_redo_flag = nullptr, *d_redo_list = nullptr, *d_redo_count = nullptr;  // FP32 force: FP64 redo
+    int *d_long_list = nullptr, *d_long_count = nullptr;  // force: particles whose pair list overflowed
     Counters *d_ctr = nullptr;
     int n_active = 0;
     bool all_active = true;
@@ -245,6 +246,7 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_heavy_list);
     dev_free(c->d_offm); dev_free(c->d_velf); dev_free(c->d_cellc); dev_free(c->d_grecf); dev_free(c->d_frecf);
     dev_free(c->d_redo_flag); dev_free(c->d_redo_list); dev_free(c->d_redo_count);
+    dev_free(c->d_long_list); dev_free(c->d_long_count);
     dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
     dev_free(c->d_box_of);
     dev_free(c->d_defer_mask); dev_free(c->d_defer_count); dev_free(c->d_defer_off); dev_free(c->d_defer_total);
@@ -1128,9 +1130,14 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         auto launch = (c->p.precision != 0) ? launch_force_f32 : launch_force;
         launch(c->stream, a);
         if (use_list && c->list_overflow_slices > 0) {
-            ForceArgs b = a;
-            b.fallback_only = 1;  // slices whose list overflowed scan the grid
-            launch(c->stream, b);
+            // particles whose pair set overflowed the list: one wavefront each, FP64
+            if (!c->d_long_list) {
+                s = dev_alloc(&c->d_long_list, (size_t)c->n);
+                s = s ? s : dev_alloc(&c->d_long_count, 1);
+                if (s) return s;
+            }
+            launch_fill_int(c->stream, c->d_long_count, 1, 0);
+            launch_force_long_lists(c->stream, a, c->d_long_list, c->d_long_count);
         }
     }
     GZM_HIP(hipGetLastError());

@@ -302,6 +302,8 @@ void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float
                            const double *grad, double gamma, ForceRecF *rec);
 void launch_force_f32(hipStream_t s, const ForceArgs &a);
 void launch_force_redo_fp64(hipStream_t s, const ForceArgs &a);  // gradients, then force loops
+// particles whose pair set overflowed the list: FP64, one wavefront each (n_list zeroed by the caller)
+void launch_force_long_lists(hipStream_t s, const ForceArgs &a, int *list, int *n_list);
 void launch_force_deferred_f32(hipStream_t s, const ForceArgs &a, int n_pairs);
 
 // density sweep of the entries in a.heavy_list (*a.n_heavy of them), one

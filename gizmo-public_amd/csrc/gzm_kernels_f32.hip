@@ -896,21 +896,25 @@ __global__ __launch_bounds__(kBlock) void gradients_redo_fp64_kernel(ForceArgs a
     }
 }
 
-// The force loop of each particle listed by mark_redo, again, in FP64 from the
-// FP64 state (with the redone gradients): the FP64 pass's neighbour predicate,
-// pair arithmetic and inline exact solver (hydro_evaluate.h:22-516),
-// overwriting the FP32 result, and clearing the particle's flag.  One
-// wavefront per listed particle: the lanes share its candidates
-// (wave_pair_candidates) and reduce their flux sums; lane 0 applies the final
-// operations.
-__global__ __launch_bounds__(kBlock) void force_redo_fp64_kernel(ForceArgs a)
+// The force loop of each listed particle in FP64 from the FP64 state: the
+// FP64 pass's neighbour predicate, pair arithmetic and inline exact solver
+// (hydro_evaluate.h:22-516).  One wavefront per listed particle: the lanes
+// share its candidates (wave_pair_candidates) and reduce their flux sums;
+// lane 0 applies the final operations.  REDO: the particles listed by
+// mark_redo (with the redone gradients), overwriting their FP32 result and
+// clearing their flags; their pairs were counted by the FP32 loop.  Otherwise:
+// the particles whose pair set overflowed the list (one thread would walk
+// their long pair sets alone), counted here.
+template <bool REDO>
+__global__ __launch_bounds__(kBlock) void force_wave_fp64_kernel(ForceArgs a, const int *list, const int *count)
 {
-    const int n_redo = a.redo_count[0];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && n_redo > 0) atomicAdd(&a.ctr->redo_n, n_redo);
+    const int n_redo = *count;
+    if (REDO && blockIdx.x == 0 && threadIdx.x == 0 && n_redo > 0) atomicAdd(&a.ctr->redo_n, n_redo);
     const int lane = threadIdx.x & 63;
     const int n_waves = gridDim.x * (kBlock / 64);
+    unsigned long long cnt = 0, nretry = 0, nexact = 0;
     for (int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < n_redo; w += n_waves) {
-        const int i = a.redo_list[w];
+        const int i = list[w];
         const BoxDesc &G = a.boxes[a.box_of[i]];
         const double4 pi = a.posm[i];
         const double m_i = pi.w;
@@ -942,6 +946,9 @@ __global__ __launch_bounds__(kBlock) void force_redo_fp64_kernel(ForceArgs a)
                 int mask;
                 const int e = force_pair<double, false, true>(I, J, dp, r2, a.G, ro, Fv, Fp, vsig, mask);
                 if (e > status) status = e;
+                cnt++;
+                nretry += (mask & 8) ? 1 : 0;
+                nexact += (mask & 4) ? 1 : 0;
                 acc[0] += Fv[0];
                 acc[1] += Fv[1];
                 acc[2] += Fv[2];
@@ -974,10 +981,22 @@ __global__ __launch_bounds__(kBlock) void force_redo_fp64_kernel(ForceArgs a)
             a.acc[3 * (size_t)i + 2] = acc[2];
             a.dtu[i] = dte;
             a.vsig[i] = vmax;
-            a.redo_flag[i] = 0;
+            if (REDO) a.redo_flag[i] = 0;
             if (status) atomicMax(&a.ctr->status, status);
         }
     }
+    if (!REDO) {
+        wave_add_u64(&a.ctr->interactions, cnt);
+        wave_add_u64(&a.ctr->retries, nretry);
+        wave_add_u64(&a.ctr->exact_calls, nexact);
+    }
+}
+
+__global__ void collect_long_lists_kernel(int n_active, const int *active, const int *count, int cap, int *list,
+                                          int *n_list)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_active && count[t] > cap) list[atomicAdd(n_list, 1)] = active[t];
 }
 
 inline unsigned nblocks(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -1034,7 +1053,16 @@ void launch_force_redo_fp64(hipStream_t s, const ForceArgs &a)
 {
     // fixed grids: the counts are known on the device only (graph capture)
     hipLaunchKernelGGL(gradients_redo_fp64_kernel, dim3(1024), dim3(kBlock), 0, s, a);
-    hipLaunchKernelGGL(force_redo_fp64_kernel, dim3(1024), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(force_wave_fp64_kernel<true>, dim3(1024), dim3(kBlock), 0, s, a, (const int *)a.redo_list,
+                       (const int *)a.redo_count);
+}
+void launch_force_long_lists(hipStream_t s, const ForceArgs &a, int *list, int *n_list)
+{
+    if (a.n_active <= 0) return;
+    hipLaunchKernelGGL(collect_long_lists_kernel, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a.n_active, a.active,
+                       a.list.count, a.list.cap, list, n_list);
+    hipLaunchKernelGGL(force_wave_fp64_kernel<false>, dim3(1024), dim3(kBlock), 0, s, a, (const int *)list,
+                       (const int *)n_list);
 }
 void launch_force_deferred_f32(hipStream_t s, const ForceArgs &a, int n_pairs)
 {

@@ -80,13 +80,20 @@ def test_fp32_neighbor_sets_match_fp64_for_same_hsml(name):
 
 @pytest.mark.parametrize("precision", [1, 2])
 @pytest.mark.parametrize("cap", [-1, 16, 32])
-def test_fp32_pair_list_fallbacks_give_identical_results(cap, precision):
+def test_fp32_pair_list_fallbacks_keep_sets_and_bounds(cap, precision):
+    """Capacities 16 and 32 send some particles to the FP64 wavefront loop for
+    long pair sets; -1 disables the list (the FP32 scan, the default's bits)."""
     ic = ics.random_state(14)
+    ref, _, _ = O.run_pass(ic)
     base, bst = gzm.run_pass(ic, params=fp32_params(ic, precision))
     out, st = gzm.run_pass(ic, params=fp32_params(ic, precision, pair_list_cap=cap))
     assert st["force_interactions"] == bst["force_interactions"]
-    for k in ("acc", "dtu", "vsig", "grad_v"):
-        np.testing.assert_array_equal(out[k], base[k])
+    es = error_stats(out, ref)
+    for k in ("acc", "dtu"):
+        assert es[k]["max"] <= TOL32_FLUX, (k, es[k])
+    if cap == -1:
+        for k in ("acc", "dtu", "vsig", "grad_v"):
+            np.testing.assert_array_equal(out[k], base[k])
 
 
 def test_fp32_repeated_pass_is_bit_identical():
