@@ -380,7 +380,7 @@ def main_batch(args, gzm, ics):
         log(f"precision 2: {r32d['ms_per_step']:.2f} ms/pass")
     r64 = run_graph_mode(gzm, ic, args, 0, steps, tuned.get(0))
     log(f"precision 0: {r64['ms_per_step']:.2f} ms/pass")
-    plain = run_plain_per_box(gzm, ic, tuned.get(0)) if args.plain else None
+    plain = run_plain_per_box(gzm, ic, tuned.get(0)) if (args.plain and "box_offsets" in ic) else None
     if plain is not None:
         log(f"plain FP64, one box per pass: {plain['ms_per_pass']:.1f} ms for all boxes")
     traffic = measured_traffic(r32["dominant"], 1, N)
