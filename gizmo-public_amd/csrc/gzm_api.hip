@@ -118,6 +118,7 @@ struct gzm_ctx {
     size_t list_entries = 0;
     int list_cap = 64;
     bool list_valid = false;
+    bool frec_fresh = false;  // the gradient stage wrote every particle's force record
     int list_overflow_slices = 0;
     uint64_t active_version = 0, list_version = 0;
     std::vector<int> last_sweep_live;  // particles of each density sweep of the last eager call
@@ -258,6 +259,7 @@ void free_particles(gzm_ctx *c)
     c->defer_result_cap = 0;
     c->list_entries = 0;
     c->list_valid = false;
+    c->frec_fresh = false;
     c->n = 0;
 }
 
@@ -822,6 +824,7 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
     c->n_active = 0;
     c->active_h.clear();
     c->list_valid = false;
+    c->frec_fresh = false;
     c->hmax_valid = false;
     return GZM_OK;
 }
@@ -830,6 +833,7 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
 gzm_status restore_state(gzm_ctx *c)
 {
     c->hmax_valid = false;
+    c->frec_fresh = false;
     GZM_HIP(hipMemcpyAsync(c->d_hsml, c->d_hsml0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_cn, c->d_cn0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_vsig, c->d_vsig0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
@@ -963,6 +967,7 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     c->have_density = true;
     c->have_gradients = false;
     c->list_valid = false;
+    c->frec_fresh = false;
     if (gp) return GZM_OK;
     Counters h;
     s = read_counters(c, kCtrDensity, &h);
@@ -1013,6 +1018,16 @@ gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     a.recf = c->d_grecf;
     a.f32 = f32_geom(c);
     a.ctr = c->d_ctr + kCtrGradients;
+    // every particle active: the gradient kernel also writes the force records
+    // (the force stage then skips its pack kernel)
+    const bool fuse_records = c->all_active && c->n_active == c->n;
+    if (fuse_records) {
+        if (c->p.precision != 0) a.frecf_out = c->d_frecf;
+        else a.frec_out = c->d_frec;
+        a.dhsml = c->d_dhsml;
+        a.fce = c->d_fce;
+        a.gamma = c->p.gamma;
+    }
     if (c->n_active > 0 && c->list_cap > 0) {
         if (c->capturing && ((size_t)c->n_active + 63) / 64 * (size_t)c->list_cap * 64 > c->list_entries) {
             set_error("pair list too small while capturing");
@@ -1032,6 +1047,7 @@ gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     }
     GZM_HIP(hipGetLastError());
     c->have_gradients = true;
+    c->frec_fresh = fuse_records && c->n_active > 0;
     c->list_valid = (c->n_active > 0 && c->list_cap > 0);
     c->list_version = c->active_version;
     if (gp) {
@@ -1058,7 +1074,9 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     if (s) return s;
     {
         Timed tm(c, kOther);
-        if (c->p.precision != 0)
+        if (c->frec_fresh) {
+            // written by the gradient stage of this pass
+        } else if (c->p.precision != 0)
             launch_pack_force_f32(c->stream, c->n, c->d_offm, c->d_velf, c->d_cellc, c->d_posm, c->d_hsml, c->d_rho,
                                   c->d_pres, c->d_dhsml, c->d_cn, c->d_fce, c->d_nvt, c->d_grad, c->p.gamma,
                                   c->d_frecf);

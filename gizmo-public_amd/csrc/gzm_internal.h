@@ -200,6 +200,12 @@ struct GradientArgs {
     const GradRecF *recf;  // FP32 kernels
     F32Geom f32;
     Counters *ctr;
+    // all particles active: the kernels also write each particle's force record
+    // (what pack_force_kernel / pack_force_f32_kernel would write), else nullptr
+    ForceRec *frec_out;
+    ForceRecF *frecf_out;
+    const double *dhsml, *fce;
+    double gamma;
 };
 
 // Pairs whose Riemann problem reaches the exact solver are taken out of the

@@ -370,6 +370,26 @@ __global__ void pack_grad_kernel(int n, const double4 *posm, const double4 *velu
     rec[k] = r;
 }
 
+// One particle's FP64 force record (shared by pack_force_kernel and the
+// gradient kernel's epilogue, so that both write the same bits)
+__device__ __forceinline__ ForceRec make_force_rec(const double4 &p, const double4 &v, double d, double P, double h,
+                                                   double dh, double cn, double fce, const double *t, const double *g,
+                                                   double gamma)
+{
+    ForceRec r;
+    r.xh = make_double4(p.x, p.y, p.z, h);
+    r.vc = make_double4(v.x, v.y, v.z, sqrt(gamma * P / d));  // eos.c:245-254
+    r.th = make_double4(d, P, p.w / d, dh);
+    r.mc = make_double4(p.w, cn, fce, P / d);
+    r.t0 = make_double4(t[0], t[1], t[2], t[3]);
+    r.t1 = make_double4(t[4], t[5], 1.0 / h, 0.0);
+    r.g0 = make_double4(g[0], g[1], g[2], g[3]);
+    r.g1 = make_double4(g[4], g[5], g[6], g[7]);
+    r.g2 = make_double4(g[8], g[9], g[10], g[11]);
+    r.g3 = make_double4(g[12], g[13], g[14], 0.0);
+    return r;
+}
+
 __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
 {
     // Pair-list entries are staged in LDS, column-major (entry c of thread x
@@ -470,6 +490,13 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
         o[6] = gv0[0]; o[7] = gv0[1]; o[8] = gv0[2];
         o[9] = gv1[0]; o[10] = gv1[1]; o[11] = gv1[2];
         o[12] = gv2[0]; o[13] = gv2[1]; o[14] = gv2[2];
+        if (a.frec_out) {  // every particle active: its force record, as pack_force_kernel writes it
+            const double g[15] = {gr[0], gr[1], gr[2], gp[0], gp[1], gp[2], gv0[0], gv0[1], gv0[2],
+                                  gv1[0], gv1[1], gv1[2], gv2[0], gv2[1], gv2[2]};
+            a.frec_out[i] = make_force_rec(make_double4(ri.xh.x, ri.xh.y, ri.xh.z, m_i),
+                                           make_double4(ri.vr.x, ri.vr.y, ri.vr.z, 0.0), rho_i, p_i, h_i,
+                                           a.dhsml[i], cn, a.fce[i], nvt, g, a.gamma);
+        }
         if (a.list.idx) a.list.count[t] = listed;
     });
     if (a.list.idx) {
@@ -494,22 +521,8 @@ __global__ void pack_force_kernel(int n, const double4 *posm, const double4 *vel
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    double4 p = posm[k], v = velu[k];
-    double d = rho[k], P = pres[k], h = hsml[k];
-    const double *t = nvt + 6 * (size_t)k;
-    const double *g = grad + 15 * (size_t)k;
-    ForceRec r;
-    r.xh = make_double4(p.x, p.y, p.z, h);
-    r.vc = make_double4(v.x, v.y, v.z, sqrt(gamma * P / d));  // eos.c:245-254
-    r.th = make_double4(d, P, p.w / d, dhsml[k]);
-    r.mc = make_double4(p.w, cn[k], fce[k], P / d);
-    r.t0 = make_double4(t[0], t[1], t[2], t[3]);
-    r.t1 = make_double4(t[4], t[5], 1.0 / h, 0.0);
-    r.g0 = make_double4(g[0], g[1], g[2], g[3]);
-    r.g1 = make_double4(g[4], g[5], g[6], g[7]);
-    r.g2 = make_double4(g[8], g[9], g[10], g[11]);
-    r.g3 = make_double4(g[12], g[13], g[14], 0.0);
-    rec[k] = r;
+    rec[k] = make_force_rec(posm[k], velu[k], rho[k], pres[k], hsml[k], dhsml[k], cn[k], fce[k], nvt + 6 * (size_t)k,
+                            grad + 15 * (size_t)k, gamma);
 }
 
 // ForceRec -> the pair-side quantities (moves only)

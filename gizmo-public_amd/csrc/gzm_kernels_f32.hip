@@ -291,6 +291,27 @@ __global__ void pack_grad_f32_kernel(int n, const float4 *offm, const float4 *ve
     rec[k] = r;
 }
 
+// One particle's FP32 force record (shared by pack_force_f32_kernel and the
+// gradient kernel's epilogue, so that both write the same bits)
+__device__ __forceinline__ ForceRecF make_force_rec_f(const float4 &o, const float4 &v, uint32_t cellc, double m,
+                                                      double d, double P, double h, double dh, double cn, double fce,
+                                                      const double *t, const double *g, double gamma)
+{
+    ForceRecF r;
+    r.xh = make_float4(o.x, o.y, o.z, (float)h);
+    r.vc = make_float4(v.x, v.y, v.z, (float)sqrt(gamma * P / d));  // eos.c:245-254
+    r.th = make_float4((float)d, (float)P, (float)(m / d), (float)dh);
+    r.t0 = make_float4((float)t[0], (float)t[1], (float)t[2], (float)t[3]);
+    r.t1 = make_float4((float)t[4], (float)t[5], (float)(1.0 / h), (float)fce);
+    r.g0 = make_float4((float)g[0], (float)g[1], (float)g[2], (float)g[3]);
+    r.g1 = make_float4((float)g[4], (float)g[5], (float)g[6], (float)g[7]);
+    r.g2 = make_float4((float)g[8], (float)g[9], (float)g[10], (float)g[11]);
+    r.g3 = make_float4((float)g[12], (float)g[13], (float)g[14], __uint_as_float(cellc));
+    r.cn = cn;
+    r.e = P / d;
+    return r;
+}
+
 template <bool TREE>
 __global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a)
 {
@@ -437,6 +458,12 @@ __global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a
         o[6] = gv0[0]; o[7] = gv0[1]; o[8] = gv0[2];
         o[9] = gv1[0]; o[10] = gv1[1]; o[11] = gv1[2];
         o[12] = gv2[0]; o[13] = gv2[1]; o[14] = gv2[2];
+        if (a.frecf_out) {  // every particle active: its force record, as pack_force_f32_kernel writes it
+            const double g[15] = {gr[0], gr[1], gr[2], gp[0], gp[1], gp[2], gv0[0], gv0[1], gv0[2],
+                                  gv1[0], gv1[1], gv1[2], gv2[0], gv2[1], gv2[2]};
+            a.frecf_out[i] = make_force_rec_f(make_float4(ri.xh.x, ri.xh.y, ri.xh.z, 0.0f), ri.vr, a.f32.cellc[i], m_i,
+                                              rho_i, p_i, h_i, a.dhsml[i], cn, a.fce[i], nvt, g, a.gamma);
+        }
         if (a.list.idx) a.list.count[t] = listed;
     });
     if (a.list.idx) {
@@ -459,23 +486,8 @@ __global__ void pack_force_f32_kernel(int n, const float4 *offm, const float4 *v
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const float4 o = offm[k], v = velf[k];
-    const double d = rho[k], P = pres[k], h = hsml[k], m = posm[k].w;
-    const double *t = nvt + 6 * (size_t)k;
-    const double *g = grad + 15 * (size_t)k;
-    ForceRecF r;
-    r.xh = make_float4(o.x, o.y, o.z, (float)h);
-    r.vc = make_float4(v.x, v.y, v.z, (float)sqrt(gamma * P / d));  // eos.c:245-254
-    r.th = make_float4((float)d, (float)P, (float)(m / d), (float)dhsml[k]);
-    r.t0 = make_float4((float)t[0], (float)t[1], (float)t[2], (float)t[3]);
-    r.t1 = make_float4((float)t[4], (float)t[5], (float)(1.0 / h), (float)fce[k]);
-    r.g0 = make_float4((float)g[0], (float)g[1], (float)g[2], (float)g[3]);
-    r.g1 = make_float4((float)g[4], (float)g[5], (float)g[6], (float)g[7]);
-    r.g2 = make_float4((float)g[8], (float)g[9], (float)g[10], (float)g[11]);
-    r.g3 = make_float4((float)g[12], (float)g[13], (float)g[14], __uint_as_float(cellc[k]));
-    r.cn = cn[k];
-    r.e = P / d;
-    rec[k] = r;
+    rec[k] = make_force_rec_f(offm[k], velf[k], cellc[k], posm[k].w, rho[k], pres[k], hsml[k], dhsml[k], cn[k], fce[k],
+                              nvt + 6 * (size_t)k, grad + 15 * (size_t)k, gamma);
 }
 
 __device__ __forceinline__ void unpack_side_f(const ForceRecF &r, PairSide<float> &S)
