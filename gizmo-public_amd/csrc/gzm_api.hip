@@ -1122,7 +1122,6 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     a.n = c->n;
     a.f32 = f32_geom(c);
     a.ctr = c->d_ctr + kCtrForce;
-    a.fallback_only = 0;
     const bool use_list = c->list_valid && c->list_version == c->active_version;
     if (use_list) {
         a.list.idx = c->d_list;

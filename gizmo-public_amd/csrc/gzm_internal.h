@@ -239,7 +239,6 @@ struct ForceArgs {
     double *vsig;
     uint8_t *wakeup;
     PairList list;         // list.idx == NULL: scan the grid for every particle
-    int fallback_only;     // scan only the slices flagged in list.overflow
     DeferArgs defer;       // list kernel: defer.mask != NULL -> exact-solver pairs deferred
     const double4 *posm;   // FP64 positions and masses (FP32 kernels)
     const double4 *velu;   // FP64 velocities (FP32 kernels: final operations)

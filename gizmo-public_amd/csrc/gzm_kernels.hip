@@ -542,9 +542,9 @@ __device__ __forceinline__ void unpack_side(const ForceRec &r, PairSide<double> 
 }
 
 // One particle's force loop.  USE_LIST: walk the pair list written by the
-// gradient kernel (particles that overflowed it are skipped and left to the
-// scanning variant); otherwise scan the grid or tree (every particle, or only
-// the overflowed ones when a.fallback_only).  Pairs that reach the exact Riemann
+// gradient kernel (particles that overflowed it are skipped and left to
+// force_wave_fp64_kernel); otherwise scan the grid or tree (no list: every
+// particle).  Pairs that reach the exact Riemann
 // solver are summed after all other pairs of the particle: THREE_PHASE (list
 // kernel with defer buffers) leaves them to force_deferred_kernel and
 // force_finish_kernel, otherwise the loop walks the pairs a second time and
@@ -558,13 +558,11 @@ __global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
     if (USE_LIST) {
         if (valid && a.list.count[t] > a.list.cap) {
             valid = false;
-            if (THREE_PHASE) {  // finished by the scanning kernel
+            if (THREE_PHASE) {  // evaluated by the wavefront loop for long pair sets
                 a.defer.count[t] = 0;
                 a.defer.raw[t] = 0;
             }
         }
-    } else if (a.fallback_only) {
-        if (valid && !(a.list.count[t] > a.list.cap)) valid = false;
     }
     unsigned long long cnt = 0, nretry = 0, nexact = 0;
     for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
@@ -883,7 +881,7 @@ void launch_force(hipStream_t s, const ForceArgs &a)
 {
     if (a.n_active <= 0) return;
     const dim3 grid(nblocks(a.n_active)), block(kBlock);
-    if (a.list.idx && !a.fallback_only) {
+    if (a.list.idx) {
         if (a.defer.mask) hipLaunchKernelGGL((force_kernel<true, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((force_kernel<true, false>), grid, block, 0, s, a);
     } else {

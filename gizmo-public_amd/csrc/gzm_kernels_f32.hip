@@ -578,8 +578,8 @@ __device__ __forceinline__ void side_i_f(const ForceRecF &ri, PairI<float> &I)
 }
 
 // USE_LIST: walk the pair list of the FP32 gradient kernel (particles that
-// overflowed it are left to the scanning variant); otherwise scan the grid or
-// tree with the same banded predicate as the gradient kernel.  Exact-solver pairs are summed
+// overflowed it are left to force_wave_fp64_kernel); otherwise (no list) scan
+// the grid or tree with the same banded predicate as the gradient kernel.  Exact-solver pairs are summed
 // after the others, by force_deferred_f32_kernel + force_finish_kernel
 // (THREE_PHASE) or by a second walk over the pairs, as in the FP64 kernel.
 template <bool USE_LIST, bool THREE_PHASE, bool TREE = false>
@@ -590,13 +590,11 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
     if (USE_LIST) {
         if (valid && a.list.count[t] > a.list.cap) {
             valid = false;
-            if (THREE_PHASE) {  // finished by the scanning kernel
+            if (THREE_PHASE) {  // evaluated by the wavefront loop for long pair sets
                 a.defer.count[t] = 0;
                 a.defer.raw[t] = 0;
             }
         }
-    } else if (a.fallback_only) {
-        if (valid && !(a.list.count[t] > a.list.cap)) valid = false;
     }
     unsigned long long cnt = 0, nretry = 0, nexact = 0;
     for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
@@ -1052,7 +1050,7 @@ void launch_force_f32(hipStream_t s, const ForceArgs &a)
 {
     if (a.n_active <= 0) return;
     const dim3 grid(nblocks(a.n_active)), block(kBlock);
-    if (a.list.idx && !a.fallback_only) {
+    if (a.list.idx) {
         if (a.defer.mask) hipLaunchKernelGGL((force_f32_kernel<true, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((force_f32_kernel<true, false>), grid, block, 0, s, a);
     } else if (a.tree.nodes) {
