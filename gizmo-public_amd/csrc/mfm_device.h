@@ -399,16 +399,33 @@ __device__ __forceinline__ void apply_nvt(const double nvt[6], double g[3])
     g[2] = nvt[2] * v0 + nvt[4] * v1 + nvt[5] * v2;
 }
 
+// FP32 only: a branch of the face construction, the limiter, the wavespeed
+// estimate or the Riemann retry whose FP32 operand lies within this relative
+// margin of its threshold may go the other way than in the FP64 pass (the
+// FP32 operands carry rounding of ~1e-7 to 1e-6 relative); the pair is then
+// evaluated in FP64 instead (kNearThreshold).
+constexpr float kDecideMargin = 1e-5f;
+
+template <typename T>
+__device__ __forceinline__ bool near_threshold(T x, T threshold, T scale)
+{
+    return sizeof(T) == sizeof(float) && !(fabs(x - threshold) > T(kDecideMargin) * fabs(scale));
+}
+
 // ---------------------------------------------------------------------------
 // face reconstruction and limiter (reimann.h:129-248), fac_minmax 0.5,
 // fac_meddev 0.375.  R side is particle i, L side is particle j.
+// The sign-preserving bounds jump where Qmax_eff (Qmax < 0) or Qmin_eff
+// (Qmin > 0) crosses zero: from 0 to Qmax/2 (Qmin/2).  In FP32 the function
+// returns true when that crossing lies within kDecideMargin of (Qmax - Qmin),
+// where the rounded operands may take the other side than in FP64.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void reconstruct(T Qi, const T gi[3], T Qj, const T gj[3], const T di[3], const T dj[3],
+__device__ __forceinline__ bool reconstruct(T Qi, const T gi[3], T Qj, const T gj[3], const T di[3], const T dj[3],
                                             T &QL, T &QR, int mode)
 {
-    if (mode == 0) { QR = Qi; QL = Qj; return; }
-    if (Qi == Qj) { QL = QR = Qi; return; }
+    if (mode == 0) { QR = Qi; QL = Qj; return false; }
+    if (Qi == Qj) { QL = QR = Qi; return false; }
     QR = Qi + gi[0] * di[0] + gi[1] * di[1] + gi[2] * di[2];
     QL = Qj + gj[0] * dj[0] + gj[1] * dj[1] + gj[2] * dj[2];
     T Qmax, Qmin;
@@ -417,8 +434,15 @@ __device__ __forceinline__ void reconstruct(T Qi, const T gi[3], T Qj, const T g
     T fac = T(0.5) * (Qmax - Qmin);
     T Qmax_eff = Qmax + fac;
     T Qmin_eff = Qmin - fac;
-    if (Qmax < 0) { if (Qmax_eff > 0) Qmax_eff = Real<T>::div(Qmax * Qmax, (Qmax - (Qmax_eff - Qmax))); }
-    if (Qmin > 0) { if (Qmin_eff < 0) Qmin_eff = Real<T>::div(Qmin * Qmin, (Qmin + (Qmin - Qmin_eff))); }
+    bool near = false;
+    if (Qmax < 0) {
+        near = near_threshold(Qmax_eff, T(0), Qmax - Qmin);
+        if (Qmax_eff > 0) Qmax_eff = Real<T>::div(Qmax * Qmax, (Qmax - (Qmax_eff - Qmax)));
+    }
+    if (Qmin > 0) {
+        near = near || near_threshold(Qmin_eff, T(0), Qmax - Qmin);
+        if (Qmin_eff < 0) Qmin_eff = Real<T>::div(Qmin * Qmin, (Qmin + (Qmin - Qmin_eff)));
+    }
     fac = T(0.375) * (Qmax - Qmin);
     T Qmed_max = Qmed + fac;
     T Qmed_min = Qmed - fac;
@@ -435,6 +459,7 @@ __device__ __forceinline__ void reconstruct(T Qi, const T gi[3], T Qj, const T g
         if (QL < Qmin_eff) QL = Qmin_eff;
         if (QL > Qmed_max) QL = Qmed_max;
     }
+    return near;
 }
 
 // ---------------------------------------------------------------------------
@@ -449,19 +474,6 @@ template <typename T>
 struct ROut {
     T P_M, S_M, Fv[3], Fp;
 };
-
-// FP32 only: a branch of the wavespeed estimate or of the Riemann retry whose
-// FP32 operand lies within this relative margin of its threshold may go the
-// other way than in the FP64 pass (the reconstructed FP32 states carry
-// rounding of ~1e-7 to 1e-6 relative); the pair is then evaluated in FP64
-// instead (kNearThreshold).
-constexpr float kDecideMargin = 1e-5f;
-
-template <typename T>
-__device__ __forceinline__ bool near_threshold(T x, T threshold, T scale)
-{
-    return sizeof(T) == sizeof(float) && !(fabs(x - threshold) > T(kDecideMargin) * fabs(scale));
-}
 
 // get_wavespeeds_and_pressure_star: Gaburov HLLC estimate, Roe, then PVRS
 // (reimann.h:527-593).  Sets P_M and S_M.  Returns true (FP32 only, with
@@ -764,7 +776,9 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
     if (sizeof(T) == sizeof(float) && !(wk_i + wk_j > 0)) return 0;
     // effective face (compute_finitevol_faces.h:14-106)
     T wt_i = I.V, wt_j = J.V;
-    if (M::div(M::div(fabs(I.V - J.V), tmin(I.V, J.V)), T(3)) > T(1.25)) {
+    const T vratio = M::div(M::div(fabs(I.V - J.V), tmin(I.V, J.V)), T(3));
+    if (near_threshold(vratio, T(1.25), T(1.25))) return DEFER ? kDeferred : kNearThreshold;
+    if (vratio > T(1.25)) {
         wt_i = wt_j = M::div(I.V * J.V * (wk_i + wk_j), (I.V * wk_i + J.V * wk_j));
     }
     T A[3];
@@ -783,7 +797,10 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
             fdp += A[k] * dp[k];
         }
     }
-    if ((J.cn * J.cn > 1.0e12 + I.cnumcrit2) || (fdp < 0)) {
+    const bool cn_switch = J.cn * J.cn > 1.0e12 + I.cnumcrit2;
+    if (!cn_switch && near_threshold(fdp, T(0), (fabs(A[0]) + fabs(A[1]) + fabs(A[2])) * r))
+        return DEFER ? kDeferred : kNearThreshold;
+    if (cn_switch || (fdp < 0)) {
         T An = M::div(-(wt_i * I.V * dwk_i + wt_j * J.V * dwk_j), r);
         A[0] = An * dp[0];
         A[1] = An * dp[1];
@@ -816,14 +833,15 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
     if (vdotf2 < 0) v2a = tmax(v2a, vdotf2 * vdotf2);
     const int recon = (leak > 1) ? 0 : 1;
     RState<T> SL, SR;
-    reconstruct(I.rho, I.gr, J.rho, J.gr, di, dj, SL.rho, SR.rho, recon);
-    reconstruct(I.P, I.gp, J.P, J.gp, di, dj, SL.p, SR.p, recon);
+    bool near = reconstruct(I.rho, I.gr, J.rho, J.gr, di, dj, SL.rho, SR.rho, recon);
+    near |= reconstruct(I.P, I.gp, J.P, J.gp, di, dj, SL.p, SR.p, recon);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        reconstruct(I.v[k], I.gv + 3 * k, J.v[k], J.gv + 3 * k, di, dj, SL.v[k], SR.v[k], recon);
+        near |= reconstruct(I.v[k], I.gv + 3 * k, J.v[k], J.gv + 3 * k, di, dj, SL.v[k], SR.v[k], recon);
         SL.v[k] -= vf[k];
         SR.v[k] -= vf[k];
     }
+    if (sizeof(T) == sizeof(float) && near) return DEFER ? kDeferred : kNearThreshold;
     const T pti = I.P + I.rho * v2a;
     const T ptj = J.P + J.rho * v2a;
     T lim = T(1.1) * T(1.0) * tmax(pti, ptj);
