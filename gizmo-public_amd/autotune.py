@@ -19,6 +19,11 @@ import gzm
 
 CELL_FACTORS = (1.0, 0.75, 0.6, 0.5)
 LIST_CAPS = (64, 48)
+# A later candidate replaces the best so far only when it is faster by more
+# than this fraction: timings of one setting on the sample vary by ~1 % from
+# run to run, and a choice made on noise would differ between otherwise
+# identical runs.
+MIN_GAIN = 0.02
 
 
 def sample_boxes(ic: dict, n_sample: int = 32) -> dict:
@@ -62,20 +67,23 @@ def time_setting(ic: dict, precision: int, cell_factor: float, list_cap: int, pa
         return (time.perf_counter() - t0) / passes
 
 
-def tune(ic: dict, precision: int, cell_factors=CELL_FACTORS, list_caps=LIST_CAPS, n_sample: int = 32) -> dict:
+def tune(ic: dict, precision: int, cell_factors=CELL_FACTORS, list_caps=LIST_CAPS, n_sample: int = 32,
+         min_gain: float = MIN_GAIN) -> dict:
     """Fastest (cell factor, list capacity) on a sample of `ic`, and the table
-    of all timings: {"cell_factor": f, "pair_list_cap": c, "table": {...}}."""
+    of all timings: {"cell_factor": f, "pair_list_cap": c, "table": {...}}.
+    Candidates are tried in the order given; one replaces the best so far
+    only when it is faster by more than `min_gain`."""
     sample = sample_boxes(ic, n_sample) if "box_offsets" in ic else ic
     table = {}
     best_f = None
     for f in cell_factors:
         table[(f, list_caps[0])] = time_setting(sample, precision, f, list_caps[0])
-        if best_f is None or table[(f, list_caps[0])] < table[(best_f, list_caps[0])]:
+        if best_f is None or table[(f, list_caps[0])] < (1 - min_gain) * table[(best_f, list_caps[0])]:
             best_f = f
     best = (best_f, list_caps[0])
     for c in list_caps[1:]:
         table[(best_f, c)] = time_setting(sample, precision, best_f, c)
-        if table[(best_f, c)] < table[best]:
+        if table[(best_f, c)] < (1 - min_gain) * table[best]:
             best = (best_f, c)
     return {"cell_factor": best[0], "pair_list_cap": best[1],
             "table": {f"cell {k[0]} cap {k[1]}": round(v * 1e3, 3) for k, v in table.items()},
