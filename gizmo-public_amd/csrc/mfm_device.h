@@ -434,15 +434,11 @@ __device__ __forceinline__ bool reconstruct(T Qi, const T gi[3], T Qj, const T g
     T fac = T(0.5) * (Qmax - Qmin);
     T Qmax_eff = Qmax + fac;
     T Qmin_eff = Qmin - fac;
-    bool near = false;
-    if (Qmax < 0) {
-        near = near_threshold(Qmax_eff, T(0), Qmax - Qmin);
-        if (Qmax_eff > 0) Qmax_eff = Real<T>::div(Qmax * Qmax, (Qmax - (Qmax_eff - Qmax)));
-    }
-    if (Qmin > 0) {
-        near = near || near_threshold(Qmin_eff, T(0), Qmax - Qmin);
-        if (Qmin_eff < 0) Qmin_eff = Real<T>::div(Qmin * Qmin, (Qmin + (Qmin - Qmin_eff)));
-    }
+    // one test covers both bounds: when Qmin <= 0 <= Qmax, Qmin_eff is at
+    // least (Qmax - Qmin) / 2 below zero and never near
+    const bool near = near_threshold((Qmax < 0) ? Qmax_eff : Qmin_eff, T(0), Qmax - Qmin);
+    if (Qmax < 0) { if (Qmax_eff > 0) Qmax_eff = Real<T>::div(Qmax * Qmax, (Qmax - (Qmax_eff - Qmax))); }
+    if (Qmin > 0) { if (Qmin_eff < 0) Qmin_eff = Real<T>::div(Qmin * Qmin, (Qmin + (Qmin - Qmin_eff))); }
     fac = T(0.375) * (Qmax - Qmin);
     T Qmed_max = Qmed + fac;
     T Qmed_min = Qmed - fac;
