@@ -124,8 +124,8 @@ struct gzm_ctx {
     std::vector<int> last_sweep_live;  // particles of each density sweep of the last eager call
     // neighbour-search tree (search = 1)
     int2 *d_t_child = nullptr, *d_t_range = nullptr;
-    int *d_t_parent = nullptr, *d_t_leaf_parent = nullptr, *d_t_split = nullptr, *d_t_esc = nullptr,
-        *d_t_leaf_esc = nullptr, *d_t_flag = nullptr;
+    int *d_t_large = nullptr, *d_t_large_count = nullptr, *d_t_split = nullptr,
+        *d_t_esc = nullptr, *d_t_leaf_esc = nullptr, *d_t_flag = nullptr;
     double4 *d_t_lo = nullptr, *d_t_hi = nullptr;
     float *d_t_wmax = nullptr;
     TreeNode *d_t_nodes = nullptr;
@@ -252,7 +252,8 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_box_of);
     dev_free(c->d_defer_mask); dev_free(c->d_defer_count); dev_free(c->d_defer_off); dev_free(c->d_defer_total);
     dev_free(c->d_defer_raw); dev_free(c->d_defer_result);
-    dev_free(c->d_t_child); dev_free(c->d_t_range); dev_free(c->d_t_parent); dev_free(c->d_t_leaf_parent);
+    dev_free(c->d_t_child); dev_free(c->d_t_range); dev_free(c->d_t_large);
+    dev_free(c->d_t_large_count);
     dev_free(c->d_t_split); dev_free(c->d_t_esc); dev_free(c->d_t_leaf_esc); dev_free(c->d_t_flag);
     dev_free(c->d_t_lo); dev_free(c->d_t_hi); dev_free(c->d_t_wmax); dev_free(c->d_t_nodes);
     c->tree = TreeArgs{};
@@ -676,8 +677,8 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
         if (c->p.search == 1) {
             s = s ? s : dev_alloc(&c->d_t_child, n);
             s = s ? s : dev_alloc(&c->d_t_range, n);
-            s = s ? s : dev_alloc(&c->d_t_parent, n);
-            s = s ? s : dev_alloc(&c->d_t_leaf_parent, n);
+            s = s ? s : dev_alloc(&c->d_t_large, n);
+            s = s ? s : dev_alloc(&c->d_t_large_count, 1);
             s = s ? s : dev_alloc(&c->d_t_split, n);
             s = s ? s : dev_alloc(&c->d_t_esc, n);
             s = s ? s : dev_alloc(&c->d_t_leaf_esc, n);
@@ -756,8 +757,8 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
         t.boxes = c->d_boxes;
         t.keys = c->d_keys_sorted;
         t.posm = c->d_posm;
-        t.child = c->d_t_child; t.range = c->d_t_range; t.parent = c->d_t_parent;
-        t.leaf_parent = c->d_t_leaf_parent; t.split_right = c->d_t_split;
+        t.child = c->d_t_child; t.range = c->d_t_range;
+        t.large = c->d_t_large; t.large_count = c->d_t_large_count; t.split_right = c->d_t_split;
         t.esc = c->d_t_esc; t.leaf_esc = c->d_t_leaf_esc; t.lo = c->d_t_lo; t.hi = c->d_t_hi; t.flag = c->d_t_flag;
         t.nodes = c->d_t_nodes;
         int largest = 1;

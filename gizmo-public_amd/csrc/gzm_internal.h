@@ -328,12 +328,12 @@ struct TreeBuild {
     const double4 *posm;
     int2 *child;               // internal nodes (n entries: at most n - 1 used)
     int2 *range;
-    int *parent;               // internal nodes: parent node, -1 at a root
-    int *leaf_parent;          // sorted slots: parent node, -1 for a box of one particle
     int *split_right;          // sorted slot s: right child of the node that splits after s
     int *esc, *leaf_esc;
     double4 *lo, *hi;          // FP64 node boxes (build only)
-    int *flag;                 // internal nodes: bottom-up arrival counters (left at 0)
+    int *flag;                 // internal nodes: launch of the bounds pass that completed the node
+    int *large;                // internal nodes of more than 32 slots, in no particular order
+    int *large_count;          // length of `large` (one int)
     TreeNode *nodes;           // the walk's records
     // node hmax refresh: max of hsml over a node's slot range from a table of
     // window maxima, level k holding max(hsml[s .. s + 2^k - 1]) rounded up to

@@ -17,12 +17,16 @@
 //   3. links:  the stackless walk's escape link of every node: the node that
 //              follows a subtree in depth-first order is the right child of
 //              the node whose split lies at the subtree's last slot;
-//   4. bounds: bottom-up union of the children's boxes and hsml maxima, one
-//              thread per leaf; the second thread to reach a node (atomic
-//              arrival counter, agent-scope fences) continues upwards.
+//   4. bounds: the boxes and hsml maxima of nodes of at most 32 slots from
+//              their slot ranges, then those of the larger nodes bottom-up,
+//              one launch per level: a node combines the children that
+//              earlier launches completed.  No fences: on a chip whose XCDs
+//              each have their own L2, a device-scope fence writes back and
+//              invalidates L2, and a chain of them per node cost 63 ms for
+//              8.4 M particles.
 //
 // The node hmax changes with every density stage; it is refreshed without
-// the bottom-up chain (whose fences cost a few ms per million particles):
+// the bottom-up pass:
 // every node covers a contiguous slot range, whose maximum hsml is the larger
 // of two overlapping power-of-two windows of a window-maximum table built in
 // log2(n) data-parallel levels.  No pass needs a host synchronisation, so
@@ -85,10 +89,7 @@ __global__ void tree_build_kernel(TreeBuild t)
     if (s >= t.n) return;
     const BoxDesc &B = t.boxes[t.box_of[s]];
     const int first = B.first, m = B.count, i = s - first;
-    if (m == 1) {
-        t.leaf_parent[s] = -1;
-        return;
-    }
+    if (m == 1) return;
     if (i >= m - 1) return;  // m - 1 internal nodes per box
     const int g0 = B.root;
     const uint64_t *k = t.keys;
@@ -115,9 +116,6 @@ __global__ void tree_build_kernel(TreeBuild t)
     const int right = (hi == gamma + 1) ? leaf_id(first + gamma + 1) : g0 + gamma + 1;
     t.child[g] = make_int2(left, right);
     t.range[g] = make_int2(first + lo, first + hi);
-    if (left < 0) t.leaf_parent[-left - 1] = g; else t.parent[left] = g;
-    if (right < 0) t.leaf_parent[-right - 1] = g; else t.parent[right] = g;
-    if (i == 0) t.parent[g] = -1;
     t.split_right[first + gamma] = right;
 }
 
@@ -137,39 +135,92 @@ __global__ void tree_links_kernel(TreeBuild t)
     }
 }
 
-// Bottom-up: node boxes and the max hsml below each node.  The second thread
-// to arrive at a node has both children final; it resets the node's counter
-// for the next build and carries on to the parent.
-__global__ void tree_bounds_kernel(TreeBuild t, const double *hsml)
+// Node boxes and the max hsml below each node, bottom-up without fences.
+// The flag of a node records the launch that completed it (0: not yet); a
+// node combines only children completed by an earlier launch, whose boxes the
+// launch boundary has made visible on every XCD.
+
+// nodes of at most kScanSlots slots take their box straight from their
+// contiguous slot range; the larger ones are combined from their children
+constexpr int kScanSlots = 32;
+// combine launches: more than the height of the tallest tree (a node's prefix
+// length grows on every level down and is below 64 key bits + 32 slot bits)
+constexpr int kCombineLaunches = 100;
+constexpr int kScanLaunch = 1;
+
+__global__ void tree_bounds_scan_kernel(TreeBuild t, const double *hsml)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= t.n) return;
-    int node = t.leaf_parent[s];
-    // depth <= 64 key bits + 32 slot bits
-    for (int depth = 0; node >= 0 && depth < 128; depth++) {
-        __threadfence();
-        if (atomicAdd(&t.flag[node], 1) == 0) return;
-        __threadfence();
-        t.flag[node] = 0;
-        const int2 c = t.child[node];
-        double4 lo[2], hi[2];
+    int g = -1;
+    bool large = false;
+    if (s < t.n) {
+        const BoxDesc &B = t.boxes[t.box_of[s]];
+        const int i = s - B.first;
+        if (i < B.count - 1) {
+            g = B.root + i;
+            const int2 r = t.range[g];
+            large = r.y - r.x + 1 > kScanSlots;
+            if (!large) {
+                double4 p = t.posm[r.x];
+                double4 lo = make_double4(p.x, p.y, p.z, hsml[r.x]);
+                double4 hi = make_double4(p.x, p.y, p.z, 0.0);
+                for (int q = r.x + 1; q <= r.y; q++) {
+                    p = t.posm[q];
+                    const double h = hsml[q];
+                    lo = make_double4(fmin(lo.x, p.x), fmin(lo.y, p.y), fmin(lo.z, p.z), h > lo.w ? h : lo.w);
+                    hi = make_double4(fmax(hi.x, p.x), fmax(hi.y, p.y), fmax(hi.z, p.z), 0.0);
+                }
+                t.lo[g] = lo;
+                t.hi[g] = hi;
+                t.flag[g] = kScanLaunch;
+            }
+        }
+    }
+    // append the large nodes to the list, one counter update per wavefront
+    const uint64_t mask = __ballot(large);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(t.large_count, __popcll(mask));
+    base = __shfl(base, leader);
+    if (large) t.large[base + __popcll(mask & ((1ull << lane) - 1))] = g;
+}
+
+__global__ void tree_bounds_combine_kernel(TreeBuild t, const double *hsml, int launch)
+{
+    const int count = *t.large_count;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        const int g = t.large[q];
+        if (__hip_atomic_load(&t.flag[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) continue;
+        const int2 c = t.child[g];
         const int ch[2] = {c.x, c.y};
+        bool ready = true;
 #pragma unroll
-        for (int q = 0; q < 2; q++) {
-            if (ch[q] < 0) {
-                const int j = -ch[q] - 1;
+        for (int k = 0; k < 2; k++) {
+            if (ch[k] >= 0) {
+                const int f = __hip_atomic_load(&t.flag[ch[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ready = ready && f != 0 && f != launch;
+            }
+        }
+        if (!ready) continue;
+        double4 lo[2], hi[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            if (ch[k] < 0) {
+                const int j = -ch[k] - 1;
                 const double4 p = t.posm[j];
-                lo[q] = make_double4(p.x, p.y, p.z, hsml[j]);
-                hi[q] = make_double4(p.x, p.y, p.z, 0.0);
+                lo[k] = make_double4(p.x, p.y, p.z, hsml[j]);
+                hi[k] = make_double4(p.x, p.y, p.z, 0.0);
             } else {
-                lo[q] = t.lo[ch[q]];
-                hi[q] = t.hi[ch[q]];
+                lo[k] = t.lo[ch[k]];
+                hi[k] = t.hi[ch[k]];
             }
         }
         const double hmax = lo[0].w > lo[1].w ? lo[0].w : lo[1].w;
-        t.lo[node] = make_double4(fmin(lo[0].x, lo[1].x), fmin(lo[0].y, lo[1].y), fmin(lo[0].z, lo[1].z), hmax);
-        t.hi[node] = make_double4(fmax(hi[0].x, hi[1].x), fmax(hi[0].y, hi[1].y), fmax(hi[0].z, hi[1].z), 0.0);
-        node = t.parent[node];
+        t.lo[g] = make_double4(fmin(lo[0].x, lo[1].x), fmin(lo[0].y, lo[1].y), fmin(lo[0].z, lo[1].z), hmax);
+        t.hi[g] = make_double4(fmax(hi[0].x, hi[1].x), fmax(hi[0].y, hi[1].y), fmax(hi[0].z, hi[1].z), 0.0);
+        __hip_atomic_store(&t.flag[g], launch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -243,7 +294,11 @@ void launch_tree_build(hipStream_t s, const TreeBuild &b, const double *hsml)
     if (b.n <= 0) return;
     hipLaunchKernelGGL(tree_build_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b);
     hipLaunchKernelGGL(tree_links_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b);
-    hipLaunchKernelGGL(tree_bounds_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b, hsml);
+    launch_fill_int(s, b.large_count, 1, 0);
+    hipLaunchKernelGGL(tree_bounds_scan_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b, hsml);
+    const int combine_blocks = std::min(blocks(b.n), 1024);
+    for (int launch = kScanLaunch + 1; launch <= kScanLaunch + kCombineLaunches; launch++)
+        hipLaunchKernelGGL(tree_bounds_combine_kernel, dim3(combine_blocks), dim3(kBlock), 0, s, b, hsml, launch);
     hipLaunchKernelGGL(pack_nodes_kernel, dim3(blocks(b.n)), dim3(kBlock), 0, s, b);
 }
 
