@@ -48,6 +48,37 @@ def test_tree_neighbour_counts_equal_grid(name):
         np.testing.assert_array_equal(ct, cg)
 
 
+def test_tree_deep_equal_key_cluster_counts_equal_grid():
+    """A tree 73 levels deep: 600 of 1728 particles moved into a cube of side
+    2^-23 at (0.5, 0.5, 0.5), below the 2^-21 resolution of the Morton keys,
+    so that they share one key and are split by slot; and 63 particles at
+    0.5 + 0.75 * 2^-k along one axis at a time (k = 1..21), each branching off
+    at its own key bit above the cluster.  Every node on that chain holds the
+    cluster, so the bounds pass completes them one combine launch after
+    another; the tree's neighbour counts still equal the grid's exactly."""
+    ic = dict(ics.random_state(12))
+    rng = np.random.default_rng(11)
+    pos = ic["pos"].copy()
+    pos[:600] = 0.5 + rng.uniform(0.0, 2.0 ** -23, size=(600, 3))
+    q = 600
+    for k in range(1, 22):
+        for ax in range(3):
+            pos[q] = 0.5
+            pos[q, ax] += 0.75 * 2.0 ** -k
+            q += 1
+    ic["pos"] = pos
+    for mode in (0, 1):
+        with gzm.MFMContext(params_for(ic)) as g:
+            g.upload(ic)
+            cg, tg = g.neighbor_counts(mode)
+        with gzm.MFMContext(params_for(ic, search=1)) as t:
+            t.upload(ic)
+            ct, tt = t.neighbor_counts(mode)
+        assert tt == tg
+        np.testing.assert_array_equal(ct, cg)
+        assert cg[:600].min() >= 599
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_tree_pass_matches_oracle(name):
     ic = CASES[name]()
