@@ -762,8 +762,10 @@ __global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
 
 // Deferred pairs of the FP32 list kernel, one per lane: exact-solver pairs
 // (FP32 with the FP64 exact solver) and near-threshold pairs, whose particle
-// this lists for the FP64 redo (its result slot stays zero).
-__global__ __launch_bounds__(kBlock) void force_deferred_f32_kernel(ForceArgs a)
+// this lists for the FP64 redo (its result slot stays zero).  Bounded to 4
+// waves per SIMD: 153 VGPRs become 128 with 14 spilled, and the extra wave
+// hides more of the divergent solver's latency (2 % faster, same bits).
+__global__ __launch_bounds__(kBlock, 4) void force_deferred_f32_kernel(ForceArgs a)
 {
     const int q = a.defer.q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int q_end = min(*a.defer.total, a.defer.q_end);
