@@ -832,52 +832,85 @@ void orc_reconstruct(double qi, const double gi[3], double qj, const double gj[3
     reconstruct_face_states(qi, gi, qj, gj, di, dj, ql, qr, mode);
 }
 
-static void get_wavespeeds_and_pressure_star(const rstate *L, const rstate *R, rout *o, const gconst *c,
-                                             const double n_unit[3],
-                                             double v_line_L, double v_line_R, double cs_L, double cs_R,
-                                             double h_L, double h_R, double press_tot_limiter)
+/* The three star-state estimates of get_wavespeeds_and_pressure_star
+ * (reimann.h:527-593), each with the MIN_REAL_NUMBER floor the reference
+ * applies after it.  S_L/S_R are the outer wave speeds the estimate used. */
+
+/* Gaburov's HLLC estimate with the Davis wave speeds (reimann.h:541-547) */
+static void pstar_hllc(const rstate *L, const rstate *R, rout *o, double v_line_L, double v_line_R,
+                       double cs_L, double cs_R, double *S_L_out, double *S_R_out)
 {
-    double S_L, S_R;
+    double PT_L = L->p, PT_R = R->p;
+    double S_L = DMIN(v_line_L, v_line_R) - DMAX(cs_L, cs_R);
+    double S_R = DMAX(v_line_L, v_line_R) + DMAX(cs_L, cs_R);
+    double rho_wt_L = L->rho * (S_L - v_line_L);
+    double rho_wt_R = R->rho * (S_R - v_line_R);
+    o->S_M = ((PT_R - PT_L) + rho_wt_L * v_line_L - rho_wt_R * v_line_R) / (rho_wt_L - rho_wt_R);
+    o->P_M = (PT_L * rho_wt_R - PT_R * rho_wt_L + rho_wt_L * rho_wt_R * (v_line_R - v_line_L)) / (rho_wt_R - rho_wt_L);
+    if (o->P_M <= MIN_REAL_NUMBER) { o->P_M = MIN_REAL_NUMBER; o->S_M = S_L = S_R = 0; }
+    *S_L_out = S_L; *S_R_out = S_R;
+}
+
+/* Roe-averaged wave speeds (reimann.h:551-576) */
+static void pstar_roe(const rstate *L, const rstate *R, rout *o, const gconst *c, const double n_unit[3],
+                      double v_line_L, double v_line_R, double cs_L, double cs_R, double h_L, double h_R,
+                      double *S_L_out, double *S_R_out)
+{
+    double PT_L = L->p, PT_R = R->p;
+    double sqrt_rho_L = sqrt(L->rho);
+    double sqrt_rho_R = sqrt(R->rho);
+    double sqrt_rho_inv = 1 / (sqrt_rho_L + sqrt_rho_R);
+    double vx_roe = (sqrt_rho_L * L->v[0] + sqrt_rho_R * R->v[0]) * sqrt_rho_inv;
+    double vy_roe = (sqrt_rho_L * L->v[1] + sqrt_rho_R * R->v[1]) * sqrt_rho_inv;
+    double vz_roe = (sqrt_rho_L * L->v[2] + sqrt_rho_R * R->v[2]) * sqrt_rho_inv;
+    double v_line_roe = vx_roe * n_unit[0] + vy_roe * n_unit[1] + vz_roe * n_unit[2];
+    double h_roe = (sqrt_rho_L * h_L + sqrt_rho_R * h_R) * sqrt_rho_inv;
+    double cs_roe = sqrt(DMAX(1.e-30, c->G9 * (h_roe - 0.5 * (vx_roe * vx_roe + vy_roe * vy_roe + vz_roe * vz_roe))));
+    double S_R = DMAX(v_line_R + cs_R, v_line_roe + cs_roe);
+    double S_L = DMIN(v_line_L - cs_L, v_line_roe - cs_roe);
+    double rho_wt_R = R->rho * (S_R - v_line_R);
+    double rho_wt_L = -L->rho * (S_L - v_line_L);
+    o->S_M = ((rho_wt_R * v_line_R + rho_wt_L * v_line_L) + (PT_L - PT_R)) / (rho_wt_R + rho_wt_L);
+    o->P_M = L->rho * (v_line_L - S_L) * (v_line_L - o->S_M) + PT_L;
+    if (o->P_M <= MIN_REAL_NUMBER) { o->P_M = MIN_REAL_NUMBER; o->S_M = S_L = S_R = 0; }
+    *S_L_out = S_L; *S_R_out = S_R;
+}
+
+/* primitive-variable (PVRS) estimate, contact speed clamped to the fastest
+ * signal (reimann.h:582-587) */
+static void pstar_pvrs(const rstate *L, const rstate *R, rout *o, double v_line_L, double v_line_R,
+                       double cs_L, double cs_R, double *S_L_out, double *S_R_out)
+{
+    double PT_L = L->p, PT_R = R->p;
+    o->P_M = 0.5 * ((PT_L + PT_R) + (v_line_L - v_line_R) * 0.25 * (L->rho + R->rho) * (cs_L + cs_R));
+    o->S_M = 0.5 * (v_line_R + v_line_L) + 2.0 * (PT_L - PT_R) / ((L->rho + R->rho) * (cs_L + cs_R));
+    double S_plus = DMAX(DMAX(fabs(v_line_L - cs_L), fabs(v_line_R - cs_R)), DMAX(fabs(v_line_L + cs_L), fabs(v_line_R + cs_R)));
+    double S_L = -S_plus, S_R = S_plus;
+    if (o->S_M < S_L) o->S_M = S_L;
+    if (o->S_M > S_R) o->S_M = S_R;
+    if (o->P_M <= MIN_REAL_NUMBER) { o->P_M = MIN_REAL_NUMBER; o->S_M = S_L = S_R = 0; }
+    *S_L_out = S_L; *S_R_out = S_R;
+}
+
+/* get_wavespeeds_and_pressure_star (reimann.h:527-593): vacuum test, then the
+ * estimates in turn while P_M is <= 0, NaN or above the limiter.  Returns the
+ * estimate that stands: 0 vacuum, 1 HLLC, 2 Roe, 3 PVRS. */
+static int get_wavespeeds_and_pressure_star(const rstate *L, const rstate *R, rout *o, const gconst *c,
+                                            const double n_unit[3],
+                                            double v_line_L, double v_line_R, double cs_L, double cs_R,
+                                            double h_L, double h_R, double press_tot_limiter,
+                                            double *S_L, double *S_R)
+{
     if ((v_line_R - v_line_L) > DMAX(cs_L, cs_R)) {
-        o->P_M = MIN_REAL_NUMBER; o->S_M = S_L = S_R = 0;
-    } else {
-        double PT_L = L->p, PT_R = R->p;
-        S_L = DMIN(v_line_L, v_line_R) - DMAX(cs_L, cs_R);
-        S_R = DMAX(v_line_L, v_line_R) + DMAX(cs_L, cs_R);
-        double rho_wt_L = L->rho * (S_L - v_line_L);
-        double rho_wt_R = R->rho * (S_R - v_line_R);
-        o->S_M = ((PT_R - PT_L) + rho_wt_L * v_line_L - rho_wt_R * v_line_R) / (rho_wt_L - rho_wt_R);
-        o->P_M = (PT_L * rho_wt_R - PT_R * rho_wt_L + rho_wt_L * rho_wt_R * (v_line_R - v_line_L)) / (rho_wt_R - rho_wt_L);
-        if (o->P_M <= MIN_REAL_NUMBER) { o->P_M = MIN_REAL_NUMBER; o->S_M = S_L = S_R = 0; }
-        if ((o->P_M <= 0) || (isnan(o->P_M)) || (o->P_M > press_tot_limiter)) {
-            double sqrt_rho_L = sqrt(L->rho);
-            double sqrt_rho_R = sqrt(R->rho);
-            double sqrt_rho_inv = 1 / (sqrt_rho_L + sqrt_rho_R);
-            double vx_roe = (sqrt_rho_L * L->v[0] + sqrt_rho_R * R->v[0]) * sqrt_rho_inv;
-            double vy_roe = (sqrt_rho_L * L->v[1] + sqrt_rho_R * R->v[1]) * sqrt_rho_inv;
-            double vz_roe = (sqrt_rho_L * L->v[2] + sqrt_rho_R * R->v[2]) * sqrt_rho_inv;
-            double v_line_roe = vx_roe * n_unit[0] + vy_roe * n_unit[1] + vz_roe * n_unit[2];
-            double h_roe = (sqrt_rho_L * h_L + sqrt_rho_R * h_R) * sqrt_rho_inv;
-            double cs_roe = sqrt(DMAX(1.e-30, c->G9 * (h_roe - 0.5 * (vx_roe * vx_roe + vy_roe * vy_roe + vz_roe * vz_roe))));
-            S_R = DMAX(v_line_R + cs_R, v_line_roe + cs_roe);
-            S_L = DMIN(v_line_L - cs_L, v_line_roe - cs_roe);
-            rho_wt_R = R->rho * (S_R - v_line_R);
-            rho_wt_L = -L->rho * (S_L - v_line_L);
-            o->S_M = ((rho_wt_R * v_line_R + rho_wt_L * v_line_L) + (PT_L - PT_R)) / (rho_wt_R + rho_wt_L);
-            o->P_M = L->rho * (v_line_L - S_L) * (v_line_L - o->S_M) + PT_L;
-            if (o->P_M <= MIN_REAL_NUMBER) { o->P_M = MIN_REAL_NUMBER; o->S_M = S_L = S_R = 0; }
-            if ((o->P_M <= 0) || (isnan(o->P_M)) || (o->P_M > press_tot_limiter)) {
-                o->P_M = 0.5 * ((PT_L + PT_R) + (v_line_L - v_line_R) * 0.25 * (L->rho + R->rho) * (cs_L + cs_R));
-                o->S_M = 0.5 * (v_line_R + v_line_L) + 2.0 * (PT_L - PT_R) / ((L->rho + R->rho) * (cs_L + cs_R));
-                double S_plus = DMAX(DMAX(fabs(v_line_L - cs_L), fabs(v_line_R - cs_R)), DMAX(fabs(v_line_L + cs_L), fabs(v_line_R + cs_R)));
-                S_L = -S_plus; S_R = S_plus;
-                if (o->S_M < S_L) o->S_M = S_L;
-                if (o->S_M > S_R) o->S_M = S_R;
-                if (o->P_M <= MIN_REAL_NUMBER) { o->P_M = MIN_REAL_NUMBER; o->S_M = S_L = S_R = 0; }
-            }
-        }
+        o->P_M = MIN_REAL_NUMBER; o->S_M = *S_L = *S_R = 0;
+        return 0;
     }
-    (void)S_L; (void)S_R;
+    pstar_hllc(L, R, o, v_line_L, v_line_R, cs_L, cs_R, S_L, S_R);
+    if (!((o->P_M <= 0) || (isnan(o->P_M)) || (o->P_M > press_tot_limiter))) return 1;
+    pstar_roe(L, R, o, c, n_unit, v_line_L, v_line_R, cs_L, cs_R, h_L, h_R, S_L, S_R);
+    if (!((o->P_M <= 0) || (isnan(o->P_M)) || (o->P_M > press_tot_limiter))) return 2;
+    pstar_pvrs(L, R, o, v_line_L, v_line_R, cs_L, cs_R, S_L, S_R);
+    return 3;
 }
 
 static void riemann_kt(const rstate *L, const rstate *R, rout *o, const double n[3],
@@ -1016,7 +1049,9 @@ static int riemann_solver(rstate L, rstate R, rout *o, const double n_unit[3], d
     double v_line_R = R.v[0] * n_unit[0] + R.v[1] * n_unit[1] + R.v[2] * n_unit[2];
     int mask = 1;
     /* HLLC_Riemann_solver (reimann.h:400-425), MFM branch */
-    get_wavespeeds_and_pressure_star(&L, &R, o, c, n_unit, v_line_L, v_line_R, cs_L, cs_R, h_L, h_R, press_tot_limiter);
+    double S_L, S_R;
+    get_wavespeeds_and_pressure_star(&L, &R, o, c, n_unit, v_line_L, v_line_R, cs_L, cs_R, h_L, h_R, press_tot_limiter,
+                                     &S_L, &S_R);
     for (int k = 0; k < 3; k++) o->Fv[k] = o->P_M * n_unit[k];
     o->Fp = o->P_M * o->S_M;
     if ((o->P_M < 0) || (isnan(o->P_M))) {
@@ -1029,6 +1064,78 @@ static int riemann_solver(rstate L, rstate R, rout *o, const double n_unit[3], d
         mask |= 4;
     }
     return mask;
+}
+
+/* Face-frame quantities that Riemann_solver derives from the two states
+ * (reimann.h:327-345): sound speeds, enthalpies, normal velocities. */
+typedef struct rprep { double cs_L, cs_R, h_L, h_R, v_line_L, v_line_R; } rprep;
+
+static rprep riemann_prep(rstate *L, rstate *R, const double n_unit[3], const gconst *c)
+{
+    rprep q;
+    L->cs = sqrt(c->G0 * L->p / L->rho);
+    R->cs = sqrt(c->G0 * R->p / R->rho);
+    L->u = L->p / (c->G9 * L->rho);
+    R->u = R->p / (c->G9 * R->rho);
+    q.cs_L = L->cs; q.cs_R = R->cs;
+    q.h_L = L->p / L->rho + L->u + 0.5 * (L->v[0] * L->v[0] + L->v[1] * L->v[1] + L->v[2] * L->v[2]);
+    q.h_R = R->p / R->rho + R->u + 0.5 * (R->v[0] * R->v[0] + R->v[1] * R->v[1] + R->v[2] * R->v[2]);
+    q.v_line_L = L->v[0] * n_unit[0] + L->v[1] * n_unit[1] + L->v[2] * n_unit[2];
+    q.v_line_R = R->v[0] * n_unit[0] + R->v[1] * n_unit[1] + R->v[2] * n_unit[2];
+    return q;
+}
+
+static void load_states(const double Lin[5], const double Rin[5], rstate *L, rstate *R)
+{
+    memset(L, 0, sizeof(*L));
+    memset(R, 0, sizeof(*R));
+    L->rho = Lin[0]; L->p = Lin[1]; L->v[0] = Lin[2]; L->v[1] = Lin[3]; L->v[2] = Lin[4];
+    R->rho = Rin[0]; R->p = Rin[1]; R->v[0] = Rin[2]; R->v[1] = Rin[3]; R->v[2] = Rin[4];
+}
+
+void orc_riemann_kt(double gamma, const double Lin[5], const double Rin[5], const double n[3], double out[6])
+{
+    gconst c = make_gconst(gamma);
+    rstate L, R;
+    load_states(Lin, Rin, &L, &R);
+    rprep q = riemann_prep(&L, &R, n, &c);
+    rout o;
+    memset(&o, 0, sizeof(o));
+    riemann_kt(&L, &R, &o, n, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R, q.h_L, q.h_R);
+    out[0] = o.P_M; out[1] = o.S_M; out[2] = o.Fv[0]; out[3] = o.Fv[1]; out[4] = o.Fv[2]; out[5] = o.Fp;
+}
+
+void orc_pstar_estimate(double gamma, const double Lin[5], const double Rin[5], const double n[3], int which,
+                        double out[4])
+{
+    gconst c = make_gconst(gamma);
+    rstate L, R;
+    load_states(Lin, Rin, &L, &R);
+    rprep q = riemann_prep(&L, &R, n, &c);
+    rout o;
+    memset(&o, 0, sizeof(o));
+    double S_L = 0, S_R = 0;
+    if (which == 0) pstar_hllc(&L, &R, &o, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R, &S_L, &S_R);
+    else if (which == 1)
+        pstar_roe(&L, &R, &o, &c, n, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R, q.h_L, q.h_R, &S_L, &S_R);
+    else pstar_pvrs(&L, &R, &o, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R, &S_L, &S_R);
+    out[0] = o.P_M; out[1] = o.S_M; out[2] = S_L; out[3] = S_R;
+}
+
+int orc_wavespeeds(double gamma, const double Lin[5], const double Rin[5], const double n[3],
+                   double press_tot_limiter, double out[4])
+{
+    gconst c = make_gconst(gamma);
+    rstate L, R;
+    load_states(Lin, Rin, &L, &R);
+    rprep q = riemann_prep(&L, &R, n, &c);
+    rout o;
+    memset(&o, 0, sizeof(o));
+    double S_L = 0, S_R = 0;
+    int which = get_wavespeeds_and_pressure_star(&L, &R, &o, &c, n, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R, q.h_L,
+                                                 q.h_R, press_tot_limiter, &S_L, &S_R);
+    out[0] = o.P_M; out[1] = o.S_M; out[2] = S_L; out[3] = S_R;
+    return which;
 }
 
 int orc_riemann(double gamma, const double Lin[5], const double Rin[5], const double n[3],

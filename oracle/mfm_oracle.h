@@ -108,6 +108,18 @@ void   orc_slope_limiter(double grad[3], double valmax, double valmin, double al
  * (1 HLLC, 2 KT, 4 exact). */
 int    orc_riemann(double gamma, const double L[5], const double R[5], const double n[3],
                    double press_tot_limiter, double out[6]);
+/* The Kurganov-Tadmor/PWK flux alone (reimann.h:484-519), MFM branch:
+ * out = P_M, S_M, F_v[3], F_p. */
+void   orc_riemann_kt(double gamma, const double L[5], const double R[5], const double n[3], double out[6]);
+/* One star-state estimate of get_wavespeeds_and_pressure_star
+ * (reimann.h:527-593): which 0 Gaburov HLLC (Davis wave speeds), 1 Roe
+ * averages, 2 PVRS; out = P_M, S_M, S_L, S_R. */
+void   orc_pstar_estimate(double gamma, const double L[5], const double R[5], const double n[3], int which,
+                          double out[4]);
+/* The whole estimate cascade with its limiter tests: out = P_M, S_M, S_L,
+ * S_R; returns the estimate that stands (0 vacuum, 1 HLLC, 2 Roe, 3 PVRS). */
+int    orc_wavespeeds(double gamma, const double L[5], const double R[5], const double n[3],
+                      double press_tot_limiter, double out[4]);
 
 /* ---- the three stages (density.c:451, gradients.c:577, hydro_toplevel.c:1045) ---- */
 int  orc_density(const orc_params *p, orc_state *s, const int *active, int n_active, orc_stats *st);

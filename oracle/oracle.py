@@ -83,6 +83,12 @@ def lib():
                                         C.c_double, C.c_int, C.c_double, C.c_double]
         L.orc_riemann.argtypes = [C.c_double, _dp, _dp, _dp, C.c_double, _dp]
         L.orc_riemann.restype = C.c_int
+        L.orc_riemann_kt.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_riemann_kt.restype = None
+        L.orc_pstar_estimate.argtypes = [C.c_double, _dp, _dp, _dp, C.c_int, _dp]
+        L.orc_pstar_estimate.restype = None
+        L.orc_wavespeeds.argtypes = [C.c_double, _dp, _dp, _dp, C.c_double, _dp]
+        L.orc_wavespeeds.restype = C.c_int
         for f in (L.orc_density, L.orc_gradients, L.orc_hydro_force):
             f.argtypes = [C.POINTER(Params), C.POINTER(State), _ip, C.c_int, C.POINTER(Stats)]
             f.restype = C.c_int
@@ -154,6 +160,38 @@ def riemann(L, R, n=(1.0, 0.0, 0.0), press_tot_limiter=1e300, gamma=5.0 / 3.0):
     out = np.zeros(6)
     mask = lib().orc_riemann(gamma, ptr(Lc), ptr(Rc), ptr(nc), press_tot_limiter, ptr(out))
     return dict(P_M=out[0], S_M=out[1], Fv=out[2:5].copy(), Fp=out[5], mask=mask)
+
+
+def _states(L, R, n):
+    return [np.ascontiguousarray(np.asarray(x, dtype=np.float64)) for x in (L, R, n)]
+
+
+def riemann_kt(L, R, n=(1.0, 0.0, 0.0), gamma=5.0 / 3.0):
+    """Kurganov-Tadmor/PWK flux alone. Returns dict(P_M, S_M, Fv, Fp)."""
+    Lc, Rc, nc = _states(L, R, n)
+    out = np.zeros(6)
+    lib().orc_riemann_kt(gamma, ptr(Lc), ptr(Rc), ptr(nc), ptr(out))
+    return dict(P_M=out[0], S_M=out[1], Fv=out[2:5].copy(), Fp=out[5])
+
+
+PSTAR_HLLC, PSTAR_ROE, PSTAR_PVRS = 0, 1, 2
+
+
+def pstar_estimate(L, R, which, n=(1.0, 0.0, 0.0), gamma=5.0 / 3.0):
+    """One star-state estimate (0 HLLC/Davis, 1 Roe, 2 PVRS): dict(P_M, S_M, S_L, S_R)."""
+    Lc, Rc, nc = _states(L, R, n)
+    out = np.zeros(4)
+    lib().orc_pstar_estimate(gamma, ptr(Lc), ptr(Rc), ptr(nc), int(which), ptr(out))
+    return dict(P_M=out[0], S_M=out[1], S_L=out[2], S_R=out[3])
+
+
+def wavespeeds(L, R, n=(1.0, 0.0, 0.0), press_tot_limiter=1e300, gamma=5.0 / 3.0):
+    """The estimate cascade: dict(P_M, S_M, S_L, S_R, which) with which 0 vacuum,
+    1 HLLC, 2 Roe, 3 PVRS."""
+    Lc, Rc, nc = _states(L, R, n)
+    out = np.zeros(4)
+    which = lib().orc_wavespeeds(gamma, ptr(Lc), ptr(Rc), ptr(nc), press_tot_limiter, ptr(out))
+    return dict(P_M=out[0], S_M=out[1], S_L=out[2], S_R=out[3], which=which)
 
 
 # ---------------------------------------------------------------------------
