@@ -1465,6 +1465,56 @@ gzm_status gzm_neighbor_counts(gzm_ctx *c, int32_t mode, const int32_t *active, 
     return GZM_OK;
 }
 
+gzm_status gzm_riemann_states(int32_t device, int32_t mode, int32_t precision, double gamma, int64_t n,
+                              const double *L, const double *R, const double *normal, const double *limiter,
+                              double *out, int32_t *mask)
+{
+    if (!L || !R || !normal || !out || !mask || (mode == 0 && !limiter)) { set_error("NULL argument"); return GZM_ERR_ARG; }
+    if (mode != 0 && mode != 1) { set_error("mode must be 0 (Riemann_solver) or 1 (KT flux)"); return GZM_ERR_ARG; }
+    if (precision < 0 || precision > 2) { set_error("precision must be 0, 1 or 2"); return GZM_ERR_ARG; }
+    if (!(gamma > 1)) { set_error("gamma must be > 1"); return GZM_ERR_ARG; }
+    if (n < 0 || n > ((int64_t)1 << 28)) { set_error("n=%lld out of range", (long long)n); return GZM_ERR_ARG; }
+    if (n == 0) return GZM_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { set_error("no HIP device"); return GZM_ERR_NO_DEVICE; }
+    if (device < 0 || device >= ndev) { set_error("device %d not present", device); return GZM_ERR_ARG; }
+    GZM_HIP(hipSetDevice(device));
+    GammaConsts G;
+    G.G0 = gamma;
+    G.G1 = ((gamma - 1.0) / (2.0 * gamma));
+    G.G2 = ((gamma + 1.0) / (2.0 * gamma));
+    G.G3 = ((2.0 * gamma / (gamma - 1.0)));
+    G.G4 = (2.0 / (gamma - 1.0));
+    G.G5 = (2.0 / (gamma + 1.0));
+    G.G6 = ((gamma - 1.0) / (gamma + 1.0));
+    G.G7 = (0.5 * (gamma - 1.0));
+    G.G8 = (1.0 / gamma);
+    G.G9 = (gamma - 1.0);
+    const size_t nn = (size_t)n;
+    double *d = nullptr;
+    int *dm = nullptr;
+    const size_t doubles = nn * (5 + 5 + 3 + 1 + 6);
+    gzm_status s = dev_alloc(&d, doubles);
+    if (s) return s;
+    s = dev_alloc(&dm, nn);
+    if (s) { dev_free(d); return s; }
+    double *dL = d, *dR = d + 5 * nn, *dn = d + 10 * nn, *dlim = d + 13 * nn, *dout = d + 14 * nn;
+    hipError_t e = hipMemcpy(dL, L, sizeof(double) * 5 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dR, R, sizeof(double) * 5 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, normal, sizeof(double) * 3 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess && limiter) e = hipMemcpy(dlim, limiter, sizeof(double) * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_riemann_states(nullptr, mode, precision, G, n, dL, dR, dn, dlim, dout, dm);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * 6 * nn, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(mask, dm, sizeof(int) * nn, hipMemcpyDeviceToHost);
+    dev_free(d);
+    dev_free(dm);
+    if (e != hipSuccess) { set_error("gzm_riemann_states: %s", hipGetErrorString(e)); return GZM_ERR_HIP; }
+    return GZM_OK;
+}
+
 gzm_status gzm_timing_get(gzm_ctx *c, gzm_timing *t)
 {
     if (!c || !t) { set_error("NULL argument"); return GZM_ERR_ARG; }

@@ -354,5 +354,9 @@ void launch_fill_int(hipStream_t s, int *p, int n, int value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value);
 void launch_fill_double(hipStream_t s, double *p, long long n, double value);
 void launch_set_flags(hipStream_t s, int n_active, const int *active, uint8_t *flags);
+// the device Riemann solver on independent face states (gzm_elements.hip):
+// mode 0 riemann_solve, mode 1 the KT flux alone; precision 0 FP64, else FP32
+void launch_riemann_states(hipStream_t s, int mode, int precision, const GammaConsts &G, int64_t n, const double *L,
+                           const double *R, const double *normal, const double *limiter, double *out, int *mask);
 
 }  // namespace gzm

@@ -559,12 +559,15 @@ __device__ __forceinline__ void side_fp64(const ForceArgs &a, int k, PairSide<do
 }
 
 // The pair (i, j) of particle i's loop returned kNearThreshold: list i for the
-// FP64 force redo and i and j for the FP64 gradient redo, each once.
+// FP64 force redo and i and j for the FP64 gradient redo, each once.  An
+// inactive j keeps its stored gradient (an output of an earlier pass, which
+// the FP64 pass reads as it is), so only an active j is listed.
 __device__ __forceinline__ void mark_redo(const ForceArgs &a, int i, int j)
 {
     const int fi = atomicOr(&a.redo_flag[i], 3);
     if (!(fi & 1)) a.redo_list[atomicAdd(&a.redo_count[0], 1)] = i;
     if (!(fi & 2)) a.redo_list[a.n + atomicAdd(&a.redo_count[1], 1)] = i;
+    if (a.is_active && !a.is_active[j]) return;
     const int fj = atomicOr(&a.redo_flag[j], 2);
     if (!(fj & 2)) a.redo_list[a.n + atomicAdd(&a.redo_count[1], 1)] = j;
 }

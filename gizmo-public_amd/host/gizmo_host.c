@@ -25,9 +25,10 @@ int FirstActiveParticle = -1;
 int *NextActiveParticle = NULL;
 
 static gzm_ctx *g_ctx = NULL;
+static gzm_params g_params;      /* parameters g_ctx was created with */
 static int g_device = 0;
 static int g_status = 0;
-static int g_resident = 0;       /* density() results live on the device */
+static int g_resident = 0;       /* the device holds this pass's particle set (uploaded by density()) */
 static gizmo_endrun_handler g_endrun = NULL;
 static char g_err[512];
 
@@ -37,6 +38,7 @@ static double *h_pos, *h_vel, *h_mass, *h_u, *h_hsml, *h_cn, *h_vsig, *h_num_ngb
 static double *h_rho, *h_pres, *h_nvt, *h_fce, *h_grho, *h_gp, *h_gv, *h_acc, *h_dtu;
 static unsigned char *h_wakeup;
 static int *h_active;
+static unsigned char *h_seen;    /* active-list walk: particles already listed */
 
 void gizmo_host_set_endrun_handler(gizmo_endrun_handler handler) { g_endrun = handler; }
 
@@ -66,6 +68,7 @@ static void free_staging(void)
     for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) { free(*arrs[k]); *arrs[k] = NULL; }
     free(h_wakeup); h_wakeup = NULL;
     free(h_active); h_active = NULL;
+    free(h_seen); h_seen = NULL;
     g_cap = 0;
 }
 
@@ -81,9 +84,10 @@ static int ensure_staging(int n)
     h_pres = malloc(N * sizeof(double)); h_nvt = malloc(9 * N * sizeof(double)); h_fce = malloc(N * sizeof(double));
     h_grho = malloc(3 * N * sizeof(double)); h_gp = malloc(3 * N * sizeof(double)); h_gv = malloc(9 * N * sizeof(double));
     h_acc = malloc(3 * N * sizeof(double)); h_dtu = malloc(N * sizeof(double));
-    h_wakeup = malloc(N); h_active = malloc(N * sizeof(int));
+    h_wakeup = malloc(N); h_active = malloc(N * sizeof(int)); h_seen = calloc(N, 1);
     if (!h_pos || !h_vel || !h_mass || !h_u || !h_hsml || !h_cn || !h_vsig || !h_num_ngb || !h_dhsml || !h_divvel ||
-        !h_rho || !h_pres || !h_nvt || !h_fce || !h_grho || !h_gp || !h_gv || !h_acc || !h_dtu || !h_wakeup || !h_active) {
+        !h_rho || !h_pres || !h_nvt || !h_fce || !h_grho || !h_gp || !h_gv || !h_acc || !h_dtu || !h_wakeup || !h_active ||
+        !h_seen) {
         free_staging();
         return -1;
     }
@@ -142,23 +146,42 @@ void gizmo_host_release(void)
     g_resident = 0;
 }
 
-/* FirstActiveParticle/NextActiveParticle -> gas indices (the loops' condition
- * P[i].Type==0; mass/density tests happen inside the library) */
-static int collect_active(void)
-{
-    int n = 0;
-    for (int i = FirstActiveParticle; i >= 0; i = NextActiveParticle[i]) {
-        if (i < N_gas && P[i].Type == 0) h_active[n++] = i;
-        if (n > N_gas) break; /* malformed list */
-    }
-    return n;
-}
-
 static int fail(int status, const char *where)
 {
     g_status = status;
     snprintf(g_err, sizeof(g_err), "%s: %s (%s)", where, gzm_status_string(status), gzm_last_error());
     return status;
+}
+
+/* FirstActiveParticle/NextActiveParticle -> gas indices in h_active (the
+ * loops' condition P[i].Type == 0; mass and density tests happen inside the
+ * library).  skip_converged: leave out particles whose TimeBin is negative,
+ * the mark density() puts on particles that have finished iterating
+ * (density_isactive, density.c:34).  A list that leaves [0, NumPart), visits a
+ * particle twice or runs longer than NumPart steps is reported through
+ * endrun instead of being followed. */
+static int collect_active(int skip_converged, int *n_out)
+{
+    int n = 0, steps = 0, bad = 0;
+    for (int i = FirstActiveParticle; i >= 0; i = NextActiveParticle[i]) {
+        if (i >= NumPart || ++steps > NumPart) { bad = 1; break; }
+        if (i >= N_gas || P[i].Type != 0) continue;
+        if (h_seen[i]) { bad = 1; break; }
+        h_seen[i] = 1;
+        if (skip_converged && P[i].TimeBin < 0) continue;
+        h_active[n++] = i;
+    }
+    /* clear the marks of the walk (listed or skipped) */
+    steps = 0;
+    for (int i = FirstActiveParticle; i >= 0 && i < NumPart && steps <= NumPart; i = NextActiveParticle[i], steps++)
+        if (i < N_gas) h_seen[i] = 0;
+    if (bad) {
+        g_status = GZM_ERR_ARG;
+        snprintf(g_err, sizeof(g_err), "malformed active list (index outside [0, %d), repeated or cyclic)", NumPart);
+        return GZM_ERR_ARG;
+    }
+    *n_out = n;
+    return 0;
 }
 
 static int ensure_ctx(void)
@@ -177,16 +200,27 @@ static int ensure_ctx(void)
     p.precision = 0;
     p.cell_size = 0;
     p.pair_list_cap = 0;
+    /* the context (stream, device buffers) lives across passes; it is made
+     * again only when a parameter it was created with changes (the first
+     * step's tolerance, a new box size or device) */
+    if (g_ctx && memcmp(&p, &g_params, sizeof(p)) == 0) return 0;
     if (g_ctx) gzm_ctx_destroy(g_ctx);
     g_ctx = NULL;
+    g_resident = 0;
     int s = gzm_ctx_create(&g_ctx, &p);
     if (s) return fail(s, "gzm_ctx_create");
+    g_params = p;
     return 0;
 }
 
-/* P[]/SphP[] -> staging; with_state also sends the density/gradient results
- * that inactive particles contribute as neighbours */
-static int upload(int with_state)
+/* P[]/SphP[] -> staging -> device: positions, velocities, masses, energies and
+ * the whole carried state (hsml, NumNgb, DhsmlNgbFactor, DivVel, density,
+ * pressure, NV_T, ConditionNumber, FaceClosureError, gradients,
+ * MaxSignalVel), so that particles outside the active list enter the
+ * gradient and force loops of active neighbours with their stored values, as
+ * the reference's loops read P[j]/SphP[j] (gradients.c:1477-2089,
+ * hydro_evaluate.h:146-498) */
+static int upload(void)
 {
     int n = N_gas;
     if (ensure_staging(n)) return fail(GZM_ERR_NOMEM, "host staging");
@@ -217,23 +251,27 @@ static int upload(int with_state)
     v.pos = h_pos; v.vel = h_vel; v.mass = h_mass; v.u = h_u;
     v.hsml = h_hsml; v.cn = h_cn; v.vsig = h_vsig;
     v.num_ngb = h_num_ngb; v.dhsml = h_dhsml; v.divvel = h_divvel; v.fce = h_fce;
-    if (with_state) {
-        v.density = h_rho; v.pressure = h_pres; v.nvt = h_nvt;
-        v.grad_rho = h_grho; v.grad_p = h_gp; v.grad_v = h_gv;
-    }
+    v.density = h_rho; v.pressure = h_pres; v.nvt = h_nvt;
+    v.grad_rho = h_grho; v.grad_p = h_gp; v.grad_v = h_gv;
     int s = gzm_upload(g_ctx, &v);
     if (s) return fail(s, "gzm_upload");
     return 0;
 }
 
-static int download(void)
+enum { DL_DENSITY = 1, DL_GRADIENTS = 2, DL_FORCE = 4 };
+
+/* device -> staging: the arrays a stage wrote (out2particle's role) */
+static int download(int what)
 {
     gzm_host_view v;
     memset(&v, 0, sizeof(v));
     v.n = N_gas;
-    v.hsml = h_hsml; v.cn = h_cn; v.vsig = h_vsig; v.num_ngb = h_num_ngb; v.dhsml = h_dhsml; v.divvel = h_divvel;
-    v.density = h_rho; v.pressure = h_pres; v.nvt = h_nvt; v.fce = h_fce;
-    v.grad_rho = h_grho; v.grad_p = h_gp; v.grad_v = h_gv; v.acc = h_acc; v.dtu = h_dtu; v.wakeup = h_wakeup;
+    if (what & DL_DENSITY) {
+        v.hsml = h_hsml; v.cn = h_cn; v.num_ngb = h_num_ngb; v.dhsml = h_dhsml; v.divvel = h_divvel;
+        v.density = h_rho; v.pressure = h_pres; v.nvt = h_nvt; v.fce = h_fce;
+    }
+    if (what & DL_GRADIENTS) { v.grad_rho = h_grho; v.grad_p = h_gp; v.grad_v = h_gv; }
+    if (what & DL_FORCE) { v.acc = h_acc; v.dtu = h_dtu; v.vsig = h_vsig; v.wakeup = h_wakeup; }
     int s = gzm_download(g_ctx, &v);
     if (s) return fail(s, "gzm_download");
     return 0;
@@ -243,13 +281,16 @@ void density(void)
 {
     g_status = 0;
     if (N_gas <= 0) return;
-    if (ensure_ctx() || upload(0)) { endrun(g_status); return; }
-    int na = collect_active();
+    if (ensure_staging(N_gas)) { fail(GZM_ERR_NOMEM, "host staging"); endrun(g_status); return; }
+    int na = 0;
+    if (collect_active(1, &na)) { endrun(g_status); return; }
+    if (ensure_ctx() || upload()) { endrun(g_status); return; }
+    g_resident = 1;
     gzm_pass_stats st;
     memset(&st, 0, sizeof(st));
     int s = gzm_density(g_ctx, h_active, na, &st);
     if (s) { fail(s, "gzm_density"); endrun(s); return; }
-    if (download()) { endrun(g_status); return; }
+    if (download(DL_DENSITY)) { endrun(g_status); return; }
     for (int a = 0; a < na; a++) {
         int i = h_active[a];
         if (!(P[i].Mass > 0)) continue;
@@ -263,23 +304,36 @@ void density(void)
         SphP[i].FaceClosureError = h_fce[i];
         memcpy(SphP[i].NV_T, h_nvt + 9 * (size_t)i, 9 * sizeof(double));
     }
+    /* "mark as active again" (density.c:869-872): a particle that entered
+     * with a negative TimeBin was left out of the iteration */
+    int steps = 0;
+    for (int i = FirstActiveParticle; i >= 0 && i < NumPart && steps <= NumPart; i = NextActiveParticle[i], steps++)
+        if (P[i].TimeBin < 0) P[i].TimeBin = -P[i].TimeBin - 1;
+}
+
+/* The gradient and force stages run on the particle set density() uploaded;
+ * called on their own (a caller that set P/SphP from elsewhere), they upload
+ * it first. */
+static int ensure_resident(void)
+{
+    if (ensure_staging(N_gas)) return fail(GZM_ERR_NOMEM, "host staging");
+    if (g_resident && g_ctx) return 0;
+    if (ensure_ctx() || upload()) return g_status;
     g_resident = 1;
+    return 0;
 }
 
 void hydro_gradient_calc(void)
 {
     g_status = 0;
     if (N_gas <= 0) return;
-    if (!g_resident) {
-        if (ensure_ctx() || upload(1)) { endrun(g_status); return; }
-        g_resident = 1;
-    }
-    int na = collect_active();
+    int na = 0;
+    if (ensure_resident() || collect_active(0, &na)) { endrun(g_status); return; }
     gzm_pass_stats st;
     memset(&st, 0, sizeof(st));
     int s = gzm_gradients(g_ctx, h_active, na, &st);
     if (s) { fail(s, "gzm_gradients"); endrun(s); return; }
-    if (download()) { endrun(g_status); return; }
+    if (download(DL_GRADIENTS)) { endrun(g_status); return; }
     for (int a = 0; a < na; a++) {
         int i = h_active[a];
         memcpy(SphP[i].Gradients.Density, h_grho + 3 * (size_t)i, 3 * sizeof(double));
@@ -292,16 +346,13 @@ void hydro_force(void)
 {
     g_status = 0;
     if (N_gas <= 0) return;
-    if (!g_resident) {
-        if (ensure_ctx() || upload(1)) { endrun(g_status); return; }
-        g_resident = 1;
-    }
-    int na = collect_active();
+    int na = 0;
+    if (ensure_resident() || collect_active(0, &na)) { endrun(g_status); return; }
     gzm_pass_stats st;
     memset(&st, 0, sizeof(st));
     int s = gzm_hydro_force(g_ctx, h_active, na, &st);
     if (s) { fail(s, "gzm_hydro_force"); endrun(s); return; }
-    if (download()) { endrun(g_status); return; }
+    if (download(DL_FORCE)) { endrun(g_status); return; }
     for (int a = 0; a < na; a++) {
         int i = h_active[a];
         for (int k = 0; k < 3; k++) SphP[i].HydroAccel[k] = h_acc[3 * (size_t)i + k];
@@ -310,6 +361,8 @@ void hydro_force(void)
     }
     for (int i = 0; i < N_gas; i++)
         if (h_wakeup[i]) SphP[i].wakeup = 1;
+    /* the next pass starts from P/SphP again (positions drift between passes) */
+    g_resident = 0;
 }
 
 void compute_hydro_densities_and_forces(void)

@@ -211,6 +211,21 @@ gzm_status gzm_run_passes(gzm_ctx *ctx, int32_t n_passes, gzm_pass_stats *stats)
 gzm_status gzm_neighbor_counts(gzm_ctx *ctx, int32_t mode, const int32_t *active, int64_t n_active,
                                int32_t *counts, int64_t *total);
 
+/* Riemann_solver() for MFM hydro (hydro/reimann.h:286-388) on n independent
+ * face states, one per GPU thread, with the same device code as the force
+ * loop: element-level checks of branches that particle sets rarely reach.
+ * Host arrays: L and R [n][5] = rho, p, v[3]; normal [n][3] (unit); limiter
+ * [n] = press_tot_limiter (hydro_core_meshless.h:182-200); out [n][6] = P_M,
+ * S_M, F_v[3], F_p; mask [n] = stages used (1 HLLC, 2 KT, 4 exact; 64 in FP32
+ * when an estimate lay within the FP32 margin of its threshold, where the
+ * force loop would redo the pair in FP64), or -1234545 on the exact
+ * solver's negative-density exit.  mode 0: the whole solver; mode 1: the
+ * Kurganov-Tadmor/PWK flux alone (reimann.h:484-519; limiter unused).
+ * precision 0: FP64; 1 or 2: FP32 with the FP64 exact solver. */
+gzm_status gzm_riemann_states(int32_t device, int32_t mode, int32_t precision, double gamma, int64_t n,
+                              const double *L, const double *R, const double *normal, const double *limiter,
+                              double *out, int32_t *mask);
+
 gzm_status gzm_timing_get(gzm_ctx *ctx, gzm_timing *timing);
 gzm_status gzm_timing_reset(gzm_ctx *ctx);
 

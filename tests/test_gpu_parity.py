@@ -4,9 +4,6 @@ Bar: neighbour sets and interaction counts bit-exact; density, hsml, NumNgb,
 pressure, signal speed within 1e-12 relative; gradients, dv/dt, du/dt within
 1e-10 of the sum of |pair terms| (see tests/parity.py).
 """
-import ctypes as C
-import os
-
 import numpy as np
 import pytest
 
@@ -175,86 +172,3 @@ def test_blast_128_full_size():
     errs, bad = check_pass(out, ref)
     print({k: f"{v:.2e}" for k, v in errs.items()})
     assert not bad, bad
-
-
-# ------------------------------------------------------- host mirror -------
-class ParticleData(C.Structure):
-    _fields_ = [("Pos", C.c_double * 3), ("Vel", C.c_double * 3), ("Mass", C.c_double), ("Hsml", C.c_double),
-                ("NumNgb", C.c_double), ("DhsmlNgbFactor", C.c_double), ("Particle_DivVel", C.c_double),
-                ("Type", C.c_int), ("TimeBin", C.c_int), ("ID", C.c_ulonglong)]
-
-
-class GasCellData(C.Structure):
-    _fields_ = [("Density", C.c_double), ("Pressure", C.c_double), ("InternalEnergy", C.c_double),
-                ("InternalEnergyPred", C.c_double), ("VelPred", C.c_double * 3), ("HydroAccel", C.c_double * 3),
-                ("DtInternalEnergy", C.c_double), ("MaxSignalVel", C.c_double), ("ConditionNumber", C.c_double),
-                ("FaceClosureError", C.c_double), ("NV_T", (C.c_double * 3) * 3),
-                ("GradDensity", C.c_double * 3), ("GradPressure", C.c_double * 3),
-                ("GradVelocity", (C.c_double * 3) * 3), ("wakeup", C.c_int)]
-
-
-class AllData(C.Structure):
-    _fields_ = [("BoxSize", C.c_double), ("DesNumNgb", C.c_double), ("MaxNumNgbDeviation", C.c_double),
-                ("MinHsml", C.c_double), ("MaxHsml", C.c_double), ("Time", C.c_double), ("TimeBegin", C.c_double),
-                ("MaxPart", C.c_int)]
-
-
-def host_lib():
-    lib = C.CDLL(os.path.join(gzm.LIB_DIR, "libgizmo_host.so"))
-    lib.gizmo_host_allocate.argtypes = [C.c_int]
-    lib.gizmo_host_last_status.restype = C.c_int
-    return lib
-
-
-def fill_host(lib, ic):
-    N = ic["pos"].shape[0]
-    assert lib.gizmo_host_allocate(N) == 0
-    P = C.cast(C.c_void_p.in_dll(lib, "P"), C.POINTER(ParticleData))
-    S = C.cast(C.c_void_p.in_dll(lib, "SphP"), C.POINTER(GasCellData))
-    A = AllData.in_dll(lib, "All")
-    A.BoxSize, A.DesNumNgb, A.MaxNumNgbDeviation = 1.0, 32.0, 0.05
-    A.MinHsml, A.MaxHsml, A.Time, A.TimeBegin = 0.0, 1e56, 0.0, 0.0
-    for i in range(N):
-        p, s = P[i], S[i]
-        p.Pos[:] = list(ic["pos"][i])
-        p.Vel[:] = list(ic["vel"][i])
-        s.VelPred[:] = list(ic["vel"][i])
-        p.Mass = ic["mass"][i]
-        p.Hsml = ic["hsml"][i]
-        p.Type = 0
-        s.InternalEnergy = s.InternalEnergyPred = ic["u"][i]
-    return P, S
-
-
-def test_host_mirror_is_a_drop_in():
-    ic = ics.random_state(10)
-    N = ic["pos"].shape[0]
-    lib = host_lib()
-    codes = []
-    handler_t = C.CFUNCTYPE(None, C.c_int, C.c_char_p)
-    handler = handler_t(lambda code, where: codes.append(code))
-    lib.gizmo_host_set_endrun_handler(handler)
-    P, S = fill_host(lib, ic)
-    lib.compute_hydro_densities_and_forces()
-    assert lib.gizmo_host_last_status() == 0 and not codes
-    out, _ = gzm.run_pass(ic)
-    for i in range(0, N, 7):
-        assert P[i].Hsml == out["hsml"][i]
-        assert S[i].Density == out["density"][i]
-        assert list(S[i].HydroAccel) == list(out["acc"][i])
-        assert S[i].DtInternalEnergy == out["dtu"][i]
-        assert list(S[i].GradPressure) == list(out["grad_p"][i])
-        assert [S[i].NV_T[a][b] for a in range(3) for b in range(3)] == list(out["nvt"][i])
-    # the reference's failure path: endrun(1155) when hsml cannot converge
-    lib.gizmo_host_release()
-    fill_host(lib, dict(ic, hsml=ic["hsml"] * 1e-3))
-    A = AllData.in_dll(lib, "All")
-    A.MaxHsml = 1e56
-    lib.gizmo_host_set_device(0)
-    # MAXITER is compiled in (150); a collapsed bracket cannot occur here, so
-    # check the status plumbing with an out-of-box position instead
-    P[0].Pos[0] = 2.0
-    lib.density()
-    assert codes and codes[-1] == 1  # GZM_ERR_ARG from gzm_upload
-    lib.gizmo_host_release()
-    lib.gizmo_host_free()
