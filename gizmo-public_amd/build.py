@@ -4,6 +4,8 @@
   lib/libgizmo_host.so  C host that keeps GIZMO's density() /
                         hydro_gradient_calc() / hydro_force() over P[]/SphP[]
                         (include/gizmo_mfm_host.h), linked against libgzm.so
+  lib/libgizmo_io.so    the HDF5 IC / snapshot layout over the host's P[]/SphP[]
+                        (include/gizmo_io.h), against the HDF5 C library
 
 Usage: python gizmo-public_amd/build.py
 """
@@ -38,6 +40,9 @@ HIP_SOURCES = ["csrc/gzm_api.hip", "csrc/gzm_kernels.hip", "csrc/gzm_kernels_f32
 # force loops round differently.
 FILE_FLAGS = {}
 HOST_SOURCES = ["host/gizmo_host.c"]
+IO_SOURCE = "host/gizmo_io.c"
+# the HDF5 C library of this image (1.10); GIZMO_HDF5_PREFIX selects another
+HDF5_PREFIX = os.environ.get("GIZMO_HDF5_PREFIX", "/opt/conda")
 
 
 def run(cmd, cwd=HERE):
@@ -84,9 +89,23 @@ def build_host():
     os.replace(tmp, os.path.join(LIB, "libgizmo_host.so"))
 
 
+def build_io():
+    inc = os.path.join(HDF5_PREFIX, "include")
+    lib = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"skipping libgizmo_io.so: no hdf5.h under {inc}", flush=True)
+        return
+    tmp = os.path.join(LIB, "libgizmo_io.so.tmp")
+    run(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", f"-I{INC}", f"-I{inc}",
+         os.path.join(HERE, IO_SOURCE), "-o", tmp, f"-L{LIB}", "-lgizmo_host", f"-L{lib}", "-lhdf5", "-lm",
+         "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{lib}"])
+    os.replace(tmp, os.path.join(LIB, "libgizmo_io.so"))
+
+
 def build_all():
     build_gzm()
     build_host()
+    build_io()
 
 
 if __name__ == "__main__":

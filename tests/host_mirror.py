@@ -125,3 +125,30 @@ def read_state(lib, n):
 def set_active(lib, idx):
     idx = np.ascontiguousarray(idx, dtype=np.int32)
     assert lib.gizmo_host_set_active(idx.ctypes.data_as(C.POINTER(C.c_int)), int(idx.size)) == 0
+
+
+_io = None
+
+
+def io_lib():
+    """libgizmo_io.so (HDF5 IC / snapshot layout over the same P[]/SphP[]);
+    None when it was not built (no HDF5 library)."""
+    global _io
+    if _io is None:
+        path = os.path.join(gzm.LIB_DIR, "libgizmo_io.so")
+        if not os.path.exists(path):
+            return None
+        host_lib()  # the host library first: libgizmo_io.so resolves P, SphP, All there
+        lib = C.CDLL(path)
+        lib.gizmo_read_ic.argtypes = [C.c_char_p, C.c_int, C.c_void_p]
+        lib.gizmo_write_snapshot.argtypes = [C.c_char_p, C.c_int]
+        lib.gizmo_io_last_error.restype = C.c_char_p
+        _io = lib
+    return _io
+
+
+class IOHeader(C.Structure):
+    _fields_ = [("npart", C.c_int * 6), ("npart_total", C.c_uint * 6), ("npart_total_high", C.c_uint * 6),
+                ("mass_table", C.c_double * 6), ("time", C.c_double), ("redshift", C.c_double),
+                ("box_size", C.c_double), ("hubble_param", C.c_double), ("num_files", C.c_int),
+                ("flag_double", C.c_int), ("comoving", C.c_int)]

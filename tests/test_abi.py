@@ -53,3 +53,15 @@ def test_status_strings_carry_reference_codes():
     assert b"1155" in lib.gzm_status_string(1155)
     assert b"1234" in lib.gzm_status_string(1234)
     assert b"1234545" in lib.gzm_status_string(1234545)
+
+
+def test_io_library_exports_its_header():
+    path = os.path.join(LIB, "libgizmo_io.so")
+    if not os.path.exists(path):
+        pytest.skip("libgizmo_io.so not built (no HDF5 C library)")
+    C.CDLL(os.path.join(LIB, "libgizmo_host.so"), mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(path)
+    names = declared_functions("gizmo_io.h")
+    assert {"gizmo_read_ic", "gizmo_write_snapshot", "gizmo_setup_smoothinglengths"} <= set(names)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
