@@ -39,7 +39,7 @@ HIP_SOURCES = ["csrc/gzm_api.hip", "csrc/gzm_kernels.hip", "csrc/gzm_kernels_f32
 # and contraction decided per inlining site made the pair-list and scanning
 # force loops round differently.
 FILE_FLAGS = {}
-HOST_SOURCES = ["host/gizmo_host.c"]
+HOST_SOURCES = ["host/gizmo_host.c", "host/gizmo_integrate.c"]
 IO_SOURCE = "host/gizmo_io.c"
 # the HDF5 C library of this image (1.10); GIZMO_HDF5_PREFIX selects another
 HDF5_PREFIX = os.environ.get("GIZMO_HDF5_PREFIX", "/opt/conda")
@@ -80,12 +80,10 @@ def build_gzm():
 
 
 def build_host():
-    src = os.path.join(HERE, HOST_SOURCES[0])
-    if not os.path.exists(src):
-        return
+    srcs = [os.path.join(HERE, s) for s in HOST_SOURCES]
     tmp = os.path.join(LIB, "libgizmo_host.so.tmp")
-    run(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", f"-I{INC}", src, "-o", tmp,
-         f"-L{LIB}", "-lgzm", "-Wl,-rpath,$ORIGIN"])
+    run(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", f"-I{INC}", *srcs, "-o", tmp,
+         f"-L{LIB}", "-lgzm", "-lm", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, os.path.join(LIB, "libgizmo_host.so"))
 
 

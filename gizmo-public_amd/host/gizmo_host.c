@@ -146,6 +146,9 @@ void gizmo_host_release(void)
     g_resident = 0;
 }
 
+double gizmo_box_side(int k) { return All.BoxSize * ((All.BoxLong[k] > 0) ? All.BoxLong[k] : 1.0); }
+double gizmo_gamma(void) { return (All.GasGamma > 1) ? All.GasGamma : 5.0 / 3.0; /* EOS_GAMMA (allvars.h:1092) */ }
+
 static int fail(int status, const char *where)
 {
     g_status = status;
@@ -188,8 +191,8 @@ static int ensure_ctx(void)
 {
     gzm_params p;
     memset(&p, 0, sizeof(p));
-    p.box[0] = p.box[1] = p.box[2] = All.BoxSize;
-    p.gamma = 5.0 / 3.0; /* EOS_GAMMA (allvars.h:1092) */
+    for (int k = 0; k < 3; k++) p.box[k] = gizmo_box_side(k);
+    p.gamma = gizmo_gamma();
     p.des_num_ngb = All.DesNumNgb;
     p.max_num_ngb_dev = All.MaxNumNgbDeviation;
     p.min_hsml = All.MinHsml;

@@ -219,6 +219,41 @@ def batch_boxes(n_boxes: int = 512, seed: int = 4, n_min: int = 4096, n_max: int
     return out
 
 
+def sod_tube(n_left: int = 64, n_cross: int = 8, rho_left: float = 1.0, p_left: float = 1.0,
+             rho_right: float = 0.125, p_right: float = 0.1, gamma: float = 5.0 / 3.0) -> dict:
+    """Sod shock tube in a periodic box of sides (1, s, s): x in [0, 1/2) holds
+    the left state on an n_left x n_cross x n_cross lattice of spacing
+    d = 1/(2 n_left), x in [1/2, 1) the right state on a lattice of spacing 2d
+    (density 1/8 with equal masses, so rho_right must be rho_left / 8); s =
+    n_cross d.  At rest, u = p / ((gamma - 1) rho).  The periodic box holds two
+    Riemann problems, at x = 1/2 and at x = 0, which do not meet before
+    t ~ 0.15 (gamma 5/3)."""
+    if abs(rho_right * 8.0 - rho_left) > 1e-12 * rho_left:
+        raise ValueError("equal-mass lattices give a density ratio of exactly 8")
+    if n_left % 2 or n_cross % 2:
+        raise ValueError("n_left and n_cross must be even")
+    d = 0.5 / n_left
+    side = n_cross * d
+
+    def lattice(nx, nc, x0, step):
+        i, j, k = np.meshgrid(np.arange(nx), np.arange(nc), np.arange(nc), indexing="ij")
+        return np.stack([x0 + (i.ravel() + 0.5) * step, (j.ravel() + 0.5) * step, (k.ravel() + 0.5) * step], axis=1)
+
+    left = lattice(n_left, n_cross, 0.0, d)
+    right = lattice(n_left // 2, n_cross // 2, 0.5, 2 * d)
+    pos = np.concatenate([left, right])
+    n = pos.shape[0]
+    m = rho_left * d ** 3
+    u = np.concatenate([np.full(len(left), p_left / ((gamma - 1) * rho_left)),
+                        np.full(len(right), p_right / ((gamma - 1) * rho_right))])
+    h = np.concatenate([np.full(len(left), (3 * 32 * d ** 3 / (4 * np.pi)) ** (1 / 3)),
+                        np.full(len(right), (3 * 32 * 8 * d ** 3 / (4 * np.pi)) ** (1 / 3))])
+    out = _state(pos, np.zeros((n, 3)), np.full(n, m), u, h, f"sod_tube_{n_left}")
+    out["box"] = np.array([1.0, side, side])
+    out["gamma"] = gamma
+    return out
+
+
 def box_slice(ic: dict, b: int) -> dict:
     """Box b of a batch_boxes set as a single-box particle set."""
     o = ic["box_offsets"]

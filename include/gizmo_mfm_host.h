@@ -21,8 +21,9 @@
  * gizmo_host_set_endrun_handler().
  *
  * Particles 0..N_gas-1 are gas (Type 0), as in GIZMO's memory layout; the
- * box is periodic with side All.BoxSize.  P[]/SphP[] must not change between
- * density() and hydro_force() of one pass, as in accel.c:57-111.
+ * box is periodic with sides All.BoxSize * All.BoxLong[k].  P[]/SphP[] must
+ * not change between density() and hydro_force() of one pass, as in
+ * accel.c:57-111.
  */
 #ifndef GIZMO_MFM_HOST_H
 #define GIZMO_MFM_HOST_H
@@ -76,6 +77,13 @@ struct global_data_all_processes {
     double Time;
     double TimeBegin;
     int MaxPart;
+    /* time integration (kick / drift / timestep) */
+    double CourantFac;       /* All.CourantFac (timestep.c:485-487) */
+    double MaxSizeTimestep;  /* All.MaxSizeTimestep (timestep.c:260) */
+    double MinEgySpec;       /* All.MinEgySpec: specific-energy floor, 0 = none (eos.c:221-231) */
+    double BoxLong[3];       /* BOX_LONG_X/Y/Z: box side k is BoxSize * BoxLong[k]; 0 counts as 1
+                                (allvars.h:1518-1560) */
+    double GasGamma;         /* adiabatic index, GAMMA_DEFAULT / EOS_GAMMA (allvars.h:1092); 0 = 5/3 */
 };
 
 extern struct particle_data *P;
@@ -97,6 +105,32 @@ typedef void (*gizmo_endrun_handler)(int code, const char *where);
 void gizmo_host_set_endrun_handler(gizmo_endrun_handler handler);
 void endrun(int code);
 
+/* time integration of the gas with equal timesteps for every particle
+ * (FORCE_EQUAL_TIMESTEPS), on top of the hydro pass: */
+/* kicks.c:255-402 (do_the_kick, gas part): u += DtInternalEnergy dt, halved
+ * instead where that would fall below u/2, floored at MinEgySpec;
+ * Vel += HydroAccel dt; mode 1 (second half kick) also sets VelPred = Vel and
+ * InternalEnergyPred = InternalEnergy */
+void gizmo_kick_gas(double dt, int mode);
+/* predict.c:120-240 (drift_particle, gas part): Pos += Vel dt, wrapped into
+ * the periodic box; VelPred += HydroAccel dt; Density *= exp(-DivVel dt) with
+ * DivVel dt clamped to +-0.3; InternalEnergyPred += DtInternalEnergy dt with
+ * the same halving floor and MinEgySpec; Pressure = (gamma - 1) rho u_pred
+ * (set_eos_pressure, eos.c:58-61) */
+void gizmo_drift_gas(double dt);
+/* timestep.c:482-491: CourantFac * Get_Particle_Size(i) / (0.5 MaxSignalVel),
+ * with Get_Particle_Size = 1.61199 Hsml / NumNgb^(1/3) (predict.c:381-395),
+ * at most MaxSizeTimestep */
+double gizmo_gas_timestep(int i);
+/* the smallest of them over the gas (FORCE_EQUAL_TIMESTEPS, timestep.c:71-77) */
+double gizmo_global_timestep(void);
+/* The main loop of run.c with equal timesteps: hydro forces at the start,
+ * then per step dt = min(global timestep, t_end - Time), first half kick,
+ * drift, Time += dt, hydro forces, second half kick; until Time reaches t_end
+ * or max_steps steps.  Returns the number of steps taken, or the negative
+ * status of a failed pass (endrun is called as well). */
+int gizmo_run_until(double t_end, int max_steps);
+
 /* helpers for embedding and tests (not in the reference) */
 int  gizmo_host_allocate(int n_gas);        /* P, SphP, NextActiveParticle for n gas particles */
 void gizmo_host_free(void);
@@ -105,6 +139,8 @@ int  gizmo_host_set_active(const int *idx, int n); /* active list = idx[0..n-1],
 int  gizmo_host_set_device(int device);     /* GPU ordinal used by the next pass */
 int  gizmo_host_last_status(void);          /* status of the last library call */
 const char *gizmo_host_last_error(void);
+double gizmo_box_side(int k);               /* periodic box side k: BoxSize * BoxLong[k] */
+double gizmo_gamma(void);                   /* adiabatic index of the gas: GasGamma, or 5/3 */
 void gizmo_host_release(void);              /* destroy the device context */
 
 #ifdef __cplusplus

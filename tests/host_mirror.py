@@ -27,7 +27,8 @@ class GasCellData(C.Structure):
 class AllData(C.Structure):
     _fields_ = [("BoxSize", C.c_double), ("DesNumNgb", C.c_double), ("MaxNumNgbDeviation", C.c_double),
                 ("MinHsml", C.c_double), ("MaxHsml", C.c_double), ("Time", C.c_double), ("TimeBegin", C.c_double),
-                ("MaxPart", C.c_int)]
+                ("MaxPart", C.c_int), ("CourantFac", C.c_double), ("MaxSizeTimestep", C.c_double),
+                ("MinEgySpec", C.c_double), ("BoxLong", C.c_double * 3), ("GasGamma", C.c_double)]
 
 
 _lib = None
@@ -45,6 +46,15 @@ def host_lib():
         lib.gizmo_host_last_status.restype = C.c_int
         lib.gizmo_host_last_error.restype = C.c_char_p
         lib.gizmo_host_set_active.argtypes = [C.POINTER(C.c_int), C.c_int]
+        lib.gizmo_kick_gas.argtypes = [C.c_double, C.c_int]
+        lib.gizmo_drift_gas.argtypes = [C.c_double]
+        lib.gizmo_gas_timestep.argtypes = [C.c_int]
+        lib.gizmo_gas_timestep.restype = C.c_double
+        lib.gizmo_global_timestep.restype = C.c_double
+        lib.gizmo_run_until.argtypes = [C.c_double, C.c_int]
+        lib.gizmo_box_side.argtypes = [C.c_int]
+        lib.gizmo_box_side.restype = C.c_double
+        lib.gizmo_gamma.restype = C.c_double
         handler_t = C.CFUNCTYPE(None, C.c_int, C.c_char_p)
         _handler = handler_t(lambda code, where: codes.append(code))
         lib.gizmo_host_set_endrun_handler(_handler)
@@ -75,6 +85,10 @@ def fill(lib, ic, state=False):
     P, S, A, _ = views(lib, N)
     A.BoxSize, A.DesNumNgb, A.MaxNumNgbDeviation = 1.0, 32.0, 0.05
     A.MinHsml, A.MaxHsml, A.Time, A.TimeBegin = 0.0, 1e56, 0.0, 0.0
+    A.CourantFac, A.MaxSizeTimestep, A.MinEgySpec, A.GasGamma = 0.2, 0.0, 0.0, 0.0
+    box = np.asarray(ic.get("box", (1.0, 1.0, 1.0)), dtype=np.float64)
+    A.BoxSize = float(box[0])
+    A.BoxLong[:] = list(box / box[0])
     P["Pos"] = ic["pos"]
     P["Vel"] = ic["vel"]
     S["VelPred"] = ic["vel"]
