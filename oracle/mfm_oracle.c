@@ -118,15 +118,133 @@ double orc_matrix_invert(const double Tin[9], double Tout[9])
 /* neighbour search: uniform cell grid (or brute force) + the exact FP64     */
 /* predicates of density.c:277-281 and hydro_evaluate.h:170-178.             */
 /* ------------------------------------------------------------------------- */
+/* Octree search mode (orc_params.tree = 1), the CPU search the reference
+ * runs (ngb_treefind_variable_threads / ngb_treefind_pairs_threads, ngb.c:102-130,
+ * with the node opening of system/ngb_codeblock_checknode.h:38-60 and node
+ * hsml maxima of force_update_hmax, forcetree_update.c:348-476): a cubic
+ * root of the largest box side, eight children per node, leaves of at most
+ * OT_BUCKET particles.  A node is opened when the particle lies within the
+ * search radius of its cube on every axis (nearest periodic image), with
+ * radius h_i for the density set and max(h_i, node hmax) for the pair set;
+ * every candidate then passes the exact predicates of nb_test, so the sets
+ * are those of the grid and of brute force. */
+#define OT_BUCKET 8
+typedef struct onode {
+    double c[3], half, hmax;
+    int child[8];  /* -1: none */
+    int first, count, leaf;
+} onode;
+
+typedef struct otree {
+    onode *node;
+    int nn, cap;
+    int *idx;      /* particles, grouped by leaf */
+} otree;
+
 typedef struct grid_t {
     int nc[3];
     double inv_s[3];
     int *start;  /* ncell+1 */
     int *idx;    /* n, ascending particle index inside every cell */
+    otree *tree; /* octree mode: the search structure (the grid is not built) */
 } grid_t;
 
-static void grid_build(grid_t *g, const orc_params *p, int n, const double *pos)
+static int ot_new(otree *t, const double c[3], double half, int first, int count)
 {
+    if (t->nn == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 1024;
+        t->node = (onode *)realloc(t->node, sizeof(onode) * (size_t)t->cap);
+    }
+    onode *o = &t->node[t->nn];
+    for (int a = 0; a < 3; a++) o->c[a] = c[a];
+    o->half = half;
+    o->hmax = 0;
+    for (int k = 0; k < 8; k++) o->child[k] = -1;
+    o->first = first;
+    o->count = count;
+    o->leaf = 1;
+    return t->nn++;
+}
+
+/* split node `id` (its particles idx[first, first+count)) into octants */
+static void ot_split(otree *t, int id, const double *pos, int *tmp, int depth)
+{
+    onode o = t->node[id];
+    if (o.count <= OT_BUCKET || depth >= 40) return;
+    int cnt[8] = {0}, off[9];
+    for (int k = 0; k < o.count; k++) {
+        const double *x = pos + 3 * (size_t)t->idx[o.first + k];
+        int oct = (x[0] >= o.c[0]) * 4 + (x[1] >= o.c[1]) * 2 + (x[2] >= o.c[2]);
+        cnt[oct]++;
+    }
+    off[0] = 0;
+    for (int q = 0; q < 8; q++) off[q + 1] = off[q] + cnt[q];
+    int fill[8];
+    for (int q = 0; q < 8; q++) fill[q] = off[q];
+    for (int k = 0; k < o.count; k++) {
+        int j = t->idx[o.first + k];
+        const double *x = pos + 3 * (size_t)j;
+        int oct = (x[0] >= o.c[0]) * 4 + (x[1] >= o.c[1]) * 2 + (x[2] >= o.c[2]);
+        tmp[fill[oct]++] = j;
+    }
+    memcpy(t->idx + o.first, tmp, sizeof(int) * (size_t)o.count);
+    t->node[id].leaf = 0;
+    for (int q = 0; q < 8; q++) {
+        if (!cnt[q]) continue;
+        double c[3] = {o.c[0] + ((q & 4) ? 0.5 : -0.5) * o.half, o.c[1] + ((q & 2) ? 0.5 : -0.5) * o.half,
+                       o.c[2] + ((q & 1) ? 0.5 : -0.5) * o.half};
+        int ch = ot_new(t, c, 0.5 * o.half, o.first + off[q], cnt[q]);
+        t->node[id].child[q] = ch;
+        ot_split(t, ch, pos, tmp, depth + 1);
+    }
+}
+
+static double ot_hmax(otree *t, int id, const double *hsml)
+{
+    onode *o = &t->node[id];
+    double m = 0;
+    if (o->leaf) {
+        for (int k = 0; k < o->count; k++) m = DMAX(m, hsml[t->idx[o->first + k]]);
+    } else {
+        for (int q = 0; q < 8; q++)
+            if (t->node[id].child[q] >= 0) m = DMAX(m, ot_hmax(t, t->node[id].child[q], hsml));
+    }
+    t->node[id].hmax = m;
+    return m;
+}
+
+static otree *ot_build(const orc_params *p, int n, const double *pos, const double *hsml)
+{
+    otree *t = (otree *)calloc(1, sizeof(otree));
+    t->idx = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *tmp = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) t->idx[i] = i;
+    double L = DMAX(p->box[0], DMAX(p->box[1], p->box[2]));
+    double c[3] = {0.5 * L, 0.5 * L, 0.5 * L};
+    int root = ot_new(t, c, 0.5 * L, 0, n);
+    ot_split(t, root, pos, tmp, 0);
+    if (hsml) ot_hmax(t, root, hsml);
+    free(tmp);
+    return t;
+}
+
+static void ot_free(otree *t)
+{
+    if (!t) return;
+    free(t->node);
+    free(t->idx);
+    free(t);
+}
+
+static void grid_build(grid_t *g, const orc_params *p, int n, const double *pos, const double *hsml)
+{
+    g->tree = NULL;
+    if (p->tree && !p->brute_force) {
+        g->tree = ot_build(p, n, pos, hsml);
+        g->start = NULL;
+        g->idx = NULL;
+        return;
+    }
     double vol = p->box[0] * p->box[1] * p->box[2];
     double s = cbrt(vol * 8.0 / (n > 0 ? n : 1)); /* ~8 particles per cell */
     long long ncell = 1;
@@ -163,6 +281,7 @@ static void grid_free(grid_t *g)
 {
     free(g->start);
     free(g->idx);
+    ot_free(g->tree);
 }
 
 /* signed nearest-image wrap, NEAREST_XYZ (allvars.h:1519,1550,1556-1559) */
@@ -235,6 +354,29 @@ static void nb_collect(nb_t *out, const orc_params *p, const grid_t *g, int n, c
         for (int j = 0; j < n; j++) nb_test(out, p, pos, hsml, i, j, h2_i, mode);
         return;
     }
+    if (g->tree) {
+        /* depth-first walk with an explicit stack */
+        const otree *t = g->tree;
+        int stack[8 * 64];
+        int sp = 0;
+        stack[sp++] = 0;
+        const double *xi = pos + 3 * (size_t)i;
+        while (sp > 0) {
+            const onode *o = &t->node[stack[--sp]];
+            const double r = (mode == 0) ? h_i : DMAX(h_i, o->hmax);
+            int far = 0;
+            for (int a = 0; a < 3 && !far; a++)
+                if (fabs(wrap1(xi[a] - o->c[a], p->box[a])) - o->half > r) far = 1;
+            if (far) continue;
+            if (o->leaf) {
+                for (int k = 0; k < o->count; k++) nb_test(out, p, pos, hsml, i, t->idx[o->first + k], h2_i, mode);
+            } else {
+                for (int q = 7; q >= 0; q--)
+                    if (o->child[q] >= 0) stack[sp++] = o->child[q];
+            }
+        }
+        return;
+    }
     int lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         double x = pos[3 * i + a];
@@ -282,7 +424,7 @@ long long orc_neighbor_counts(const orc_params *p, int n, const double *pos, con
                               const int *active, int n_active, int mode, int *counts)
 {
     grid_t g;
-    grid_build(&g, p, n, pos);
+    grid_build(&g, p, n, pos, hsml);
     double hmax = max_hsml(n, hsml);
     long long total = 0;
     set_threads(p);
@@ -314,7 +456,7 @@ long long orc_neighbor_lists(const orc_params *p, int n, const double *pos, cons
     free(counts);
     if (total > capacity) return -total;
     grid_t g;
-    grid_build(&g, p, n, pos);
+    grid_build(&g, p, n, pos, hsml);
     double hmax = max_hsml(n, hsml);
 #pragma omp parallel
     {
@@ -367,7 +509,7 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
     }
 
     grid_t g;
-    grid_build(&g, p, n, pos);
+    grid_build(&g, p, n, pos, hsml);
     int iter = 0;
     long long dens_int = 0;
     long long ntot;
@@ -673,7 +815,7 @@ int orc_gradients(const orc_params *p, orc_state *s, const int *active, int n_ac
 {
     const int n = s->n;
     grid_t g;
-    grid_build(&g, p, n, s->pos);
+    grid_build(&g, p, n, s->pos, s->hsml);
     double hmax = max_hsml(n, s->hsml);
     long long nint = 0;
     set_threads(p);
@@ -1381,7 +1523,7 @@ int orc_hydro_force(const orc_params *p, orc_state *s, const int *active, int n_
     const int n = s->n;
     const gconst c = make_gconst(p->gamma);
     grid_t g;
-    grid_build(&g, p, n, s->pos);
+    grid_build(&g, p, n, s->pos, s->hsml);
     double hmax = max_hsml(n, s->hsml);
     long long nint = 0, nretry = 0, nexact = 0;
     int status = 0;

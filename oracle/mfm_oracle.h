@@ -32,7 +32,9 @@ typedef struct orc_params {
     int first_step;         /* All.Time == All.TimeBegin (density.c:509) */
     int maxiter;            /* MAXITER (allvars.h:1294) */
     int nthreads;           /* OpenMP threads (<=0: runtime default) */
-    int brute_force;        /* 1: O(N^2) neighbour search (truth), 0: cell grid */
+    int brute_force;        /* 1: O(N^2) neighbour search (truth), 0: cell grid or octree */
+    int tree;               /* 1: octree walk with node hsml maxima, the reference's search
+                               (ngb.c:102-130); 0: cell grid */
 } orc_params;
 
 typedef struct orc_state {

@@ -32,6 +32,7 @@ class Params(C.Structure):
         ("maxiter", C.c_int),
         ("nthreads", C.c_int),
         ("brute_force", C.c_int),
+        ("tree", C.c_int),
     ]
 
 
@@ -107,7 +108,7 @@ def ptr(a: np.ndarray, t=_dp):
 
 def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_ngb_dev=0.05,
                 min_hsml=0.0, max_hsml=1e56, first_step=1, maxiter=150, nthreads=0,
-                brute_force=0) -> Params:
+                brute_force=0, tree=0) -> Params:
     p = Params()
     p.box[:] = list(box)
     p.gamma = gamma
@@ -119,6 +120,7 @@ def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_
     p.maxiter = maxiter
     p.nthreads = nthreads
     p.brute_force = brute_force
+    p.tree = tree
     return p
 
 

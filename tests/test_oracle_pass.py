@@ -37,10 +37,26 @@ def test_grid_neighbor_sets_equal_brute_force(mode):
     pos[60] = pos[61]                                       # coincident pair
     hsml = rng.uniform(0.03, 0.12, size=N)
     ref = brute_neighbors(pos, hsml, mode)
-    for brute in (0, 1):
-        offs, idx = O.neighbor_lists(pos, hsml, mode, params=O.make_params(brute_force=brute))
+    for brute, tree in ((0, 0), (1, 0), (0, 1)):
+        offs, idx = O.neighbor_lists(pos, hsml, mode, params=O.make_params(brute_force=brute, tree=tree))
         for i in range(N):
             np.testing.assert_array_equal(idx[offs[i]:offs[i + 1]], ref[i])
+
+
+@pytest.mark.parametrize("maker", [lambda: ics.random_state(12), lambda: ics.plummer_clusters(6000, 6)])
+def test_octree_search_pass_equals_grid_pass(maker):
+    """The octree walk (the reference's search, CPU baseline mode) finds the
+    same sets as the grid, so the pass takes the same sweeps and interaction
+    counts; outputs agree to summation order."""
+    from parity import check_pass
+
+    ic = maker()
+    ref, st, _ = O.run_pass(ic)
+    out, st2, _ = O.run_pass(ic, params=O.make_params(tree=1))
+    assert (st2.iterations, st2.dens_interactions, st2.grad_interactions, st2.force_interactions) == \
+           (st.iterations, st.dens_interactions, st.grad_interactions, st.force_interactions)
+    errs, bad = check_pass(out, ref)
+    assert not bad, bad
 
 
 def _pass(ic, **kw):
