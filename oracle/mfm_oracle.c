@@ -488,6 +488,111 @@ static inline double ncorr_from_cn(double cn)
     return ncorr_ngb;
 }
 
+/* One hsml update of the iteration body (density.c:594-846) from the sweep's
+ * normalised neighbour number and DhsmlNgbFactor, with the sweep's target
+ * desnumngb and window half-width desnumngbdev (before its growth with the
+ * sweep number, which this applies).  Updates h and
+ * the brackets Left/Right; returns HSML_DONE (inside the window, or held at
+ * Min/MaxHsml), HSML_BRACKETED (the bracket is narrower than 1e-3 of Left:
+ * accepted as it is), HSML_COLLAPSED (no bracket at all: density.c:780-783
+ * ends the run) or HSML_AGAIN (sweep again with the new h). */
+enum { HSML_DONE = 0, HSML_AGAIN = 1, HSML_BRACKETED = 2, HSML_COLLAPSED = -1 };
+
+static int hsml_step(double NumNgb, double Dhsml, double *h_io, double *Left_io, double *Right_io, int iter,
+                     double desnumngb, double desnumngbdev, double minsoft, double maxsoft)
+{
+    double h = *h_io, Left = *Left_io, Right = *Right_io;
+    int redo = 0, maxflag = 0, minflag = 0, result = HSML_AGAIN;
+    /* the window grows with the sweep so that the iteration ends (density.c:594) */
+    if (iter > 1)
+        desnumngbdev = DMIN(0.25 * desnumngb, desnumngbdev * exp(0.1 * log(desnumngb / (16. * desnumngbdev)) * (double)iter));
+    if ((NumNgb < (desnumngb - desnumngbdev) && h < 0.999 * maxsoft) ||
+        (NumNgb > (desnumngb + desnumngbdev) && h > 1.001 * minsoft)) redo = 1;
+    if ((h >= 0.999 * maxsoft) && (NumNgb < (desnumngb - desnumngbdev))) {
+        redo = 0;
+        if (h == maxsoft) maxflag = 0;
+        else { redo = 1; h = maxsoft; maxflag = 1; }
+    }
+    if ((h <= 1.001 * minsoft) && (NumNgb > (desnumngb + desnumngbdev))) {
+        redo = 0;
+        if (h == minsoft) minflag = 0;
+        else { redo = 1; h = minsoft; minflag = 1; }
+    }
+    if (!redo) return HSML_DONE;
+    if (Left > 0 && Right > 0 && (Right - Left) < 1.0e-3 * Left) { result = HSML_BRACKETED; goto out; }
+    if ((maxflag == 0) && (minflag == 0)) {
+        if (NumNgb < (desnumngb - desnumngbdev)) Left = DMAX(h, Left);
+        else {
+            if (Right != 0) { if (h < Right) Right = h; }
+            else Right = h;
+        }
+        if (Right > 0 && Left > 0) {
+            double maxjump = 0;
+            if (iter > 1) maxjump = 0.2 * log(Right / Left);
+            if (NumNgb > 1) {
+                double jumpvar = Dhsml * log(desnumngb / NumNgb) / NUMDIMS;
+                if (iter > 1) { if (fabs(jumpvar) < maxjump) { if (jumpvar < 0) jumpvar = -maxjump; else jumpvar = maxjump; } }
+                h *= exp(jumpvar);
+            } else {
+                h *= 2.0;
+            }
+            if ((h < Right) && (h > Left)) {
+                if (iter > 1) {
+                    double hfac = exp(maxjump);
+                    if (h > Right / hfac) h = Right / hfac;
+                    if (h < Left * hfac) h = Left * hfac;
+                }
+            } else {
+                if (h > Right) h = Right;
+                if (h < Left) h = Left;
+                h = pow(h * Left * Right, 1.0 / 3.0);
+            }
+        } else {
+            if (Right == 0 && Left == 0) { result = HSML_COLLAPSED; goto out; }
+            if (Right == 0 && Left > 0) {
+                double fac_lim, fac;
+                if (NumNgb > 1) fac_lim = log(desnumngb / NumNgb) / NUMDIMS; else fac_lim = 1.4;
+                if ((NumNgb < 2 * desnumngb) && (NumNgb > 0.1 * desnumngb)) {
+                    double slope = Dhsml;
+                    if (iter > 2 && slope < 1) slope = 0.5 * (slope + 1);
+                    fac = fac_lim * slope;
+                    if (iter >= 4) { if (Dhsml == 1) fac *= 10; }
+                    if (fac < fac_lim + 0.231) h *= exp(fac);
+                    else h *= exp(fac_lim + 0.231);
+                } else h *= exp(fac_lim);
+            }
+            if (Right > 0 && Left == 0) {
+                double fac_lim, fac;
+                if (NumNgb > 1) fac_lim = log(desnumngb / NumNgb) / NUMDIMS; else fac_lim = 1.4;
+                if (fac_lim < -1.535) fac_lim = -1.535;
+                if ((NumNgb < 2 * desnumngb) && (NumNgb > 0.1 * desnumngb)) {
+                    double slope = Dhsml;
+                    if (iter > 2 && slope < 1) slope = 0.5 * (slope + 1);
+                    fac = fac_lim * slope;
+                    if (iter >= 4) { if (Dhsml == 1) fac *= 10; }
+                    if (fac > fac_lim - 0.231) h *= exp(fac);
+                    else h *= exp(fac_lim - 0.231);
+                } else h *= exp(fac_lim);
+            }
+        }
+    }
+    if (h < minsoft) h = minsoft;
+    if (minflag == 1) h = minsoft;
+    if (h > maxsoft) h = maxsoft;
+    if (maxflag == 1) h = maxsoft;
+out:
+    *h_io = h;
+    *Left_io = Left;
+    *Right_io = Right;
+    return result;
+}
+
+int orc_hsml_step(double num_ngb, double dhsml, double *h, double *left, double *right, int iter,
+                  double des_num_ngb, double des_num_ngb_dev, double min_hsml, double max_hsml)
+{
+    return hsml_step(num_ngb, dhsml, h, left, right, iter, des_num_ngb, des_num_ngb_dev, min_hsml, max_hsml);
+}
+
 int orc_density(const orc_params *p, orc_state *s, const int *active, int n_active, orc_stats *st)
 {
     const int n = s->n;
@@ -625,97 +730,20 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                 }
                 desnumngb = p->des_num_ngb * ncorr_ngb;
                 desnumngbdev = desnumngbdev_0 * ncorr_ngb;
-                if (iter > 1)
-                    desnumngbdev = DMIN(0.25 * desnumngb,
-                                        desnumngbdev * exp(0.1 * log(desnumngb / (16. * desnumngbdev)) * (double)iter));
 
-                int redo = 0, maxflag = 0, minflag = 0;
-                if ((NumNgb < (desnumngb - desnumngbdev) && h < 0.999 * maxsoft) ||
-                    (NumNgb > (desnumngb + desnumngbdev) && h > 1.001 * minsoft)) redo = 1;
-                if ((h >= 0.999 * maxsoft) && (NumNgb < (desnumngb - desnumngbdev))) {
-                    redo = 0;
-                    if (h == maxsoft) maxflag = 0;
-                    else { redo = 1; h = maxsoft; maxflag = 1; }
-                }
-                if ((h <= 1.001 * minsoft) && (NumNgb > (desnumngb + desnumngbdev))) {
-                    redo = 0;
-                    if (h == minsoft) minflag = 0;
-                    else { redo = 1; h = minsoft; minflag = 1; }
-                }
-                if (redo == 0) cn_stored = ConditionNumber; /* density.c:705-719 */
-
-                if (redo) {
-                    npleft++;
-                    if (Left[i] > 0 && Right[i] > 0 && (Right[i] - Left[i]) < 1.0e-3 * Left[i]) {
-                        npleft--;
+                {
+                    const int r = hsml_step(NumNgb, Dhsml, &h, &Left[i], &Right[i], iter, desnumngb, desnumngbdev,
+                                            minsoft, maxsoft);
+                    if (r == HSML_DONE || r == HSML_BRACKETED) {
                         done[i] = 1;
-                        cn_stored = ConditionNumber;
-                        goto store;
+                        cn_stored = ConditionNumber; /* density.c:705-719, 737-738 */
+                    } else if (r == HSML_COLLAPSED) {
+                        npleft++;
+                        bad++;
+                    } else {
+                        npleft++;
                     }
-                    if ((maxflag == 0) && (minflag == 0)) {
-                        if (NumNgb < (desnumngb - desnumngbdev)) Left[i] = DMAX(h, Left[i]);
-                        else {
-                            if (Right[i] != 0) { if (h < Right[i]) Right[i] = h; }
-                            else Right[i] = h;
-                        }
-                        if (Right[i] > 0 && Left[i] > 0) {
-                            double maxjump = 0;
-                            if (iter > 1) maxjump = 0.2 * log(Right[i] / Left[i]);
-                            if (NumNgb > 1) {
-                                double jumpvar = Dhsml * log(desnumngb / NumNgb) / NUMDIMS;
-                                if (iter > 1) { if (fabs(jumpvar) < maxjump) { if (jumpvar < 0) jumpvar = -maxjump; else jumpvar = maxjump; } }
-                                h *= exp(jumpvar);
-                            } else {
-                                h *= 2.0;
-                            }
-                            if ((h < Right[i]) && (h > Left[i])) {
-                                if (iter > 1) {
-                                    double hfac = exp(maxjump);
-                                    if (h > Right[i] / hfac) h = Right[i] / hfac;
-                                    if (h < Left[i] * hfac) h = Left[i] * hfac;
-                                }
-                            } else {
-                                if (h > Right[i]) h = Right[i];
-                                if (h < Left[i]) h = Left[i];
-                                h = pow(h * Left[i] * Right[i], 1.0 / 3.0);
-                            }
-                        } else {
-                            if (Right[i] == 0 && Left[i] == 0) { bad++; goto store; } /* density.c:780-783 terminate */
-                            if (Right[i] == 0 && Left[i] > 0) {
-                                double fac_lim, fac;
-                                if (NumNgb > 1) fac_lim = log(desnumngb / NumNgb) / NUMDIMS; else fac_lim = 1.4;
-                                if ((NumNgb < 2 * desnumngb) && (NumNgb > 0.1 * desnumngb)) {
-                                    double slope = Dhsml;
-                                    if (iter > 2 && slope < 1) slope = 0.5 * (slope + 1);
-                                    fac = fac_lim * slope;
-                                    if (iter >= 4) { if (Dhsml == 1) fac *= 10; }
-                                    if (fac < fac_lim + 0.231) h *= exp(fac);
-                                    else h *= exp(fac_lim + 0.231);
-                                } else h *= exp(fac_lim);
-                            }
-                            if (Right[i] > 0 && Left[i] == 0) {
-                                double fac_lim, fac;
-                                if (NumNgb > 1) fac_lim = log(desnumngb / NumNgb) / NUMDIMS; else fac_lim = 1.4;
-                                if (fac_lim < -1.535) fac_lim = -1.535;
-                                if ((NumNgb < 2 * desnumngb) && (NumNgb > 0.1 * desnumngb)) {
-                                    double slope = Dhsml;
-                                    if (iter > 2 && slope < 1) slope = 0.5 * (slope + 1);
-                                    fac = fac_lim * slope;
-                                    if (iter >= 4) { if (Dhsml == 1) fac *= 10; }
-                                    if (fac > fac_lim - 0.231) h *= exp(fac);
-                                    else h *= exp(fac_lim - 0.231);
-                                } else h *= exp(fac_lim);
-                            }
-                        }
-                    }
-                    if (h < minsoft) h = minsoft;
-                    if (minflag == 1) h = minsoft;
-                    if (h > maxsoft) h = maxsoft;
-                    if (maxflag == 1) h = maxsoft;
-                } else {
-                    done[i] = 1;
                 }
-            store:
                 /* out2particle assignments of this sweep (density.c:177-244) plus the
                  * in-place updates of the iteration body */
                 hsml[i] = h;

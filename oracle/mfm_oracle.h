@@ -123,6 +123,15 @@ void   orc_pstar_estimate(double gamma, const double L[5], const double R[5], co
 int    orc_wavespeeds(double gamma, const double L[5], const double R[5], const double n[3],
                       double press_tot_limiter, double out[4]);
 
+/* One hsml update of density()'s iteration (density.c:594-846) from the
+ * sweep's normalised neighbour number and DhsmlNgbFactor, for the target
+ * des_num_ngb and window half-width des_num_ngb_dev (grown with the sweep
+ * number iter here, as the reference does).  Updates h and the
+ * brackets; returns 0 converged (or held at Min/MaxHsml), 1 sweep again,
+ * 2 accepted by the bracket-width test, -1 no bracket (the run ends). */
+int    orc_hsml_step(double num_ngb, double dhsml, double *h, double *left, double *right, int iter,
+                     double des_num_ngb, double des_num_ngb_dev, double min_hsml, double max_hsml);
+
 /* ---- the three stages (density.c:451, gradients.c:577, hydro_toplevel.c:1045) ---- */
 int  orc_density(const orc_params *p, orc_state *s, const int *active, int n_active, orc_stats *st);
 int  orc_gradients(const orc_params *p, orc_state *s, const int *active, int n_active, orc_stats *st);

@@ -90,6 +90,9 @@ def lib():
         L.orc_pstar_estimate.restype = None
         L.orc_wavespeeds.argtypes = [C.c_double, _dp, _dp, _dp, C.c_double, _dp]
         L.orc_wavespeeds.restype = C.c_int
+        L.orc_hsml_step.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_double]
+        L.orc_hsml_step.restype = C.c_int
         for f in (L.orc_density, L.orc_gradients, L.orc_hydro_force):
             f.argtypes = [C.POINTER(Params), C.POINTER(State), _ip, C.c_int, C.POINTER(Stats)]
             f.restype = C.c_int
@@ -194,6 +197,16 @@ def wavespeeds(L, R, n=(1.0, 0.0, 0.0), press_tot_limiter=1e300, gamma=5.0 / 3.0
     out = np.zeros(4)
     which = lib().orc_wavespeeds(gamma, ptr(Lc), ptr(Rc), ptr(nc), press_tot_limiter, ptr(out))
     return dict(P_M=out[0], S_M=out[1], S_L=out[2], S_R=out[3], which=which)
+
+
+HSML_DONE, HSML_AGAIN, HSML_BRACKETED, HSML_COLLAPSED = 0, 1, 2, -1
+
+
+def hsml_step(num_ngb, dhsml, h, left=0.0, right=0.0, iter=0, des=32.0, dev=0.05, min_hsml=0.0, max_hsml=1e56):
+    """One hsml update of density()'s iteration: (result, h, left, right)."""
+    hh, ll, rr = C.c_double(h), C.c_double(left), C.c_double(right)
+    r = lib().orc_hsml_step(num_ngb, dhsml, C.byref(hh), C.byref(ll), C.byref(rr), iter, des, dev, min_hsml, max_hsml)
+    return r, hh.value, ll.value, rr.value
 
 
 # ---------------------------------------------------------------------------

@@ -9,6 +9,7 @@ Reference functions covered (restated in oracle/mfm_oracle.c):
   face volume-weight switch, fallback hydro/compute_finitevol_faces.h:21,96-104
   MFM entropic switch                 hydro/hydro_core_meshless.h:335-374
   ncorr from ConditionNumber and FCE  hydro/density.c:565-595
+  hsml update of the iteration        hydro/density.c:655-846
 """
 import math
 
@@ -432,3 +433,74 @@ def test_face_closure_error_raises_the_neighbour_target():
     np.testing.assert_array_equal(corner, edge.all(axis=1))  # exactly the 8 corners
     assert np.all(n_eff[corner] > 32.0 * 1.1)
     assert np.all(np.abs(n_eff[~corner] - 32.0) <= dev[~corner] * (1 + 1e-12))
+
+
+# ------------------------------------------------------ the hsml update -----
+def test_hsml_step_inside_the_window_keeps_hsml():
+    r, h, l, rr = O.hsml_step(32.02, 1.0, 0.7, 0.6, 0.8)
+    assert (r, h, l, rr) == (O.HSML_DONE, 0.7, 0.6, 0.8)
+
+
+@pytest.mark.parametrize("n_ngb,dhsml,expect", [
+    # without a bracket the step is the Newton step of a uniform medium,
+    # N ~ h^3: h -> h (32/N)^(1/3) (slope DhsmlNgbFactor = 1) ...
+    (16.0, 1.0, 2.0 ** (1 / 3)),
+    (64.0, 1.0, 0.5 ** (1 / 3)),
+    # ... with a slope the step scales, capped 0.231 beyond the uniform step
+    (16.0, 0.5, 2.0 ** (1 / 6)),
+    (4.0, 2.0, 2.0 * math.exp(0.231)),
+    (16.0, 2.15, 2.0 ** (1 / 3) * math.exp(0.231)),
+    (60.0, 3.0, (32 / 60) ** (1 / 3) * math.exp(-0.231)),
+    # outside (0.1, 2) x 32 the uniform step alone, shrinking by at most e^-1.535
+    (2.0, 5.0, 16.0 ** (1 / 3)),
+    (100.0, 5.0, (32 / 100) ** (1 / 3)),
+    (1e6, 1.0, math.exp(-1.535)),
+    # fewer than one neighbour: e^1.4
+    (0.5, 1.0, math.exp(1.4)),
+])
+def test_hsml_step_one_sided(n_ngb, dhsml, expect):
+    r, h, l, rr = O.hsml_step(n_ngb, dhsml, 1.0)
+    assert r == O.HSML_AGAIN
+    assert h == pytest.approx(expect, rel=1e-14)
+    # the current hsml becomes the bracket on its side
+    assert (l, rr) == ((1.0, 0.0) if n_ngb < 32 else (0.0, 1.0))
+
+
+def test_hsml_step_bracketed():
+    # inside the bracket: the Newton step of a uniform medium, accepted
+    r, h, l, rr = O.hsml_step(30.0, 1.0, 1.0, 0.9, 1.2)
+    assert (r, l, rr) == (O.HSML_AGAIN, 1.0, 1.2) and h == pytest.approx((32 / 30) ** (1 / 3), rel=1e-14)
+    # past the bracket: the step is clamped to it and replaced by the geometric
+    # mean of (clamped h, Left, Right) = (1.1 * 1.0 * 1.1)^(1/3)
+    r, h, l, rr = O.hsml_step(20.0, 1.0, 1.0, 0.9, 1.1)
+    assert h == pytest.approx(1.21 ** (1 / 3), rel=1e-14)
+    # from the third sweep a step smaller than a fifth of the bracket's log
+    # width is widened to it: h = Left * (Right/Left)^0.2
+    r, h, l, rr = O.hsml_step(31.5, 1.0, 1.0, 0.95, 1.5, iter=2)
+    assert (l, rr) == (1.0, 1.5) and h == pytest.approx(1.5 ** 0.2, rel=1e-14)
+    # a bracket narrower than 1e-3 of Left is accepted as it stands
+    assert O.hsml_step(30.0, 1.0, 1.0002, 1.0, 1.0005) == (O.HSML_BRACKETED, 1.0002, 1.0, 1.0005)
+
+
+def test_hsml_step_window_grows_with_the_sweep():
+    """The window half-width grows from the third sweep on as
+    dev (des / (16 dev))^(0.1 iter), at most des/4 (density.c:594): for
+    des = 32, dev = 0.05 that is 0.05 * 40^(0.1 iter) -- 0.151 at sweep 3,
+    0.455 at sweep 6 and the cap of 8 from sweep 15."""
+    for it, width in ((1, 0.05), (3, 0.05 * 40 ** 0.3), (6, 0.05 * 40 ** 0.6), (15, 8.0)):
+        inside, outside = 32.0 + 0.98 * width, 32.0 - 1.02 * width
+        assert O.hsml_step(inside, 1.0, 1.0, iter=it)[0] == O.HSML_DONE, it
+        assert O.hsml_step(outside, 1.0, 1.0, iter=it)[0] == O.HSML_AGAIN, it
+
+
+def test_hsml_step_limits_and_collapse():
+    # below the target at MaxHsml: set to MaxHsml and swept once more, then kept
+    r, h, _, _ = O.hsml_step(10.0, 1.0, 0.9995, max_hsml=1.0)
+    assert (r, h) == (O.HSML_AGAIN, 1.0)
+    assert O.hsml_step(10.0, 1.0, 1.0, max_hsml=1.0)[0] == O.HSML_DONE
+    # above the target at MinHsml: likewise
+    r, h, _, _ = O.hsml_step(90.0, 1.0, 0.1001, min_hsml=0.1)
+    assert (r, h) == (O.HSML_AGAIN, 0.1)
+    assert O.hsml_step(90.0, 1.0, 0.1, min_hsml=0.1)[0] == O.HSML_DONE
+    # no bracket on either side (h = 0): the iteration cannot go on
+    assert O.hsml_step(10.0, 1.0, 0.0)[0] == O.HSML_COLLAPSED
