@@ -10,6 +10,7 @@ Reference functions covered (restated in oracle/mfm_oracle.c):
   MFM entropic switch                 hydro/hydro_core_meshless.h:335-374
   ncorr from ConditionNumber and FCE  hydro/density.c:565-595
   hsml update of the iteration        hydro/density.c:655-846
+  WAKEUP flags of inactive neighbours hydro/hydro_evaluate.h:485-495
 """
 import math
 
@@ -504,3 +505,55 @@ def test_hsml_step_limits_and_collapse():
     assert O.hsml_step(90.0, 1.0, 0.1, min_hsml=0.1)[0] == O.HSML_DONE
     # no bracket on either side (h = 0): the iteration cannot go on
     assert O.hsml_step(10.0, 1.0, 0.0)[0] == O.HSML_COLLAPSED
+
+
+# ------------------------------------------------------------- WAKEUP -------
+def test_wakeup_flags_inactive_neighbours_of_a_fast_signal():
+    """An inactive neighbour j of an active particle is flagged when the
+    pair's signal speed exceeds WAKEUP = 4.1 times j's stored MaxSignalVel
+    (hydro_evaluate.h:485-495, allvars.h:78).  A perfect 10^3 lattice at
+    rest with uniform u: every pair's signal speed is c_i + c_j = 2c, the
+    MaxSignalVel of the full pass.  The active set is the slab x < 0.2; the
+    inactive particles store 2c/4.11 (flagged beside an active particle) or
+    2c/4.09 (not flagged) by the parity of their y row, and so do the active
+    ones, which are never flagged.  One active particle moves along -y at
+    0.1c: its pairs with neighbours ahead of it approach, and the signal speed
+    2c + 0.1c cos(theta) >= 2.05c passes 4.1 * 2c/4.09 = 2.005c, while the
+    pairs beside and behind it (no approach) do not, whatever the particle's
+    largest signal speed.  Inactive particles out of every active kernel are
+    never flagged."""
+    n = 10
+    ic = ics.lattice_gaussian(n, sigma_v=0.0, jitter=0.0)
+    full, _, _ = O.run_pass(ic)
+    h = full["hsml"]
+    assert np.ptp(h) == 0.0
+    c = np.sqrt(GAMMA * full["pressure"] / full["density"])
+    np.testing.assert_allclose(full["vsig"], 2 * c, rtol=1e-14)
+
+    pos = ic["pos"]
+    N = pos.shape[0]
+    ix, iy = np.rint(pos[:, 0] * n - 0.5).astype(int), np.rint(pos[:, 1] * n - 0.5).astype(int)
+    active = np.flatnonzero(ix < 2).astype(np.int32)
+    is_active = ix < 2
+    k = int(np.flatnonzero((ix == 1) & (iy == 4) & (np.rint(pos[:, 2] * n - 0.5) == 5))[0])
+    state = dict(ic)
+    state.update({key: full[key] for key in full if not key.endswith("_l1") and key != "wakeup"})
+    slow = iy % 2 == 0
+    state["vsig"] = np.where(slow, full["vsig"] / 4.11, full["vsig"] / 4.09)
+    vel = np.zeros((N, 3))
+    vel[k, 1] = -0.1 * c[k]
+    state["vel"] = vel
+    out, _, _ = O.run_pass(state, active=active)
+
+    d = pos[None, :, :] - pos[active][:, None, :]
+    d -= np.rint(d)                                   # periodic unit box
+    r2 = (d ** 2).sum(axis=2)
+    assert np.abs(r2 - h[0] ** 2).min() > 1e-9       # no pair on the kernel edge
+    inside = (r2 < h[0] ** 2) & (r2 > 0)
+    near_active = inside.any(axis=0)
+    row_k = int(np.flatnonzero(active == k)[0])
+    ahead_of_k = inside[row_k] & (d[row_k, :, 1] < 0)
+    want = ~is_active & near_active & (slow | ahead_of_k)
+    assert want.sum() > 50 and (want & ~slow).sum() == 3  # x row 2, y row 3
+    assert (~is_active & near_active & ~slow & ~ahead_of_k).sum() > 50
+    np.testing.assert_array_equal(out["wakeup"].astype(bool), want)
