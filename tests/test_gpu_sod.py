@@ -31,17 +31,29 @@ def test_sod_shock_tube_on_the_device(lib):
     errs = sod_errors(P, S, 0.1, ic["gamma"])
     print(f"device: steps={steps} L1 rho, u, p = {errs} dE/E={(e1 - e0) / e0:.2e}")
     assert errs[0] < SOD_L1["density"] and errs[1] < SOD_L1["velocity"] and errs[2] < SOD_L1["pressure"]
-    gpu = {"pos": P["Pos"].copy(), "rho": S["Density"].copy(), "vel": P["Vel"].copy(), "steps": steps}
     lib.gizmo_host_release()
     lib.gizmo_host_free()
-    # the same loop with the oracle pass: the same number of steps and, after
-    # ~90 steps through a shock, the same fields to well inside the
-    # discretisation error (rounding differences of the two passes grow
-    # where the hsml iteration or a limiter decides on a threshold)
+
+
+def test_sod_device_loop_follows_the_oracle_loop(lib):
+    """The device loop and the oracle loop step for step to t = 0.02 (18
+    steps).  The comparison stops early because the reference algorithm itself
+    amplifies rounding: in the oracle's own loop a relative change of 1e-15 in
+    the y positions grows to a mean density change of 9e-8 (max 6e-6, positions
+    1.2e-6) by t = 0.02 and of 8e-3 (max 0.42, positions 0.06) by t = 0.1, as
+    particles cross the hsml tolerance window and the limiter's switches in
+    another order.  The bounds are ten times the t = 0.02 figures."""
     ic, N, P, S, A = sod_setup(lib, 64, 8)
-    steps = run_sod_with_oracle(lib, ic, 0.1)
-    assert steps == gpu["steps"]
-    d_rho = np.abs(gpu["rho"] - S["Density"]).mean() / S["Density"].mean()
-    d_x = np.abs(gpu["pos"] - P["Pos"]).max()
-    print(f"device vs oracle loop: mean |d rho|/rho {d_rho:.2e}, max |dx| {d_x:.2e}")
-    assert d_rho < 1e-3 and d_x < 1e-4
+    steps = lib.gizmo_run_until(0.02, 400)
+    assert steps == 18 and lib.gizmo_host_last_status() == 0
+    gpu = {"pos": P["Pos"].copy(), "rho": S["Density"].copy()}
+    lib.gizmo_host_release()
+    lib.gizmo_host_free()
+    ic, N, P, S, A = sod_setup(lib, 64, 8)
+    assert run_sod_with_oracle(lib, ic, 0.02) == steps
+    d_rho = np.abs(gpu["rho"] - S["Density"]) / S["Density"]
+    d_x = np.abs(gpu["pos"] - P["Pos"])
+    d_x = np.minimum(d_x, np.asarray(ic["box"]) - d_x)  # periodic images
+    print(f"device vs oracle loop at t=0.02: mean |d rho|/rho {d_rho.mean():.2e} (max {d_rho.max():.2e}), "
+          f"max |dx| {d_x.max():.2e}")
+    assert d_rho.mean() < 1e-6 and d_rho.max() < 6e-5 and d_x.max() < 1.2e-5
