@@ -14,6 +14,7 @@
 #include <rocprim/device/device_select.hpp>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/gzm.h"
@@ -1230,9 +1231,106 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     return GZM_OK;
 }
 
+// Arguments of the two Riemann-state entry points, checked before any device
+// work; G receives the gamma constants.
+gzm_status riemann_args(int32_t device, int32_t mode, int32_t precision, double gamma, int64_t n, int64_t n_max,
+                        const double *L, const double *R, const double *normal, const double *limiter,
+                        const double *out, const int32_t *mask, GammaConsts &G)
+{
+    if (!L || !R || !normal || !out || !mask || (mode == 0 && !limiter)) { set_error("NULL argument"); return GZM_ERR_ARG; }
+    if (mode != 0 && mode != 1) { set_error("mode must be 0 (Riemann_solver) or 1 (KT flux)"); return GZM_ERR_ARG; }
+    if (precision < 0 || precision > 2) { set_error("precision must be 0, 1 or 2"); return GZM_ERR_ARG; }
+    if (!(gamma > 1)) { set_error("gamma must be > 1"); return GZM_ERR_ARG; }
+    if (n < 0 || n > n_max) {
+        set_error("n=%lld out of range [0, %lld]", (long long)n, (long long)n_max);
+        return GZM_ERR_ARG;
+    }
+    if (n == 0) return GZM_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { set_error("no HIP device"); return GZM_ERR_NO_DEVICE; }
+    if (device < 0 || device >= ndev) { set_error("device %d not present", device); return GZM_ERR_ARG; }
+    GZM_HIP(hipSetDevice(device));
+    G.G0 = gamma;
+    G.G1 = ((gamma - 1.0) / (2.0 * gamma));
+    G.G2 = ((gamma + 1.0) / (2.0 * gamma));
+    G.G3 = ((2.0 * gamma / (gamma - 1.0)));
+    G.G4 = (2.0 / (gamma - 1.0));
+    G.G5 = (2.0 / (gamma + 1.0));
+    G.G6 = ((gamma - 1.0) / (gamma + 1.0));
+    G.G7 = (0.5 * (gamma - 1.0));
+    G.G8 = (1.0 / gamma);
+    G.G9 = (gamma - 1.0);
+    return GZM_OK;
+}
+
+// doubles of device memory per face state: L[5], R[5], normal[3], limiter, out[6]
+constexpr size_t kRiemannDoubles = 5 + 5 + 3 + 1 + 6;
+
+// Copy n face states into the device buffers d (n * kRiemannDoubles doubles)
+// and dm (n ints), solve them on the null stream, copy the results back.
+gzm_status riemann_run(int32_t mode, int32_t precision, const GammaConsts &G, int64_t n, const double *L,
+                       const double *R, const double *normal, const double *limiter, double *out, int32_t *mask,
+                       double *d, int *dm)
+{
+    const size_t nn = (size_t)n;
+    double *dL = d, *dR = d + 5 * nn, *dn = d + 10 * nn, *dlim = d + 13 * nn, *dout = d + 14 * nn;
+    hipError_t e = hipMemcpy(dL, L, sizeof(double) * 5 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dR, R, sizeof(double) * 5 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, normal, sizeof(double) * 3 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess && limiter) e = hipMemcpy(dlim, limiter, sizeof(double) * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_riemann_states(nullptr, mode, precision, G, n, dL, dR, dn, dlim, dout, dm);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * 6 * nn, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(mask, dm, sizeof(int) * nn, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { set_error("Riemann states: %s", hipGetErrorString(e)); return GZM_ERR_HIP; }
+    return GZM_OK;
+}
+
+// Largest n of gzm_riemann_states_noalloc, and the size of its staging area:
+// the face states, results and masks of one call (kNoAllocStates * (20
+// doubles + 1 int) = 10 MiB + 256 KiB), which every device the library
+// runs on holds until the process ends.
+constexpr int64_t kNoAllocStates = GZM_RIEMANN_NOALLOC_MAX;
+constexpr size_t kNoAllocDoubles = (size_t)kNoAllocStates * kRiemannDoubles + (size_t)kNoAllocStates / 2;
+
+}  // namespace
+
+// Device array behind gzm_riemann_states_noalloc.  In host code this name is a
+// host-side stand-in, not a device address: host code uses only the address
+// that hipGetSymbolAddress returns for the current device (scratch_of_device).
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current: looked
+// up on the first call on that device, then kept per device.
+gzm_status scratch_of_device(int device, double **ptr)
+{
+    if (device >= kMaxScratchDevices) { set_error("device %d beyond %d", device, kMaxScratchDevices); return GZM_ERR_ARG; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+            return GZM_ERR_HIP;
+        }
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *ptr = g_scratch_ptr[device];
+    return GZM_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
 
 gzm_status gzm_upload(gzm_ctx *c, const gzm_host_view *v)
 {
@@ -1469,50 +1567,35 @@ gzm_status gzm_riemann_states(int32_t device, int32_t mode, int32_t precision, d
                               const double *L, const double *R, const double *normal, const double *limiter,
                               double *out, int32_t *mask)
 {
-    if (!L || !R || !normal || !out || !mask || (mode == 0 && !limiter)) { set_error("NULL argument"); return GZM_ERR_ARG; }
-    if (mode != 0 && mode != 1) { set_error("mode must be 0 (Riemann_solver) or 1 (KT flux)"); return GZM_ERR_ARG; }
-    if (precision < 0 || precision > 2) { set_error("precision must be 0, 1 or 2"); return GZM_ERR_ARG; }
-    if (!(gamma > 1)) { set_error("gamma must be > 1"); return GZM_ERR_ARG; }
-    if (n < 0 || n > ((int64_t)1 << 28)) { set_error("n=%lld out of range", (long long)n); return GZM_ERR_ARG; }
-    if (n == 0) return GZM_OK;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { set_error("no HIP device"); return GZM_ERR_NO_DEVICE; }
-    if (device < 0 || device >= ndev) { set_error("device %d not present", device); return GZM_ERR_ARG; }
-    GZM_HIP(hipSetDevice(device));
     GammaConsts G;
-    G.G0 = gamma;
-    G.G1 = ((gamma - 1.0) / (2.0 * gamma));
-    G.G2 = ((gamma + 1.0) / (2.0 * gamma));
-    G.G3 = ((2.0 * gamma / (gamma - 1.0)));
-    G.G4 = (2.0 / (gamma - 1.0));
-    G.G5 = (2.0 / (gamma + 1.0));
-    G.G6 = ((gamma - 1.0) / (gamma + 1.0));
-    G.G7 = (0.5 * (gamma - 1.0));
-    G.G8 = (1.0 / gamma);
-    G.G9 = (gamma - 1.0);
+    gzm_status s = riemann_args(device, mode, precision, gamma, n, (int64_t)1 << 28, L, R, normal, limiter, out, mask, G);
+    if (s || n == 0) return s;
     const size_t nn = (size_t)n;
     double *d = nullptr;
     int *dm = nullptr;
-    const size_t doubles = nn * (5 + 5 + 3 + 1 + 6);
-    gzm_status s = dev_alloc(&d, doubles);
+    s = dev_alloc(&d, nn * kRiemannDoubles);
     if (s) return s;
     s = dev_alloc(&dm, nn);
     if (s) { dev_free(d); return s; }
-    double *dL = d, *dR = d + 5 * nn, *dn = d + 10 * nn, *dlim = d + 13 * nn, *dout = d + 14 * nn;
-    hipError_t e = hipMemcpy(dL, L, sizeof(double) * 5 * nn, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dR, R, sizeof(double) * 5 * nn, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dn, normal, sizeof(double) * 3 * nn, hipMemcpyHostToDevice);
-    if (e == hipSuccess && limiter) e = hipMemcpy(dlim, limiter, sizeof(double) * nn, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        launch_riemann_states(nullptr, mode, precision, G, n, dL, dR, dn, dlim, dout, dm);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * 6 * nn, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(mask, dm, sizeof(int) * nn, hipMemcpyDeviceToHost);
+    s = riemann_run(mode, precision, G, n, L, R, normal, limiter, out, mask, d, dm);
     dev_free(d);
     dev_free(dm);
-    if (e != hipSuccess) { set_error("gzm_riemann_states: %s", hipGetErrorString(e)); return GZM_ERR_HIP; }
-    return GZM_OK;
+    return s;
+}
+
+gzm_status gzm_riemann_states_noalloc(int32_t device, int32_t mode, int32_t precision, double gamma, int64_t n,
+                                      const double *L, const double *R, const double *normal,
+                                      const double *limiter, double *out, int32_t *mask)
+{
+    GammaConsts G;
+    gzm_status s = riemann_args(device, mode, precision, gamma, n, kNoAllocStates, L, R, normal, limiter, out, mask, G);
+    if (s || n == 0) return s;
+    double *d = nullptr;
+    s = scratch_of_device(device, &d);
+    if (s) return s;
+    // riemann_run's layout for n states inside the area of kNoAllocStates, the masks after all the doubles
+    return riemann_run(mode, precision, G, n, L, R, normal, limiter, out, mask, d,
+                       reinterpret_cast<int *>(d + (size_t)kNoAllocStates * kRiemannDoubles));
 }
 
 gzm_status gzm_timing_get(gzm_ctx *c, gzm_timing *t)

@@ -25,7 +25,7 @@ EXPORTED_SYMBOLS = (
     "gzm_status_string", "gzm_last_error", "gzm_ctx_create", "gzm_ctx_destroy", "gzm_upload",
     "gzm_density", "gzm_gradients", "gzm_hydro_force", "gzm_download", "gzm_restore_initial_state",
     "gzm_neighbor_counts", "gzm_timing_get", "gzm_timing_reset", "gzm_upload_batch", "gzm_run_passes",
-    "gzm_riemann_states",
+    "gzm_riemann_states", "gzm_riemann_states_noalloc",
 )
 
 
@@ -120,8 +120,8 @@ def load_library():
     L.gzm_upload_batch.argtypes = [C.c_void_p, C.POINTER(HostView), C.c_int32, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int64)]
     L.gzm_run_passes.argtypes = [C.c_void_p, C.c_int32, C.POINTER(PassStats)]
-    L.gzm_riemann_states.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int64, _dp, _dp, _dp, _dp,
-                                     _dp, _ip]
+    for f in (L.gzm_riemann_states, L.gzm_riemann_states_noalloc):
+        f.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int64, _dp, _dp, _dp, _dp, _dp, _ip]
     for name in EXPORTED_SYMBOLS:
         f = getattr(L, name)
         if name not in ("gzm_status_string", "gzm_last_error"):
@@ -299,10 +299,12 @@ class MFMContext:
         _check(self.L.gzm_timing_reset(self.ctx), "gzm_timing_reset")
 
 
-def riemann_states(L, R, normal, limiter=None, mode=0, precision=0, gamma=5.0 / 3.0, device=0):
+def riemann_states(L, R, normal, limiter=None, mode=0, precision=0, gamma=5.0 / 3.0, device=0, noalloc=False):
     """The device Riemann solver (mode 0) or KT flux (mode 1) on arrays of face
     states L, R [n][5] = rho, p, v[3], unit normals [n][3] and pressure
-    limiters [n]; returns (out [n][6] = P_M, S_M, F_v[3], F_p; stage masks)."""
+    limiters [n]; returns (out [n][6] = P_M, S_M, F_v[3], F_p; stage masks).
+    noalloc: through gzm_riemann_states_noalloc (n <= 65536, no device
+    allocation)."""
     L = np.ascontiguousarray(L, dtype=np.float64).reshape(-1, 5)
     R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 5)
     nrm = np.ascontiguousarray(normal, dtype=np.float64).reshape(-1, 3)
@@ -311,8 +313,9 @@ def riemann_states(L, R, normal, limiter=None, mode=0, precision=0, gamma=5.0 / 
     out = np.zeros((n, 6))
     mask = np.zeros(n, dtype=np.int32)
     lib = load_library()
-    _check(lib.gzm_riemann_states(device, mode, precision, gamma, n, _p(L), _p(R), _p(nrm), _p(lim), _p(out),
-                                  _p(mask, _ip)), "gzm_riemann_states")
+    name = "gzm_riemann_states_noalloc" if noalloc else "gzm_riemann_states"
+    _check(getattr(lib, name)(device, mode, precision, gamma, n, _p(L), _p(R), _p(nrm), _p(lim), _p(out),
+                              _p(mask, _ip)), name)
     return out, mask
 
 

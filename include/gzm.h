@@ -226,6 +226,20 @@ gzm_status gzm_riemann_states(int32_t device, int32_t mode, int32_t precision, d
                               const double *L, const double *R, const double *normal, const double *limiter,
                               double *out, int32_t *mask);
 
+/* gzm_riemann_states without device allocation: the same arguments, kernel
+ * and results, with the face states staged through a fixed device array of
+ * the library (one per GPU, 10 MiB + 256 KiB, held until the process ends)
+ * instead of buffers allocated per call.  n is at most 65536
+ * (GZM_RIEMANN_NOALLOC_MAX); a larger n returns GZM_ERR_ARG.  No counterpart
+ * in the reference: for callers that cannot call hipMalloc (harnesses that
+ * forbid it, tight loops).  Every call on one GPU uses that GPU's array, so
+ * calls that overlap on one GPU (from several host threads) race; calls are
+ * synchronous, so sequential calls are safe. */
+#define GZM_RIEMANN_NOALLOC_MAX 65536
+gzm_status gzm_riemann_states_noalloc(int32_t device, int32_t mode, int32_t precision, double gamma, int64_t n,
+                                      const double *L, const double *R, const double *normal,
+                                      const double *limiter, double *out, int32_t *mask);
+
 gzm_status gzm_timing_get(gzm_ctx *ctx, gzm_timing *timing);
 gzm_status gzm_timing_reset(gzm_ctx *ctx);
 
