@@ -770,6 +770,15 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
     // tiny but positive and the face vanishes, so give it no face here instead
     // of the 0/0 of the volume-weighted switch below.
     if (sizeof(T) == sizeof(float) && !(wk_i + wk_j > 0)) return 0;
+    // FP32 only: a kernel weight near the edge of its support, w ~ (1 - u)^3,
+    // carries the FP32 geometry's relative error ~3e-7 / (1 - u).  With
+    // pressures of similar size such a pair weighs nothing in i's sums; across
+    // a pressure contrast above 1e4 (a blast front) it can carry most of i's
+    // energy flux with that error, so the pair is decided in FP64 by the
+    // caller (at contrast C the FP32 error would reach ~(2C)^(1/3) 1.5e-7 of
+    // the particle's sum).
+    if (sizeof(T) == sizeof(float) && (tmax(I.P, J.P) > T(1e4) * tmin(I.P, J.P)))
+        return DEFER ? kDeferred : kNearThreshold;
     // effective face (compute_finitevol_faces.h:14-106)
     T wt_i = I.V, wt_j = J.V;
     const T vratio = M::div(M::div(fabs(I.V - J.V), tmin(I.V, J.V)), T(3));
@@ -881,6 +890,15 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
         for (int k = 0; k < 3; k++) Fv[k] = Anorm * ro.Fv[k];
         // MFM adiabatic face correction (hydro_core_meshless.h:335-374)
         const T sm_c = M::div(fabs(ro.S_M), tmin(I.cs, J.cs));
+        // FP32: the switch's decisions (|S_M|/c against 0.5 and 1e-3, and the
+        // signs of the energy transfer to j or i) flip the pair's energy flux
+        // by du_new - du_old; one within the FP32 margin of its threshold is
+        // decided in FP64 by the caller.  S_M of nearly equal states comes
+        // from a pressure difference: its FP32 error is ~eps c / gamma in
+        // absolute terms whatever its size, so |S_M| / c carries ~1e-7
+        // absolute error and the margins below hold that with room (0.03 +
+        // threshold scales kDecideMargin to 3e-4 of 1e-3 and 1e-5 of 0.5).
+        if (!(leak > 1) && near_threshold(sm_c, T(0.5), T(0.5 + 0.03))) return DEFER ? kDeferred : kNearThreshold;
         if ((sm_c < T(0.5)) || (leak > 1)) {
             int use = 1;
             const T fpm = ro.P_M * Anorm;
@@ -890,14 +908,25 @@ __device__ __forceinline__ int force_pair(const PairI<T> &I, const PairSide<T> &
             const T du_old = fpm * (ro.S_M + fadv);
             const T du_new = T(0.5) * (pdv_i - pdv_j + fpm * (fvi + fvj));
             const double cnum2 = J.cn * J.cn;
+            if (cnum2 < I.cnumcrit2 && (I.e != J.e) && near_threshold(sm_c, T(1.e-3), T(1.e-3 + 0.03)))
+                return DEFER ? kDeferred : kNearThreshold;
             if (sm_c > T(1.e-3) && cnum2 < I.cnumcrit2) {
                 if (I.e != J.e) {
+                    const T fpv = (I.e > J.e) ? fpm * fvj : fpm * fvi;
+                    // dtoj = -du_old + fpm fvj, dtoi = du_old - fpm fvi: compared with 0 and with
+                    // -du_new + fpm fvj (+du_new - fpm fvi); the S_M inside du_old carries
+                    // ~eps c absolute error (above), hence the fpm c term
+                    const T scale = fabs(du_old) + fabs(fpv) + fabs(du_new) + T(0.1) * fpm * tmin(I.cs, J.cs);
                     if (I.e > J.e) {
                         const T dtoj = -du_old + fpm * fvj;
+                        if (near_threshold(dtoj, T(0), scale) || near_threshold(dtoj, -du_new + fpm * fvj, scale))
+                            return DEFER ? kDeferred : kNearThreshold;
                         if (dtoj > 0) use = 0;
                         else if (dtoj < 0) { if (dtoj > -du_new + fpm * fvj) use = 0; }
                     } else {
                         const T dtoi = +du_old - fpm * fvi;
+                        if (near_threshold(dtoi, T(0), scale) || near_threshold(dtoi, +du_new - fpm * fvi, scale))
+                            return DEFER ? kDeferred : kNearThreshold;
                         if (dtoi > 0) use = 0;
                         else if (dtoi < 0) { if (dtoi > +du_new - fpm * fvi) use = 0; }
                     }

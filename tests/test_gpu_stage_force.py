@@ -72,7 +72,8 @@ def test_force_stage_on_identical_inputs(name, precision):
     e_acc = l1_errs(out["acc"], ref["acc"], ref["acc_l1"]).max(axis=1)
     tied = ref["tie_flag"].astype(bool)
     print(name, precision, f"n_tied={int(tied.sum())}", f"dtu max {e_dtu.max():.2e}",
-          f"dtu tied max {e_dtu[tied].max() if tied.any() else 0:.2e}", f"acc max {e_acc.max():.2e}")
+          f"dtu tied max {e_dtu[tied].max() if tied.any() else 0:.2e}", f"acc max {e_acc.max():.2e}",
+          f"fp64 redo {gst['fp64_force_particles']}", f"worst {int(np.argmax(e_dtu))}")
     if name != "blast_128":
         assert tied.sum() > 0.5 * len(tied)  # uniform u: most particles meet a P/rho tie
     assert e_dtu.max() <= tol, (e_dtu.max(), int(np.argmax(e_dtu)))
