@@ -103,7 +103,8 @@ def parse():
                     help="batch workload: also time precision 2 (FP32 density pair sums)")
     ap.add_argument("--blast", type=int, default=1, help="batch workload: also time the 128^3 blast case")
     ap.add_argument("--autotune", type=int, default=1,
-                    help="batch workload: pick cell size and pair-list capacity per mode on a sample of boxes")
+                    help="batch workload: pick cell size, pair-list capacity, workgroup sizes and the density "
+                         "load-balance thresholds per mode on a sample of boxes")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="instead of the pass, copy a buffer of this many MiB three times and exit: the "
                          "known-size reference that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -302,7 +303,7 @@ def run_plain_per_box(gzm, ic, tuned=None):
     import ics
     params = dict(precision=0)
     if tuned:
-        params.update(cell_size=-tuned["cell_factor"], pair_list_cap=tuned["pair_list_cap"])
+        params.update(tuned["params"])
     nb = len(ic["box_offsets"]) - 1
     total_s, inter = 0.0, 0
     for b in range(nb):
@@ -329,8 +330,7 @@ def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
     HIP graph (timed with events around the launch)."""
     N = ic["pos"].shape[0]
     if tuned:
-        params = gzm.make_params(cell_size=-tuned["cell_factor"], pair_list_cap=tuned["pair_list_cap"],
-                                 precision=precision)
+        params = gzm.make_params(precision=precision, **tuned["params"])
     else:
         params = gzm.make_params(cell_size=args.cell_size, precision=precision)
     ctx = gzm.MFMContext(params)
@@ -426,7 +426,7 @@ def main_batch(args, gzm, ics):
         t0 = time.perf_counter()
         for prec in (1, 0):
             tuned[prec] = autotune.tune(ic, prec)
-            log(f"autotune precision {prec}: {tuned[prec]['cell_factor']}, cap {tuned[prec]['pair_list_cap']}")
+            log(f"autotune precision {prec}: {tuned[prec]['setting']}")
         tune_s = time.perf_counter() - t0
     r32 = run_graph_mode(gzm, ic, args, 1, steps, tuned.get(1))
     log(f"precision 1: {r32['ms_per_step']:.2f} ms/pass")

@@ -1,13 +1,26 @@
 """Autotuner of the MFM pass's launch parameters on the GPU.
 
 For a particle set (one box or a batch) it times eager passes through the C
-ABI for each candidate setting and returns the fastest: the grid cell side
-as a factor of each box's mean initial hsml (gzm_params.cell_size < 0), which
-trades candidate tests per neighbour against cells walked, and the capacity
-of the gradient-to-force pair list (neighbours per particle kept in its row;
-particles with more pairs fall back to the grid scan).  A batch is tuned on a
-sample of its boxes spread over the size range, so that tuning costs a small
-fraction of a full pass.
+ABI for candidate settings of each axis in turn (coordinate descent from the
+library's defaults) and returns the fastest setting it met:
+
+  cell_factor     grid cell side as a factor of each box's mean initial hsml
+                  (gzm_params.cell_size < 0): candidate tests per neighbour
+                  against cells walked
+  pair_list_cap   neighbours per particle kept in the gradient-to-force pair
+                  list (longer pair sets take the wavefront fallback)
+  block_size      workgroup size of the density-sweep, gradient and force
+                  loops, one axis each (threads per particle are fixed at one,
+                  so this trades scheduling granularity against tail effects)
+  density_budget  candidates a density-sweep thread tests before its particle
+                  moves to a wavefront of its own
+  wave_sweep_max  density sweeps over at most this many particles give every
+                  particle a wavefront (neighbours per wavefront rather than
+                  per thread)
+
+A batch is tuned on a sample of its boxes spread over the size range, so that
+tuning costs a small fraction of a full pass.  None of these settings changes
+a neighbour set; they change summation order only, within the parity bar.
 """
 from __future__ import annotations
 
@@ -17,13 +30,31 @@ import numpy as np
 
 import gzm
 
-CELL_FACTORS = (1.0, 0.75, 0.6, 0.5)
+CELL_FACTORS = (1.0, 0.75, 0.6, 0.5, 0.42, 0.35)
 LIST_CAPS = (64, 48)
+BLOCK_SIZES = (256, 128, 64)
+DENSITY_BUDGETS = (512, 256, 1024)
+WAVE_SWEEP_MAX = (4096, 1024, 16384)
+DEFAULTS = dict(cell_factor=1.0, pair_list_cap=64, block_density=256, block_gradients=256, block_force=256,
+                density_budget=512, wave_sweep_max=4096)
+# axes in the order they are tuned, each with its candidates (the first is
+# the default the descent starts from)
+AXES = (("cell_factor", CELL_FACTORS), ("pair_list_cap", LIST_CAPS), ("block_density", BLOCK_SIZES),
+        ("block_gradients", BLOCK_SIZES), ("block_force", BLOCK_SIZES), ("density_budget", DENSITY_BUDGETS),
+        ("wave_sweep_max", WAVE_SWEEP_MAX))
 # A later candidate replaces the best so far only when it is faster by more
 # than this fraction: timings of one setting on the sample vary by ~1 % from
 # run to run, and a choice made on noise would differ between otherwise
 # identical runs.
 MIN_GAIN = 0.02
+
+
+def make_params_kwargs(setting: dict) -> dict:
+    """gzm.make_params keyword arguments of an autotuner setting."""
+    return dict(cell_size=-float(setting["cell_factor"]), pair_list_cap=int(setting["pair_list_cap"]),
+                block_size=(int(setting["block_density"]), int(setting["block_gradients"]),
+                            int(setting["block_force"])),
+                density_budget=int(setting["density_budget"]), wave_sweep_max=int(setting["wave_sweep_max"]))
 
 
 def sample_boxes(ic: dict, n_sample: int = 32) -> dict:
@@ -46,11 +77,10 @@ def sample_boxes(ic: dict, n_sample: int = 32) -> dict:
     return out
 
 
-def time_setting(ic: dict, precision: int, cell_factor: float, list_cap: int, passes: int = 2) -> float:
+def time_setting(ic: dict, precision: int, setting: dict, passes: int = 2) -> float:
     """Mean wall time (s) of `passes` eager passes after one warm-up pass."""
     box = tuple(ic["box"]) if "box" in ic and np.ndim(ic["box"]) == 1 else (1.0, 1.0, 1.0)
-    params = gzm.make_params(box=box, precision=precision, cell_size=-float(cell_factor),
-                             pair_list_cap=int(list_cap))
+    params = gzm.make_params(box=box, precision=precision, **make_params_kwargs(setting))
     with gzm.MFMContext(params) as ctx:
         ctx.upload(ic)
 
@@ -67,24 +97,38 @@ def time_setting(ic: dict, precision: int, cell_factor: float, list_cap: int, pa
         return (time.perf_counter() - t0) / passes
 
 
-def tune(ic: dict, precision: int, cell_factors=CELL_FACTORS, list_caps=LIST_CAPS, n_sample: int = 32,
-         min_gain: float = MIN_GAIN) -> dict:
-    """Fastest (cell factor, list capacity) on a sample of `ic`, and the table
-    of all timings: {"cell_factor": f, "pair_list_cap": c, "table": {...}}.
-    Candidates are tried in the order given; one replaces the best so far
-    only when it is faster by more than `min_gain`."""
+def _label(setting: dict) -> str:
+    return " ".join(f"{k}={setting[k]}" for k, _ in AXES)
+
+
+def tune(ic: dict, precision: int, axes=AXES, n_sample: int = 32, min_gain: float = MIN_GAIN) -> dict:
+    """Fastest setting on a sample of `ic` by coordinate descent over `axes`,
+    and the table of every timing: {"setting": {...}, "params": make_params
+    kwargs, "cell_factor", "pair_list_cap", "table": {label: ms},
+    "axes": {axis: {candidate: ms}}}.  Along an axis the candidates are tried
+    in the order given, the others held at the best setting so far; one
+    replaces the best only when it is faster by more than `min_gain`."""
     sample = sample_boxes(ic, n_sample) if "box_offsets" in ic else ic
-    table = {}
-    best_f = None
-    for f in cell_factors:
-        table[(f, list_caps[0])] = time_setting(sample, precision, f, list_caps[0])
-        if best_f is None or table[(f, list_caps[0])] < (1 - min_gain) * table[(best_f, list_caps[0])]:
-            best_f = f
-    best = (best_f, list_caps[0])
-    for c in list_caps[1:]:
-        table[(best_f, c)] = time_setting(sample, precision, best_f, c)
-        if table[(best_f, c)] < (1 - min_gain) * table[best]:
-            best = (best_f, c)
-    return {"cell_factor": best[0], "pair_list_cap": best[1],
-            "table": {f"cell {k[0]} cap {k[1]}": round(v * 1e3, 3) for k, v in table.items()},
+    best = dict(DEFAULTS)
+    timed = {}
+
+    def measure(setting):
+        key = _label(setting)
+        if key not in timed:
+            timed[key] = time_setting(sample, precision, setting)
+        return timed[key]
+
+    best_t = measure(best)
+    per_axis = {}
+    for name, candidates in axes:
+        per_axis[name] = {}
+        for v in candidates:
+            trial = dict(best, **{name: v})
+            t = measure(trial)
+            per_axis[name][str(v)] = round(t * 1e3, 3)
+            if t < (1 - min_gain) * best_t:
+                best, best_t = trial, t
+    return {"setting": best, "params": make_params_kwargs(best), "cell_factor": best["cell_factor"],
+            "pair_list_cap": best["pair_list_cap"], "ms": round(best_t * 1e3, 3),
+            "table": {k: round(v * 1e3, 3) for k, v in timed.items()}, "axes": per_axis,
             "sample_particles": int(sample["pos"].shape[0])}

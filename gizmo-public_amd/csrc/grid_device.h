@@ -21,6 +21,11 @@ __device__ __forceinline__ bool visit_more(F &f, A... args)
 }
 
 constexpr int kBlock = 256;
+// largest workgroup of the per-particle loop kernels (density sweeps,
+// gradients, force), whose size gzm_params.block_size picks from 64, 128 and
+// 256 per stage.  Not 512: a bound of 512 threads makes the compiler fit the
+// 256-VGPR scanning force kernels to two waves per SIMD, with 3-5x the spills.
+constexpr int kMaxBlock = 256;
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, each with its own L2
 constexpr int kListChunk = 16;   // pair-list entries a lane stages before one 64-byte store
 
@@ -405,14 +410,15 @@ __device__ __forceinline__ void minmax_check(T x, T &mn, T &mx)
     mx = gt ? x : mx;
 }
 
-// One lane's staged list chunk (column `lane` of `stage`) -> 64 bytes of its row.
-__device__ __forceinline__ void flush_list_chunk(const int *stage, int *dst)
+// One lane's staged list chunk (column threadIdx.x of `stage`, rows `stride`
+// ints apart: the workgroup size) -> 64 bytes of its row.
+__device__ __forceinline__ void flush_list_chunk(const int *stage, int *dst, int stride)
 {
     int4 *d = reinterpret_cast<int4 *>(dst);
 #pragma unroll
     for (int q = 0; q < kListChunk / 4; q++)
-        d[q] = make_int4(stage[(4 * q + 0) * kBlock + threadIdx.x], stage[(4 * q + 1) * kBlock + threadIdx.x],
-                         stage[(4 * q + 2) * kBlock + threadIdx.x], stage[(4 * q + 3) * kBlock + threadIdx.x]);
+        d[q] = make_int4(stage[(4 * q + 0) * stride + threadIdx.x], stage[(4 * q + 1) * stride + threadIdx.x],
+                         stage[(4 * q + 2) * stride + threadIdx.x], stage[(4 * q + 3) * stride + threadIdx.x]);
 }
 
 }  // namespace gzm

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/gzm.h"
+#include "grid_device.h"
 #include "gzm_internal.h"
 
 using namespace gzm;
@@ -90,6 +91,10 @@ struct gzm_ctx {
     int *d_active_caller = nullptr, *d_active = nullptr, *d_live = nullptr, *d_live2 = nullptr, *d_flags = nullptr;
     int *d_count = nullptr, *d_count_heavy = nullptr;
     int *d_heavy_list = nullptr;  // density sweeps: live positions past the candidate budget
+    // precision 2: particles whose iteration is redone in FP64 (kEdgeBand),
+    // their count, and every particle's hsml at the start of the stage
+    int *d_edge_list = nullptr, *d_n_edge = nullptr;
+    double *d_hsml_entry = nullptr;
     double *d_hmax = nullptr;
     GradRec *d_grec = nullptr;
     ForceRec *d_frec = nullptr;
@@ -118,11 +123,14 @@ struct gzm_ctx {
     uint8_t *d_list_over = nullptr;
     size_t list_entries = 0;
     int list_cap = 64;
+    int block[3] = {kBlock, kBlock, kBlock};  // workgroup size of the density, gradient and force loops
+    int wave_sweep_max = kWaveSweepMax;       // sweeps over at most this many particles: a wavefront each
     bool list_valid = false;
     bool frec_fresh = false;  // the gradient stage wrote every particle's force record
     int list_overflow_slices = 0;
     uint64_t active_version = 0, list_version = 0;
     std::vector<int> last_sweep_live;  // particles of each density sweep of the last eager call
+    std::vector<int> last_edge_live;   // precision 2: particles of each FP64 redo sweep of the last eager call
     // neighbour-search tree (search = 1)
     int2 *d_t_child = nullptr, *d_t_range = nullptr;
     int *d_t_large = nullptr, *d_t_large_count = nullptr, *d_t_split = nullptr,
@@ -246,6 +254,7 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_active_caller); dev_free(c->d_active); dev_free(c->d_live); dev_free(c->d_live2);
     dev_free(c->d_flags); dev_free(c->d_grec); dev_free(c->d_frec);
     dev_free(c->d_heavy_list);
+    dev_free(c->d_edge_list); dev_free(c->d_n_edge); dev_free(c->d_hsml_entry);
     dev_free(c->d_offm); dev_free(c->d_velf); dev_free(c->d_cellc); dev_free(c->d_grecf); dev_free(c->d_frecf);
     dev_free(c->d_redo_flag); dev_free(c->d_redo_list); dev_free(c->d_redo_count);
     dev_free(c->d_long_list); dev_free(c->d_long_count);
@@ -472,6 +481,13 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
         return GZM_ERR_ARG;
     }
     if (params->pair_list_cap > 4096) { set_error("pair_list_cap %d exceeds 4096", params->pair_list_cap); return GZM_ERR_ARG; }
+    for (int k = 0; k < 3; k++) {
+        const int b = params->block_size[k];
+        if (b != 0 && b != 64 && b != 128 && b != kMaxBlock) {
+            set_error("block_size[%d] = %d: must be 0 (%d), 64, 128 or %d", k, b, kBlock, kMaxBlock);
+            return GZM_ERR_ARG;
+        }
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { set_error("no HIP device"); return GZM_ERR_NO_DEVICE; }
     if (params->device < 0 || params->device >= ndev) { set_error("device %d not present", params->device); return GZM_ERR_ARG; }
@@ -481,6 +497,8 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
     // whole 64-byte chunks per row (the gradient kernel stores 16 entries at a time)
     c->list_cap = params->pair_list_cap == 0 ? 64 : params->pair_list_cap;
     if (c->list_cap > 0) c->list_cap = (c->list_cap + 15) / 16 * 16;
+    for (int k = 0; k < 3; k++) c->block[k] = params->block_size[k] ? params->block_size[k] : kBlock;
+    c->wave_sweep_max = params->wave_sweep_max == 0 ? kWaveSweepMax : (params->wave_sweep_max < 0 ? 0 : params->wave_sweep_max);
     memset(&c->timing, 0, sizeof(c->timing));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
@@ -675,6 +693,11 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
             s = s ? s : dev_alloc(&c->d_redo_list, 2 * (size_t)n);
             s = s ? s : dev_alloc(&c->d_redo_count, 2);
         }
+        if (c->p.precision == 2) {
+            s = s ? s : dev_alloc(&c->d_edge_list, n);
+            s = s ? s : dev_alloc(&c->d_n_edge, 1);
+            s = s ? s : dev_alloc(&c->d_hsml_entry, n);
+        }
         if (c->p.search == 1) {
             s = s ? s : dev_alloc(&c->d_t_child, n);
             s = s ? s : dev_alloc(&c->d_t_range, n);
@@ -850,6 +873,7 @@ struct Plan {
     bool overflow;
     int64_t deferred;             // exact-solver pairs deferred out of the force loop
     std::vector<int> sweep_live;  // particles of each density sweep
+    std::vector<int> edge_live;   // precision 2: particles of each FP64 redo sweep
 };
 
 gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
@@ -872,6 +896,13 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         s = compact(c, c->d_active, c->d_flags, na, c->d_live, gp ? nullptr : &n_live);
         if (s) return s;
     }
+    const bool fp32_sums = c->p.precision == 2;
+    if (fp32_sums && na > 0) {
+        // the stage's entry hsml, from which an FP64 redo restarts, and an empty redo list
+        GZM_HIP(hipMemcpyAsync(c->d_hsml_entry, c->d_hsml, sizeof(double) * (size_t)c->n, hipMemcpyDeviceToDevice,
+                               c->stream));
+        launch_fill_int(c->stream, c->d_n_edge, 1, 0);
+    }
     DensitySweepArgs a;
     memset(&a, 0, sizeof(a));
     a.boxes = c->d_boxes;
@@ -893,12 +924,18 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     a.n_heavy = c->d_count_heavy;
     a.f32 = f32_geom(c);
     a.ctr = ctr;
+    a.block = c->block[0];
+    if (fp32_sums) {
+        a.edge_list = c->d_edge_list;
+        a.n_edge = c->d_n_edge;
+    }
+    bool sums_fp32 = fp32_sums;  // false for the FP64 redo sweeps
     // one sweep over up to `cap` live entries: the budgeted launch (entries
     // past the budget are listed on the device), then one wavefront per listed
     // entry
-    // (a sweep over at most kWaveSweepMax entries gives every entry a wavefront)
+    // (a sweep over at most wave_sweep_max entries gives every entry a wavefront)
     auto sweep = [&](int cap, int expected) -> gzm_status {
-        if (expected <= kWaveSweepMax) {
+        if (expected <= c->wave_sweep_max) {
             Timed tm(c, kDensityWave);
             DensitySweepArgs w = a;
             w.wave_all = 1;
@@ -908,7 +945,7 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         if (a.budget) launch_fill_int(c->stream, c->d_count_heavy, 1, 0);
         {
             Timed tm(c, kDensity);
-            if (c->p.precision == 2) launch_density_sweep_f32(c->stream, a);
+            if (sums_fp32) launch_density_sweep_f32(c->stream, a);
             else launch_density_sweep(c->stream, a);
         }
         if (a.budget) {
@@ -918,9 +955,13 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         return GZM_OK;
     };
     int iter = 0, sweeps = 0;
-    if (!gp) c->last_sweep_live.clear();
+    if (!gp) {
+        c->last_sweep_live.clear();
+        c->last_edge_live.clear();
+    }
     int64_t updates = 0;
     int status = GZM_OK;
+    int n_edge = 0;
     if (gp) {
         // captured: every sweep is sized for all active particles and reads the
         // live count from the device; the probe pass fixed the number of sweeps
@@ -960,6 +1001,75 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
             }
         }
     }
+    // Precision 2: the particles that converged next to a kernel edge
+    // (kEdgeBand) repeat their whole iteration with FP64 sums, from the entry
+    // hsml, so that their hsml and neighbour sets are the FP64 pass's.  Their
+    // sweeps count into the misc counter slot, not the density interactions.
+    if (fp32_sums && na > 0 && status == GZM_OK) {
+        sums_fp32 = false;
+        a.edge_list = nullptr;
+        a.n_edge = nullptr;
+        a.ctr = c->d_ctr + kCtrMisc;
+        if (gp) {
+            const int ne = gp->edge_live.empty() ? 0 : gp->edge_live[0];
+            if (ne > 0) {
+                launch_edge_reset(c->stream, ne, c->d_n_edge, c->d_edge_list, c->d_hsml_entry, c->d_hsml, c->d_left,
+                                  c->d_right);
+                GZM_HIP(hipMemcpyAsync(c->d_live, c->d_edge_list, sizeof(int) * (size_t)ne, hipMemcpyDeviceToDevice,
+                                       c->stream));
+                GZM_HIP(hipMemcpyAsync(c->d_count, c->d_n_edge, sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+                for (int sw = 0; sw < (int)gp->edge_live.size(); sw++) {
+                    a.hp.iter = sw;
+                    a.live = c->d_live;
+                    a.n_live = ne;
+                    a.n_live_dev = c->d_count;
+                    s = sweep(ne, gp->edge_live[sw]);
+                    if (s) return s;
+                    s = compact(c, c->d_live, c->d_flags, ne, c->d_live2, nullptr);
+                    if (s) return s;
+                    std::swap(c->d_live, c->d_live2);
+                }
+                launch_flag_unconverged(c->stream, c->d_count, ctr);
+            }
+        } else {
+            GZM_HIP(hipMemcpyAsync(&n_edge, c->d_n_edge, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            GZM_HIP(hipStreamSynchronize(c->stream));
+            if (n_edge > 0) {
+                s = reset_counters(c, kCtrMisc);
+                if (s) return s;
+                {
+                    Timed tm(c, kOther);
+                    launch_edge_reset(c->stream, n_edge, c->d_n_edge, c->d_edge_list, c->d_hsml_entry, c->d_hsml,
+                                      c->d_left, c->d_right);
+                }
+                GZM_HIP(hipMemcpyAsync(c->d_live, c->d_edge_list, sizeof(int) * (size_t)n_edge,
+                                       hipMemcpyDeviceToDevice, c->stream));
+                n_live = n_edge;
+                int it = 0;
+                while (n_live > 0) {
+                    a.hp.iter = it;
+                    a.live = c->d_live;
+                    a.n_live = n_live;
+                    a.n_live_dev = nullptr;
+                    c->last_edge_live.push_back(n_live);
+                    s = sweep(n_live, n_live);
+                    if (s) return s;
+                    GZM_HIP(hipGetLastError());
+                    int n_next = 0;
+                    s = compact(c, c->d_live, c->d_flags, n_live, c->d_live2, &n_next);
+                    if (s) return s;
+                    std::swap(c->d_live, c->d_live2);
+                    n_live = n_next;
+                    if (n_live > 0 && ++it > c->p.maxiter) { status = GZM_ERR_HSML_NOT_CONVERGED; break; }
+                }
+                Counters hm;
+                s = read_counters(c, kCtrMisc, &hm);
+                if (s) return s;
+                if (hm.status == 6 && status == GZM_OK) status = GZM_ERR_BRACKET;
+            }
+        }
+        a.ctr = ctr;
+    }
     if (na > 0) {
         Timed tm(c, kOther);
         launch_density_finalize(c->stream, na, c->d_active, c->d_posm, c->d_velu, c->d_hsml, c->d_num_ngb,
@@ -980,6 +1090,7 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         stats->density_sweeps = sweeps;
         stats->density_interactions = (int64_t)h.interactions;
         stats->density_updates = updates;
+        stats->density_fp64_redo = n_edge;
         stats->status = status;
     }
     if (status != GZM_OK) set_error("%s", gzm_status_string(status));
@@ -1004,6 +1115,7 @@ gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     }
     GradientArgs a;
     memset(&a, 0, sizeof(a));
+    a.block = c->block[1];
     a.boxes = c->d_boxes;
     a.box_of = c->d_box_of;
     a.tree = c->tree;
@@ -1093,6 +1205,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     }
     ForceArgs a;
     memset(&a, 0, sizeof(a));
+    a.block = c->block[2];
     a.boxes = c->d_boxes;
     a.box_of = c->d_box_of;
     a.tree = c->tree;
@@ -1395,7 +1508,8 @@ gzm_status gzm_run_passes(gzm_ctx *c, int32_t n_passes, gzm_pass_stats *stats)
     s = s ? s : gradients_impl(c, nullptr, &probe);
     s = s ? s : force_impl(c, nullptr, &probe);
     if (s) return s;
-    const Plan gp = {probe.density_sweeps, c->list_overflow_slices > 0, c->last_deferred, c->last_sweep_live};
+    const Plan gp = {probe.density_sweeps, c->list_overflow_slices > 0, c->last_deferred, c->last_sweep_live,
+                     c->last_edge_live};
     {
         // the captured compactions run over the whole active list
         size_t bytes = 0;

@@ -141,6 +141,11 @@ constexpr int kDensityBudget = 512;
 // a sweep over at most this many particles runs one wavefront per particle:
 // they all fit on the GPU at once, and a walk shared by 64 lanes is shorter
 constexpr int kWaveSweepMax = 4096;
+// Relative band around h^2 in which a neighbour of an FP32-summed particle
+// (precision 2) may lie on the other side of the FP64 pass's kernel edge: the
+// converged hsml of FP32 sums differs from the FP64 pass's by up to ~3e-7
+// (measured 2.3e-7 on 31.5 M particles), and FP32 r^2 carries up to 2e-6 h^2.
+constexpr double kEdgeBand = 1e-5;
 
 struct DensitySweepArgs {
     const BoxDesc *boxes;  // per box
@@ -164,8 +169,16 @@ struct DensitySweepArgs {
     int *heavy_list;
     int *n_heavy;
     int wave_all;          // wavefront-per-particle launch over all live entries (a small sweep)
+    // FP32 density sums (precision 2): a particle that converges with a
+    // candidate whose r^2 lies within kEdgeBand of h^2 is appended to
+    // edge_list (count n_edge); the stage then redoes its whole iteration in
+    // FP64, so that its hsml and neighbour sets are those of the FP64 pass.
+    // NULL in the FP64 kernels' own sweeps.
+    int *edge_list;
+    int *n_edge;
     F32Geom f32;           // FP32-bulk kernels only
     Counters *ctr;
+    int block;             // threads per workgroup of the sweep kernel (64 .. kMaxBlock)
 };
 
 // Pair list: row t (entry t of the active list) holds the force-loop pair
@@ -206,6 +219,7 @@ struct GradientArgs {
     ForceRecF *frecf_out;
     const double *dhsml, *fce;
     double gamma;
+    int block;             // threads per workgroup of the gradient kernel (64 .. kMaxBlock)
 };
 
 // Pairs whose Riemann problem reaches the exact solver are taken out of the
@@ -257,6 +271,7 @@ struct ForceArgs {
     int n;
     F32Geom f32;
     Counters *ctr;
+    int block;             // threads per workgroup of the force loop kernel (64 .. kMaxBlock)
 };
 
 // launchers (gzm_kernels.hip); all asynchronous on `stream`
@@ -280,6 +295,10 @@ void launch_map_active(hipStream_t s, int n_active, const int *active_caller, co
 void launch_density_init(hipStream_t s, int n_active, const int *active, const double4 *posm, double *hsml,
                          double *left, double *right, double max_hsml, int *live_flag);
 void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a);
+// precision 2: the listed particles (count on the device) restart their hsml
+// iteration from the stage's entry hsml, with empty brackets
+void launch_edge_reset(hipStream_t s, int cap, const int *n_edge, const int *edge_list, const double *hsml_entry,
+                       double *hsml, double *left, double *right);
 void launch_density_finalize(hipStream_t s, int n_active, const int *active, const double4 *posm,
                              const double4 *velu, const double *hsml, double *num_ngb, double *rho,
                              double *pres, double gamma_minus_1);

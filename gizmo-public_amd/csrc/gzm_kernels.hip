@@ -184,15 +184,18 @@ __global__ void density_init_kernel(int n_active, const int *active, const doubl
     if ((h < 0) || !isfinite(h) || (h > 0.99 * max_hsml)) hsml[i] = 0.99 * max_hsml;  // density.c:484
 }
 
-// one density pair (density.c:252-347): j's terms into S when r_ij < h_i
+// one density pair (density.c:252-347): j's terms into S when r_ij < h_i.
+// EDGE: also set `edge` when r_ij^2 lies within kEdgeBand of h_i^2.
+template <bool EDGE = false>
 __device__ __forceinline__ bool density_pair(const BoxDesc &G, const double4 &pi, const double4 &vi,
                                              const DensitySweepArgs &a, int j, double h2, double hinv,
-                                             double hinv3, double hinv4, DensitySums &S)
+                                             double hinv3, double hinv4, DensitySums &S, bool *edge = nullptr)
 {
     const double4 pj = a.posm[j];
     double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
     wrap_dp(G.box, dp);
     const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+    if (EDGE && pj.w > 0 && fabs(r2 - h2) <= kEdgeBand * h2) *edge = true;
     if (!(r2 < h2 && pj.w > 0)) return false;
     const double r = sqrt(r2);
     const double u = r * hinv;
@@ -218,8 +221,9 @@ __device__ __forceinline__ bool density_pair(const BoxDesc &G, const double4 &pi
     return true;
 }
 
-// the hsml update of one particle from its sums, and its outputs
-__device__ __forceinline__ void density_store(const DensitySweepArgs &a, int t, int i, double h, const DensitySums &S)
+// the hsml update of one particle from its sums, and its outputs; returns
+// hsml_update's result (1 sweep again, 0 done, -1 bracket failure)
+__device__ __forceinline__ int density_store(const DensitySweepArgs &a, int t, int i, double h, const DensitySums &S)
 {
     double left = a.left[i], right = a.right[i];
     DensityResult R;
@@ -237,14 +241,15 @@ __device__ __forceinline__ void density_store(const DensitySweepArgs &a, int t, 
     for (int k = 0; k < 6; k++) a.nvt[6 * (size_t)i + k] = R.nvt[k];
     a.redo[t] = (res == 1) ? 1 : 0;
     if (res < 0) atomicMax(&a.ctr->status, (int)6);
+    return res;
 }
 
 // 4 waves per SIMD: fits 128 VGPRs without spills, and the extra wave hides
 // the candidate loads (sweeps 10 % faster than at the compiler's 146 VGPRs)
 template <bool TREE>
-__global__ __launch_bounds__(kBlock, 4) void density_sweep_kernel(DensitySweepArgs a)
+__global__ __launch_bounds__(kMaxBlock, 4) void density_sweep_kernel(DensitySweepArgs a)
 {
-    const int t = xcd_block() * kBlock + threadIdx.x;
+    const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
     const int n_live = a.n_live_dev ? *a.n_live_dev : a.n_live;
     const bool valid = t < n_live;
     if (!valid && a.n_live_dev && t < a.n_live) a.redo[t] = 0;  // captured pass: clear up to the capacity
@@ -325,17 +330,37 @@ __global__ __launch_bounds__(kBlock) void density_heavy_kernel(DensitySweepArgs 
         DensitySums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         const double xs[3] = {pi.x, pi.y, pi.z};
         unsigned long long c = 0;
+        bool edge = false;
         wave_candidates(a.tree, G, xs, h, lane, [&](int j) {
-            if (density_pair(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S)) c++;
+            if (a.edge_list) {
+                if (density_pair<true>(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S, &edge)) c++;
+            } else {
+                if (density_pair(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S)) c++;
+            }
         });
+        const bool any_edge = __any(edge);
         S.ngb = wave_sum(S.ngb); S.rho = wave_sum(S.rho); S.dhsml = wave_sum(S.dhsml);
         S.divvel = wave_sum(S.divvel);
         S.t00 = wave_sum(S.t00); S.t01 = wave_sum(S.t01); S.t02 = wave_sum(S.t02);
         S.t11 = wave_sum(S.t11); S.t12 = wave_sum(S.t12); S.t22 = wave_sum(S.t22);
         S.f0 = wave_sum(S.f0); S.f1 = wave_sum(S.f1); S.f2 = wave_sum(S.f2);
         wave_add_u64(&a.ctr->interactions, c);
-        if (lane == 0) density_store(a, t, i, h, S);
+        if (lane == 0) {
+            const int res = density_store(a, t, i, h, S);
+            if (res == 0 && any_edge && a.edge_list) a.edge_list[atomicAdd(a.n_edge, 1)] = i;
+        }
     }
+}
+
+__global__ void edge_reset_kernel(int cap, const int *n_edge, const int *edge_list, const double *hsml_entry,
+                                  double *hsml, double *left, double *right)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cap || t >= *n_edge) return;
+    const int i = edge_list[t];
+    hsml[i] = hsml_entry[i];
+    left[i] = 0;
+    right[i] = 0;
 }
 
 __global__ void density_finalize_kernel(int n_active, const int *active, const double4 *posm, const double4 *velu,
@@ -392,14 +417,16 @@ __device__ __forceinline__ ForceRec make_force_rec(const double4 &p, const doubl
     return r;
 }
 
-__global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
+template <int BS>
+__global__ __launch_bounds__(BS) void gradients_kernel(GradientArgs a)
 {
     // Pair-list entries are staged in LDS, column-major (entry c of thread x
-    // at stage[c * kBlock + x], so same-c stores of a wave hit distinct banks),
+    // at stage[c * BS + x], so same-c stores of a wave hit distinct banks),
     // and leave as whole 64-byte chunks: single 4-byte stores spread over the
-    // loop would reach HBM as partial-line write-backs.
-    __shared__ int stage[kListChunk * kBlock];
-    const int t = xcd_block() * kBlock + threadIdx.x;
+    // loop would reach HBM as partial-line write-backs.  BS = blockDim.x.
+    __shared__ int stage[kListChunk * BS];
+    constexpr int bs = BS;
+    const int t = xcd_block() * bs + threadIdx.x;
     const bool valid = t < a.n_active;
     unsigned long long cnt = 0;
     int listed = 0;
@@ -433,8 +460,8 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
                 if (list_row) {
                     if (listed < a.list.cap) {
                         const int c = listed & (kListChunk - 1);
-                        stage[c * kBlock + threadIdx.x] = j;
-                        if (c == kListChunk - 1) flush_list_chunk(stage, list_row + (listed - c));
+                        stage[c * bs + threadIdx.x] = j;
+                        if (c == kListChunk - 1) flush_list_chunk(stage, list_row + (listed - c), bs);
                     }
                     listed++;
                 }
@@ -462,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void gradients_kernel(GradientArgs a)
             });
             // the partly filled last chunk (entries past `listed` are never read)
             if (list_row && listed < a.list.cap && (listed & (kListChunk - 1)))
-                flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)));
+                flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)), bs);
         } else if ((m_i > 0) && (rho_i > 0)) {
             listed = a.list.cap + 1;  // the force loop still runs for i: send it to the scan
         }
@@ -553,9 +580,9 @@ __device__ __forceinline__ void unpack_side(const ForceRec &r, PairSide<double> 
 // evaluates them in place -- the same order, so every variant gives the same
 // bits.
 template <bool USE_LIST, bool THREE_PHASE>
-__global__ __launch_bounds__(kBlock) void force_kernel(ForceArgs a)
+__global__ __launch_bounds__(kMaxBlock) void force_kernel(ForceArgs a)
 {
-    const int t = xcd_block() * kBlock + threadIdx.x;
+    const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
     bool valid = t < a.n_active;
     if (USE_LIST) {
         if (valid && a.list.count[t] > a.list.cap) {
@@ -790,6 +817,7 @@ __global__ __launch_bounds__(kBlock) void neighbor_count_kernel(const BoxDesc *b
 }
 
 inline unsigned nblocks(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline unsigned nblocks(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
 
 }  // namespace
@@ -853,8 +881,16 @@ void launch_density_init(hipStream_t s, int n_active, const int *active, const d
 void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a)
 {
     if (a.n_live <= 0) return;
-    if (a.tree.nodes) hipLaunchKernelGGL(density_sweep_kernel<true>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(density_sweep_kernel<false>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+    const int b = a.block;
+    if (a.tree.nodes) hipLaunchKernelGGL(density_sweep_kernel<true>, dim3(nblocks(a.n_live, b)), dim3(b), 0, s, a);
+    else hipLaunchKernelGGL(density_sweep_kernel<false>, dim3(nblocks(a.n_live, b)), dim3(b), 0, s, a);
+}
+void launch_edge_reset(hipStream_t s, int cap, const int *n_edge, const int *edge_list, const double *hsml_entry,
+                       double *hsml, double *left, double *right)
+{
+    if (cap > 0)
+        hipLaunchKernelGGL(edge_reset_kernel, dim3(nblocks(cap)), dim3(kBlock), 0, s, cap, n_edge, edge_list,
+                           hsml_entry, hsml, left, right);
 }
 void launch_density_finalize(hipStream_t s, int n_active, const int *active, const double4 *posm,
                              const double4 *velu, const double *hsml, double *num_ngb, double *rho, double *pres,
@@ -870,7 +906,11 @@ void launch_pack_grad(hipStream_t s, int n, const double4 *posm, const double4 *
 }
 void launch_gradients(hipStream_t s, const GradientArgs &a)
 {
-    if ((a.n_active) > 0) hipLaunchKernelGGL(gradients_kernel, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    if (a.n_active <= 0) return;
+    const dim3 grid(nblocks(a.n_active, a.block)), block(a.block);
+    if (a.block == 64) hipLaunchKernelGGL(gradients_kernel<64>, grid, block, 0, s, a);
+    else if (a.block == 128) hipLaunchKernelGGL(gradients_kernel<128>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(gradients_kernel<kBlock>, grid, block, 0, s, a);
 }
 void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
                        const double *rho, const double *pres, const double *dhsml, const double *cn,
@@ -882,7 +922,7 @@ void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 
 void launch_force(hipStream_t s, const ForceArgs &a)
 {
     if (a.n_active <= 0) return;
-    const dim3 grid(nblocks(a.n_active)), block(kBlock);
+    const dim3 grid(nblocks(a.n_active, a.block)), block(a.block);
     if (a.list.idx) {
         if (a.defer.mask) hipLaunchKernelGGL((force_kernel<true, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((force_kernel<true, false>), grid, block, 0, s, a);

@@ -139,9 +139,9 @@ __device__ inline bool hsml_step_near(const HsmlParams &P, const HsmlStep &X, do
 }
 
 template <bool TREE>
-__global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepArgs a)
+__global__ __launch_bounds__(kMaxBlock) void density_sweep_f32_kernel(DensitySweepArgs a)
 {
-    const int t = xcd_block() * kBlock + threadIdx.x;
+    const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
     const int n_live = a.n_live_dev ? *a.n_live_dev : a.n_live;
     const bool valid = t < n_live;
     if (!valid && a.n_live_dev && t < a.n_live) a.redo[t] = 0;  // captured pass: clear up to the capacity
@@ -165,6 +165,10 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
         int visited = 0;
         bool heavy = false;
         unsigned long long lane_cnt = 0;
+        // a candidate within kEdgeBand of the kernel edge (FP32 r^2 carries up
+        // to 2e-6 h^2 of its own, inside the band)
+        const float edge_lo = (float)(h2 * (1 - kEdgeBand)), edge_hi = (float)(h2 * (1 + kEdgeBand));
+        bool edge = false;
         auto visit = [&](int k, float sh0, float sh1, float sh2, bool exact_geom) {
             const float4 oj = a.f32.offm[k];
             float dp[3];
@@ -187,6 +191,7 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
                 dp[0] = (float)d64[0]; dp[1] = (float)d64[1]; dp[2] = (float)d64[2];
                 r2 = (float)r2e;
             }
+            if ((oj.w > 0) && (r2 >= edge_lo) && (r2 < edge_hi)) edge = true;
             if (!in || !(oj.w > 0)) return;
             lane_cnt++;
             const float r = Real<float>::root(r2);
@@ -253,6 +258,7 @@ __global__ __launch_bounds__(kBlock) void density_sweep_f32_kernel(DensitySweepA
         }
         cnt += lane_cnt;
         const int res = hsml_step(a.hp, X, h, left, right, R.cn_stored);
+        if (res == 0 && edge && a.edge_list) a.edge_list[atomicAdd(a.n_edge, 1)] = i;
         a.hsml[i] = h;
         a.left[i] = left;
         a.right[i] = right;
@@ -312,11 +318,12 @@ __device__ __forceinline__ ForceRecF make_force_rec_f(const float4 &o, const flo
     return r;
 }
 
-template <bool TREE>
-__global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a)
+template <bool TREE, int BS>
+__global__ __launch_bounds__(BS, 4) void gradients_f32_kernel(GradientArgs a)
 {
-    __shared__ int stage[kListChunk * kBlock];
-    const int t = xcd_block() * kBlock + threadIdx.x;
+    __shared__ int stage[kListChunk * BS];  // as in gradients_kernel; BS = blockDim.x
+    constexpr int bs = BS;
+    const int t = xcd_block() * bs + threadIdx.x;
     const bool valid = t < a.n_active;
     unsigned long long cnt = 0;
     int listed = 0;
@@ -387,8 +394,8 @@ __global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a
                 if (list_row) {
                     if (listed < a.list.cap) {
                         const int c = listed & (kListChunk - 1);
-                        stage[c * kBlock + threadIdx.x] = k;
-                        if (c == kListChunk - 1) flush_list_chunk(stage, list_row + (listed - c));
+                        stage[c * bs + threadIdx.x] = k;
+                        if (c == kListChunk - 1) flush_list_chunk(stage, list_row + (listed - c), bs);
                     }
                     listed++;
                 }
@@ -428,7 +435,7 @@ __global__ __launch_bounds__(kBlock, 4) void gradients_f32_kernel(GradientArgs a
                 }, h_i);
             }
             if (list_row && listed < a.list.cap && (listed & (kListChunk - 1)))
-                flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)));
+                flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)), bs);
         } else if ((m_i > 0) && (rho_i > 0)) {
             listed = a.list.cap + 1;  // the force loop still runs for i: send it to the scan
         }
@@ -586,9 +593,9 @@ __device__ __forceinline__ void side_i_f(const ForceRecF &ri, PairI<float> &I)
 // after the others, by force_deferred_f32_kernel + force_finish_kernel
 // (THREE_PHASE) or by a second walk over the pairs, as in the FP64 kernel.
 template <bool USE_LIST, bool THREE_PHASE, bool TREE = false>
-__global__ __launch_bounds__(kBlock) void force_f32_kernel(ForceArgs a)
+__global__ __launch_bounds__(kMaxBlock) void force_f32_kernel(ForceArgs a)
 {
-    const int t = xcd_block() * kBlock + threadIdx.x;
+    const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
     bool valid = t < a.n_active;
     if (USE_LIST) {
         if (valid && a.list.count[t] > a.list.cap) {
@@ -1015,6 +1022,7 @@ __global__ void collect_long_lists_kernel(int n_active, const int *active, const
 }
 
 inline unsigned nblocks(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline unsigned nblocks(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
 }  // namespace
 
@@ -1028,8 +1036,9 @@ void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const doubl
 void launch_density_sweep_f32(hipStream_t s, const DensitySweepArgs &a)
 {
     if (a.n_live <= 0) return;
-    if (a.tree.nodes) hipLaunchKernelGGL(density_sweep_f32_kernel<true>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(density_sweep_f32_kernel<false>, dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+    const int b = a.block;
+    if (a.tree.nodes) hipLaunchKernelGGL(density_sweep_f32_kernel<true>, dim3(nblocks(a.n_live, b)), dim3(b), 0, s, a);
+    else hipLaunchKernelGGL(density_sweep_f32_kernel<false>, dim3(nblocks(a.n_live, b)), dim3(b), 0, s, a);
 }
 void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const double *hsml,
                           const double *rho, const double *pres, GradRecF *rec)
@@ -1039,8 +1048,16 @@ void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4
 void launch_gradients_f32(hipStream_t s, const GradientArgs &a)
 {
     if (a.n_active <= 0) return;
-    if (a.tree.nodes) hipLaunchKernelGGL(gradients_f32_kernel<true>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(gradients_f32_kernel<false>, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+    const dim3 grid(nblocks(a.n_active, a.block)), block(a.block);
+    if (a.tree.nodes) {
+        if (a.block == 64) hipLaunchKernelGGL((gradients_f32_kernel<true, 64>), grid, block, 0, s, a);
+        else if (a.block == 128) hipLaunchKernelGGL((gradients_f32_kernel<true, 128>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((gradients_f32_kernel<true, kBlock>), grid, block, 0, s, a);
+    } else {
+        if (a.block == 64) hipLaunchKernelGGL((gradients_f32_kernel<false, 64>), grid, block, 0, s, a);
+        else if (a.block == 128) hipLaunchKernelGGL((gradients_f32_kernel<false, 128>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((gradients_f32_kernel<false, kBlock>), grid, block, 0, s, a);
+    }
 }
 void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const uint32_t *cellc,
                            const double4 *posm, const double *hsml, const double *rho, const double *pres,
@@ -1054,7 +1071,7 @@ void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float
 void launch_force_f32(hipStream_t s, const ForceArgs &a)
 {
     if (a.n_active <= 0) return;
-    const dim3 grid(nblocks(a.n_active)), block(kBlock);
+    const dim3 grid(nblocks(a.n_active, a.block)), block(a.block);
     if (a.list.idx) {
         if (a.defer.mask) hipLaunchKernelGGL((force_f32_kernel<true, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((force_f32_kernel<true, false>), grid, block, 0, s, a);

@@ -45,6 +45,8 @@ class Params(C.Structure):
         ("pair_list_cap", C.c_int32),
         ("search", C.c_int32),
         ("density_budget", C.c_int32),
+        ("block_size", C.c_int32 * 3),
+        ("wave_sweep_max", C.c_int32),
     ]
 
 
@@ -70,6 +72,7 @@ class PassStats(C.Structure):
         ("exact_solver_calls", C.c_int64),
         ("density_updates", C.c_int64),
         ("fp64_force_particles", C.c_int64),
+        ("density_fp64_redo", C.c_int64),
     ]
 
 
@@ -142,12 +145,15 @@ def _p(a, t=_dp):
 
 def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_ngb_dev=0.05,
                 min_hsml=0.0, max_hsml=1e56, first_step=1, maxiter=150, device=0, cell_size=0.0,
-                pair_list_cap=0, precision=0, search=0, density_budget=0) -> Params:
+                pair_list_cap=0, precision=0, search=0, density_budget=0, block_size=(0, 0, 0),
+                wave_sweep_max=0) -> Params:
     """precision 0: plain FP64 pass; 1: FP64 density stage, FP32 pair
     arithmetic in gradients and force with FP64 matrix inversion, limiter,
     exact Riemann solver and flux accumulation; 2: as 1 with FP32 density
     pair sums.  search 0: cell grid; 1: tree with per-node hsml maxima
-    (clustered sets)."""
+    (clustered sets).  block_size: workgroup sizes of the density,
+    gradient and force loops (0 = 256); wave_sweep_max: density sweeps over at
+    most this many particles run a wavefront per particle (0 = 4096)."""
     p = Params()
     p.box[:] = list(box)
     p.gamma = gamma
@@ -163,6 +169,8 @@ def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_
     p.pair_list_cap = pair_list_cap
     p.search = search
     p.density_budget = density_budget
+    p.block_size[:] = [int(b) for b in block_size]
+    p.wave_sweep_max = wave_sweep_max
     return p
 
 
@@ -328,7 +336,7 @@ def run_pass(ic: dict, active=None, params: Params | None = None, stages=("densi
         if "density" in stages:
             s = ctx.density(active)
             stats.update(density_sweeps=s.density_sweeps, density_interactions=s.density_interactions,
-                         density_updates=s.density_updates)
+                         density_updates=s.density_updates, density_fp64_redo=s.density_fp64_redo)
         if "gradients" in stages:
             s = ctx.gradients(active)
             stats.update(gradient_interactions=s.gradient_interactions)

@@ -86,6 +86,14 @@ typedef struct gzm_params {
                                (load balance where a few hsml jump far in one iteration):
                                0 = default (512), < 0 = no limit; results do not depend
                                on it */
+    int32_t block_size[3];  /* threads per workgroup of the per-particle loop kernels of
+                               [0] the density sweeps, [1] the gradient loop, [2] the force
+                               loop: 64, 128 or 256; 0 = 256.  Results do not depend on
+                               it */
+    int32_t wave_sweep_max; /* a density sweep over at most this many particles gives each
+                               one a wavefront whose lanes share its walk: 0 = default
+                               (4096), < 0 = never.  Sums then run in another order (lane
+                               partial sums), within the parity bar */
 } gzm_params;
 
 /* Host arrays of one particle set, caller order.  Inputs are read by
@@ -132,8 +140,12 @@ typedef struct gzm_pass_stats {
     int64_t exact_solver_calls;      /* pairs that reached the exact solver */
     int64_t density_updates;         /* sum over sweeps of the particles swept */
     int64_t fp64_force_particles;    /* FP32-bulk force loop: particles whose loop was redone
-                                        in FP64 because a pair's wavespeed-estimate or Riemann
-                                        retry branch lay within FP32 rounding of its threshold */
+                                        in FP64 because one of its pairs' branch decisions lay
+                                        within FP32 rounding of its threshold */
+    int64_t density_fp64_redo;       /* precision 2: particles whose hsml iteration was redone
+                                        with FP64 sums because they converged with a neighbour
+                                        within 1e-5 of the kernel edge (their sweeps are not in
+                                        density_interactions) */
 } gzm_pass_stats;
 
 /* Per-kernel device time (HIP events on the context stream), accumulated

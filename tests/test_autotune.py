@@ -1,33 +1,58 @@
 """Selection rule of the launch-parameter autotuner, with the GPU timings
-replaced by a fixed table (runs on the CPU)."""
+replaced by a model of the setting (runs on the CPU)."""
 import numpy as np
 
 import autotune
 
 
-def _tune(monkeypatch, times):
-    monkeypatch.setattr(autotune, "time_setting", lambda sample, precision, f, c: times[(f, c)])
+def _tune(monkeypatch, cost):
+    calls = []
+
+    def fake(sample, precision, setting):
+        calls.append(dict(setting))
+        return cost(setting)
+
+    monkeypatch.setattr(autotune, "time_setting", fake)
     ic = {"pos": np.zeros((8, 3))}
-    return autotune.tune(ic, precision=1)
+    return autotune.tune(ic, precision=1), calls
 
 
-def _table(cell_ms, cap48_ms):
-    t = {(f, 64): ms for f, ms in zip(autotune.CELL_FACTORS, cell_ms)}
-    for f in autotune.CELL_FACTORS:
-        t[(f, 48)] = cap48_ms
-    return t
+def _model(cell=None, cap48=1.0, blocks=None, budget=None, wave=None):
+    """time = product of per-axis factors (1.0 = the default's)"""
+    cell = cell or {}
+    blocks = blocks or {}
+    budget = budget or {}
+    wave = wave or {}
+
+    def cost(s):
+        t = 10.0 * cell.get(s["cell_factor"], 1.0) * (cap48 if s["pair_list_cap"] == 48 else 1.0)
+        for ax in ("block_density", "block_gradients", "block_force"):
+            t *= blocks.get((ax, s[ax]), 1.0)
+        return t * budget.get(s["density_budget"], 1.0) * wave.get(s["wave_sweep_max"], 1.0)
+    return cost
 
 
-def test_clear_gain_is_taken(monkeypatch):
-    r = _tune(monkeypatch, _table([10.4, 12.7, 13.2, 8.2], 7.9))
-    assert (r["cell_factor"], r["pair_list_cap"]) == (0.5, 48)
-    assert r["table"]["cell 0.5 cap 48"] == 7.9e3
+def test_coordinate_descent_takes_each_clear_gain(monkeypatch):
+    cost = _model(cell={0.75: 0.9, 0.6: 0.85, 0.5: 0.8, 0.42: 0.78, 0.35: 0.83}, cap48=0.97,
+                  blocks={("block_force", 128): 0.95, ("block_density", 64): 1.2},
+                  budget={1024: 0.9}, wave={16384: 1.1})
+    r, calls = _tune(monkeypatch, cost)
+    s = r["setting"]
+    assert s["cell_factor"] == 0.42          # interior optimum of the cell axis
+    assert s["pair_list_cap"] == 48
+    assert (s["block_density"], s["block_gradients"], s["block_force"]) == (256, 256, 128)
+    assert s["density_budget"] == 1024 and s["wave_sweep_max"] == 4096
+    assert r["params"]["cell_size"] == -0.42 and r["params"]["block_size"] == (256, 256, 128)
+    # every candidate of every axis appears in the per-axis table, none timed twice
+    for name, cands in autotune.AXES:
+        assert set(r["axes"][name]) == {str(v) for v in cands}
+    labels = [autotune._label(c) for c in calls]
+    assert len(labels) == len(set(labels))
 
 
 def test_gain_within_noise_keeps_the_earlier_setting(monkeypatch):
-    # cap 48 is 0.9 % faster than cap 64 at cell 0.5: within run-to-run noise
-    r = _tune(monkeypatch, _table([10.4, 12.7, 13.2, 8.155], 8.083))
-    assert (r["cell_factor"], r["pair_list_cap"]) == (0.5, 64)
-    # a cell factor 1 % faster than the first one does not replace it either
-    r = _tune(monkeypatch, _table([10.0, 9.9, 10.2, 10.1], 9.0))
-    assert (r["cell_factor"], r["pair_list_cap"]) == (1.0, 48)
+    # every alternative within 2 % of the default: nothing changes
+    cost = _model(cell={0.75: 0.99, 0.5: 0.985}, cap48=0.99, blocks={("block_gradients", 64): 0.985})
+    r, _ = _tune(monkeypatch, cost)
+    assert r["setting"] == autotune.DEFAULTS
+    assert r["params"]["block_size"] == (256, 256, 256)
