@@ -9,10 +9,12 @@ neighbours, every particle active.  One step = one full MFM pass over all
 boxes (hsml iteration from the stored initial guess, gradients, force) on
 data resident in HBM; the K timed steps are K passes captured in one HIP graph
 and launched once (gzm_run_passes), timed with HIP events around the launch.
-`value` is the FP32-bulk pass (precision 1: FP32 pair arithmetic; FP64 hsml
-iteration, NV_T inversion, limiter, exact Riemann solver and flux sums); the
-plain FP64 pass on the same boxes is reported beside it, and the max / rms
-error of every FP32 output against the FP64 oracle run on all boxes.
+`value` is the FP32-bulk pass (precision 1: FP32 gradient and force pair
+arithmetic; FP64 hsml iteration, NV_T inversion, limiter, exact Riemann solver
+and flux sums), with the FP64 pass's interaction count; precision 2 (FP32
+density sums too) and the plain FP64 pass on the same boxes are reported
+beside it, with the max / rms error of every FP32 output against the FP64
+oracle run on all boxes, and dv/dt and du/dt on identical inputs.
 
 Interactions per pass I = sum over density sweeps of |{j : r_ij < h_i}| +
 |pair set| of the gradient loop + |pair set| of the force loop, counted by the
@@ -384,6 +386,8 @@ def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
         "force_interactions": st.force_interactions // steps,
         "exact_solver_calls": st.exact_solver_calls // steps,
         "fp64_force_particles": st.fp64_force_particles // steps,
+        "density_fp64_redo": int(s1.density_fp64_redo),
+        "precision": precision,
     }
 
 
@@ -405,6 +409,45 @@ def oracle_boxes(O, ic, cores):
     secs = time.perf_counter() - t0
     ref = {k: np.concatenate([o[k] for o in outs]) for k in outs[0] if isinstance(outs[0][k], np.ndarray)}
     return ref, inter, secs
+
+
+IDENTICAL_KEYS = ("hsml", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "cn", "fce",
+                  "grad_rho", "grad_p", "grad_v", "vsig")
+
+
+def force_on_identical_inputs(gzm, O, ic, state_out, cores):
+    """The FP32-bulk force stage (GPU) and the oracle's force stage on the FP64
+    pass's own density and gradient outputs, so that every P/rho tie of the
+    entropic switch resolves alike: dv/dt and du/dt errors over every
+    particle, ties included, relative to the sum of |pair terms|."""
+    import ics
+    from parity import TOL32_FLUX, l1_errs
+    st = dict(ic)
+    for k in IDENTICAL_KEYS:
+        st[k] = state_out[k]
+    offs = st.get("box_offsets")
+    prm = O.make_params(nthreads=cores)
+    parts = []
+    rng = [(0, st["pos"].shape[0])] if offs is None else [(int(offs[b]), int(offs[b + 1])) for b in range(len(offs) - 1)]
+    for b, (lo, hi) in enumerate(rng):
+        one = dict(st) if offs is None else ics.box_slice(st, b)
+        for k in IDENTICAL_KEYS:
+            one[k] = np.asarray(st[k])[lo:hi]
+        r, _, _ = O.run_pass(one, params=prm if offs is not None else
+                             O.make_params(box=tuple(st["box"]), nthreads=cores), stages=("force",))
+        parts.append(r)
+    ref = {k: np.concatenate([p[k] for p in parts]) for k in ("acc", "dtu", "acc_l1", "dtu_l1", "tie_flag")}
+    out, gst = gzm.run_pass(st, params=gzm.make_params(precision=1, box=tuple(st.get("box", (1.0, 1.0, 1.0)))),
+                            stages=("force",))
+    e_dtu = l1_errs(out["dtu"], ref["dtu"], ref["dtu_l1"])
+    e_acc = l1_errs(out["acc"], ref["acc"], ref["acc_l1"]).max(axis=1)
+    tied = ref["tie_flag"].astype(bool)
+    stat = lambda e: {"max": float(f"{e.max():.3e}"), "rms": float(f"{np.sqrt(np.mean(e * e)):.3e}"),
+                      "n_over": int((e > TOL32_FLUX).sum())}
+    return {"dtu": stat(e_dtu), "dtu_tied_only": stat(e_dtu[tied]) if tied.any() else None, "acc": stat(e_acc),
+            "n_tied": int(tied.sum()), "n_particles": int(len(e_dtu)),
+            "fp64_force_particles": int(gst["fp64_force_particles"]),
+            "inputs": "density and gradients of the FP64 GPU pass, given unchanged to both force stages"}
 
 
 def main_batch(args, gzm, ics):
@@ -438,15 +481,26 @@ def main_batch(args, gzm, ics):
     plain = run_plain_per_box(gzm, ic, tuned.get(0)) if (args.plain and "box_offsets" in ic) else None
     if plain is not None:
         log(f"plain FP64, one box per pass: {plain['ms_per_pass']:.1f} ms for all boxes")
-    traffic = measured_traffic(r32["dominant"], 1, N)
+    # The headline is precision 1 (FP64 density sweeps, FP32 gradient and
+    # force pairs): its inputs to every threshold decision are the FP64
+    # pass's, so its FP64 redo of near-threshold pairs recovers the FP64
+    # decisions.  Precision 2 (FP32 density sums) is reported beside it.
+    # Every mode's rate is the FP64 pass's interaction count (the oracle's:
+    # the neighbour sets are bit-exact) over that mode's time per pass.
+    head = r32
+    inter = r64["interactions_per_pass"]
+    for r in (r32, r32d, r64):
+        if r is not None:
+            r["value"] = inter / (r["ms_per_step"] * 1e-3)
+    traffic = measured_traffic(head["dominant"], head["precision"], N)
     result = {
         "metric": "particle-neighbor interactions/s, MFM pass; FP32-batched vs FP64 at stated error",
-        "value": r32["value"],
+        "value": head["value"],
         "unit": "interactions/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": r32["ms_per_step"],
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -460,8 +514,8 @@ def main_batch(args, gzm, ics):
             "gamma": 5.0 / 3.0,
             "des_num_ngb": 32,
             "passes_per_graph": steps,
-            "interactions_per_pass": int(r32["interactions_per_pass"]),
-            "density_sweeps": r32["density_sweeps"],
+            "interactions_per_pass": int(inter),
+            "density_sweeps": head["density_sweeps"],
             "parallelism": "single GPU",
         },
         "fp64_same_workload": {
@@ -473,13 +527,18 @@ def main_batch(args, gzm, ics):
             "kernels_ms_per_pass_eager": r64["kernels_ms_per_pass_eager"],
         },
         # precision alone: FP32-bulk vs FP64, both batched in one graph
-        "fp32_speedup_vs_fp64": r32["value"] / r64["value"],
+        "fp32_speedup_vs_fp64": head["value"] / r64["value"],
         # the plain form (FP64, one box per pass through the eager API) and the
         # chosen form's speed-up over it (FP32-bulk and batching together)
         "fp64_plain_per_box": plain,
-        "fp32_batched_speedup_vs_plain_fp64": (None if plain is None else r32["value"] / plain["value"]),
+        "fp32_batched_speedup_vs_plain_fp64": (None if plain is None else head["value"] / plain["value"]),
         "plain_interactions_equal_batched": (None if plain is None else
                                              bool(plain["interactions_per_pass"] == r64["interactions_per_pass"])),
+        # pair sets: the gradient and force interactions of each FP32 mode
+        # equal the FP64 pass's (density sweeps of precision 2 are its own)
+        "fp32_pair_sets_equal_fp64": bool(
+            head["grad_interactions"] == r64["grad_interactions"] and
+            head["force_interactions"] == r64["force_interactions"]),
         "fp32_interactions_equal_fp64": bool(r32["interactions_per_pass"] == r64["interactions_per_pass"]),
         "fp32_density_too": (None if r32d is None else {
             "precision": 2,
@@ -487,27 +546,30 @@ def main_batch(args, gzm, ics):
             "ms_per_step": r32d["ms_per_step"],
             "speedup_vs_fp64": r32d["value"] / r64["value"],
             "interactions_per_pass": int(r32d["interactions_per_pass"]),
+            "pair_sets_equal_fp64": bool(r32d["grad_interactions"] == r64["grad_interactions"] and
+                                         r32d["force_interactions"] == r64["force_interactions"]),
+            "fp64_density_redo_particles_per_pass": r32d["density_fp64_redo"],
             "density_sweeps": r32d["density_sweeps"],
             "kernels_ms_per_pass_eager": r32d["kernels_ms_per_pass_eager"],
         }),
         "exact_solver_calls_per_pass": r64["exact_solver_calls"],
-        # FP32 force loops redone in FP64: a pair's wavespeed or retry branch within FP32 rounding of its threshold
-        "fp64_force_particles_per_pass": r32["fp64_force_particles"],
+        # FP32 force loops redone in FP64: a pair's branch decision within FP32 rounding of its threshold
+        "fp64_force_particles_per_pass": head["fp64_force_particles"],
         "autotune": ({"fp32": tuned[1], "fp64": tuned[0], "seconds": round(tune_s, 1)} if tuned else None),
-        "kernels_ms_per_pass_eager": r32["kernels_ms_per_pass_eager"],
+        "kernels_ms_per_pass_eager": head["kernels_ms_per_pass_eager"],
         "roofline": {
             "bound": "hbm",
-            "kernel": r32["dominant"],
-            "achieved": r32["achieved_gbs"],
+            "kernel": head["dominant"],
+            "achieved": head["achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": r32["achieved_gbs"] / HBM_PEAK_GBS,
+            "frac": head["achieved_gbs"] / HBM_PEAK_GBS,
             "traffic": traffic,
-            "bytes_per_launch": r32["bytes_per_launch"],
-            "mean_launch_ms": r32["mean_launch_ms"],
+            "bytes_per_launch": head["bytes_per_launch"],
+            "mean_launch_ms": head["mean_launch_ms"],
         },
         # rocprof VALU counters of the dominant kernel on this workload (committed summary)
-        "valu": measured_valu(r32["dominant"], 1, N),
+        "valu": measured_valu(head["dominant"], head["precision"], N),
     }
     if args.cpu_repeats > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -536,9 +598,13 @@ def main_batch(args, gzm, ics):
             return {k: (v if isinstance(v, int) else {q: (float(f"{x:.3e}") if isinstance(x, float) else x)
                                                       for q, x in v.items()})
                     for k, v in es.items()}
-        result["error_vs_fp64_oracle"] = fmt(error_stats(r32["out"], ref))
+        result["error_vs_fp64_oracle"] = fmt(error_stats(head["out"], ref))
         if r32d is not None:
             result["fp32_density_too"]["error_vs_fp64_oracle"] = fmt(error_stats(r32d["out"], ref))
+        # du/dt with P/rho ties included: both force stages on the FP64
+        # pass's own density and gradient outputs (identical inputs)
+        log("force stage on identical inputs: oracle and FP32-bulk GPU")
+        result["force_on_identical_inputs"] = force_on_identical_inputs(gzm, O, ic, r64["out"], cores)
         errs = compare_pass(r64["out"], ref)
         result["fp64_same_workload"]["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
         result["fp64_same_workload"]["interactions_equal"] = bool(cpu_int == r64["interactions_per_pass"])

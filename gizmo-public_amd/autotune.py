@@ -10,8 +10,9 @@ library's defaults) and returns the fastest setting it met:
   pair_list_cap   neighbours per particle kept in the gradient-to-force pair
                   list (longer pair sets take the wavefront fallback)
   block_size      workgroup size of the density-sweep, gradient and force
-                  loops, one axis each (threads per particle are fixed at one,
-                  so this trades scheduling granularity against tail effects)
+                  loops (64, 128, 256, 512), one axis each (threads per
+                  particle are fixed at one, so this trades scheduling
+                  granularity against tail effects)
   density_budget  candidates a density-sweep thread tests before its particle
                   moves to a wavefront of its own
   wave_sweep_max  density sweeps over at most this many particles give every
@@ -32,7 +33,7 @@ import gzm
 
 CELL_FACTORS = (1.0, 0.75, 0.6, 0.5, 0.42, 0.35)
 LIST_CAPS = (64, 48)
-BLOCK_SIZES = (256, 128, 64)
+BLOCK_SIZES = (256, 128, 64, 512)
 DENSITY_BUDGETS = (512, 256, 1024)
 WAVE_SWEEP_MAX = (4096, 1024, 16384)
 DEFAULTS = dict(cell_factor=1.0, pair_list_cap=64, block_density=256, block_gradients=256, block_force=256,

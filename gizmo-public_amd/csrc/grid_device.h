@@ -22,10 +22,11 @@ __device__ __forceinline__ bool visit_more(F &f, A... args)
 
 constexpr int kBlock = 256;
 // largest workgroup of the per-particle loop kernels (density sweeps,
-// gradients, force), whose size gzm_params.block_size picks from 64, 128 and
-// 256 per stage.  Not 512: a bound of 512 threads makes the compiler fit the
-// 256-VGPR scanning force kernels to two waves per SIMD, with 3-5x the spills.
-constexpr int kMaxBlock = 256;
+// gradients, the pair-list force loops), whose size gzm_params.block_size
+// picks from 64, 128, 256 and 512 per stage.  The scanning force kernels stay
+// at 256: a bound of 512 threads makes the compiler fit their 256 VGPRs to two
+// waves per SIMD, with 3-5x the spills.
+constexpr int kMaxBlock = 512;
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, each with its own L2
 constexpr int kListChunk = 16;   // pair-list entries a lane stages before one 64-byte store
 
@@ -369,25 +370,6 @@ __device__ __forceinline__ void force_final_ops(double acc[3], double &dte, cons
     acc[2] /= m_i;
     dte /= m_i;
     if (h_i >= 0.99 * max_hsml) dte = 0;
-}
-
-// position of the r-th set bit of m (r < popcount(m))
-__device__ __forceinline__ int nth_set_bit(uint64_t m, int r)
-{
-    for (int k = 0; k < r; k++) m &= m - 1;
-    return __builtin_ctzll(m);
-}
-
-// last index t in [0, n) with offset[t] <= q (offset non-decreasing, offset[0] = 0)
-__device__ __forceinline__ int owner_of(const int *offset, int n, int q)
-{
-    int lo = 0, hi = n;  // answer in [lo, hi)
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (offset[mid] <= q) lo = mid;
-        else hi = mid;
-    }
-    return lo;
 }
 
 // wavefront-wide sum, then one atomic per wavefront

@@ -116,6 +116,8 @@ struct gzm_ctx {
     int *d_defer_count = nullptr, *d_defer_off = nullptr, *d_defer_total = nullptr;
     uint8_t *d_defer_raw = nullptr;
     double *d_defer_result = nullptr;
+    int *d_defer_owner = nullptr;      // per deferred pair of a round: active entry and list position
+    uint8_t *d_defer_kpos = nullptr;
     size_t defer_result_cap = 0;   // deferred pairs the result buffer holds
     int64_t last_deferred = 0;     // deferred pairs of the last force call
     // pair list written by the gradient kernel, read by the force kernel
@@ -261,7 +263,7 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
     dev_free(c->d_box_of);
     dev_free(c->d_defer_mask); dev_free(c->d_defer_count); dev_free(c->d_defer_off); dev_free(c->d_defer_total);
-    dev_free(c->d_defer_raw); dev_free(c->d_defer_result);
+    dev_free(c->d_defer_raw); dev_free(c->d_defer_result); dev_free(c->d_defer_owner); dev_free(c->d_defer_kpos);
     dev_free(c->d_t_child); dev_free(c->d_t_range); dev_free(c->d_t_large);
     dev_free(c->d_t_large_count);
     dev_free(c->d_t_split); dev_free(c->d_t_esc); dev_free(c->d_t_leaf_esc); dev_free(c->d_t_flag);
@@ -483,8 +485,8 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
     if (params->pair_list_cap > 4096) { set_error("pair_list_cap %d exceeds 4096", params->pair_list_cap); return GZM_ERR_ARG; }
     for (int k = 0; k < 3; k++) {
         const int b = params->block_size[k];
-        if (b != 0 && b != 64 && b != 128 && b != kMaxBlock) {
-            set_error("block_size[%d] = %d: must be 0 (%d), 64, 128 or %d", k, b, kBlock, kMaxBlock);
+        if (b != 0 && b != 64 && b != 128 && b != 256 && b != kMaxBlock) {
+            set_error("block_size[%d] = %d: must be 0 (%d), 64, 128, 256 or %d", k, b, kBlock, kMaxBlock);
             return GZM_ERR_ARG;
         }
     }
@@ -1298,8 +1300,12 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
             const size_t want = (size_t)std::min<int64_t>(D, std::max<int64_t>(1 << 20, 2 * (int64_t)c->n));
             if (want > c->defer_result_cap) {
                 dev_free(c->d_defer_result);
+                dev_free(c->d_defer_owner);
+                dev_free(c->d_defer_kpos);
                 c->defer_result_cap = 0;
                 s = dev_alloc(&c->d_defer_result, want * 5);
+                s = s ? s : dev_alloc(&c->d_defer_owner, want);
+                s = s ? s : dev_alloc(&c->d_defer_kpos, want);
                 if (s) return s;
                 c->defer_result_cap = want;
             }
@@ -1308,11 +1314,14 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         if (D > 0 && c->defer_result_cap == 0) { set_error("deferred-pair buffer missing while capturing"); return GZM_ERR_STATE; }
         const int64_t chunk = D > 0 ? (int64_t)c->defer_result_cap : 1;
         a.defer.result = c->d_defer_result;
+        a.defer.owner = c->d_defer_owner;
+        a.defer.kpos = c->d_defer_kpos;
         for (int64_t q0 = 0; q0 == 0 || q0 < D; q0 += chunk) {
             const int64_t q1 = std::min<int64_t>(D, q0 + chunk);
             a.defer.q_begin = (int)q0;
             a.defer.q_end = (int)q1;
             Timed tm(c, kForceDeferred);
+            if (q1 > q0) launch_defer_owners(c->stream, a);
             if (c->p.precision != 0) launch_force_deferred_f32(c->stream, a, (int)(q1 - q0));
             else launch_force_deferred(c->stream, a, (int)(q1 - q0));
             launch_force_finish(c->stream, a);

@@ -234,6 +234,8 @@ struct DeferArgs {
     const int *offset;     // exclusive scan of count
     const int *total;      // sum of count (device)
     double *result;        // per deferred pair of the round: dv/dt terms (3), du/dt term, v_sig
+    int *owner;            // per deferred pair of the round: its active entry t and list position k,
+    uint8_t *kpos;         //   written by launch_defer_owners before the round's pair kernel
     int q_begin, q_end;    // deferred pairs handled by this round
 };
 
@@ -311,6 +313,8 @@ void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 
 void launch_force(hipStream_t s, const ForceArgs &a);  // list-driven if a.list.idx, else scanning
 // deferred exact-solver pairs [a.defer.q_begin, a.defer.q_end), then the per-particle finish
 void launch_force_deferred(hipStream_t s, const ForceArgs &a, int n_pairs);
+// owner entry and list position of each deferred pair of the round [q_begin, q_end)
+void launch_defer_owners(hipStream_t s, const ForceArgs &a);
 void launch_force_finish(hipStream_t s, const ForceArgs &a);
 void launch_defer_total(hipStream_t s, int n_active, const int *offset, const int *count, int *total);
 // FP32-bulk kernels (gzm_kernels_f32.hip)

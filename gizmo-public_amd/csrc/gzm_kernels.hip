@@ -246,8 +246,8 @@ __device__ __forceinline__ int density_store(const DensitySweepArgs &a, int t, i
 
 // 4 waves per SIMD: fits 128 VGPRs without spills, and the extra wave hides
 // the candidate loads (sweeps 10 % faster than at the compiler's 146 VGPRs)
-template <bool TREE>
-__global__ __launch_bounds__(kMaxBlock, 4) void density_sweep_kernel(DensitySweepArgs a)
+template <bool TREE, int MAXB>
+__global__ __launch_bounds__(MAXB, 4) void density_sweep_kernel(DensitySweepArgs a)
 {
     const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
     const int n_live = a.n_live_dev ? *a.n_live_dev : a.n_live;
@@ -580,7 +580,7 @@ __device__ __forceinline__ void unpack_side(const ForceRec &r, PairSide<double> 
 // evaluates them in place -- the same order, so every variant gives the same
 // bits.
 template <bool USE_LIST, bool THREE_PHASE>
-__global__ __launch_bounds__(kMaxBlock) void force_kernel(ForceArgs a)
+__global__ __launch_bounds__((USE_LIST && THREE_PHASE) ? kMaxBlock : kBlock) void force_kernel(ForceArgs a)
 {
     const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
     bool valid = t < a.n_active;
@@ -708,11 +708,11 @@ __global__ __launch_bounds__(kBlock) void force_deferred_kernel(ForceArgs a)
     const int q = a.defer.q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int q_end = min(*a.defer.total, a.defer.q_end);
     const bool valid = q < q_end;
-    const int t = valid ? owner_of(a.defer.offset, a.n_active, q) : 0;
+    const int t = valid ? a.defer.owner[q - a.defer.q_begin] : 0;
     unsigned long long nretry = 0, nexact = 0;
     for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = a.active[t];
-        const int k = nth_set_bit(a.defer.mask[t], q - a.defer.offset[t]);
+        const int k = a.defer.kpos[q - a.defer.q_begin];
         const int j = a.list.idx[(size_t)t * a.list.cap + k];
         const ForceRec ri = a.rec[i];
         PairI<double> I;
@@ -881,9 +881,16 @@ void launch_density_init(hipStream_t s, int n_active, const int *active, const d
 void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a)
 {
     if (a.n_live <= 0) return;
+    // workgroups of 512 take their own instantiation: the bound shapes the register allocation
     const int b = a.block;
-    if (a.tree.nodes) hipLaunchKernelGGL(density_sweep_kernel<true>, dim3(nblocks(a.n_live, b)), dim3(b), 0, s, a);
-    else hipLaunchKernelGGL(density_sweep_kernel<false>, dim3(nblocks(a.n_live, b)), dim3(b), 0, s, a);
+    const dim3 grid(nblocks(a.n_live, b)), block(b);
+    if (b > kBlock) {
+        if (a.tree.nodes) hipLaunchKernelGGL((density_sweep_kernel<true, kMaxBlock>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((density_sweep_kernel<false, kMaxBlock>), grid, block, 0, s, a);
+    } else {
+        if (a.tree.nodes) hipLaunchKernelGGL((density_sweep_kernel<true, kBlock>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((density_sweep_kernel<false, kBlock>), grid, block, 0, s, a);
+    }
 }
 void launch_edge_reset(hipStream_t s, int cap, const int *n_edge, const int *edge_list, const double *hsml_entry,
                        double *hsml, double *left, double *right)
@@ -910,6 +917,7 @@ void launch_gradients(hipStream_t s, const GradientArgs &a)
     const dim3 grid(nblocks(a.n_active, a.block)), block(a.block);
     if (a.block == 64) hipLaunchKernelGGL(gradients_kernel<64>, grid, block, 0, s, a);
     else if (a.block == 128) hipLaunchKernelGGL(gradients_kernel<128>, grid, block, 0, s, a);
+    else if (a.block == 512) hipLaunchKernelGGL(gradients_kernel<512>, grid, block, 0, s, a);
     else hipLaunchKernelGGL(gradients_kernel<kBlock>, grid, block, 0, s, a);
 }
 void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
@@ -922,7 +930,9 @@ void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 
 void launch_force(hipStream_t s, const ForceArgs &a)
 {
     if (a.n_active <= 0) return;
-    const dim3 grid(nblocks(a.n_active, a.block)), block(a.block);
+    // only the three-phase list kernel takes workgroups above kBlock
+    const int b = (a.list.idx && a.defer.mask) ? a.block : min(a.block, kBlock);
+    const dim3 grid(nblocks(a.n_active, b)), block(b);
     if (a.list.idx) {
         if (a.defer.mask) hipLaunchKernelGGL((force_kernel<true, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((force_kernel<true, false>), grid, block, 0, s, a);
@@ -930,6 +940,37 @@ void launch_force(hipStream_t s, const ForceArgs &a)
         hipLaunchKernelGGL((force_kernel<false, false>), grid, block, 0, s, a);
     }
 }
+namespace {
+
+// one thread per active entry: its deferred pairs q = offset + r (r-th set
+// bit of its mask) that fall in the round get their owner entry and bit
+__global__ __launch_bounds__(kBlock) void defer_owners_kernel(ForceArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_active) return;
+    const int c = a.defer.count[t];
+    if (c == 0) return;
+    const int q0 = a.defer.offset[t];
+    if (q0 >= a.defer.q_end || q0 + c <= a.defer.q_begin) return;
+    uint64_t m = a.defer.mask[t];
+    for (int r = 0; r < c && m; r++) {
+        const int k = __builtin_ctzll(m);
+        m &= m - 1;
+        const int q = q0 + r;
+        if (q >= a.defer.q_begin && q < a.defer.q_end) {
+            a.defer.owner[q - a.defer.q_begin] = t;
+            a.defer.kpos[q - a.defer.q_begin] = (uint8_t)k;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_defer_owners(hipStream_t s, const ForceArgs &a)
+{
+    if (a.n_active > 0) hipLaunchKernelGGL(defer_owners_kernel, dim3(nblocks(a.n_active)), dim3(kBlock), 0, s, a);
+}
+
 void launch_force_deferred(hipStream_t s, const ForceArgs &a, int n_pairs)
 {
     if (n_pairs > 0) hipLaunchKernelGGL(force_deferred_kernel, dim3(nblocks(n_pairs)), dim3(kBlock), 0, s, a);

@@ -138,8 +138,8 @@ __device__ inline bool hsml_step_near(const HsmlParams &P, const HsmlStep &X, do
     return rs[0] != rs[1] || !(fabs(hs[1] - hs[0]) <= 1e-4 * h);
 }
 
-template <bool TREE>
-__global__ __launch_bounds__(kMaxBlock) void density_sweep_f32_kernel(DensitySweepArgs a)
+template <bool TREE, int MAXB>
+__global__ __launch_bounds__(MAXB) void density_sweep_f32_kernel(DensitySweepArgs a)
 {
     const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
     const int n_live = a.n_live_dev ? *a.n_live_dev : a.n_live;
@@ -593,7 +593,7 @@ __device__ __forceinline__ void side_i_f(const ForceRecF &ri, PairI<float> &I)
 // after the others, by force_deferred_f32_kernel + force_finish_kernel
 // (THREE_PHASE) or by a second walk over the pairs, as in the FP64 kernel.
 template <bool USE_LIST, bool THREE_PHASE, bool TREE = false>
-__global__ __launch_bounds__(kMaxBlock) void force_f32_kernel(ForceArgs a)
+__global__ __launch_bounds__((USE_LIST && THREE_PHASE) ? kMaxBlock : kBlock) void force_f32_kernel(ForceArgs a)
 {
     const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
     bool valid = t < a.n_active;
@@ -780,11 +780,11 @@ __global__ __launch_bounds__(kBlock, 4) void force_deferred_f32_kernel(ForceArgs
     const int q = a.defer.q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int q_end = min(*a.defer.total, a.defer.q_end);
     const bool valid = q < q_end;
-    const int t = valid ? owner_of(a.defer.offset, a.n_active, q) : 0;
+    const int t = valid ? a.defer.owner[q - a.defer.q_begin] : 0;
     unsigned long long nretry = 0, nexact = 0;
     for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
         const int i = a.active[t];
-        const int k = nth_set_bit(a.defer.mask[t], q - a.defer.offset[t]);
+        const int k = a.defer.kpos[q - a.defer.q_begin];
         const int j = a.list.idx[(size_t)t * a.list.cap + k];
         const ForceRecF ri = a.recf[i];
         PairI<float> I;
@@ -1036,9 +1036,16 @@ void launch_pack_geom_f32(hipStream_t s, int n, const double4 *posm, const doubl
 void launch_density_sweep_f32(hipStream_t s, const DensitySweepArgs &a)
 {
     if (a.n_live <= 0) return;
+    // workgroups of 512 take their own instantiation: the bound shapes the register allocation
     const int b = a.block;
-    if (a.tree.nodes) hipLaunchKernelGGL(density_sweep_f32_kernel<true>, dim3(nblocks(a.n_live, b)), dim3(b), 0, s, a);
-    else hipLaunchKernelGGL(density_sweep_f32_kernel<false>, dim3(nblocks(a.n_live, b)), dim3(b), 0, s, a);
+    const dim3 grid(nblocks(a.n_live, b)), block(b);
+    if (b > kBlock) {
+        if (a.tree.nodes) hipLaunchKernelGGL((density_sweep_f32_kernel<true, kMaxBlock>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((density_sweep_f32_kernel<false, kMaxBlock>), grid, block, 0, s, a);
+    } else {
+        if (a.tree.nodes) hipLaunchKernelGGL((density_sweep_f32_kernel<true, kBlock>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((density_sweep_f32_kernel<false, kBlock>), grid, block, 0, s, a);
+    }
 }
 void launch_pack_grad_f32(hipStream_t s, int n, const float4 *offm, const float4 *velf, const double *hsml,
                           const double *rho, const double *pres, GradRecF *rec)
@@ -1052,10 +1059,12 @@ void launch_gradients_f32(hipStream_t s, const GradientArgs &a)
     if (a.tree.nodes) {
         if (a.block == 64) hipLaunchKernelGGL((gradients_f32_kernel<true, 64>), grid, block, 0, s, a);
         else if (a.block == 128) hipLaunchKernelGGL((gradients_f32_kernel<true, 128>), grid, block, 0, s, a);
+        else if (a.block == 512) hipLaunchKernelGGL((gradients_f32_kernel<true, 512>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((gradients_f32_kernel<true, kBlock>), grid, block, 0, s, a);
     } else {
         if (a.block == 64) hipLaunchKernelGGL((gradients_f32_kernel<false, 64>), grid, block, 0, s, a);
         else if (a.block == 128) hipLaunchKernelGGL((gradients_f32_kernel<false, 128>), grid, block, 0, s, a);
+        else if (a.block == 512) hipLaunchKernelGGL((gradients_f32_kernel<false, 512>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((gradients_f32_kernel<false, kBlock>), grid, block, 0, s, a);
     }
 }
@@ -1071,7 +1080,9 @@ void launch_pack_force_f32(hipStream_t s, int n, const float4 *offm, const float
 void launch_force_f32(hipStream_t s, const ForceArgs &a)
 {
     if (a.n_active <= 0) return;
-    const dim3 grid(nblocks(a.n_active, a.block)), block(a.block);
+    // only the three-phase list kernel takes workgroups above kBlock
+    const int b = (a.list.idx && a.defer.mask) ? a.block : min(a.block, kBlock);
+    const dim3 grid(nblocks(a.n_active, b)), block(b);
     if (a.list.idx) {
         if (a.defer.mask) hipLaunchKernelGGL((force_f32_kernel<true, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((force_f32_kernel<true, false>), grid, block, 0, s, a);

@@ -473,7 +473,8 @@ struct ROut {
 
 // get_wavespeeds_and_pressure_star: Gaburov HLLC estimate, Roe, then PVRS
 // (reimann.h:527-593).  Sets P_M and S_M.  Returns true (FP32 only, with
-// P_M and S_M unset) when the vacuum test or a P_M > limiter test is within
+// P_M and S_M unset) when the vacuum test or a P_M > limiter test -- also the
+// one after PVRS, which decides whether the exact solver runs -- is within
 // kDecideMargin of its threshold.
 template <typename T>
 __device__ inline bool wavespeeds_pstar(const RState<T> &L, const RState<T> &R, const GammaConsts &G, const T n[3],
@@ -519,6 +520,7 @@ __device__ inline bool wavespeeds_pstar(const RState<T> &L, const RState<T> &R, 
             if (o.S_M < -S_plus) o.S_M = -S_plus;
             if (o.S_M > S_plus) o.S_M = S_plus;
             if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
+            if (near_threshold(o.P_M, limiter, limiter)) return true;
         }
     }
     return false;
@@ -697,6 +699,9 @@ __device__ __forceinline__ int riemann_solve(const RState<T> &L, const RState<T>
             o = riemann_kt<T>(L.rho, L.p, L.v[0], L.v[1], L.v[2], R.rho, R.p, R.v[0], R.v[1], R.v[2], n[0], n[1], n[2],
                               vL, vR, cL, cR, hL, hR);
         mask |= 2;
+        // FP32: the exact solver's P_M and S_M replace the estimate's in the
+        // entropic switch, so whether it runs matters; decided in FP64 near the limiter
+        if (near_threshold(o.P_M, limiter, limiter)) return mask | kNearBit;
     }
     if ((o.P_M < 0) || isnan(o.P_M) || (o.P_M > limiter)) {
         if (DEFER) return mask | kDeferBit;

@@ -88,7 +88,8 @@ typedef struct gzm_params {
                                on it */
     int32_t block_size[3];  /* threads per workgroup of the per-particle loop kernels of
                                [0] the density sweeps, [1] the gradient loop, [2] the force
-                               loop: 64, 128 or 256; 0 = 256.  Results do not depend on
+                               loop: 64, 128, 256 or 512 (the force loop's grid-scanning
+                               fallbacks stay at 256); 0 = 256.  Results do not depend on
                                it */
     int32_t wave_sweep_max; /* a density sweep over at most this many particles gives each
                                one a wavefront whose lanes share its walk: 0 = default

@@ -38,6 +38,7 @@ SETTINGS = {
     "blocks_64": dict(block_size=(64, 64, 64)),
     "blocks_128": dict(block_size=(128, 128, 128)),
     "mixed_blocks": dict(block_size=(64, 256, 128)),
+    "blocks_512": dict(block_size=(512, 512, 512)),
     "no_wave_sweeps": dict(wave_sweep_max=-1, density_budget=-1),
     "wave_every_sweep": dict(wave_sweep_max=1 << 20),
     "small_budget": dict(density_budget=16, wave_sweep_max=16),
