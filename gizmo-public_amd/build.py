@@ -33,7 +33,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["csrc/gzm_api.hip", "csrc/gzm_kernels.hip", "csrc/gzm_kernels_f32.hip", "csrc/gzm_tree.hip",
-               "csrc/gzm_elements.hip"]
+               "csrc/gzm_elements.hip", "csrc/gzm_mfv.hip"]
 # per-file flags after HIP_FLAGS.  The FP32-bulk kernels stay at
 # -ffp-contract=off too: fused multiply-adds gained nothing measurable there,
 # and contraction decided per inlining site made the pair-list and scanning

@@ -95,6 +95,10 @@ struct gzm_ctx {
     // their count, and every particle's hsml at the start of the stage
     int *d_edge_list = nullptr, *d_n_edge = nullptr;
     double *d_hsml_entry = nullptr;
+    // method 1 (MFV): ParticleVel [3n], integer timesteps as doubles, DtMass,
+    // dMass (and its value at upload), GravWorkTerm [3n]
+    double *d_pvel = nullptr, *d_ti = nullptr, *d_dtmass = nullptr, *d_dmass = nullptr, *d_dmass0 = nullptr,
+           *d_gravwork = nullptr;
     double *d_hmax = nullptr;
     GradRec *d_grec = nullptr;
     ForceRec *d_frec = nullptr;
@@ -257,6 +261,8 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_flags); dev_free(c->d_grec); dev_free(c->d_frec);
     dev_free(c->d_heavy_list);
     dev_free(c->d_edge_list); dev_free(c->d_n_edge); dev_free(c->d_hsml_entry);
+    dev_free(c->d_pvel); dev_free(c->d_ti); dev_free(c->d_dtmass); dev_free(c->d_dmass); dev_free(c->d_dmass0);
+    dev_free(c->d_gravwork);
     dev_free(c->d_offm); dev_free(c->d_velf); dev_free(c->d_cellc); dev_free(c->d_grecf); dev_free(c->d_frecf);
     dev_free(c->d_redo_flag); dev_free(c->d_redo_list); dev_free(c->d_redo_count);
     dev_free(c->d_long_list); dev_free(c->d_long_count);
@@ -482,6 +488,18 @@ gzm_status gzm_ctx_create(gzm_ctx **out, const gzm_params *params)
         set_error("search must be 0 (cell grid) or 1 (tree)");
         return GZM_ERR_ARG;
     }
+    if (params->method != 0 && params->method != 1) {
+        set_error("method must be 0 (meshless finite mass) or 1 (meshless finite volume)");
+        return GZM_ERR_ARG;
+    }
+    if (params->method == 1 && params->precision != 0) {
+        set_error("method 1 (meshless finite volume) runs in FP64 only: precision must be 0");
+        return GZM_ERR_ARG;
+    }
+    if (params->method == 1 && !(params->timebase_interval >= 0 && isfinite(params->timebase_interval))) {
+        set_error("timebase_interval must be finite and >= 0");
+        return GZM_ERR_ARG;
+    }
     if (params->pair_list_cap > 4096) { set_error("pair_list_cap %d exceeds 4096", params->pair_list_cap); return GZM_ERR_ARG; }
     for (int k = 0; k < 3; k++) {
         const int b = params->block_size[k];
@@ -695,6 +713,14 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
             s = s ? s : dev_alloc(&c->d_redo_list, 2 * (size_t)n);
             s = s ? s : dev_alloc(&c->d_redo_count, 2);
         }
+        if (c->p.method == 1) {
+            s = s ? s : dev_alloc(&c->d_pvel, (size_t)n * 3);
+            s = s ? s : dev_alloc(&c->d_ti, n);
+            s = s ? s : dev_alloc(&c->d_dtmass, n);
+            s = s ? s : dev_alloc(&c->d_dmass, n);
+            s = s ? s : dev_alloc(&c->d_dmass0, n);
+            s = s ? s : dev_alloc(&c->d_gravwork, (size_t)n * 3);
+        }
         if (c->p.precision == 2) {
             s = s ? s : dev_alloc(&c->d_edge_list, n);
             s = s ? s : dev_alloc(&c->d_n_edge, 1);
@@ -842,6 +868,25 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
     if (v->dtu) { s = upload_sorted(c, v->dtu, c->d_dtu, 1); if (s) return s; }
     else launch_fill_double(c->stream, c->d_dtu, n, 0.0);
     launch_fill_u8(c->stream, c->d_wakeup, n, 0);
+    if (c->p.method == 1) {
+        // MFV state: the integer timesteps as doubles (exact below 2^53, as the
+        // reference converts them, hydro_evaluate.h:82), ParticleVel, and the
+        // mass fluxes of particles that stay inactive
+        if (v->ti_step) {
+            std::vector<double> ti(n);
+            for (int i = 0; i < n; i++) ti[i] = (double)v->ti_step[i];
+            s = upload_sorted(c, ti.data(), c->d_ti, 1);
+            if (s) return s;
+        } else launch_fill_double(c->stream, c->d_ti, n, 1.0);
+        struct MOpt { const double *h; double *d; int w; } mopts[] = {
+            {v->particle_vel, c->d_pvel, 3}, {v->dt_mass, c->d_dtmass, 1}, {v->d_mass, c->d_dmass, 1},
+            {v->grav_work, c->d_gravwork, 3}};
+        for (auto &o : mopts) {
+            if (o.h) { s = upload_sorted(c, o.h, o.d, o.w); if (s) return s; }
+            else launch_fill_double(c->stream, o.d, (long long)n * o.w, 0.0);
+        }
+        GZM_HIP(hipMemcpyAsync(c->d_dmass0, c->d_dmass, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
     GZM_HIP(hipGetLastError());
     GZM_HIP(hipStreamSynchronize(c->stream));
     collect_timings(c);
@@ -864,6 +909,8 @@ gzm_status restore_state(gzm_ctx *c)
     GZM_HIP(hipMemcpyAsync(c->d_hsml, c->d_hsml0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_cn, c->d_cn0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     GZM_HIP(hipMemcpyAsync(c->d_vsig, c->d_vsig0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    if (c->d_dmass0)
+        GZM_HIP(hipMemcpyAsync(c->d_dmass, c->d_dmass0, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     return GZM_OK;
 }
 
@@ -1074,6 +1121,9 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     }
     if (na > 0) {
         Timed tm(c, kOther);
+        if (c->p.method == 1)
+            launch_mfv_particle_vel(c->stream, c->d_boxes, c->d_box_of, c->tree, na, c->d_active, c->d_posm,
+                                    c->d_velu, c->d_hsml, c->d_rho, c->d_pvel);
         launch_density_finalize(c->stream, na, c->d_active, c->d_posm, c->d_velu, c->d_hsml, c->d_num_ngb,
                                 c->d_rho, c->d_pres, c->G.G9);
     }
@@ -1246,8 +1296,9 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         a.list.overflow = c->d_list_over;
         a.list.cap = c->list_cap;
     }
+    const bool mfv = c->p.method == 1;
     // exact-solver pairs leave the list loop for a kernel of their own (lists of <= 64)
-    const bool three_phase = use_list && c->list_cap <= 64 && c->n_active > 0;
+    const bool three_phase = !mfv && use_list && c->list_cap <= 64 && c->n_active > 0;
     if (three_phase) {
         s = ensure_defer(c);
         if (s) return s;
@@ -1261,9 +1312,20 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         Timed tm(c, kForce);
         // a kernel, not a memset: this stage may be captured into a graph
         if (c->p.precision != 0) launch_fill_int(c->stream, c->d_redo_count, 2, 0);
-        auto launch = (c->p.precision != 0) ? launch_force_f32 : launch_force;
-        launch(c->stream, a);
-        if (use_list && c->list_overflow_slices > 0) {
+        if (mfv) {
+            MfvArgs m;
+            m.pvel = c->d_pvel;
+            m.ti = c->d_ti;
+            m.timebase = c->p.timebase_interval;
+            m.dt_mass = c->d_dtmass;
+            m.d_mass = c->d_dmass;
+            m.grav_work = c->d_gravwork;
+            launch_force_mfv(c->stream, a, m);  // overflowed pair sets scan in the same kernel
+        } else {
+            auto launch = (c->p.precision != 0) ? launch_force_f32 : launch_force;
+            launch(c->stream, a);
+        }
+        if (!mfv && use_list && c->list_overflow_slices > 0) {
             // particles whose pair set overflowed the list: one wavefront each, FP64
             if (!c->d_long_list) {
                 s = dev_alloc(&c->d_long_list, (size_t)c->n);
@@ -1609,9 +1671,11 @@ gzm_status gzm_download(gzm_ctx *c, gzm_host_view *v)
     struct Out { double *h; const double *d; int w; } outs[] = {
         {v->hsml, c->d_hsml, 1}, {v->cn, c->d_cn, 1}, {v->vsig, c->d_vsig, 1}, {v->num_ngb, c->d_num_ngb, 1},
         {v->dhsml, c->d_dhsml, 1}, {v->divvel, c->d_divvel, 1}, {v->density, c->d_rho, 1},
-        {v->pressure, c->d_pres, 1}, {v->fce, c->d_fce, 1}, {v->acc, c->d_acc, 3}, {v->dtu, c->d_dtu, 1}};
+        {v->pressure, c->d_pres, 1}, {v->fce, c->d_fce, 1}, {v->acc, c->d_acc, 3}, {v->dtu, c->d_dtu, 1},
+        {v->particle_vel, c->d_pvel, 3}, {v->dt_mass, c->d_dtmass, 1}, {v->d_mass, c->d_dmass, 1},
+        {v->grav_work, c->d_gravwork, 3}};
     for (auto &o : outs) {
-        if (!o.h) continue;
+        if (!o.h || !o.d) continue;
         gzm_status s = download_sorted(c, o.d, o.h, o.w);
         if (s) return s;
     }

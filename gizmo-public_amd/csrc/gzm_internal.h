@@ -78,6 +78,22 @@ struct alignas(64) ForceRec {
     double4 g3;    // grad v row 2 (x, y, z), 0
 };
 
+// ForceRec -> the pair-side quantities (moves only)
+__device__ __forceinline__ void unpack_side(const ForceRec &r, PairSide<double> &S)
+{
+    S.v[0] = r.vc.x; S.v[1] = r.vc.y; S.v[2] = r.vc.z; S.cs = r.vc.w;
+    S.h = r.xh.w; S.hinv = r.t1.z;
+    S.rho = r.th.x; S.P = r.th.y; S.V = r.th.z; S.dhsml = r.th.w;
+    S.cn = r.mc.y; S.fce = r.mc.z; S.e = r.mc.w;
+    S.nvt[0] = r.t0.x; S.nvt[1] = r.t0.y; S.nvt[2] = r.t0.z; S.nvt[3] = r.t0.w;
+    S.nvt[4] = r.t1.x; S.nvt[5] = r.t1.y;
+    S.gr[0] = r.g0.x; S.gr[1] = r.g0.y; S.gr[2] = r.g0.z;
+    S.gp[0] = r.g0.w; S.gp[1] = r.g1.x; S.gp[2] = r.g1.y;
+    S.gv[0] = r.g1.z; S.gv[1] = r.g1.w; S.gv[2] = r.g2.x;
+    S.gv[3] = r.g2.y; S.gv[4] = r.g2.z; S.gv[5] = r.g2.w;
+    S.gv[6] = r.g3.x; S.gv[7] = r.g3.y; S.gv[8] = r.g3.z;
+}
+
 // Per-particle record read by the gradient loop for the neighbour j.
 struct alignas(32) GradRec {
     double4 xh;    // x, y, z, hsml
@@ -276,6 +292,17 @@ struct ForceArgs {
     int block;             // threads per workgroup of the force loop kernel (64 .. kMaxBlock)
 };
 
+// MFV force loop: ParticleVel (3 per particle), the integer timestep of each
+// particle as a double (null: equal for all), the physical time of one
+// integer unit, and the MFV outputs DtMass, dMass (neighbours add their
+// shares atomically), GravWorkTerm (3 per particle)
+struct MfvArgs {
+    const double *pvel;
+    const double *ti;
+    double timebase;
+    double *dt_mass, *d_mass, *grav_work;
+};
+
 // launchers (gzm_kernels.hip); all asynchronous on `stream`
 void launch_cell_keys(hipStream_t s, int n, const double *pos, const BoxDesc *boxes, const int *box_caller,
                       uint64_t *keys, int *iota);
@@ -373,6 +400,13 @@ void launch_tree_build(hipStream_t s, const TreeBuild &b, const double *hsml);
 void launch_tree_hmax(hipStream_t s, const TreeBuild &b, const double *hsml);
 // per-cell max hsml (rounded up to FP32) of the cells' slot ranges
 void launch_cell_hmax(hipStream_t s, int ncell, const int2 *range, const double *hsml, float *hmax);
+// MFV (gzm_mfv.hip): ParticleVel after the density sweeps, before the
+// finalize (rho_sum = the sweeps' density sums); the force loop with its
+// preloop
+void launch_mfv_particle_vel(hipStream_t s, const BoxDesc *boxes, const int *box_of, const TreeArgs &tree,
+                             int n_active, const int *active, const double4 *posm, const double4 *velu,
+                             const double *hsml, const double *rho_sum, double *pvel);
+void launch_force_mfv(hipStream_t s, const ForceArgs &a, const MfvArgs &m);
 void launch_fill_int(hipStream_t s, int *p, int n, int value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value);
 void launch_fill_double(hipStream_t s, double *p, long long n, double value);

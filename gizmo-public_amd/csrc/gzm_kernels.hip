@@ -554,21 +554,6 @@ __global__ void pack_force_kernel(int n, const double4 *posm, const double4 *vel
                             grad + 15 * (size_t)k, gamma);
 }
 
-// ForceRec -> the pair-side quantities (moves only)
-__device__ __forceinline__ void unpack_side(const ForceRec &r, PairSide<double> &S)
-{
-    S.v[0] = r.vc.x; S.v[1] = r.vc.y; S.v[2] = r.vc.z; S.cs = r.vc.w;
-    S.h = r.xh.w; S.hinv = r.t1.z;
-    S.rho = r.th.x; S.P = r.th.y; S.V = r.th.z; S.dhsml = r.th.w;
-    S.cn = r.mc.y; S.fce = r.mc.z; S.e = r.mc.w;
-    S.nvt[0] = r.t0.x; S.nvt[1] = r.t0.y; S.nvt[2] = r.t0.z; S.nvt[3] = r.t0.w;
-    S.nvt[4] = r.t1.x; S.nvt[5] = r.t1.y;
-    S.gr[0] = r.g0.x; S.gr[1] = r.g0.y; S.gr[2] = r.g0.z;
-    S.gp[0] = r.g0.w; S.gp[1] = r.g1.x; S.gp[2] = r.g1.y;
-    S.gv[0] = r.g1.z; S.gv[1] = r.g1.w; S.gv[2] = r.g2.x;
-    S.gv[3] = r.g2.y; S.gv[4] = r.g2.z; S.gv[5] = r.g2.w;
-    S.gv[6] = r.g3.x; S.gv[7] = r.g3.y; S.gv[8] = r.g3.z;
-}
 
 // One particle's force loop.  USE_LIST: walk the pair list written by the
 // gradient kernel (particles that overflowed it are skipped and left to

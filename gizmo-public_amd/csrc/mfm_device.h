@@ -476,15 +476,20 @@ struct ROut {
 // P_M and S_M unset) when the vacuum test or a P_M > limiter test -- also the
 // one after PVRS, which decides whether the exact solver runs -- is within
 // kDecideMargin of its threshold.
+//
+// S_LR (may be null): the outer wave speeds of the estimate that stands (both
+// 0 on the vacuum test and where P_M is floored), which the MFV fluxes read.
 template <typename T>
 __device__ inline bool wavespeeds_pstar(const RState<T> &L, const RState<T> &R, const GammaConsts &G, const T n[3],
-                                        T vL, T vR, T cL, T cR, T hL, T hR, T limiter, ROut<T> &o)
+                                        T vL, T vR, T cL, T cR, T hL, T hR, T limiter, ROut<T> &o,
+                                        T *S_LR = nullptr)
 {
     using M = Real<T>;
     const T cmax = tmax(cL, cR);
     if (near_threshold(vR - vL, cmax, cmax)) return true;
     if ((vR - vL) > cmax) {
         o.P_M = M::tiny; o.S_M = 0;
+        if (S_LR) S_LR[0] = S_LR[1] = 0;
         return false;
     }
     T PT_L = L.p, PT_R = R.p;
@@ -495,7 +500,8 @@ __device__ inline bool wavespeeds_pstar(const RState<T> &L, const RState<T> &R, 
     o.S_M = M::div(((PT_R - PT_L) + rwL * vL - rwR * vR), (rwL - rwR));
     o.P_M = M::div((PT_L * rwR - PT_R * rwL + rwL * rwR * (vR - vL)), (rwR - rwL));
     if (near_threshold(o.P_M, limiter, limiter)) return true;
-    if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
+    if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; S_L = S_R = 0; }
+    if (S_LR) { S_LR[0] = S_L; S_LR[1] = S_R; }
     if ((o.P_M <= 0) || isnan(o.P_M) || (o.P_M > limiter)) {
         T sL = M::root(L.rho), sR = M::root(R.rho);
         T sinv = M::rcp(sL + sR);
@@ -512,14 +518,18 @@ __device__ inline bool wavespeeds_pstar(const RState<T> &L, const RState<T> &R, 
         o.S_M = M::div(((rwR * vR + rwL * vL) + (PT_L - PT_R)), (rwR + rwL));
         o.P_M = L.rho * (vL - S_L) * (vL - o.S_M) + PT_L;
         if (near_threshold(o.P_M, limiter, limiter)) return true;
-        if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
+        if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; S_L = S_R = 0; }
+        if (S_LR) { S_LR[0] = S_L; S_LR[1] = S_R; }
         if ((o.P_M <= 0) || isnan(o.P_M) || (o.P_M > limiter)) {
             o.P_M = T(0.5) * ((PT_L + PT_R) + (vL - vR) * T(0.25) * (L.rho + R.rho) * (cL + cR));
             o.S_M = T(0.5) * (vR + vL) + M::div(T(2.0) * (PT_L - PT_R), ((L.rho + R.rho) * (cL + cR)));
             T S_plus = tmax(tmax(fabs(vL - cL), fabs(vR - cR)), tmax(fabs(vL + cL), fabs(vR + cR)));
-            if (o.S_M < -S_plus) o.S_M = -S_plus;
-            if (o.S_M > S_plus) o.S_M = S_plus;
-            if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; }
+            S_L = -S_plus;
+            S_R = S_plus;
+            if (o.S_M < S_L) o.S_M = S_L;
+            if (o.S_M > S_R) o.S_M = S_R;
+            if (o.P_M <= M::tiny) { o.P_M = M::tiny; o.S_M = 0; S_L = S_R = 0; }
+            if (S_LR) { S_LR[0] = S_L; S_LR[1] = S_R; }
             if (near_threshold(o.P_M, limiter, limiter)) return true;
         }
     }
@@ -575,6 +585,74 @@ struct ExactOut {
     int status;
 };
 
+// iterative_Riemann_solver (reimann.h:1034-1100) with guess_for_pressure
+// (1101-1138): star pressure and contact speed of two states with positive
+// densities.  Returns false (P, S unset) when the states open a vacuum or
+// the iteration does not converge.
+__device__ __forceinline__ bool exact_star(double rL, double pL, double rR, double pR, double vL, double vR, double cL,
+                                           double cR, const GammaConsts &G, double &P_out, double &S_out)
+{
+    double dvel = vR - vL;
+    double check_vel = G.G4 * (cR + cL) - dvel;
+    if (check_vel < 0) return false;
+    // guess_for_pressure (reimann.h:1101-1138)
+    double Pg;
+    {
+        double pv = 0.5 * (pL + pR) - 0.125 * (vR - vL) * (pL + pR) * (cL + cR);
+        double pmin = dmin(pL, pR), pmax = dmax(pL, pR);
+        if (pmin <= 0) Pg = 0.5 * (pmin + pmax);
+        else {
+            double qrat = pmax / pmin;
+            if (qrat <= 2.0 && (pmin <= pv && pv <= pmax)) Pg = pv;
+            else if (pv < pmin) {
+                double pnu = (cL + cR) - G.G7 * (vR - vL);
+                double pde = cL / pow(pL, G.G1) + cR / pow(pR, G.G1);
+                Pg = pow(pnu / pde, G.G3);
+            } else {
+                double gel = sqrt((G.G5 / rL) / (G.G6 * pL + pv));
+                double ger = sqrt((G.G5 / rR) / (G.G6 * pR + pv));
+                double x = (gel * pL + ger * pR - (vR - vL)) / (gel + ger);
+                if (x < pmin || x > pmax) x = pmin;
+                Pg = x;
+            }
+        }
+    }
+    double tol = 100.0, W_L = 0, W_R = 0, Z_L, Z_R;
+    int niter = 0;
+    while ((tol > 1.e-6) && (niter < 1000)) {
+        double Pg_prev = Pg;
+        if (Pg > pL) {
+            double a0 = G.G5 / rL, a1 = G.G6 * pL, a2 = sqrt(a0 / (Pg + a1));
+            W_L = (Pg - pL) * a2;
+            Z_L = a2 * (1.0 - 0.5 * (Pg - pL) / (a1 + Pg));
+        } else {
+            double pratio = Pg / pL;
+            W_L = G.G4 * cL * (pow(pratio, G.G1) - 1);
+            Z_L = 1 / (rL * cL) * pow(Pg / pL, -G.G2);
+        }
+        if (Pg > pR) {
+            double a0 = G.G5 / rR, a1 = G.G6 * pR, a2 = sqrt(a0 / (Pg + a1));
+            W_R = (Pg - pR) * a2;
+            Z_R = a2 * (1.0 - 0.5 * (Pg - pR) / (a1 + Pg));
+        } else {
+            double pratio = Pg / pR;
+            W_R = G.G4 * cR * (pow(pratio, G.G1) - 1);
+            Z_R = 1 / (rR * cR) * pow(pratio, -G.G2);
+        }
+        if (niter < 500) Pg -= (W_L + W_R + dvel) / (Z_L + Z_R);
+        else Pg -= 0.5 * (W_L + W_R + dvel) / (Z_L + Z_R);
+        if (Pg < 0.1 * Pg_prev) Pg = 0.1 * Pg_prev;
+        tol = 2.0 * fabs((Pg - Pg_prev) / (Pg + Pg_prev));
+        niter++;
+    }
+    if (niter < 1000) {
+        P_out = Pg;
+        S_out = 0.5 * (vL + vR) + 0.5 * (W_R - W_L);
+        return true;
+    }
+    return false;
+}
+
 __device__ __forceinline__ ExactOut riemann_exact_body(double rL, double pL, double rR, double pR, double vL,
                                                       double vR, double cL, double cR, double P_M_in, double S_M_in,
                                                       const GammaConsts &G)
@@ -582,66 +660,7 @@ __device__ __forceinline__ ExactOut riemann_exact_body(double rL, double pL, dou
     ExactOut o = {P_M_in, S_M_in, 0};
     if ((pL == 0 && pR == 0) || (rL == 0 && rR == 0)) { o.P_M = o.S_M = 0; return o; }
     if ((rL > 0) && (rR > 0)) {
-        double dvel = vR - vL;
-        double check_vel = G.G4 * (cR + cL) - dvel;
-        bool converged = false;
-        if (!(check_vel < 0)) {
-            // guess_for_pressure (reimann.h:1101-1138)
-            double Pg;
-            {
-                double pv = 0.5 * (pL + pR) - 0.125 * (vR - vL) * (pL + pR) * (cL + cR);
-                double pmin = dmin(pL, pR), pmax = dmax(pL, pR);
-                if (pmin <= 0) Pg = 0.5 * (pmin + pmax);
-                else {
-                    double qrat = pmax / pmin;
-                    if (qrat <= 2.0 && (pmin <= pv && pv <= pmax)) Pg = pv;
-                    else if (pv < pmin) {
-                        double pnu = (cL + cR) - G.G7 * (vR - vL);
-                        double pde = cL / pow(pL, G.G1) + cR / pow(pR, G.G1);
-                        Pg = pow(pnu / pde, G.G3);
-                    } else {
-                        double gel = sqrt((G.G5 / rL) / (G.G6 * pL + pv));
-                        double ger = sqrt((G.G5 / rR) / (G.G6 * pR + pv));
-                        double x = (gel * pL + ger * pR - (vR - vL)) / (gel + ger);
-                        if (x < pmin || x > pmax) x = pmin;
-                        Pg = x;
-                    }
-                }
-            }
-            double tol = 100.0, W_L = 0, W_R = 0, Z_L, Z_R;
-            int niter = 0;
-            while ((tol > 1.e-6) && (niter < 1000)) {
-                double Pg_prev = Pg;
-                if (Pg > pL) {
-                    double a0 = G.G5 / rL, a1 = G.G6 * pL, a2 = sqrt(a0 / (Pg + a1));
-                    W_L = (Pg - pL) * a2;
-                    Z_L = a2 * (1.0 - 0.5 * (Pg - pL) / (a1 + Pg));
-                } else {
-                    double pratio = Pg / pL;
-                    W_L = G.G4 * cL * (pow(pratio, G.G1) - 1);
-                    Z_L = 1 / (rL * cL) * pow(Pg / pL, -G.G2);
-                }
-                if (Pg > pR) {
-                    double a0 = G.G5 / rR, a1 = G.G6 * pR, a2 = sqrt(a0 / (Pg + a1));
-                    W_R = (Pg - pR) * a2;
-                    Z_R = a2 * (1.0 - 0.5 * (Pg - pR) / (a1 + Pg));
-                } else {
-                    double pratio = Pg / pR;
-                    W_R = G.G4 * cR * (pow(pratio, G.G1) - 1);
-                    Z_R = 1 / (rR * cR) * pow(pratio, -G.G2);
-                }
-                if (niter < 500) Pg -= (W_L + W_R + dvel) / (Z_L + Z_R);
-                else Pg -= 0.5 * (W_L + W_R + dvel) / (Z_L + Z_R);
-                if (Pg < 0.1 * Pg_prev) Pg = 0.1 * Pg_prev;
-                tol = 2.0 * fabs((Pg - Pg_prev) / (Pg + Pg_prev));
-                niter++;
-            }
-            if (niter < 1000) {
-                o.P_M = Pg;
-                o.S_M = 0.5 * (vL + vR) + 0.5 * (W_R - W_L);
-                converged = true;
-            }
-        }
+        const bool converged = exact_star(rL, pL, rR, pR, vL, vR, cL, cR, G, o.P_M, o.S_M);
         if (!converged) {
             // vacuum inside the fan, sampled at S = 0 (reimann.h:818-846)
             double S_L = vL + G.G4 * cL, S_R = vR - G.G4 * cR;

@@ -47,6 +47,8 @@ class Params(C.Structure):
         ("density_budget", C.c_int32),
         ("block_size", C.c_int32 * 3),
         ("wave_sweep_max", C.c_int32),
+        ("method", C.c_int32),
+        ("timebase_interval", C.c_double),
     ]
 
 
@@ -58,6 +60,8 @@ class HostView(C.Structure):
         ("num_ngb", _dp), ("dhsml", _dp), ("divvel", _dp), ("density", _dp), ("pressure", _dp),
         ("nvt", _dp), ("fce", _dp), ("grad_rho", _dp), ("grad_p", _dp), ("grad_v", _dp),
         ("acc", _dp), ("dtu", _dp), ("wakeup", _u8p),
+        ("ti_step", C.POINTER(C.c_int64)),
+        ("particle_vel", _dp), ("dt_mass", _dp), ("d_mass", _dp), ("grav_work", _dp),
     ]
 
 
@@ -146,14 +150,17 @@ def _p(a, t=_dp):
 def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_ngb_dev=0.05,
                 min_hsml=0.0, max_hsml=1e56, first_step=1, maxiter=150, device=0, cell_size=0.0,
                 pair_list_cap=0, precision=0, search=0, density_budget=0, block_size=(0, 0, 0),
-                wave_sweep_max=0) -> Params:
+                wave_sweep_max=0, method=0, timebase_interval=2.0 ** -20) -> Params:
     """precision 0: plain FP64 pass; 1: FP64 density stage, FP32 pair
     arithmetic in gradients and force with FP64 matrix inversion, limiter,
     exact Riemann solver and flux accumulation; 2: as 1 with FP32 density
     pair sums.  search 0: cell grid; 1: tree with per-node hsml maxima
     (clustered sets).  block_size: workgroup sizes of the density,
     gradient and force loops (0 = 256); wave_sweep_max: density sweeps over at
-    most this many particles run a wavefront per particle (0 = 4096)."""
+    most this many particles run a wavefront per particle (0 = 4096).
+    method 0: meshless finite mass; 1: meshless finite volume (FP64 only),
+    whose mass exchange uses timebase_interval as the physical time of one
+    integer time unit."""
     p = Params()
     p.box[:] = list(box)
     p.gamma = gamma
@@ -171,7 +178,12 @@ def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_
     p.density_budget = density_budget
     p.block_size[:] = [int(b) for b in block_size]
     p.wave_sweep_max = wave_sweep_max
+    p.method = method
+    p.timebase_interval = timebase_interval
     return p
+
+
+METHOD_MFM, METHOD_MFV = 0, 1
 
 
 class MFMContext:
@@ -229,8 +241,13 @@ class MFMContext:
         v.u = _p(arr("u", required=True))
         v.hsml = _p(arr("hsml", required=True))
         for name in ("cn", "vsig", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "fce",
-                     "grad_rho", "grad_p", "grad_v", "acc", "dtu"):
+                     "grad_rho", "grad_p", "grad_v", "acc", "dtu", "particle_vel", "dt_mass", "d_mass",
+                     "grav_work"):
             setattr(v, name, _p(arr(name)))
+        if ic.get("ti_step") is not None:
+            ti = np.ascontiguousarray(ic["ti_step"], dtype=np.int64)
+            keep["ti_step"] = ti
+            v.ti_step = ti.ctypes.data_as(C.POINTER(C.c_int64))
         if ic.get("box_offsets") is not None:
             offs = np.ascontiguousarray(ic["box_offsets"], dtype=np.int64)
             sizes = np.ascontiguousarray(ic["box_sizes"], dtype=np.float64).reshape(-1, 3)
@@ -282,6 +299,9 @@ class MFMContext:
             "grad_p": np.zeros((N, 3)), "grad_v": np.zeros((N, 3, 3)), "acc": np.zeros((N, 3)),
             "dtu": np.zeros(N), "wakeup": np.zeros(N, dtype=np.uint8),
         }
+        if self.params.method == METHOD_MFV:
+            out.update(particle_vel=np.zeros((N, 3)), dt_mass=np.zeros(N), d_mass=np.zeros(N),
+                       grav_work=np.zeros((N, 3)))
         v = HostView()
         v.n = N
         for k, a in out.items():

@@ -95,6 +95,15 @@ typedef struct gzm_params {
                                one a wavefront whose lanes share its walk: 0 = default
                                (4096), < 0 = never.  Sums then run in another order (lane
                                partial sums), within the parity bar */
+    int32_t method;         /* 0: meshless finite mass (HYDRO_MESHLESS_FINITE_MASS, default);
+                               1: meshless finite volume (HYDRO_MESHLESS_FINITE_VOLUME) with
+                               the smoothed mesh motion HYDRO_FIX_MESH_MOTION = 5
+                               (allvars.h:94-98): density() also sets ParticleVel, and
+                               hydro_force() exchanges mass (DtMass, dMass, GravWorkTerm).
+                               Method 1 runs in FP64 (precision 0) only */
+    double timebase_interval; /* method 1: physical time of one integer time unit,
+                               UNIT_INTEGERTIME_IN_PHYSICAL (allvars.h:908), which turns
+                               the mass flux into the exchanged mass dMass */
 } gzm_params;
 
 /* Host arrays of one particle set, caller order.  Inputs are read by
@@ -129,6 +138,18 @@ typedef struct gzm_host_view {
     double *acc;
     double *dtu;
     uint8_t *wakeup;
+    /* method 1 (MFV) only; each may be NULL.  ti_step: P[].dt_step, the
+     * integer timestep (GET_PARTICLE_INTEGERTIME with WAKEUP, allvars.h:914),
+     * NULL = equal for every particle; particle_vel: SphP[].ParticleVel [n][3]
+     * (out after density; in at upload for particles that stay inactive);
+     * dt_mass, grav_work [n][3]: SphP[].DtMass, GravWorkTerm (out); d_mass:
+     * SphP[].dMass (in at upload and out: inactive particles keep their value
+     * plus the shares their active neighbours give them) */
+    const int64_t *ti_step;
+    double *particle_vel;
+    double *dt_mass;
+    double *d_mass;
+    double *grav_work;
 } gzm_host_view;
 
 typedef struct gzm_pass_stats {
@@ -197,7 +218,11 @@ gzm_status gzm_density(gzm_ctx *ctx, const int32_t *active, int64_t n_active, gz
 gzm_status gzm_gradients(gzm_ctx *ctx, const int32_t *active, int64_t n_active, gzm_pass_stats *stats);
 
 /* hydro_force(): MFM faces, HLLC fluxes, dv/dt, du/dt, v_sig, wakeup flags
- * (hydro_toplevel.c:961-1057, hydro_evaluate.h:22-516). */
+ * (hydro_toplevel.c:961-1057, hydro_evaluate.h:22-516); with method 1 the MFV
+ * fluxes of mass, momentum and energy, DtMass, dMass and GravWorkTerm
+ * (hydro_core_meshless.h with HYDRO_MESHLESS_FINITE_VOLUME, hydro_evaluate.h:
+ * 379-401).  dMass receives each neighbour's share by atomic adds, so its
+ * last bits depend on their order; every other output is deterministic. */
 gzm_status gzm_hydro_force(gzm_ctx *ctx, const int32_t *active, int64_t n_active, gzm_pass_stats *stats);
 
 /* Copy results back into the caller-order host arrays of `view`. */

@@ -603,6 +603,7 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
     unsigned char *done = (unsigned char *)calloc((size_t)n + 1, 1);
     double maxsoft = p->max_hsml, minsoft = p->min_hsml;
     int status = 0;
+    const int mfv = (p->method == 1) && (s->particle_vel != NULL);
     set_threads(p);
 
     /* density.c:470-490: active gas with mass>0, clamp the initial hsml */
@@ -634,6 +635,7 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                 nb_collect(&b, p, &g, n, pos, hsml, i, h, h, 0);
                 double Ngb = 0, Rho = 0, DhsmlNgb = 0, DivVel = 0, NVT[3][3] = {{0}};
                 double divvel_l1 = 0, f_l1[3] = {0, 0, 0};
+                double PVel[3] = {0, 0, 0}, pvel_l1[3] = {0, 0, 0}; /* MFV: sum of m_j W_ij v_j */
                 for (int q = 0; q < b.cnt; q++) {
                     int j = b.j[q];
                     if (mass[j] <= 0) continue;
@@ -646,6 +648,12 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                     Ngb += wk;
                     Rho += mj_wk;
                     DhsmlNgb += -(NUMDIMS * hinv * wk + u * dwk);
+                    if (mfv && r == 0) { /* density.c:291-293 */
+                        for (int kv = 0; kv < 3; kv++) {
+                            PVel[kv] += mj_wk * vel[3 * j + kv];
+                            pvel_l1[kv] += fabs(mj_wk * vel[3 * j + kv]);
+                        }
+                    }
                     if (r > 0) {
                         NVT[0][0] += wk * dp[0] * dp[0];
                         NVT[0][1] += wk * dp[0] * dp[1];
@@ -659,6 +667,13 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                         double dv0 = vel[3 * i + 0] - vel[3 * j + 0];
                         double dv1 = vel[3 * i + 1] - vel[3 * j + 1];
                         double dv2 = vel[3 * i + 2] - vel[3 * j + 2];
+                        if (mfv) { /* density.c:331-334: v_i - (v_i - v_j) */
+                            const double dvv[3] = {dv0, dv1, dv2};
+                            for (int kv = 0; kv < 3; kv++) {
+                                PVel[kv] += mj_wk * (vel[3 * i + kv] - dvv[kv]);
+                                pvel_l1[kv] += fabs(mj_wk * (vel[3 * i + kv] - dvv[kv]));
+                            }
+                        }
                         DivVel -= dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2) / r;
                         divvel_l1 += fabs(dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2) / r);
                         f_l1[0] += fabs(wk * dp[0]);
@@ -756,6 +771,10 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
                 s->cn[i] = cn_stored;
                 if (s->divvel_l1) s->divvel_l1[i] = divvel_l1;
                 if (s->fce_l1) s->fce_l1[i] = FCE_l1;
+                if (mfv) {
+                    for (int kv = 0; kv < 3; kv++) s->particle_vel[3 * (size_t)i + kv] = PVel[kv];
+                    if (s->pvel_l1) for (int kv = 0; kv < 3; kv++) s->pvel_l1[3 * (size_t)i + kv] = pvel_l1[kv];
+                }
             }
             nb_free(&b);
         }
@@ -773,6 +792,18 @@ int orc_density(const orc_params *p, orc_state *s, const int *active, int n_acti
     for (int a = 0; a < n_active; a++) {
         int i = active[a];
         if (!(mass[i] > 0)) continue;
+        if (mfv && (s->density[i] > 0)) {
+            /* mesh motion 5 (density.c:886-894): 0.7 VelPred + 0.3 the kernel-weighted
+             * mean velocity, the weights' sum being the density sum still held here */
+            const double eps_pvel = 0.3;
+            for (int k = 0; k < 3; k++) {
+                double *w = s->particle_vel + 3 * (size_t)i + k;
+                if (s->pvel_l1)
+                    s->pvel_l1[3 * (size_t)i + k] = (1. - eps_pvel) * fabs(vel[3 * i + k]) +
+                                                    eps_pvel * s->pvel_l1[3 * (size_t)i + k] / s->density[i];
+                *w = (1. - eps_pvel) * vel[3 * i + k] + eps_pvel * *w / s->density[i];
+            }
+        }
         double h = hsml[i];
         if ((h > 0) && (s->num_ngb[i] > 0)) s->density[i] = mass[i] * s->num_ngb[i] / (NORM_COEFF * pow(h, NUMDIMS));
         else if (h <= 0) s->density[i] = 0;
@@ -1323,6 +1354,334 @@ int orc_riemann(double gamma, const double Lin[5], const double Rin[5], const do
 }
 
 /* ------------------------------------------------------------------------- */
+/* Riemann problem for MFV hydro (HYDRO_MESHLESS_FINITE_VOLUME): the solver   */
+/* returns the fluxes of mass, momentum and total energy through a face at    */
+/* rest in the frame of the states, reimann.h:286-388 with the MFV branches   */
+/* of 400-425, 484-519, 603-675, 685-724, 731-1030, 1146-1166                 */
+/* ------------------------------------------------------------------------- */
+typedef struct routv { double P_M, S_M, Frho, Fv[3], Fp; } routv;
+
+/* HLLC_fluxes (reimann.h:603-675): the four-wave HLLC flux sampled at the
+ * face (frame speed 0) from the star estimate's P_M, S_M and outer speeds */
+static void hllc_fluxes_mfv(const rstate *L, const rstate *R, routv *o, const double n[3], double vL, double vR,
+                            double hL, double hR, double S_L, double S_R)
+{
+    const double P_M = o->P_M, S_M = o->S_M, vf = 0;
+    if ((P_M <= 0) || isnan(P_M)) return;
+    if ((S_M == S_L) || (S_M == S_R) || (S_M == vf)) {
+        o->Frho = 0;
+        o->Fp = P_M * S_M;
+        for (int k = 0; k < 3; k++) o->Fv[k] = P_M * n[k];
+        return;
+    }
+    if (vf < S_L) { /* left state */
+        o->Frho = L->rho * (vL - vf);
+        o->Fp = L->rho * hL * (vL - vf) + L->p * vf;
+        for (int k = 0; k < 3; k++) o->Fv[k] = o->Frho * L->v[k] + L->p * n[k];
+    } else if ((S_L <= vf) && (vf <= S_M)) { /* left star state */
+        double nfac = L->rho * (S_L - vL) / (S_L - S_M);
+        if (nfac < 0) nfac = 0;
+        o->Frho = L->rho * (vL - S_L) + nfac * (S_L - vf);
+        double eK = L->rho * hL - L->p;
+        o->Fp = (L->rho * hL * vL - eK * S_L) +
+                (S_L - vf) * nfac * (eK / L->rho + (S_M - vL) * (S_M + L->p / (L->rho * (S_L - vL))));
+        double pfac = nfac * (S_L - vf) * (S_M - vL) + L->p;
+        for (int k = 0; k < 3; k++) o->Fv[k] = o->Frho * L->v[k] + pfac * n[k];
+    } else if ((S_M <= vf) && (vf <= S_R)) { /* right star state */
+        double nfac = R->rho * (S_R - vR) / (S_R - S_M);
+        if (nfac < 0) nfac = 0;
+        o->Frho = R->rho * (vR - S_R) + nfac * (S_R - vf);
+        double eK = R->rho * hR - R->p;
+        o->Fp = (R->rho * hR * vR - eK * S_R) +
+                (S_R - vf) * nfac * (eK / R->rho + (S_M - vR) * (S_M + R->p / (R->rho * (S_R - vR))));
+        double pfac = nfac * (S_R - vf) * (S_M - vR) + R->p;
+        for (int k = 0; k < 3; k++) o->Fv[k] = o->Frho * R->v[k] + pfac * n[k];
+    } else { /* right state */
+        o->Frho = R->rho * (vR - vf);
+        o->Fp = R->rho * hR * (vR - vf) + R->p * vf;
+        for (int k = 0; k < 3; k++) o->Fv[k] = o->Frho * R->v[k] + R->p * n[k];
+    }
+}
+
+/* Kurganov-Tadmor/PWK flux, MFV branch (reimann.h:484-502, 503-507) */
+static void riemann_kt_mfv(const rstate *L, const rstate *R, routv *o, const double n[3], double vL, double vR,
+                           double cL, double cR, double hL, double hR)
+{
+    double thr = 0.001 * 0.5 * (L->rho + R->rho) * 0.5 * (cL + cR);
+    double dv2 = 0;
+    for (int k = 0; k < 3; k++) {
+        double d = R->rho * R->v[k] - L->rho * L->v[k];
+        dv2 += d * d;
+    }
+    double alpha = fabs(R->rho * vR - L->rho * vL) / sqrt(thr * thr + dv2);
+    double S_L = alpha * cL; if (vL > 0) S_L += vL; else S_L -= vL;
+    double S_R = alpha * cR; if (vR > 0) S_R += vR; else S_R -= vR;
+    double S_M = DMAX(S_L, S_R);
+    o->S_M = DMAX(cL + fabs(vL), cR + fabs(vR));
+    double frl = L->rho * (vL + S_M);
+    double frr = R->rho * (vR - S_M);
+    o->P_M = 0.5 * (R->p + L->p);
+    o->Frho = 0.5 * (frl + frr);
+    for (int k = 0; k < 3; k++) o->Fv[k] = 0.5 * (frl * L->v[k] + frr * R->v[k]) + (o->P_M) * n[k];
+    o->Fp = 0.5 * (frl * hL + frr * hR + S_M * (R->p - L->p));
+}
+
+/* sample_reimann_vaccum_left (reimann.h:731-771): vacuum on the left, the
+ * right state's rarefaction sampled at speed S; Frho/Fp/Fv hold the face
+ * state (density, pressure, velocity) until face_to_flux_mfv */
+static void sample_vacuum_left(double S, const rstate *R, routv *o, const gconst *c, const double n[3], double vR,
+                               double cR)
+{
+    double S_R = vR - c->G4 * cR;
+    if (S_R > S) {
+        o->P_M = 0; o->S_M = S_R; o->Frho = 0;
+    } else if (vR + cR > S) {
+        double C_eff = c->G5 * (cR - c->G7 * (vR - S));
+        o->P_M = R->p * pow(C_eff / cR, c->G3);
+        o->S_M = c->G5 * (-cR + c->G7 * vR + S);
+        o->Frho = R->rho * pow(C_eff / cR, c->G4);
+    } else {
+        o->P_M = R->p; o->S_M = vR; o->Frho = R->rho;
+    }
+    o->Fp = o->P_M;
+    for (int k = 0; k < 3; k++) o->Fv[k] = R->v[k] + (o->S_M - vR) * n[k];
+}
+
+/* sample_reimann_vaccum_right (reimann.h:779-820), mirror of the above */
+static void sample_vacuum_right(double S, const rstate *L, routv *o, const gconst *c, const double n[3], double vL,
+                                double cL)
+{
+    double S_L = vL + c->G4 * cL;
+    if (S_L < S) {
+        o->P_M = 0; o->S_M = S_L; o->Frho = 0;
+    } else if (vL - cL < S) {
+        double C_eff = c->G5 * (cL + c->G7 * (vL - S));
+        o->P_M = L->p * pow(C_eff / cL, c->G3);
+        o->S_M = c->G5 * (cL + c->G7 * vL + S);
+        o->Frho = L->rho * pow(C_eff / cL, c->G4);
+    } else {
+        o->P_M = L->p; o->S_M = vL; o->Frho = L->rho;
+    }
+    o->Fp = o->P_M;
+    for (int k = 0; k < 3; k++) o->Fv[k] = L->v[k] + (o->S_M - vL) * n[k];
+}
+
+/* sample_reimann_vaccum_internal (reimann.h:828-858): vacuum created inside
+ * the fan.  The velocity between the fans keeps the reference's (1 - n_k)
+ * weighting of the tangential part. */
+static void sample_vacuum_internal(double S, const rstate *L, const rstate *R, routv *o, const gconst *c,
+                                   const double n[3], double vL, double vR, double cL, double cR)
+{
+    double S_L = vL + c->G4 * cL;
+    double S_R = vR - c->G4 * cR;
+    if (S <= S_L) sample_vacuum_right(S, L, o, c, n, vL, cL);
+    else if (S >= S_R) sample_vacuum_left(S, R, o, c, n, vR, cR);
+    else {
+        o->P_M = 0; o->S_M = S; o->Frho = 0; o->Fp = o->P_M;
+        for (int k = 0; k < 3; k++)
+            o->Fv[k] = (L->v[k] + (R->v[k] - L->v[k]) * (S - S_L) / (S_R - S_L)) * (1 - n[k]) + S * n[k];
+    }
+}
+
+/* sample_reimann_standard (reimann.h:866-1030): the exact solution's state
+ * at speed S given the star pressure P_M and contact speed S_M */
+static void sample_standard(double S, const rstate *L, const rstate *R, routv *o, const gconst *c, const double n[3],
+                            double vL, double vR, double cL, double cR)
+{
+    if (S <= o->S_M) { /* left of the contact */
+        if (o->P_M <= L->p) { /* left rarefaction */
+            if (S <= vL - cL) {
+                o->Fp = L->p; o->Frho = L->rho;
+                for (int k = 0; k < 3; k++) o->Fv[k] = L->v[k];
+                return;
+            }
+            double C_eff_L = cL * pow(o->P_M / L->p, c->G1);
+            if (S > o->S_M - C_eff_L) { /* left star state */
+                o->Frho = L->rho * pow(o->P_M / L->p, c->G8);
+                o->Fp = o->P_M;
+                for (int k = 0; k < 3; k++) o->Fv[k] = L->v[k] + (o->S_M - vL) * n[k];
+                return;
+            }
+            double S_eff = c->G5 * (cL + c->G7 * vL + S); /* inside the fan */
+            double C_eff = c->G5 * (cL + c->G7 * (vL - S));
+            o->Frho = L->rho * pow(C_eff / cL, c->G4);
+            o->Fp = L->p * pow(C_eff / cL, c->G3);
+            for (int k = 0; k < 3; k++) o->Fv[k] = L->v[k] + (S_eff - vL) * n[k];
+            return;
+        }
+        if (L->p > 0) { /* left shock */
+            double pml = o->P_M / L->p;
+            double S_L = vL - cL * sqrt(c->G2 * pml + c->G1);
+            if (S <= S_L) {
+                o->Fp = L->p; o->Frho = L->rho;
+                for (int k = 0; k < 3; k++) o->Fv[k] = L->v[k];
+                return;
+            }
+            o->Frho = L->rho * (pml + c->G6) / (pml * c->G6 + 1.0);
+        } else {
+            o->Frho = L->rho / c->G6;
+        }
+        o->Fp = o->P_M;
+        for (int k = 0; k < 3; k++) o->Fv[k] = L->v[k] + (o->S_M - vL) * n[k];
+        return;
+    }
+    /* right of the contact */
+    if (o->P_M > R->p) { /* right shock */
+        if (R->p > 0) {
+            double pmr = o->P_M / R->p;
+            double S_R = vR + cR * sqrt(c->G2 * pmr + c->G1);
+            if (S >= S_R) {
+                o->Fp = R->p; o->Frho = R->rho;
+                for (int k = 0; k < 3; k++) o->Fv[k] = R->v[k];
+                return;
+            }
+            o->Frho = R->rho * (pmr + c->G6) / (pmr * c->G6 + 1.0);
+        } else {
+            o->Frho = R->rho / c->G6;
+        }
+        o->Fp = o->P_M;
+        for (int k = 0; k < 3; k++) o->Fv[k] = R->v[k] + (o->S_M - vR) * n[k];
+        return;
+    }
+    if (S >= vR + cR) { /* right rarefaction: right data state */
+        o->Fp = R->p; o->Frho = R->rho;
+        for (int k = 0; k < 3; k++) o->Fv[k] = R->v[k];
+        return;
+    }
+    double C_eff_R = cR * pow(o->P_M / R->p, c->G1);
+    if (S <= o->S_M + C_eff_R) { /* right star state */
+        o->Frho = R->rho * pow(o->P_M / R->p, c->G8);
+        o->Fp = o->P_M;
+        for (int k = 0; k < 3; k++) o->Fv[k] = R->v[k] + (o->S_M - vR) * n[k];
+        return;
+    }
+    double S_eff = c->G5 * (-cR + c->G7 * vR + S); /* inside the fan */
+    double C_eff = c->G5 * (cR - c->G7 * (vR - S));
+    o->Frho = R->rho * pow(C_eff / cR, c->G4);
+    o->Fp = R->p * pow(C_eff / cR, c->G3);
+    for (int k = 0; k < 3; k++) o->Fv[k] = R->v[k] + (S_eff - vR) * n[k];
+}
+
+/* convert_face_to_flux (reimann.h:1146-1166): face state (rho, P, v) to the
+ * Euler fluxes through a face at rest */
+static void face_to_flux_mfv(routv *o, const double n[3], const gconst *c)
+{
+    double rho = o->Frho, P = o->Fp, v[3], v_line = 0, h = 0;
+    for (int k = 0; k < 3; k++) {
+        v[k] = o->Fv[k];
+        v_line += v[k] * n[k];
+        h += v[k] * v[k];
+    }
+    v_line -= 0;
+    h *= 0.5 * rho;
+    h += (c->G0 / c->G9) * P;
+    o->Fp = h * v_line;
+    o->Frho = rho * v_line;
+    for (int k = 0; k < 3; k++) o->Fv[k] = o->Frho * v[k] + P * n[k];
+}
+
+/* Riemann_solver_exact with the MFV sampling (reimann.h:685-724) */
+static int riemann_exact_mfv(const rstate *L, const rstate *R, routv *o, const gconst *c, const double n[3],
+                             double vL, double vR, double cL, double cR)
+{
+    if ((L->p == 0 && R->p == 0) || (L->rho == 0 && R->rho == 0)) {
+        o->P_M = o->S_M = 0;
+        o->Frho = o->Fp = o->Fv[0] = o->Fv[1] = o->Fv[2] = 0;
+        return 0;
+    }
+    if ((L->rho > 0) && (R->rho > 0)) {
+        rout q;
+        q.P_M = o->P_M; q.S_M = o->S_M;
+        if (iterative_Riemann_solver(L, R, &q, c, vL, vR, cL, cR)) {
+            o->P_M = q.P_M; o->S_M = q.S_M;
+            sample_standard(0.0, L, R, o, c, n, vL, vR, cL, cR);
+        } else {
+            sample_vacuum_internal(0.0, L, R, o, c, n, vL, vR, cL, cR);
+        }
+    } else {
+        if ((L->rho < 0) || (R->rho < 0)) return 1234545;
+        if (L->rho > 0) sample_vacuum_right(0.0, L, o, c, n, vL, cL);
+        if (R->rho > 0) sample_vacuum_left(0.0, R, o, c, n, vR, cR);
+    }
+    face_to_flux_mfv(o, n, c);
+    return 0;
+}
+
+/* Riemann_solver (reimann.h:286-388) for MFV hydro: HLLC with the HLLC
+ * fluxes, the KT flux on a failed estimate, the exact solver past the
+ * limiter.  Returns the stage mask (1 HLLC, 2 KT, 4 exact). */
+static int riemann_solver_mfv(rstate L, rstate R, routv *o, const double n[3], double press_tot_limiter,
+                              const gconst *c, int *err)
+{
+    if ((L.p < 0 && R.p < 0) || (L.rho < 0) || (R.rho < 0)) { o->P_M = 0; return 0; }
+    rprep q = riemann_prep(&L, &R, n, c);
+    int mask = 1;
+    rout e;
+    e.P_M = o->P_M; e.S_M = o->S_M;
+    double S_L, S_R;
+    get_wavespeeds_and_pressure_star(&L, &R, &e, c, n, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R, q.h_L, q.h_R,
+                                     press_tot_limiter, &S_L, &S_R);
+    o->P_M = e.P_M; o->S_M = e.S_M;
+    if ((o->P_M >= 0) && !isnan(o->P_M))
+        hllc_fluxes_mfv(&L, &R, o, n, q.v_line_L, q.v_line_R, q.h_L, q.h_R, S_L, S_R);
+    if ((o->P_M < 0) || isnan(o->P_M)) {
+        riemann_kt_mfv(&L, &R, o, n, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R, q.h_L, q.h_R);
+        mask |= 2;
+    }
+    if ((o->P_M < 0) || isnan(o->P_M) || (o->P_M > press_tot_limiter)) {
+        int ee = riemann_exact_mfv(&L, &R, o, c, n, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R);
+        if (ee && err) *err = ee;
+        mask |= 4;
+    }
+    return mask;
+}
+
+/* element exports: out = P_M, S_M, F_rho, F_v[3], F_p */
+int orc_riemann_mfv(double gamma, const double Lin[5], const double Rin[5], const double n[3],
+                    double press_tot_limiter, double out[7])
+{
+    gconst c = make_gconst(gamma);
+    rstate L, R;
+    load_states(Lin, Rin, &L, &R);
+    routv o;
+    memset(&o, 0, sizeof(o));
+    int err = 0;
+    int mask = riemann_solver_mfv(L, R, &o, n, press_tot_limiter, &c, &err);
+    out[0] = o.P_M; out[1] = o.S_M; out[2] = o.Frho;
+    out[3] = o.Fv[0]; out[4] = o.Fv[1]; out[5] = o.Fv[2]; out[6] = o.Fp;
+    return err ? -err : mask;
+}
+
+void orc_riemann_kt_mfv(double gamma, const double Lin[5], const double Rin[5], const double n[3], double out[7])
+{
+    gconst c = make_gconst(gamma);
+    rstate L, R;
+    load_states(Lin, Rin, &L, &R);
+    rprep q = riemann_prep(&L, &R, n, &c);
+    routv o;
+    memset(&o, 0, sizeof(o));
+    riemann_kt_mfv(&L, &R, &o, n, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R, q.h_L, q.h_R);
+    out[0] = o.P_M; out[1] = o.S_M; out[2] = o.Frho;
+    out[3] = o.Fv[0]; out[4] = o.Fv[1]; out[5] = o.Fv[2]; out[6] = o.Fp;
+}
+
+/* The exact solver alone (reimann.h:685-724 with 1146-1166): the Godunov
+ * flux of the exact solution sampled at the face */
+int orc_riemann_exact_mfv(double gamma, const double Lin[5], const double Rin[5], const double n[3], double out[7])
+{
+    gconst c = make_gconst(gamma);
+    rstate L, R;
+    load_states(Lin, Rin, &L, &R);
+    rprep q = riemann_prep(&L, &R, n, &c);
+    routv o;
+    memset(&o, 0, sizeof(o));
+    int e = riemann_exact_mfv(&L, &R, &o, &c, n, q.v_line_L, q.v_line_R, q.cs_L, q.cs_R);
+    out[0] = o.P_M; out[1] = o.S_M; out[2] = o.Frho;
+    out[3] = o.Fv[0]; out[4] = o.Fv[1]; out[5] = o.Fv[2]; out[6] = o.Fp;
+    return e;
+}
+
+/* ------------------------------------------------------------------------- */
 /* hydro_force(): hydro_toplevel.c:961-1057 (preloop, finalize 652-767),      */
 /* hydro_evaluate.h:22-516, hydro_core_meshless.h:8-385,                      */
 /* compute_finitevol_faces.h:14-106                                           */
@@ -1527,6 +1886,159 @@ static int force_pair(const orc_params *p, const orc_state *s, const force_i_t *
     return status;
 }
 
+/* One (i,j) interaction of the MFV force loop: hydro_evaluate.h:170-298,
+ * compute_finitevol_faces.h:14-106 and hydro_core_meshless.h:8-334 with the
+ * HYDRO_MESHLESS_FINITE_VOLUME branches: the face moves with
+ * rinv (-s_i v_j + s_j w_i), w_i = ParticleVel of i and v_j = VelPred of j
+ * (hydro_evaluate.h:195-196 sets the neighbour's element velocity to its
+ * VelPred), the signal speed takes 3 |v.r|/r, the limiter is doubled, the
+ * fluxes carry mass and are boosted back from the face frame, and there is
+ * no entropic face correction.  Writes Fv[3], Fp, Frho (times the face area),
+ * vsig; Fv_abs / Fp_abs (may be NULL): the sums of |terms| behind them. */
+static int force_pair_mfv(const orc_params *p, const orc_state *s, const force_i_t *fi, const gconst *c, int j,
+                          const double dp[3], double r2, routv *ro, const double w_i[3], double Fv_out[3],
+                          double *Fp_out, double *Frho_out, double *vsig_out, int *mask_out, double Fv_abs[3],
+                          double *Fp_abs_out, double *Frho_abs_out)
+{
+    int status = 0, mask = 0;
+    const double *v_i = fi->v;
+    double rho_i = fi->rho;
+    double h_j = s->hsml[j];
+    double r = sqrt(r2);
+    double rinv = 1 / r;
+    const double *v_j = s->vel + 3 * (size_t)j;
+    double dv[3] = {v_i[0] - v_j[0], v_i[1] - v_j[1], v_i[2] - v_j[2]};
+    double rho_j = s->density[j], P_j = s->pressure[j];
+    double sound_i = fi->sound;
+    double sound_j = sqrt(p->gamma * P_j / rho_j);
+    double vsig = sound_i + sound_j;
+    double vdotr2 = dp[0] * dv[0] + dp[1] * dv[1] + dp[2] * dv[2];
+    if (vdotr2 < 0) vsig -= 3 * 1.0 * vdotr2 * rinv; /* hydro_evaluate.h:240-242, fac_mu = 1 */
+    double wk_i, dwk_i, wk_j, dwk_j, u, hinv_j, hinv3_j, hinv4_j;
+    if (r < fi->h) { u = r * fi->hinv; kernel_main(u, fi->hinv3, fi->hinv4, &wk_i, &dwk_i, 0); }
+    else { dwk_i = 0; wk_i = 0; }
+    if (r < h_j) {
+        kernel_hinv(h_j, &hinv_j, &hinv3_j, &hinv4_j);
+        u = r * hinv_j;
+        kernel_main(u, hinv3_j, hinv4_j, &wk_j, &dwk_j, 0);
+    } else { dwk_j = 0; wk_j = 0; }
+
+    double Fluxes_rho = 0, Fluxes_p = 0, Fluxes_v[3] = {0, 0, 0}, Fp_abs = 0, Fva[3] = {0, 0, 0}, Frho_abs = 0;
+    double leak_vs_tol = 0.5 * (fi->fce + s->fce[j]);
+    double face_vel_i = 0, face_vel_j = 0, Face_Area_Norm = 0;
+    double V_i = fi->V, V_j = s->mass[j] / rho_j;
+    const double *T_i = fi->T, *T_j = s->nvt + 9 * (size_t)j;
+    double cn_j = s->cn[j];
+    double Face_Area_Vec[3];
+    double wt_i = V_i, wt_j = V_j;
+    if ((fabs(V_i - V_j) / DMIN(V_i, V_j)) / NUMDIMS > 1.25) {
+        wt_i = wt_j = V_i * V_j * (wk_i + wk_j) / (V_i * wk_i + V_j * wk_j);
+    }
+    double facenormal_dot_dp = 0;
+    for (int k = 0; k < 3; k++) {
+        Face_Area_Vec[k] = wk_i * wt_i * (T_i[3 * k + 0] * dp[0] + T_i[3 * k + 1] * dp[1] + T_i[3 * k + 2] * dp[2])
+                         + wk_j * wt_j * (T_j[3 * k + 0] * dp[0] + T_j[3 * k + 1] * dp[1] + T_j[3 * k + 2] * dp[2]);
+        Face_Area_Norm += Face_Area_Vec[k] * Face_Area_Vec[k];
+        facenormal_dot_dp += Face_Area_Vec[k] * dp[k];
+    }
+    if ((cn_j * cn_j > 1.0e12 + fi->cnumcrit2) || (facenormal_dot_dp < 0)) {
+        Face_Area_Norm = -(wt_i * V_i * dwk_i + wt_j * V_j * dwk_j) / r;
+        Face_Area_Vec[0] = Face_Area_Norm * dp[0];
+        Face_Area_Vec[1] = Face_Area_Norm * dp[1];
+        Face_Area_Vec[2] = Face_Area_Norm * dp[2];
+        Face_Area_Norm = Face_Area_Norm * Face_Area_Norm * r2;
+    }
+    Face_Area_Norm = sqrt(Face_Area_Norm);
+    if (Face_Area_Norm != 0) {
+        double n_unit[3];
+        for (int k = 0; k < 3; k++) n_unit[k] = Face_Area_Vec[k] / Face_Area_Norm;
+        double s_i = 0.5 * r, s_j = -0.5 * r, s_star_ij = 0;
+        s_i = s_star_ij - s_i;
+        s_j = s_star_ij - s_j;
+        double distance_from_i[3], distance_from_j[3], v_frame[3];
+        distance_from_i[0] = dp[0] * rinv; distance_from_i[1] = dp[1] * rinv; distance_from_i[2] = dp[2] * rinv;
+        for (int k = 0; k < 3; k++) { distance_from_j[k] = distance_from_i[k] * s_j; distance_from_i[k] *= s_i; }
+        for (int k = 0; k < 3; k++) v_frame[k] = rinv * (-s_i * v_j[k] + s_j * w_i[k]);
+        for (int k = 0; k < 3; k++) { face_vel_i += v_i[k] * n_unit[k]; face_vel_j += v_j[k] * n_unit[k]; }
+        double v2_approach = 0;
+        double vdotr2_phys = vdotr2;
+        vdotr2_phys *= 1 / (r * 1.0);
+        if (vdotr2_phys < 0) v2_approach = vdotr2_phys * vdotr2_phys;
+        double vdotf2_phys = face_vel_i - face_vel_j;
+        if (vdotf2_phys < 0) v2_approach = DMAX(v2_approach, vdotf2_phys * vdotf2_phys);
+        int recon_mode = 1;
+        if (leak_vs_tol > 1) recon_mode = 0;
+        rstate RL, RR;
+        memset(&RL, 0, sizeof(RL));
+        memset(&RR, 0, sizeof(RR));
+        reconstruct_face_states(rho_i, fi->gr, rho_j, s->grad_rho + 3 * (size_t)j,
+                                distance_from_i, distance_from_j, &RL.rho, &RR.rho, recon_mode);
+        reconstruct_face_states(fi->P, fi->gp, P_j, s->grad_p + 3 * (size_t)j,
+                                distance_from_i, distance_from_j, &RL.p, &RR.p, recon_mode);
+        for (int k = 0; k < 3; k++) {
+            reconstruct_face_states(v_i[k], fi->gv + 3 * k, v_j[k], s->grad_v + 9 * (size_t)j + 3 * k,
+                                    distance_from_i, distance_from_j, &RL.v[k], &RR.v[k], recon_mode);
+            RL.v[k] -= v_frame[k];
+            RR.v[k] -= v_frame[k];
+        }
+        double press_i_tot = fi->P + rho_i * v2_approach;
+        double press_j_tot = P_j + rho_j * v2_approach;
+        double press_tot_limiter = 1.1 * 1.0 * DMAX(press_i_tot, press_j_tot);
+        press_tot_limiter *= 2.0; /* hydro_core_meshless.h:194-196 */
+        if (recon_mode == 0)
+            press_tot_limiter = DMAX(press_tot_limiter, DMAX(DMAX(fi->P, P_j), 2. * DMAX(rho_i, rho_j) * v2_approach));
+        int err = 0;
+        mask = riemann_solver_mfv(RL, RR, ro, n_unit, press_tot_limiter, c, &err);
+        if ((ro->P_M < 0) || (isnan(ro->P_M)) || (ro->P_M > 1.4 * press_tot_limiter)) {
+            mask |= 8;
+            RR.p = fi->P; RL.p = P_j;
+            RR.rho = rho_i; RL.rho = rho_j;
+            for (int k = 0; k < 3; k++) { RR.v[k] = v_i[k] - v_frame[k]; RL.v[k] = v_j[k] - v_frame[k]; }
+            mask |= riemann_solver_mfv(RL, RR, ro, n_unit, 1.4 * press_tot_limiter, c, &err);
+            if ((ro->P_M < 0) || (isnan(ro->P_M))) {
+                RR.p = fi->P; RL.p = P_j;
+                RR.rho = rho_i; RL.rho = rho_j;
+                for (int k = 0; k < 3; k++) { RR.v[k] = 0; RL.v[k] = 0; }
+                mask |= riemann_solver_mfv(RL, RR, ro, n_unit, 2.0 * press_tot_limiter, c, &err);
+                if ((ro->P_M < 0) || (isnan(ro->P_M))) status = 1234;
+            }
+        }
+        if (err) status = err;
+        if ((ro->P_M > 0) && (!isnan(ro->P_M))) {
+            /* error scale of the mass flux: the terms rho (v - S) it is made of,
+             * ~rho (|v| + c) in the face frame.  Where the flux cancels to
+             * rounding (a uniform flow), |F_rho| itself is no scale at all. */
+            double dvi = 0, dvj = 0;
+            for (int k = 0; k < 3; k++) { dvi += fabs(v_i[k] - v_frame[k]); dvj += fabs(v_j[k] - v_frame[k]); }
+            const double frho_abs = fabs(ro->Frho) + DMAX(rho_i, rho_j) * (dvi + dvj + sound_i + sound_j);
+            Frho_abs = Face_Area_Norm * frho_abs;
+            /* de-boost from the face frame (hydro_core_meshless.h:285-295) */
+            Fp_abs = fabs(ro->Fp);
+            for (int k = 0; k < 3; k++) {
+                Fva[k] = fabs(ro->Fv[k]) + fabs(v_frame[k]) * frho_abs;
+                Fp_abs += fabs(v_frame[k] * ro->Fv[k]) + (0.5 * v_frame[k] * v_frame[k]) * frho_abs;
+                ro->Fp += v_frame[k] * ro->Fv[k];
+                ro->Fp += (0.5 * v_frame[k] * v_frame[k]) * ro->Frho;
+                ro->Fv[k] += v_frame[k] * ro->Frho;
+            }
+            Fluxes_rho = Face_Area_Norm * ro->Frho;
+            Fluxes_p = Face_Area_Norm * ro->Fp;
+            for (int k = 0; k < 3; k++) Fluxes_v[k] = Face_Area_Norm * ro->Fv[k];
+            Fp_abs *= Face_Area_Norm;
+            for (int k = 0; k < 3; k++) Fva[k] *= Face_Area_Norm;
+        }
+    }
+    Fv_out[0] = Fluxes_v[0]; Fv_out[1] = Fluxes_v[1]; Fv_out[2] = Fluxes_v[2];
+    *Fp_out = Fluxes_p;
+    *Frho_out = Fluxes_rho;
+    *vsig_out = vsig;
+    if (mask_out) *mask_out = mask;
+    if (Fv_abs) for (int k = 0; k < 3; k++) Fv_abs[k] = Fva[k];
+    if (Fp_abs_out) *Fp_abs_out = Fp_abs;
+    if (Frho_abs_out) *Frho_abs_out = Frho_abs;
+    return status;
+}
+
 int orc_force_pair(const orc_params *p, const orc_state *s, int i, int j, double out[6])
 {
     const gconst c = make_gconst(p->gamma);
@@ -1546,6 +2058,129 @@ int orc_force_pair(const orc_params *p, const orc_state *s, int i, int j, double
     return stt;
 }
 
+/* One active particle's MFV force loop and final operations
+ * (hydro_evaluate.h:146-498 with the mass exchange of 379-401,
+ * hydro_toplevel.c:652-767 with the MFV terms of 661-665 and 742-752). */
+static int mfv_particle(const orc_params *p, orc_state *s, const gconst *c, const grid_t *g, double hmax, nb_t *b,
+                        int i, long long *nint, long long *nretry, long long *nexact)
+{
+    const int n = s->n;
+    int status = 0;
+    double Acc[3] = {0, 0, 0}, DtE = 0, DtMass = 0, dMass = 0, GravWork[3] = {0, 0, 0};
+    double MaxSignalVel = MIN_REAL_NUMBER;
+    double acc_l1[3] = {0, 0, 0}, dte_l1 = 0, dtmass_l1 = 0, dmass_l1 = 0, gw_l1[3] = {0, 0, 0};
+    double m_i = s->mass[i], rho_i = s->density[i];
+    const double *w_i = s->particle_vel + 3 * (size_t)i;
+    const long long ti_i = s->ti_step ? s->ti_step[i] : 1;
+    const double dt_i = (double)ti_i * p->timebase_interval; /* hydro_evaluate.h:82 */
+    force_i_t fi;
+    force_i_setup(p, s, i, &fi);
+    if ((m_i > 0) && (rho_i > 0)) {
+        double out_vsig = fi.sound;
+        routv ro;
+        memset(&ro, 0, sizeof(ro));
+        nb_collect(b, p, g, n, s->pos, s->hsml, i, fi.h, DMAX(fi.h, hmax), 1);
+        for (int q = 0; q < b->cnt; q++) {
+            int j = b->j[q];
+            if (s->mass[j] <= 0) continue;
+            if (s->density[j] <= 0) continue;
+            (*nint)++;
+            const double *dp = b->dp + 3 * q;
+            double Fv[3], Fp, Frho, vsig, Fva[3], Fpa, Frhoa;
+            int mask = 0;
+            int e = force_pair_mfv(p, s, &fi, c, j, dp, b->r2[q], &ro, w_i, Fv, &Fp, &Frho, &vsig, &mask, Fva, &Fpa,
+                                   &Frhoa);
+            if (e > status) status = e;
+            if (mask & 8) (*nretry)++;
+            if (mask & 4) (*nexact)++;
+            /* mass exchange (hydro_evaluate.h:379-392), j_is_active_for_fluxes = 0 */
+            const long long ti_j = s->ti_step ? s->ti_step[j] : 1;
+            double dmass_holder = Frho * dt_i, dmass_limiter;
+            if (dmass_holder > 0) dmass_limiter = s->mass[j]; else dmass_limiter = m_i;
+            dmass_limiter *= 0.1;
+            if (fabs(dmass_holder) > dmass_limiter) dmass_holder *= dmass_limiter / fabs(dmass_holder);
+            if (ti_i < ti_j) {
+                dMass += dmass_holder;
+#pragma omp atomic
+                s->d_mass[j] -= dmass_holder;
+                dmass_l1 += Frhoa * dt_i;
+                if (s->dmass_l1) {
+#pragma omp atomic
+                    s->dmass_l1[j] += Frhoa * dt_i;
+                }
+            }
+            if (ti_i == ti_j) {
+                dMass += 0.5 * dmass_holder;
+#pragma omp atomic
+                s->d_mass[j] -= 0.5 * dmass_holder;
+                dmass_l1 += 0.5 * Frhoa * dt_i;
+                if (s->dmass_l1) {
+#pragma omp atomic
+                    s->dmass_l1[j] += 0.5 * Frhoa * dt_i;
+                }
+            }
+            DtMass += Frho;
+            dtmass_l1 += Frhoa;
+            for (int k = 0; k < 3; k++) {
+                GravWork[k] += Frho * dp[k];
+                gw_l1[k] += Frhoa * fabs(dp[k]);
+            }
+            for (int k = 0; k < 3; k++) Acc[k] += Fv[k];
+            DtE += Fp;
+            for (int k = 0; k < 3; k++) acc_l1[k] += Fva[k];
+            dte_l1 += Fpa;
+            if (vsig > out_vsig) out_vsig = vsig;
+            if (s->wakeup && !s->is_active[j]) {
+                if (vsig > WAKEUP * s->vsig[j]) {
+#pragma omp atomic write
+                    s->wakeup[j] = 1;
+                }
+            }
+        }
+        if (MaxSignalVel < out_vsig) MaxSignalVel = out_vsig;
+    }
+    if (m_i > 0) {
+        /* hydro_final_operations_and_cleanup (hydro_toplevel.c:661-665, 738-764) */
+        const double *v = fi.v;
+        double v2_p = MaxSignalVel * MaxSignalVel;
+        for (int k = 0; k < 3; k++) v2_p += (v[k] - w_i[k]) * (v[k] - w_i[k]);
+        MaxSignalVel = sqrt(v2_p);
+        for (int k = 0; k < 3; k++) {
+            DtE -= (v[k] / 1.0) * Acc[k];
+            DtE += 0.5 * (v[k] / 1.0) * (v[k] / 1.0) * DtMass;
+            Acc[k] -= (v[k] / 1.0) * DtMass;
+            Acc[k] /= m_i;
+            dte_l1 += fabs(v[k]) * acc_l1[k] + 0.5 * v[k] * v[k] * dtmass_l1;
+            acc_l1[k] += fabs(v[k]) * dtmass_l1;
+            acc_l1[k] /= m_i;
+        }
+        DtE -= s->u[i] * DtMass;
+        dte_l1 += fabs(s->u[i]) * dtmass_l1;
+        DtE /= m_i;
+        dte_l1 /= m_i;
+        if (fi.h >= 0.99 * p->max_hsml) DtE = 0;
+    }
+    s->acc[3 * (size_t)i + 0] = Acc[0];
+    s->acc[3 * (size_t)i + 1] = Acc[1];
+    s->acc[3 * (size_t)i + 2] = Acc[2];
+    s->dtu[i] = DtE;
+    s->vsig[i] = MaxSignalVel;
+    s->dt_mass[i] = DtMass;
+#pragma omp atomic
+    s->d_mass[i] += dMass;
+    for (int k = 0; k < 3; k++) s->grav_work[3 * (size_t)i + k] = GravWork[k];
+    if (s->acc_l1) for (int k = 0; k < 3; k++) s->acc_l1[3 * (size_t)i + k] = acc_l1[k];
+    if (s->dtu_l1) s->dtu_l1[i] = dte_l1;
+    if (s->dtmass_l1) s->dtmass_l1[i] = dtmass_l1;
+    if (s->dmass_l1) {
+#pragma omp atomic
+        s->dmass_l1[i] += dmass_l1;
+    }
+    if (s->gravwork_l1) for (int k = 0; k < 3; k++) s->gravwork_l1[3 * (size_t)i + k] = gw_l1[k];
+    if (s->tie_flag) s->tie_flag[i] = 0;
+    return status;
+}
+
 int orc_hydro_force(const orc_params *p, orc_state *s, const int *active, int n_active, orc_stats *st)
 {
     const int n = s->n;
@@ -1555,6 +2190,19 @@ int orc_hydro_force(const orc_params *p, orc_state *s, const int *active, int n_
     double hmax = max_hsml(n, s->hsml);
     long long nint = 0, nretry = 0, nexact = 0;
     int status = 0;
+    const int mfv = (p->method == 1);
+    if (mfv) {
+        /* hydro_force_initial_operations_preloop (hydro_toplevel.c:968-983):
+         * DtMass, dMass, GravWorkTerm of the active particles start at zero;
+         * dMass of every neighbour, active or not, then receives its share */
+        for (int a = 0; a < n_active; a++) {
+            int i = active[a];
+            s->dt_mass[i] = 0;
+            s->d_mass[i] = 0;
+            for (int k = 0; k < 3; k++) s->grav_work[3 * (size_t)i + k] = 0;
+            if (s->dmass_l1) s->dmass_l1[i] = 0;
+        }
+    }
     set_threads(p);
 #pragma omp parallel reduction(+ : nint, nretry, nexact) reduction(max : status)
     {
@@ -1562,6 +2210,11 @@ int orc_hydro_force(const orc_params *p, orc_state *s, const int *active, int n_
 #pragma omp for schedule(dynamic, 64)
         for (int a = 0; a < n_active; a++) {
             int i = active[a];
+            if (mfv) {
+                int e = mfv_particle(p, s, &c, &g, hmax, &b, i, &nint, &nretry, &nexact);
+                if (e > status) status = e;
+                continue;
+            }
             /* preloop (hydro_toplevel.c:968-976) */
             double Acc[3] = {0, 0, 0}, DtE = 0, MaxSignalVel = MIN_REAL_NUMBER;
             double acc_l1[3] = {0, 0, 0}, dte_l1 = 0;

@@ -35,6 +35,11 @@ typedef struct orc_params {
     int brute_force;        /* 1: O(N^2) neighbour search (truth), 0: cell grid or octree */
     int tree;               /* 1: octree walk with node hsml maxima, the reference's search
                                (ngb.c:102-130); 0: cell grid */
+    int method;             /* 0: meshless finite mass (HYDRO_MESHLESS_FINITE_MASS); 1: meshless
+                               finite volume (HYDRO_MESHLESS_FINITE_VOLUME) with the smoothed
+                               mesh motion HYDRO_FIX_MESH_MOTION = 5 (allvars.h:94-98) */
+    double timebase_interval; /* MFV: physical time of one integer time unit,
+                               UNIT_INTEGERTIME_IN_PHYSICAL (allvars.h:908) */
 } orc_params;
 
 typedef struct orc_state {
@@ -85,6 +90,23 @@ typedef struct orc_state {
      * unjittered lattice) the differences are pure rounding and grad_l1 is
      * itself rounding-sized. */
     double *grad_in;          /* [n][15] */
+    /* MFV only (method 1): ParticleVel (density output, force input) [n][3];
+     * DtMass, dMass (in/out: inactive particles accumulate their share),
+     * GravWorkTerm [n][3]; ti_step = P[].dt_step, the integer timestep
+     * (NULL: equal for every particle) */
+    double *particle_vel;
+    double *dt_mass;
+    double *d_mass;
+    double *grav_work;
+    const long long *ti_step;
+    /* optional: sums of |terms| of the MFV outputs (error scales; a pair's
+     * mass-flux term counts as rho (|v - v_face| + c) times the face area,
+     * the size of the terms the flux is computed from, so that a flux that
+     * cancels to rounding in a uniform flow has a meaningful scale) */
+    double *pvel_l1;          /* [n][3] */
+    double *dtmass_l1;        /* [n] */
+    double *dmass_l1;         /* [n] */
+    double *gravwork_l1;      /* [n][3] */
 } orc_state;
 
 typedef struct orc_stats {
@@ -113,6 +135,17 @@ int    orc_riemann(double gamma, const double L[5], const double R[5], const dou
 /* The Kurganov-Tadmor/PWK flux alone (reimann.h:484-519), MFM branch:
  * out = P_M, S_M, F_v[3], F_p. */
 void   orc_riemann_kt(double gamma, const double L[5], const double R[5], const double n[3], double out[6]);
+/* Riemann_solver() for an MFV run (reimann.h:286-388 with the MFV branches):
+ * out = P_M, S_M, F_rho, F_v[3], F_p in the frame of the states; returns the
+ * stage mask (1 HLLC with the HLLC fluxes, 2 KT, 4 exact) or -1234545. */
+int    orc_riemann_mfv(double gamma, const double L[5], const double R[5], const double n[3],
+                       double press_tot_limiter, double out[7]);
+/* The KT flux alone, MFV branch (reimann.h:484-507): out as above. */
+void   orc_riemann_kt_mfv(double gamma, const double L[5], const double R[5], const double n[3], double out[7]);
+/* The exact solver alone with the MFV sampling at the face (reimann.h:685-724,
+ * 731-1030, 1146-1166): out as above; returns 0 or 1234545. */
+int    orc_riemann_exact_mfv(double gamma, const double L[5], const double R[5], const double n[3],
+                             double out[7]);
 /* One star-state estimate of get_wavespeeds_and_pressure_star
  * (reimann.h:527-593): which 0 Gaburov HLLC (Davis wave speeds), 1 Roe
  * averages, 2 PVRS; out = P_M, S_M, S_L, S_R. */

@@ -33,6 +33,8 @@ class Params(C.Structure):
         ("nthreads", C.c_int),
         ("brute_force", C.c_int),
         ("tree", C.c_int),
+        ("method", C.c_int),
+        ("timebase_interval", C.c_double),
     ]
 
 
@@ -47,6 +49,9 @@ class State(C.Structure):
         ("wakeup", _ub), ("is_active", _ub),
         ("divvel_l1", _dp), ("fce_l1", _dp), ("grad_l1", _dp), ("acc_l1", _dp), ("dtu_l1", _dp),
         ("tie_flag", _ub), ("grad_in", _dp),
+        ("particle_vel", _dp), ("dt_mass", _dp), ("d_mass", _dp), ("grav_work", _dp),
+        ("ti_step", C.POINTER(C.c_longlong)),
+        ("pvel_l1", _dp), ("dtmass_l1", _dp), ("dmass_l1", _dp), ("gravwork_l1", _dp),
     ]
 
 
@@ -90,6 +95,12 @@ def lib():
         L.orc_pstar_estimate.restype = None
         L.orc_wavespeeds.argtypes = [C.c_double, _dp, _dp, _dp, C.c_double, _dp]
         L.orc_wavespeeds.restype = C.c_int
+        L.orc_riemann_mfv.argtypes = [C.c_double, _dp, _dp, _dp, C.c_double, _dp]
+        L.orc_riemann_mfv.restype = C.c_int
+        L.orc_riemann_kt_mfv.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_riemann_kt_mfv.restype = None
+        L.orc_riemann_exact_mfv.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_riemann_exact_mfv.restype = C.c_int
         L.orc_hsml_step.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double,
                                     C.c_double, C.c_double]
         L.orc_hsml_step.restype = C.c_int
@@ -109,9 +120,12 @@ def ptr(a: np.ndarray, t=_dp):
     return a.ctypes.data_as(t)
 
 
+METHOD_MFM, METHOD_MFV = 0, 1
+
+
 def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_ngb_dev=0.05,
                 min_hsml=0.0, max_hsml=1e56, first_step=1, maxiter=150, nthreads=0,
-                brute_force=0, tree=0) -> Params:
+                brute_force=0, tree=0, method=METHOD_MFM, timebase_interval=2.0 ** -20) -> Params:
     p = Params()
     p.box[:] = list(box)
     p.gamma = gamma
@@ -124,6 +138,8 @@ def make_params(box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, des_num_ngb=32.0, max_num_
     p.nthreads = nthreads
     p.brute_force = brute_force
     p.tree = tree
+    p.method = method
+    p.timebase_interval = timebase_interval
     return p
 
 
@@ -177,6 +193,39 @@ def riemann_kt(L, R, n=(1.0, 0.0, 0.0), gamma=5.0 / 3.0):
     out = np.zeros(6)
     lib().orc_riemann_kt(gamma, ptr(Lc), ptr(Rc), ptr(nc), ptr(out))
     return dict(P_M=out[0], S_M=out[1], Fv=out[2:5].copy(), Fp=out[5])
+
+
+def _mfv_out(out, mask=None):
+    d = dict(P_M=out[0], S_M=out[1], Frho=out[2], Fv=out[3:6].copy(), Fp=out[6])
+    if mask is not None:
+        d["mask"] = mask
+    return d
+
+
+def riemann_mfv(L, R, n=(1.0, 0.0, 0.0), press_tot_limiter=1e300, gamma=5.0 / 3.0):
+    """Riemann_solver() of an MFV run: dict(P_M, S_M, Frho, Fv, Fp, mask), fluxes
+    through a face at rest in the frame of L and R."""
+    Lc, Rc, nc = _states(L, R, n)
+    out = np.zeros(7)
+    mask = lib().orc_riemann_mfv(gamma, ptr(Lc), ptr(Rc), ptr(nc), press_tot_limiter, ptr(out))
+    return _mfv_out(out, mask)
+
+
+def riemann_kt_mfv(L, R, n=(1.0, 0.0, 0.0), gamma=5.0 / 3.0):
+    """The KT flux alone, MFV branch: dict(P_M, S_M, Frho, Fv, Fp)."""
+    Lc, Rc, nc = _states(L, R, n)
+    out = np.zeros(7)
+    lib().orc_riemann_kt_mfv(gamma, ptr(Lc), ptr(Rc), ptr(nc), ptr(out))
+    return _mfv_out(out)
+
+
+def riemann_exact_mfv(L, R, n=(1.0, 0.0, 0.0), gamma=5.0 / 3.0):
+    """The exact solver with the face sampling (Godunov flux): dict(P_M, S_M,
+    Frho, Fv, Fp, mask) with mask the status (0 or 1234545)."""
+    Lc, Rc, nc = _states(L, R, n)
+    out = np.zeros(7)
+    e = lib().orc_riemann_exact_mfv(gamma, ptr(Lc), ptr(Rc), ptr(nc), ptr(out))
+    return _mfv_out(out, e)
 
 
 PSTAR_HLLC, PSTAR_ROE, PSTAR_PVRS = 0, 1, 2
@@ -246,6 +295,16 @@ class OracleState:
         self.acc_l1 = z(N, 3)
         self.dtu_l1 = z(N)
         self.tie_flag = np.zeros(N, dtype=np.uint8)
+        self.particle_vel = np.ascontiguousarray(ic.get("particle_vel", self.vel), dtype=np.float64).copy()
+        self.dt_mass = np.ascontiguousarray(ic.get("dt_mass", z(N)), dtype=np.float64).copy()
+        self.d_mass = np.ascontiguousarray(ic.get("d_mass", z(N)), dtype=np.float64).copy()
+        self.grav_work = np.ascontiguousarray(ic.get("grav_work", z(N, 3)), dtype=np.float64).copy()
+        self.ti_step = (None if ic.get("ti_step") is None
+                        else np.ascontiguousarray(ic["ti_step"], dtype=np.int64).copy())
+        self.pvel_l1 = z(N, 3)
+        self.dtmass_l1 = z(N)
+        self.dmass_l1 = z(N)
+        self.gravwork_l1 = z(N, 3)
         if active is None:
             active = np.arange(N, dtype=np.int32)
         self.active = np.ascontiguousarray(active, dtype=np.int32)
@@ -263,12 +322,19 @@ class OracleState:
         for name in ("divvel_l1", "fce_l1", "grad_l1", "acc_l1", "dtu_l1", "grad_in"):
             setattr(s, name, ptr(getattr(self, name)))
         s.tie_flag = ptr(self.tie_flag, _ub)
+        for name in ("particle_vel", "dt_mass", "d_mass", "grav_work", "pvel_l1", "dtmass_l1", "dmass_l1",
+                     "gravwork_l1"):
+            setattr(s, name, ptr(getattr(self, name)))
+        if self.ti_step is not None:
+            s.ti_step = self.ti_step.ctypes.data_as(C.POINTER(C.c_longlong))
         return s
 
     def outputs(self) -> dict:
         names = ("hsml", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "cn", "fce",
                  "grad_rho", "grad_p", "grad_v", "acc", "dtu", "vsig", "wakeup",
-                 "divvel_l1", "fce_l1", "grad_l1", "acc_l1", "dtu_l1", "tie_flag", "grad_in")
+                 "divvel_l1", "fce_l1", "grad_l1", "acc_l1", "dtu_l1", "tie_flag", "grad_in",
+                 "particle_vel", "dt_mass", "d_mass", "grav_work", "pvel_l1", "dtmass_l1", "dmass_l1",
+                 "gravwork_l1")
         return {k: getattr(self, k).copy() for k in names}
 
 

@@ -152,3 +152,33 @@ def error_stats(out, ref, active=None):
     stats["n_tied"] = int(tie.sum())
     stats["n_particles"] = int(len(sel(ref, "hsml")))
     return stats
+
+
+# MFV outputs (method 1): ParticleVel against the sum of |terms| of its
+# weighted mean, DtMass / dMass / GravWorkTerm against the sums of |pair
+# terms| (dMass: a particle's own shares and those its neighbours give it)
+MFV_FIELDS = (("particle_vel", "pvel_l1"), ("dt_mass", "dtmass_l1"), ("d_mass", "dmass_l1"),
+              ("grav_work", "gravwork_l1"))
+
+
+def compare_mfv(out, ref, active=None):
+    """Errors of the MFV outputs of a GPU pass against the oracle's (over the
+    active particles, or every particle for d_mass, which inactive particles
+    receive too); plus compare_pass's errors of the shared outputs."""
+    e = compare_pass(out, ref, active)
+    idx = slice(None) if active is None else np.asarray(active)
+    for k, l1 in MFV_FIELDS:
+        sel = slice(None) if k == "d_mass" else idx
+        e[k] = l1_err(np.asarray(out[k])[sel], np.asarray(ref[k])[sel], np.asarray(ref[l1])[sel])
+    return e
+
+
+def check_mfv(out, ref, active=None, tol_density=TOL_DENSITY, tol_flux=TOL_FLUX):
+    e = compare_mfv(out, ref, active)
+    limits = {k: tol_density for k in ("hsml", "density", "pressure", "num_ngb", "dhsml", "cn", "vsig")}
+    limits.update({k: tol_flux for k in ("nvt", "divvel", "fce", "gradients", "acc", "dtu")})
+    limits.update({k: tol_flux for k, _ in MFV_FIELDS})
+    limits["dtu_tied"] = 0.0  # MFV has no entropic switch, so no tied particles
+    limits["n_tied"] = 0
+    bad = {k: (v, limits[k]) for k, v in e.items() if not (v <= limits[k])}
+    return e, bad
