@@ -58,6 +58,11 @@ BYTES_PER_PARTICLE = {"density": 64 + 136, "gradients": 128 + 120, "force": 272 
 BYTES_PER_PARTICLE_F32 = {"density": 40 + 136, "gradients": 96 + 120, "force": 160 + 40}
 
 
+# meshless finite-volume pass: every particle on the integer timestep 2^12
+# of a 2^-20 time unit (dt = 1/256), which sets the exchanged masses
+MFV_TIMEBASE = 2.0 ** -20
+MFV_TI_STEP = 2 ** 12
+
 _T0 = time.perf_counter()
 
 
@@ -104,6 +109,8 @@ def parse():
     ap.add_argument("--fp32-density", type=int, default=1,
                     help="batch workload: also time precision 2 (FP32 density pair sums)")
     ap.add_argument("--blast", type=int, default=1, help="batch workload: also time the 128^3 blast case")
+    ap.add_argument("--mfv", type=int, default=1,
+                    help="batch workload: also time the meshless finite-volume pass (FP64) on the same boxes")
     ap.add_argument("--autotune", type=int, default=1,
                     help="batch workload: pick cell size, pair-list capacity, workgroup sizes and the density "
                          "load-balance thresholds per mode on a sample of boxes")
@@ -327,14 +334,16 @@ def run_plain_per_box(gzm, ic, tuned=None):
             "boxes": nb}
 
 
-def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
+def run_graph_mode(gzm, ic, args, precision, steps, tuned=None, method=0):
     """Warm-up eager passes (per-kernel HIP events), then `steps` passes in one
-    HIP graph (timed with events around the launch)."""
+    HIP graph (timed with events around the launch).  method 1: the
+    meshless finite-volume pass."""
     N = ic["pos"].shape[0]
+    mk = dict(precision=precision, method=method, timebase_interval=MFV_TIMEBASE)
     if tuned:
-        params = gzm.make_params(precision=precision, **tuned["params"])
+        params = gzm.make_params(**mk, **tuned["params"])
     else:
-        params = gzm.make_params(cell_size=args.cell_size, precision=precision)
+        params = gzm.make_params(cell_size=args.cell_size, **mk)
     ctx = gzm.MFMContext(params)
     ctx.upload(ic)
     ctx.timing_reset()
@@ -387,20 +396,22 @@ def run_graph_mode(gzm, ic, args, precision, steps, tuned=None):
         "exact_solver_calls": st.exact_solver_calls // steps,
         "fp64_force_particles": st.fp64_force_particles // steps,
         "density_fp64_redo": int(s1.density_fp64_redo),
+        "riemann_retries": st.riemann_retries // steps,
         "precision": precision,
     }
 
 
-def oracle_boxes(O, ic, cores):
+def oracle_boxes(O, ic, cores, method=0):
     """FP64 oracle pass box by box; returns concatenated outputs, interactions, seconds."""
     import ics
+    mk = dict(nthreads=cores, method=method, timebase_interval=MFV_TIMEBASE)
     if "box_offsets" not in ic:
         t0 = time.perf_counter()
-        ref, st, _ = O.run_pass(ic, params=O.make_params(box=tuple(ic["box"]), nthreads=cores))
+        ref, st, _ = O.run_pass(ic, params=O.make_params(box=tuple(ic["box"]), **mk))
         return ref, st.dens_interactions + st.grad_interactions + st.force_interactions, time.perf_counter() - t0
     offs = ic["box_offsets"]
     outs, inter, t0 = [], 0, time.perf_counter()
-    prm = O.make_params(nthreads=cores)
+    prm = O.make_params(**mk)
     for b in range(len(offs) - 1):
         one = ics.box_slice(ic, b)
         ref, st, _ = O.run_pass(one, params=prm)
@@ -478,6 +489,12 @@ def main_batch(args, gzm, ics):
         log(f"precision 2: {r32d['ms_per_step']:.2f} ms/pass")
     r64 = run_graph_mode(gzm, ic, args, 0, steps, tuned.get(0))
     log(f"precision 0: {r64['ms_per_step']:.2f} ms/pass")
+    ic_mfv = None
+    rmfv = None
+    if args.mfv:
+        ic_mfv = dict(ic, ti_step=np.full(N, MFV_TI_STEP, dtype=np.int64))
+        rmfv = run_graph_mode(gzm, ic_mfv, args, 0, steps, tuned.get(0), method=1)
+        log(f"meshless finite volume (FP64): {rmfv['ms_per_step']:.2f} ms/pass")
     plain = run_plain_per_box(gzm, ic, tuned.get(0)) if (args.plain and "box_offsets" in ic) else None
     if plain is not None:
         log(f"plain FP64, one box per pass: {plain['ms_per_pass']:.1f} ms for all boxes")
@@ -552,6 +569,20 @@ def main_batch(args, gzm, ics):
             "density_sweeps": r32d["density_sweeps"],
             "kernels_ms_per_pass_eager": r32d["kernels_ms_per_pass_eager"],
         }),
+        # the meshless finite-volume variant (gzm_params.method 1), FP64, same
+        # boxes and neighbour sets, mass fluxes through every face
+        "mfv_same_workload": (None if rmfv is None else {
+            "value": rmfv["value"],
+            "unit": "interactions/s",
+            "ms_per_step": rmfv["ms_per_step"],
+            "interactions_per_pass": int(rmfv["interactions_per_pass"]),
+            "interactions_equal_mfm": bool(rmfv["interactions_per_pass"] == r64["interactions_per_pass"]),
+            "exact_solver_calls_per_pass": int(rmfv["exact_solver_calls"]),
+            "riemann_retries_per_pass": int(rmfv["riemann_retries"]),
+            "vs_mfm_fp64": rmfv["value"] / r64["value"],
+            "kernels_ms_per_pass_eager": rmfv["kernels_ms_per_pass_eager"],
+            "timesteps": f"every particle on integer step {MFV_TI_STEP} of {MFV_TIMEBASE:g} time units",
+        }),
         "exact_solver_calls_per_pass": r64["exact_solver_calls"],
         # FP32 force loops redone in FP64: a pair's branch decision within FP32 rounding of its threshold
         "fp64_force_particles_per_pass": head["fp64_force_particles"],
@@ -575,7 +606,7 @@ def main_batch(args, gzm, ics):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle as O
-        from parity import compare_pass, error_stats
+        from parity import compare_mfv, compare_pass, error_stats
 
         cores = cpu_cores()
         if "box_offsets" in ic:
@@ -608,6 +639,14 @@ def main_batch(args, gzm, ics):
         errs = compare_pass(r64["out"], ref)
         result["fp64_same_workload"]["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
         result["fp64_same_workload"]["interactions_equal"] = bool(cpu_int == r64["interactions_per_pass"])
+        if rmfv is not None:
+            log(f"CPU oracle, meshless finite volume, on all {nb} box(es)")
+            ref_v, int_v, secs_v = oracle_boxes(O, ic_mfv, cores, method=1)
+            log(f"CPU oracle (MFV): {secs_v:.1f} s")
+            ev = compare_mfv(rmfv["out"], ref_v)
+            result["mfv_same_workload"]["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in ev.items()}
+            result["mfv_same_workload"]["interactions_equal_oracle"] = bool(int_v == rmfv["interactions_per_pass"])
+            result["mfv_same_workload"]["cpu_oracle_value"] = int_v / secs_v
     if args.blast and args.workload == "batch":
         b_args = argparse.Namespace(**vars(args))
         b_args.steps = 5

@@ -1297,8 +1297,18 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         a.list.cap = c->list_cap;
     }
     const bool mfv = c->p.method == 1;
+    MfvArgs m;
+    memset(&m, 0, sizeof(m));
+    if (mfv) {
+        m.pvel = c->d_pvel;
+        m.ti = c->d_ti;
+        m.timebase = c->p.timebase_interval;
+        m.dt_mass = c->d_dtmass;
+        m.d_mass = c->d_dmass;
+        m.grav_work = c->d_gravwork;
+    }
     // exact-solver pairs leave the list loop for a kernel of their own (lists of <= 64)
-    const bool three_phase = !mfv && use_list && c->list_cap <= 64 && c->n_active > 0;
+    const bool three_phase = use_list && c->list_cap <= 64 && c->n_active > 0;
     if (three_phase) {
         s = ensure_defer(c);
         if (s) return s;
@@ -1313,19 +1323,13 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         // a kernel, not a memset: this stage may be captured into a graph
         if (c->p.precision != 0) launch_fill_int(c->stream, c->d_redo_count, 2, 0);
         if (mfv) {
-            MfvArgs m;
-            m.pvel = c->d_pvel;
-            m.ti = c->d_ti;
-            m.timebase = c->p.timebase_interval;
-            m.dt_mass = c->d_dtmass;
-            m.d_mass = c->d_dmass;
-            m.grav_work = c->d_gravwork;
-            launch_force_mfv(c->stream, a, m);  // overflowed pair sets scan in the same kernel
+            launch_mfv_preloop(c->stream, a, m);
+            launch_force_mfv(c->stream, a, m);
         } else {
             auto launch = (c->p.precision != 0) ? launch_force_f32 : launch_force;
             launch(c->stream, a);
         }
-        if (!mfv && use_list && c->list_overflow_slices > 0) {
+        if ((!mfv || three_phase) && use_list && c->list_overflow_slices > 0) {
             // particles whose pair set overflowed the list: one wavefront each, FP64
             if (!c->d_long_list) {
                 s = dev_alloc(&c->d_long_list, (size_t)c->n);
@@ -1333,7 +1337,8 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
                 if (s) return s;
             }
             launch_fill_int(c->stream, c->d_long_count, 1, 0);
-            launch_force_long_lists(c->stream, a, c->d_long_list, c->d_long_count);
+            if (mfv) launch_force_mfv_overflow(c->stream, a, m, c->d_long_list, c->d_long_count);
+            else launch_force_long_lists(c->stream, a, c->d_long_list, c->d_long_count);
         }
     }
     GZM_HIP(hipGetLastError());
@@ -1365,7 +1370,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
                 dev_free(c->d_defer_owner);
                 dev_free(c->d_defer_kpos);
                 c->defer_result_cap = 0;
-                s = dev_alloc(&c->d_defer_result, want * 5);
+                s = dev_alloc(&c->d_defer_result, want * (mfv ? 9 : 5));
                 s = s ? s : dev_alloc(&c->d_defer_owner, want);
                 s = s ? s : dev_alloc(&c->d_defer_kpos, want);
                 if (s) return s;
@@ -1384,9 +1389,14 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
             a.defer.q_end = (int)q1;
             Timed tm(c, kForceDeferred);
             if (q1 > q0) launch_defer_owners(c->stream, a);
-            if (c->p.precision != 0) launch_force_deferred_f32(c->stream, a, (int)(q1 - q0));
-            else launch_force_deferred(c->stream, a, (int)(q1 - q0));
-            launch_force_finish(c->stream, a);
+            if (mfv) {
+                launch_force_mfv_deferred(c->stream, a, m, (int)(q1 - q0));
+                launch_force_mfv_finish(c->stream, a, m);
+            } else {
+                if (c->p.precision != 0) launch_force_deferred_f32(c->stream, a, (int)(q1 - q0));
+                else launch_force_deferred(c->stream, a, (int)(q1 - q0));
+                launch_force_finish(c->stream, a);
+            }
             if (D == 0) break;
         }
         GZM_HIP(hipGetLastError());

@@ -250,6 +250,7 @@ struct DeferArgs {
     const int *offset;     // exclusive scan of count
     const int *total;      // sum of count (device)
     double *result;        // per deferred pair of the round: dv/dt terms (3), du/dt term, v_sig
+                           // (MFV: 9 per pair, see force_mfv_deferred_kernel)
     int *owner;            // per deferred pair of the round: its active entry t and list position k,
     uint8_t *kpos;         //   written by launch_defer_owners before the round's pair kernel
     int q_begin, q_end;    // deferred pairs handled by this round
@@ -406,7 +407,14 @@ void launch_cell_hmax(hipStream_t s, int ncell, const int2 *range, const double 
 void launch_mfv_particle_vel(hipStream_t s, const BoxDesc *boxes, const int *box_of, const TreeArgs &tree,
                              int n_active, const int *active, const double4 *posm, const double4 *velu,
                              const double *hsml, const double *rho_sum, double *pvel);
+void launch_mfv_preloop(hipStream_t s, const ForceArgs &a, const MfvArgs &m);
+// list kernel with deferral when a.defer.mask is set (then also
+// launch_force_mfv_overflow for the pair sets the list could not hold, and
+// the deferred pairs' rounds), else the one-pass list or scanning kernel
 void launch_force_mfv(hipStream_t s, const ForceArgs &a, const MfvArgs &m);
+void launch_force_mfv_overflow(hipStream_t s, const ForceArgs &a, const MfvArgs &m, int *rows, int *n_rows);
+void launch_force_mfv_deferred(hipStream_t s, const ForceArgs &a, const MfvArgs &m, int n_pairs);
+void launch_force_mfv_finish(hipStream_t s, const ForceArgs &a, const MfvArgs &m);
 void launch_fill_int(hipStream_t s, int *p, int n, int value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value);
 void launch_fill_double(hipStream_t s, double *p, long long n, double value);

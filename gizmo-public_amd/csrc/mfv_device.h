@@ -197,9 +197,9 @@ __device__ inline void face_to_flux_mfv(ROutV &o, const double n[3], const Gamma
 
 // Riemann_solver_exact with the MFV sampling (reimann.h:685-724); returns 0
 // or 1234545
-__device__ __noinline__ int riemann_exact_mfv(const RState<double> &L, const RState<double> &R, ROutV &o,
-                                              const GammaConsts &G, const double *n, double vL, double vR,
-                                              double cL, double cR)
+__device__ __forceinline__ int riemann_exact_mfv_body(const RState<double> &L, const RState<double> &R, ROutV &o,
+                                                      const GammaConsts &G, const double *n, double vL, double vR,
+                                                      double cL, double cR)
 {
     if ((L.p == 0 && R.p == 0) || (L.rho == 0 && R.rho == 0)) {
         o.P_M = o.S_M = 0;
@@ -233,9 +233,42 @@ __device__ __noinline__ int riemann_exact_mfv(const RState<double> &L, const RSt
     return 0;
 }
 
+// out of line for the loops where it is rare (scalar arguments, result by value)
+struct ExactOutV {
+    ROutV o;
+    int status;
+};
+__device__ __noinline__ ExactOutV riemann_exact_mfv(double rL, double pL, double vLx, double vLy, double vLz, double rR,
+                                                    double pR, double vRx, double vRy, double vRz, double n0,
+                                                    double n1, double n2, double vL, double vR, double cL, double cR,
+                                                    GammaConsts G)
+{
+    const RState<double> L = {rL, pL, {vLx, vLy, vLz}}, R = {rR, pR, {vRx, vRy, vRz}};
+    const double n[3] = {n0, n1, n2};
+    ExactOutV e;
+    e.o = ROutV{0, 0, 0, {0, 0, 0}, 0};
+    e.status = riemann_exact_mfv_body(L, R, e.o, G, n, vL, vR, cL, cR);
+    return e;
+}
+
+__device__ __noinline__ ROutV riemann_kt_mfv_ool(double rL, double pL, double vLx, double vLy, double vLz, double rR,
+                                                 double pR, double vRx, double vRy, double vRz, double n0, double n1,
+                                                 double n2, double vL, double vR, double cL, double cR, double hL,
+                                                 double hR)
+{
+    const RState<double> L = {rL, pL, {vLx, vLy, vLz}}, R = {rR, pR, {vRx, vRy, vRz}};
+    const double n[3] = {n0, n1, n2};
+    ROutV o = {0, 0, 0, {0, 0, 0}, 0};
+    riemann_kt_mfv(L, R, o, n, vL, vR, cL, cR, hL, hR);
+    return o;
+}
+
 // Riemann_solver() for MFV hydro (reimann.h:286-388): HLLC with the HLLC
 // fluxes, the KT flux on a failed estimate, the exact solver past the
-// limiter.  Returns the stage mask (1 HLLC, 2 KT, 4 exact).
+// limiter.  Returns the stage mask (1 HLLC, 2 KT, 4 exact).  DEFER: stop
+// where the exact solver would run and return kDeferBit.  INLINE: the KT flux
+// and the exact solver inline (every lane may take them), else out of line.
+template <bool DEFER = false, bool INLINE = false>
 __device__ inline int riemann_solve_mfv(const RState<double> &L, const RState<double> &R, const double n[3],
                                         double limiter, const GammaConsts &G, ROutV &o, int &err)
 {
@@ -254,11 +287,23 @@ __device__ inline int riemann_solve_mfv(const RState<double> &L, const RState<do
     o.S_M = e.S_M;
     if ((o.P_M >= 0) && !isnan(o.P_M)) hllc_fluxes_mfv(L, R, o, n, vL, vR, hL, hR, S_LR[0], S_LR[1]);
     if ((o.P_M < 0) || isnan(o.P_M)) {
-        riemann_kt_mfv(L, R, o, n, vL, vR, cL, cR, hL, hR);
+        if (INLINE) riemann_kt_mfv(L, R, o, n, vL, vR, cL, cR, hL, hR);
+        else
+            o = riemann_kt_mfv_ool(L.rho, L.p, L.v[0], L.v[1], L.v[2], R.rho, R.p, R.v[0], R.v[1], R.v[2], n[0], n[1],
+                                   n[2], vL, vR, cL, cR, hL, hR);
         mask |= 2;
     }
     if ((o.P_M < 0) || isnan(o.P_M) || (o.P_M > limiter)) {
-        const int ee = riemann_exact_mfv(L, R, o, G, n, vL, vR, cL, cR);
+        if (DEFER) return mask | kDeferBit;
+        int ee;
+        if (INLINE) {
+            ee = riemann_exact_mfv_body(L, R, o, G, n, vL, vR, cL, cR);
+        } else {
+            const ExactOutV e = riemann_exact_mfv(L.rho, L.p, L.v[0], L.v[1], L.v[2], R.rho, R.p, R.v[0], R.v[1],
+                                                  R.v[2], n[0], n[1], n[2], vL, vR, cL, cR, G);
+            o = e.o;
+            ee = e.status;
+        }
         if (ee) err = ee;
         mask |= 4;
     }
@@ -274,7 +319,11 @@ __device__ inline int riemann_solve_mfv(const RState<double> &L, const RState<do
 // (hydro_core_meshless.h:194-196), the fluxes are boosted back from the face
 // frame (285-295) and there is no entropic face correction.  Writes the
 // pair's Fv[3], Fp, Frho times the face area, vsig and the stage mask (bit3
-// the first-order retry); returns 0, 1234 or 1234545.
+// the first-order retry); returns 0, 1234 or 1234545.  DEFER: returns
+// kDeferred, with no flux terms, for a pair that would call the exact solver
+// (every first-order retry follows an exact solution, so retries happen only
+// in the caller's later evaluation of such a pair with DEFER = false).
+template <bool DEFER = false, bool INLINE = false>
 __device__ inline int force_pair_mfv(const PairI<double> &I, const PairSide<double> &J, const double w_i[3],
                                      const double dp[3], double r2, const GammaConsts &G, double Fv[3], double &Fp,
                                      double &Frho, double &vsig, int &mask)
@@ -356,7 +405,8 @@ __device__ inline int force_pair_mfv(const PairI<double> &I, const PairSide<doub
     int err = 0;
     double lim_a = lim;
     for (int attempt = 0; attempt < 3; attempt++) {
-        mask |= riemann_solve_mfv(SL, SR, n, lim_a, G, ro, err);
+        mask |= riemann_solve_mfv<DEFER, INLINE>(SL, SR, n, lim_a, G, ro, err);
+        if (DEFER && (mask & kDeferBit)) return kDeferred;
         const bool bad = (ro.P_M < 0) || isnan(ro.P_M);
         if (attempt == 0) {
             if (!(bad || (ro.P_M > 1.4 * lim))) break;
