@@ -11,6 +11,7 @@
  */
 #include "gizmo_mfm_host.h"
 
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -27,6 +28,7 @@ int *NextActiveParticle = NULL;
 static gzm_ctx *g_ctx = NULL;
 static gzm_params g_params;      /* parameters g_ctx was created with */
 static int g_device = 0;
+static int g_method = 0;         /* 0 MFM, 1 MFV */
 static int g_status = 0;
 static int g_resident = 0;       /* the device holds this pass's particle set (uploaded by density()) */
 static gizmo_endrun_handler g_endrun = NULL;
@@ -36,6 +38,8 @@ static char g_err[512];
 static int g_cap = 0;
 static double *h_pos, *h_vel, *h_mass, *h_u, *h_hsml, *h_cn, *h_vsig, *h_num_ngb, *h_dhsml, *h_divvel;
 static double *h_rho, *h_pres, *h_nvt, *h_fce, *h_grho, *h_gp, *h_gv, *h_acc, *h_dtu;
+static double *h_pvel, *h_dtm, *h_dm, *h_gw;  /* MFV: ParticleVel, DtMass, dMass, GravWorkTerm */
+static int64_t *h_ti;                          /* MFV: integer timesteps */
 static unsigned char *h_wakeup;
 static int *h_active;
 static unsigned char *h_seen;    /* active-list walk: particles already listed */
@@ -61,11 +65,20 @@ int gizmo_host_set_device(int device)
     return 0;
 }
 
+int gizmo_host_set_method(int method)
+{
+    if (method != 0 && method != 1) return -1;
+    g_method = method;
+    return 0;
+}
+
 static void free_staging(void)
 {
     double **arrs[] = {&h_pos, &h_vel, &h_mass, &h_u, &h_hsml, &h_cn, &h_vsig, &h_num_ngb, &h_dhsml, &h_divvel,
-                       &h_rho, &h_pres, &h_nvt, &h_fce, &h_grho, &h_gp, &h_gv, &h_acc, &h_dtu};
+                       &h_rho, &h_pres, &h_nvt, &h_fce, &h_grho, &h_gp, &h_gv, &h_acc, &h_dtu,
+                       &h_pvel, &h_dtm, &h_dm, &h_gw};
     for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) { free(*arrs[k]); *arrs[k] = NULL; }
+    free(h_ti); h_ti = NULL;
     free(h_wakeup); h_wakeup = NULL;
     free(h_active); h_active = NULL;
     free(h_seen); h_seen = NULL;
@@ -84,10 +97,12 @@ static int ensure_staging(int n)
     h_pres = malloc(N * sizeof(double)); h_nvt = malloc(9 * N * sizeof(double)); h_fce = malloc(N * sizeof(double));
     h_grho = malloc(3 * N * sizeof(double)); h_gp = malloc(3 * N * sizeof(double)); h_gv = malloc(9 * N * sizeof(double));
     h_acc = malloc(3 * N * sizeof(double)); h_dtu = malloc(N * sizeof(double));
+    h_pvel = malloc(3 * N * sizeof(double)); h_dtm = malloc(N * sizeof(double)); h_dm = malloc(N * sizeof(double));
+    h_gw = malloc(3 * N * sizeof(double)); h_ti = malloc(N * sizeof(int64_t));
     h_wakeup = malloc(N); h_active = malloc(N * sizeof(int)); h_seen = calloc(N, 1);
     if (!h_pos || !h_vel || !h_mass || !h_u || !h_hsml || !h_cn || !h_vsig || !h_num_ngb || !h_dhsml || !h_divvel ||
         !h_rho || !h_pres || !h_nvt || !h_fce || !h_grho || !h_gp || !h_gv || !h_acc || !h_dtu || !h_wakeup || !h_active ||
-        !h_seen) {
+        !h_seen || !h_pvel || !h_dtm || !h_dm || !h_gw || !h_ti) {
         free_staging();
         return -1;
     }
@@ -203,6 +218,8 @@ static int ensure_ctx(void)
     p.precision = 0;
     p.cell_size = 0;
     p.pair_list_cap = 0;
+    p.method = g_method;
+    p.timebase_interval = g_method ? All.Timebase_interval : 0;
     /* the context (stream, device buffers) lives across passes; it is made
      * again only when a parameter it was created with changes (the first
      * step's tolerance, a new box size or device) */
@@ -247,6 +264,16 @@ static int upload(void)
         memcpy(h_grho + 3 * (size_t)i, SphP[i].Gradients.Density, 3 * sizeof(double));
         memcpy(h_gp + 3 * (size_t)i, SphP[i].Gradients.Pressure, 3 * sizeof(double));
         memcpy(h_gv + 9 * (size_t)i, SphP[i].Gradients.Velocity, 9 * sizeof(double));
+        if (g_method) {
+            /* GET_PARTICLE_INTEGERTIME: 2^TimeBin, 0 for bin 0 (allvars.h:909-912); a
+             * negative bin is density()'s mark of a converged particle */
+            const int bin = (P[i].TimeBin < 0) ? -P[i].TimeBin - 1 : P[i].TimeBin;
+            h_ti[i] = bin ? ((int64_t)1 << bin) : 0;
+            memcpy(h_pvel + 3 * (size_t)i, SphP[i].ParticleVel, 3 * sizeof(double));
+            memcpy(h_gw + 3 * (size_t)i, SphP[i].GravWorkTerm, 3 * sizeof(double));
+            h_dtm[i] = SphP[i].DtMass;
+            h_dm[i] = SphP[i].dMass;
+        }
     }
     gzm_host_view v;
     memset(&v, 0, sizeof(v));
@@ -256,6 +283,9 @@ static int upload(void)
     v.num_ngb = h_num_ngb; v.dhsml = h_dhsml; v.divvel = h_divvel; v.fce = h_fce;
     v.density = h_rho; v.pressure = h_pres; v.nvt = h_nvt;
     v.grad_rho = h_grho; v.grad_p = h_gp; v.grad_v = h_gv;
+    if (g_method) {
+        v.ti_step = h_ti; v.particle_vel = h_pvel; v.dt_mass = h_dtm; v.d_mass = h_dm; v.grav_work = h_gw;
+    }
     int s = gzm_upload(g_ctx, &v);
     if (s) return fail(s, "gzm_upload");
     return 0;
@@ -272,9 +302,13 @@ static int download(int what)
     if (what & DL_DENSITY) {
         v.hsml = h_hsml; v.cn = h_cn; v.num_ngb = h_num_ngb; v.dhsml = h_dhsml; v.divvel = h_divvel;
         v.density = h_rho; v.pressure = h_pres; v.nvt = h_nvt; v.fce = h_fce;
+        if (g_method) v.particle_vel = h_pvel;
     }
     if (what & DL_GRADIENTS) { v.grad_rho = h_grho; v.grad_p = h_gp; v.grad_v = h_gv; }
-    if (what & DL_FORCE) { v.acc = h_acc; v.dtu = h_dtu; v.vsig = h_vsig; v.wakeup = h_wakeup; }
+    if (what & DL_FORCE) {
+        v.acc = h_acc; v.dtu = h_dtu; v.vsig = h_vsig; v.wakeup = h_wakeup;
+        if (g_method) { v.dt_mass = h_dtm; v.d_mass = h_dm; v.grav_work = h_gw; }
+    }
     int s = gzm_download(g_ctx, &v);
     if (s) return fail(s, "gzm_download");
     return 0;
@@ -306,6 +340,7 @@ void density(void)
         SphP[i].ConditionNumber = h_cn[i];
         SphP[i].FaceClosureError = h_fce[i];
         memcpy(SphP[i].NV_T, h_nvt + 9 * (size_t)i, 9 * sizeof(double));
+        if (g_method) memcpy(SphP[i].ParticleVel, h_pvel + 3 * (size_t)i, 3 * sizeof(double));
     }
     /* "mark as active again" (density.c:869-872): a particle that entered
      * with a negative TimeBin was left out of the iteration */
@@ -361,7 +396,15 @@ void hydro_force(void)
         for (int k = 0; k < 3; k++) SphP[i].HydroAccel[k] = h_acc[3 * (size_t)i + k];
         SphP[i].DtInternalEnergy = h_dtu[i];
         SphP[i].MaxSignalVel = h_vsig[i];
+        if (g_method) {
+            SphP[i].DtMass = h_dtm[i];
+            memcpy(SphP[i].GravWorkTerm, h_gw + 3 * (size_t)i, 3 * sizeof(double));
+        }
     }
+    /* dMass: active particles' own sums, and the shares active neighbours
+     * gave to any particle (SphP[j].dMass -= ..., hydro_evaluate.h:384-392) */
+    if (g_method)
+        for (int i = 0; i < N_gas; i++) SphP[i].dMass = h_dm[i];
     for (int i = 0; i < N_gas; i++)
         if (h_wakeup[i]) SphP[i].wakeup = 1;
     /* the next pass starts from P/SphP again (positions drift between passes) */

@@ -66,6 +66,14 @@ struct gas_cell_data {
         MyDouble Velocity[3][3];
     } Gradients;
     int wakeup;              /* PPPZ[i].wakeup (hydro_evaluate.h:485-495) */
+    /* HYDRO_MESHLESS_FINITE_VOLUME fields (allvars.h gas_cell_data), used
+     * after gizmo_host_set_method(1): density() sets ParticleVel, hydro_force()
+     * DtMass and GravWorkTerm of the active particles and adds to dMass of
+     * every particle an active neighbour exchanged mass with */
+    MyDouble ParticleVel[3];
+    MyDouble DtMass;
+    MyDouble dMass;
+    MyDouble GravWorkTerm[3];
 };
 
 struct global_data_all_processes {
@@ -84,6 +92,9 @@ struct global_data_all_processes {
     double BoxLong[3];       /* BOX_LONG_X/Y/Z: box side k is BoxSize * BoxLong[k]; 0 counts as 1
                                 (allvars.h:1518-1560) */
     double GasGamma;         /* adiabatic index, GAMMA_DEFAULT / EOS_GAMMA (allvars.h:1092); 0 = 5/3 */
+    double Timebase_interval; /* physical time of one integer time unit (UNIT_INTEGERTIME_IN_PHYSICAL,
+                                allvars.h:908, without comoving integration); the MFV mass exchange
+                                uses it with each particle's integer step 2^TimeBin */
 };
 
 extern struct particle_data *P;
@@ -137,6 +148,12 @@ void gizmo_host_free(void);
 void gizmo_host_all_active(void);           /* active list = every particle, in index order */
 int  gizmo_host_set_active(const int *idx, int n); /* active list = idx[0..n-1], in that order */
 int  gizmo_host_set_device(int device);     /* GPU ordinal used by the next pass */
+/* hydro method of the next passes: 0 meshless finite mass (default), 1 meshless
+ * finite volume with HYDRO_FIX_MESH_MOTION = 5 (FP64); the reference fixes it at
+ * compile time (HYDRO_MESHLESS_FINITE_MASS / _VOLUME).  The kick / drift /
+ * timestep helpers below integrate MFM runs only.  Returns 0, or -1 for
+ * another value. */
+int  gizmo_host_set_method(int method);
 int  gizmo_host_last_status(void);          /* status of the last library call */
 const char *gizmo_host_last_error(void);
 double gizmo_box_side(int k);               /* periodic box side k: BoxSize * BoxLong[k] */

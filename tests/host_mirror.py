@@ -21,14 +21,17 @@ class GasCellData(C.Structure):
                 ("DtInternalEnergy", C.c_double), ("MaxSignalVel", C.c_double), ("ConditionNumber", C.c_double),
                 ("FaceClosureError", C.c_double), ("NV_T", (C.c_double * 3) * 3),
                 ("GradDensity", C.c_double * 3), ("GradPressure", C.c_double * 3),
-                ("GradVelocity", (C.c_double * 3) * 3), ("wakeup", C.c_int)]
+                ("GradVelocity", (C.c_double * 3) * 3), ("wakeup", C.c_int),
+                ("ParticleVel", C.c_double * 3), ("DtMass", C.c_double), ("dMass", C.c_double),
+                ("GravWorkTerm", C.c_double * 3)]
 
 
 class AllData(C.Structure):
     _fields_ = [("BoxSize", C.c_double), ("DesNumNgb", C.c_double), ("MaxNumNgbDeviation", C.c_double),
                 ("MinHsml", C.c_double), ("MaxHsml", C.c_double), ("Time", C.c_double), ("TimeBegin", C.c_double),
                 ("MaxPart", C.c_int), ("CourantFac", C.c_double), ("MaxSizeTimestep", C.c_double),
-                ("MinEgySpec", C.c_double), ("BoxLong", C.c_double * 3), ("GasGamma", C.c_double)]
+                ("MinEgySpec", C.c_double), ("BoxLong", C.c_double * 3), ("GasGamma", C.c_double),
+                ("Timebase_interval", C.c_double)]
 
 
 _lib = None
@@ -46,6 +49,7 @@ def host_lib():
         lib.gizmo_host_last_status.restype = C.c_int
         lib.gizmo_host_last_error.restype = C.c_char_p
         lib.gizmo_host_set_active.argtypes = [C.POINTER(C.c_int), C.c_int]
+        lib.gizmo_host_set_method.argtypes = [C.c_int]
         lib.gizmo_kick_gas.argtypes = [C.c_double, C.c_int]
         lib.gizmo_drift_gas.argtypes = [C.c_double]
         lib.gizmo_gas_timestep.argtypes = [C.c_int]
@@ -133,6 +137,8 @@ def read_state(lib, n):
         "grad_p": S["GradPressure"].copy(), "grad_v": S["GradVelocity"].copy(), "acc": S["HydroAccel"].copy(),
         "dtu": S["DtInternalEnergy"].copy(), "vsig": S["MaxSignalVel"].copy(),
         "wakeup": S["wakeup"].astype(np.uint8).copy(), "timebin": P["TimeBin"].copy(),
+        "particle_vel": S["ParticleVel"].copy(), "dt_mass": S["DtMass"].copy(), "d_mass": S["dMass"].copy(),
+        "grav_work": S["GravWorkTerm"].copy(),
     }
 
 
