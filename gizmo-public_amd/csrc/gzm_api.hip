@@ -967,6 +967,7 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     a.hsml = c->d_hsml; a.left = c->d_left; a.right = c->d_right; a.cn = c->d_cn;
     a.num_ngb = c->d_num_ngb; a.dhsml = c->d_dhsml; a.divvel = c->d_divvel; a.rho = c->d_rho;
     a.fce = c->d_fce; a.nvt = c->d_nvt;
+    a.pvel = (c->p.method == 1) ? c->d_pvel : nullptr;
     a.redo = c->d_flags;
     a.budget = c->p.density_budget == 0 ? kDensityBudget : (c->p.density_budget < 0 ? 0 : c->p.density_budget);
     a.heavy_list = c->d_heavy_list;
@@ -1121,11 +1122,8 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     }
     if (na > 0) {
         Timed tm(c, kOther);
-        if (c->p.method == 1)
-            launch_mfv_particle_vel(c->stream, c->d_boxes, c->d_box_of, c->tree, na, c->d_active, c->d_posm,
-                                    c->d_velu, c->d_hsml, c->d_rho, c->d_pvel);
         launch_density_finalize(c->stream, na, c->d_active, c->d_posm, c->d_velu, c->d_hsml, c->d_num_ngb,
-                                c->d_rho, c->d_pres, c->G.G9);
+                                c->d_rho, c->d_pres, c->G.G9, (c->p.method == 1) ? c->d_pvel : nullptr);
     }
     GZM_HIP(hipGetLastError());
     c->have_density = true;
@@ -1793,6 +1791,40 @@ gzm_status gzm_riemann_states_noalloc(int32_t device, int32_t mode, int32_t prec
     // riemann_run's layout for n states inside the area of kNoAllocStates, the masks after all the doubles
     return riemann_run(mode, precision, G, n, L, R, normal, limiter, out, mask, d,
                        reinterpret_cast<int *>(d + (size_t)kNoAllocStates * kRiemannDoubles));
+}
+
+gzm_status gzm_riemann_states_mfv(int32_t device, int32_t mode, double gamma, int64_t n, const double *L,
+                                  const double *R, const double *normal, const double *limiter, double *out,
+                                  int32_t *mask)
+{
+    if (mode < 0 || mode > 2) { set_error("mode must be 0 (Riemann_solver), 1 (KT flux) or 2 (exact solver)"); return GZM_ERR_ARG; }
+    GammaConsts G;
+    // riemann_args checks pointers, n, gamma and the device (mode 0 needs the limiter)
+    gzm_status s = riemann_args(device, mode == 0 ? 0 : 1, 0, gamma, n, (int64_t)1 << 27, L, R, normal, limiter, out,
+                                mask, G);
+    if (s || n == 0) return s;
+    const size_t nn = (size_t)n;
+    double *d = nullptr;
+    int *dm = nullptr;
+    s = dev_alloc(&d, nn * (5 + 5 + 3 + 1 + 7));
+    if (s) return s;
+    s = dev_alloc(&dm, nn);
+    if (s) { dev_free(d); return s; }
+    double *dL = d, *dR = d + 5 * nn, *dn = d + 10 * nn, *dlim = d + 13 * nn, *dout = d + 14 * nn;
+    hipError_t e = hipMemcpy(dL, L, sizeof(double) * 5 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dR, R, sizeof(double) * 5 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, normal, sizeof(double) * 3 * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess && limiter) e = hipMemcpy(dlim, limiter, sizeof(double) * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_riemann_states_mfv(nullptr, mode, G, n, dL, dR, dn, dlim, dout, dm);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * 7 * nn, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(mask, dm, sizeof(int) * nn, hipMemcpyDeviceToHost);
+    dev_free(d);
+    dev_free(dm);
+    if (e != hipSuccess) { set_error("MFV Riemann states: %s", hipGetErrorString(e)); return GZM_ERR_HIP; }
+    return GZM_OK;
 }
 
 gzm_status gzm_timing_get(gzm_ctx *c, gzm_timing *t)

@@ -175,6 +175,9 @@ struct DensitySweepArgs {
     const double4 *velu;   // vx, vy, vz, u
     double *hsml, *left, *right, *cn;
     double *num_ngb, *dhsml, *divvel, *rho, *fce, *nvt;  // nvt: 6 per particle
+    // MFV (FP64 sweeps only): the sweep's sum of m_j W_ij v_j per particle, 3
+    // per particle; the finalize turns the last one into ParticleVel.  NULL: MFM
+    double *pvel;
     int *redo;             // per live entry: 1 = sweep again
     // Load balance: a particle whose walk passes `budget` candidates (0: no
     // limit) stops and appends its live position to heavy_list (count
@@ -329,9 +332,11 @@ void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a);
 // iteration from the stage's entry hsml, with empty brackets
 void launch_edge_reset(hipStream_t s, int cap, const int *n_edge, const int *edge_list, const double *hsml_entry,
                        double *hsml, double *left, double *right);
+// pvel (MFV, else NULL): the last sweep's sums of m_j W_ij v_j become
+// ParticleVel before the density sum they are divided by is replaced
 void launch_density_finalize(hipStream_t s, int n_active, const int *active, const double4 *posm,
                              const double4 *velu, const double *hsml, double *num_ngb, double *rho,
-                             double *pres, double gamma_minus_1);
+                             double *pres, double gamma_minus_1, double *pvel);
 void launch_pack_grad(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
                       const double *rho, const double *pres, GradRec *rec);
 void launch_gradients(hipStream_t s, const GradientArgs &a);
@@ -401,12 +406,7 @@ void launch_tree_build(hipStream_t s, const TreeBuild &b, const double *hsml);
 void launch_tree_hmax(hipStream_t s, const TreeBuild &b, const double *hsml);
 // per-cell max hsml (rounded up to FP32) of the cells' slot ranges
 void launch_cell_hmax(hipStream_t s, int ncell, const int2 *range, const double *hsml, float *hmax);
-// MFV (gzm_mfv.hip): ParticleVel after the density sweeps, before the
-// finalize (rho_sum = the sweeps' density sums); the force loop with its
-// preloop
-void launch_mfv_particle_vel(hipStream_t s, const BoxDesc *boxes, const int *box_of, const TreeArgs &tree,
-                             int n_active, const int *active, const double4 *posm, const double4 *velu,
-                             const double *hsml, const double *rho_sum, double *pvel);
+// MFV force loop (gzm_mfv.hip): the preloop of its outputs, then
 void launch_mfv_preloop(hipStream_t s, const ForceArgs &a, const MfvArgs &m);
 // list kernel with deferral when a.defer.mask is set (then also
 // launch_force_mfv_overflow for the pair sets the list could not hold, and
@@ -415,6 +415,9 @@ void launch_force_mfv(hipStream_t s, const ForceArgs &a, const MfvArgs &m);
 void launch_force_mfv_overflow(hipStream_t s, const ForceArgs &a, const MfvArgs &m, int *rows, int *n_rows);
 void launch_force_mfv_deferred(hipStream_t s, const ForceArgs &a, const MfvArgs &m, int n_pairs);
 void launch_force_mfv_finish(hipStream_t s, const ForceArgs &a, const MfvArgs &m);
+// the MFV Riemann solver on n independent face states (out [n][7])
+void launch_riemann_states_mfv(hipStream_t s, int mode, const GammaConsts &G, int64_t n, const double *L,
+                               const double *R, const double *normal, const double *limiter, double *out, int *mask);
 void launch_fill_int(hipStream_t s, int *p, int n, int value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, int n, uint8_t value);
 void launch_fill_double(hipStream_t s, double *p, long long n, double value);

@@ -185,11 +185,14 @@ __global__ void density_init_kernel(int n_active, const int *active, const doubl
 }
 
 // one density pair (density.c:252-347): j's terms into S when r_ij < h_i.
-// EDGE: also set `edge` when r_ij^2 lies within kEdgeBand of h_i^2.
-template <bool EDGE = false>
+// EDGE: also set `edge` when r_ij^2 lies within kEdgeBand of h_i^2.  MFV:
+// also m_j W_ij v_j into pv (density.c:291-293 for r = 0, 331-334 as
+// v_i - (v_i - v_j) otherwise), the sum behind ParticleVel.
+template <bool EDGE = false, bool MFV = false>
 __device__ __forceinline__ bool density_pair(const BoxDesc &G, const double4 &pi, const double4 &vi,
                                              const DensitySweepArgs &a, int j, double h2, double hinv,
-                                             double hinv3, double hinv4, DensitySums &S, bool *edge = nullptr)
+                                             double hinv3, double hinv4, DensitySums &S, bool *edge = nullptr,
+                                             double *pv = nullptr)
 {
     const double4 pj = a.posm[j];
     double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
@@ -216,7 +219,19 @@ __device__ __forceinline__ bool density_pair(const BoxDesc &G, const double4 &pi
         S.f2 += wk * dp[2];
         const double4 vj = a.velu[j];
         const double dv0 = vi.x - vj.x, dv1 = vi.y - vj.y, dv2 = vi.z - vj.z;
+        if (MFV) {
+            const double mj_wk = pj.w * wk;
+            pv[0] += mj_wk * (vi.x - dv0);
+            pv[1] += mj_wk * (vi.y - dv1);
+            pv[2] += mj_wk * (vi.z - dv2);
+        }
         S.divvel -= dwk * (dp[0] * dv0 + dp[1] * dv1 + dp[2] * dv2) / r;
+    } else if (MFV) {
+        const double4 vj = a.velu[j];
+        const double mj_wk = pj.w * wk;
+        pv[0] += mj_wk * vj.x;
+        pv[1] += mj_wk * vj.y;
+        pv[2] += mj_wk * vj.z;
     }
     return true;
 }
@@ -246,7 +261,7 @@ __device__ __forceinline__ int density_store(const DensitySweepArgs &a, int t, i
 
 // 4 waves per SIMD: fits 128 VGPRs without spills, and the extra wave hides
 // the candidate loads (sweeps 10 % faster than at the compiler's 146 VGPRs)
-template <bool TREE, int MAXB>
+template <bool TREE, int MAXB, bool MFV = false>
 __global__ __launch_bounds__(MAXB, 4) void density_sweep_kernel(DensitySweepArgs a)
 {
     const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
@@ -263,13 +278,14 @@ __global__ __launch_bounds__(MAXB, 4) void density_sweep_kernel(DensitySweepArgs
         kernel_hinv(h, hinv, hinv3, hinv4);
         const double h2 = h * h;
         DensitySums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double pv[3] = {0, 0, 0};
         const double xs[3] = {pi.x, pi.y, pi.z};
         const int budget = a.budget;
         int visited = 0;
         bool heavy = false;
         unsigned long long lane_cnt = 0;
         auto pair = [&](int j) {
-            if (density_pair(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S)) lane_cnt++;
+            if (density_pair<false, MFV>(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S, nullptr, pv)) lane_cnt++;
         };
         if (TREE) {
             tree_walk(a.tree, G, xs, h, false, [&](int j) -> bool {
@@ -299,6 +315,11 @@ __global__ __launch_bounds__(MAXB, 4) void density_sweep_kernel(DensitySweepArgs
         }
         cnt += lane_cnt;
         density_store(a, t, i, h, S);
+        if (MFV) {
+            a.pvel[3 * (size_t)i + 0] = pv[0];
+            a.pvel[3 * (size_t)i + 1] = pv[1];
+            a.pvel[3 * (size_t)i + 2] = pv[2];
+        }
     });
     wave_add_u64(&a.ctr->interactions, cnt);
 }
@@ -308,6 +329,7 @@ __global__ __launch_bounds__(MAXB, 4) void density_sweep_kernel(DensitySweepArgs
 // wavefront (another summation order than a single lane's, FP64 throughout).
 // A fixed grid strides over the list, so the launch is sized independently of
 // the (device-side) count.
+template <bool MFV = false>
 __global__ __launch_bounds__(kBlock) void density_heavy_kernel(DensitySweepArgs a)
 {
     const int lane = threadIdx.x & 63;
@@ -328,6 +350,7 @@ __global__ __launch_bounds__(kBlock) void density_heavy_kernel(DensitySweepArgs 
         kernel_hinv(h, hinv, hinv3, hinv4);
         const double h2 = h * h;
         DensitySums S = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double pv[3] = {0, 0, 0};
         const double xs[3] = {pi.x, pi.y, pi.z};
         unsigned long long c = 0;
         bool edge = false;
@@ -335,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void density_heavy_kernel(DensitySweepArgs 
             if (a.edge_list) {
                 if (density_pair<true>(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S, &edge)) c++;
             } else {
-                if (density_pair(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S)) c++;
+                if (density_pair<false, MFV>(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S, nullptr, pv)) c++;
             }
         });
         const bool any_edge = __any(edge);
@@ -344,10 +367,16 @@ __global__ __launch_bounds__(kBlock) void density_heavy_kernel(DensitySweepArgs 
         S.t00 = wave_sum(S.t00); S.t01 = wave_sum(S.t01); S.t02 = wave_sum(S.t02);
         S.t11 = wave_sum(S.t11); S.t12 = wave_sum(S.t12); S.t22 = wave_sum(S.t22);
         S.f0 = wave_sum(S.f0); S.f1 = wave_sum(S.f1); S.f2 = wave_sum(S.f2);
+        if (MFV) { pv[0] = wave_sum(pv[0]); pv[1] = wave_sum(pv[1]); pv[2] = wave_sum(pv[2]); }
         wave_add_u64(&a.ctr->interactions, c);
         if (lane == 0) {
             const int res = density_store(a, t, i, h, S);
             if (res == 0 && any_edge && a.edge_list) a.edge_list[atomicAdd(a.n_edge, 1)] = i;
+            if (MFV) {
+                a.pvel[3 * (size_t)i + 0] = pv[0];
+                a.pvel[3 * (size_t)i + 1] = pv[1];
+                a.pvel[3 * (size_t)i + 2] = pv[2];
+            }
         }
     }
 }
@@ -364,13 +393,23 @@ __global__ void edge_reset_kernel(int cap, const int *n_edge, const int *edge_li
 }
 
 __global__ void density_finalize_kernel(int n_active, const int *active, const double4 *posm, const double4 *velu,
-                                        const double *hsml, double *num_ngb, double *rho, double *pres, double gm1)
+                                        const double *hsml, double *num_ngb, double *rho, double *pres, double gm1,
+                                        double *pvel)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_active) return;
     int i = active[t];
     double m = posm[i].w;
     if (!(m > 0)) return;
+    if (pvel && rho[i] > 0) {
+        // MFV mesh motion 5 (density.c:886-894): 0.7 VelPred + 0.3 the kernel-weighted
+        // mean velocity, over the density sum the last sweep left in rho
+        const double4 v = velu[i];
+        const double eps_pvel = 0.3, d_sum = rho[i];
+        const double vv[3] = {v.x, v.y, v.z};
+        for (int k = 0; k < 3; k++)
+            pvel[3 * (size_t)i + k] = (1. - eps_pvel) * vv[k] + eps_pvel * pvel[3 * (size_t)i + k] / d_sum;
+    }
     double h = hsml[i], nn = num_ngb[i], d;
     // density.c:967-978 (MFM: own mass times the kernel number density)
     if ((h > 0) && (nn > 0)) d = m * nn / (kNormCoeff * pow(h, 3.0));
@@ -869,7 +908,10 @@ void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a)
     // workgroups of 512 take their own instantiation: the bound shapes the register allocation
     const int b = a.block;
     const dim3 grid(nblocks(a.n_live, b)), block(b);
-    if (b > kBlock) {
+    if (a.pvel) {  // MFV: one instantiation per search, 256-thread workgroups
+        if (a.tree.nodes) hipLaunchKernelGGL((density_sweep_kernel<true, kBlock, true>), dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((density_sweep_kernel<false, kBlock, true>), dim3(nblocks(a.n_live)), dim3(kBlock), 0, s, a);
+    } else if (b > kBlock) {
         if (a.tree.nodes) hipLaunchKernelGGL((density_sweep_kernel<true, kMaxBlock>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((density_sweep_kernel<false, kMaxBlock>), grid, block, 0, s, a);
     } else {
@@ -886,10 +928,10 @@ void launch_edge_reset(hipStream_t s, int cap, const int *n_edge, const int *edg
 }
 void launch_density_finalize(hipStream_t s, int n_active, const int *active, const double4 *posm,
                              const double4 *velu, const double *hsml, double *num_ngb, double *rho, double *pres,
-                             double gm1)
+                             double gm1, double *pvel)
 {
     if ((n_active) > 0) hipLaunchKernelGGL(density_finalize_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, n_active, active, posm,
-                       velu, hsml, num_ngb, rho, pres, gm1);
+                       velu, hsml, num_ngb, rho, pres, gm1, pvel);
 }
 void launch_pack_grad(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
                       const double *rho, const double *pres, GradRec *rec)
@@ -973,7 +1015,9 @@ void launch_density_heavy(hipStream_t s, const DensitySweepArgs &a, int max_entr
     if (max_entries <= 0) return;
     // up to 2048 workgroups of 4 wavefronts stride over the entries
     const long long blocks = ((long long)max_entries + 3) / 4;
-    hipLaunchKernelGGL(density_heavy_kernel, dim3((unsigned)(blocks < 2048 ? blocks : 2048)), dim3(kBlock), 0, s, a);
+    const dim3 grid((unsigned)(blocks < 2048 ? blocks : 2048));
+    if (a.pvel) hipLaunchKernelGGL(density_heavy_kernel<true>, grid, dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(density_heavy_kernel<false>, grid, dim3(kBlock), 0, s, a);
 }
 
 namespace {

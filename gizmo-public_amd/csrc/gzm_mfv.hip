@@ -1,8 +1,8 @@
 // gzm_mfv.hip -- HIP kernels of the meshless finite-volume (MFV) variant of
-// the hydro pass for gfx950 (FP64): the smoothed mesh velocity after the
-// density iteration, and the force loop with mass fluxes.
+// the hydro pass for gfx950 (FP64): the force loop with mass fluxes.
 //
-// The density iteration, the gradients and the pair list are those of the
+// The density iteration (which also sums the smoothed mesh velocity,
+// density_pair<.., MFV>), the gradients and the pair list are those of the
 // MFM pass.  The force loop keeps its gather form (one thread per active
 // particle, its flux sums in registers) with one scatter: the reference gives
 // each neighbour its share of the exchanged mass (SphP[j].dMass -= ...,
@@ -16,57 +16,6 @@
 namespace gzm {
 
 namespace {
-
-// ParticleVel of the active particles with mass (HYDRO_FIX_MESH_MOTION = 5):
-// the density loop's sum of m_j W_ij v_j over {j : r_ij < h_i} (density.c:
-// 291-293 for r = 0, 331-334 otherwise, as v_i - (v_i - v_j)) mixed with the
-// particle's own velocity, 0.7 v_i + 0.3 sum / (sum of m_j W_ij)
-// (density.c:886-894).  Runs after the last sweep, whose hsml and density
-// sum it reads, and before the finalize overwrites that sum.
-__global__ __launch_bounds__(kBlock) void mfv_particle_vel_kernel(const BoxDesc *boxes, const int *box_of,
-                                                                  TreeArgs tree, int n_active, const int *active,
-                                                                  const double4 *posm, const double4 *velu,
-                                                                  const double *hsml, const double *rho_sum,
-                                                                  double *pvel)
-{
-    const int t = xcd_block() * (int)blockDim.x + threadIdx.x;
-    bool valid = t < n_active;
-    const int i = valid ? active[t] : 0;
-    if (valid && !(posm[i].w > 0)) valid = false;
-    for_box(boxes, box_of, i, valid, [&](const BoxDesc &G) {
-        const double4 pi = posm[i];
-        const double4 vi = velu[i];
-        const double h = hsml[i];
-        double hinv, hinv3, hinv4;
-        kernel_hinv(h, hinv, hinv3, hinv4);
-        const double h2 = h * h;
-        const double xs[3] = {pi.x, pi.y, pi.z};
-        const double v_i[3] = {vi.x, vi.y, vi.z};
-        double s[3] = {0, 0, 0};
-        for_each_neighbour_candidate(tree, G, xs, h, false, [&](int j) {
-            const double4 pj = posm[j];
-            double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-            wrap_dp(G.box, dp);
-            const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-            if (!(r2 < h2 && pj.w > 0)) return;
-            const double r = sqrt(r2);
-            double wk, dwk;
-            kernel_wdw(r * hinv, hinv3, hinv4, wk, dwk);
-            const double mj_wk = pj.w * wk;
-            const double4 vj = velu[j];
-            const double v_j[3] = {vj.x, vj.y, vj.z};
-            if (r > 0) {
-                for (int k = 0; k < 3; k++) s[k] += mj_wk * (v_i[k] - (v_i[k] - v_j[k]));
-            } else {
-                for (int k = 0; k < 3; k++) s[k] += mj_wk * v_j[k];
-            }
-        });
-        const double d = rho_sum[i];
-        const double eps_pvel = 0.3;
-        for (int k = 0; k < 3; k++)
-            pvel[3 * (size_t)i + k] = (d > 0) ? (1. - eps_pvel) * v_i[k] + eps_pvel * s[k] / d : s[k];
-    });
-}
 
 // hydro_force_initial_operations_preloop, MFV fields (hydro_toplevel.c:
 // 976-978): DtMass, dMass and GravWorkTerm of the active particles start at
@@ -334,18 +283,57 @@ __global__ void collect_overflow_rows_kernel(int n_active, const int *count, int
     if (t < n_active && count[t] > cap) rows[atomicAdd(n_rows, 1)] = t;
 }
 
+// The MFV Riemann solver on independent face states, one per thread (the
+// force loop's inline functions): mode 0 Riemann_solver with the HLLC
+// fluxes, 1 the KT flux alone, 2 the exact solver with its face sampling
+// alone; out [n][7] = P_M, S_M, F_rho, F_v[3], F_p; mask = stages (mode 0),
+// or the exact solver's status negated.
+__global__ __launch_bounds__(kBlock) void riemann_states_mfv_kernel(int mode, GammaConsts G, int64_t n,
+                                                                    const double *L, const double *R,
+                                                                    const double *normal, const double *limiter,
+                                                                    double *out, int *mask)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    RState<double> SL, SR;
+    SL.rho = L[5 * t + 0]; SL.p = L[5 * t + 1];
+    SR.rho = R[5 * t + 0]; SR.p = R[5 * t + 1];
+    double nv[3];
+    for (int k = 0; k < 3; k++) {
+        SL.v[k] = L[5 * t + 2 + k];
+        SR.v[k] = R[5 * t + 2 + k];
+        nv[k] = normal[3 * t + k];
+    }
+    ROutV o = {0, 0, 0, {0, 0, 0}, 0};
+    int m = 0;
+    if (mode == 0) {
+        int err = 0;
+        m = riemann_solve_mfv<false, true>(SL, SR, nv, limiter[t], G, o, err);
+        if (err) m = -err;
+    } else {
+        // the face-frame quantities of riemann_solve_mfv
+        const double cL = sqrt(G.G0 * SL.p / SL.rho), cR = sqrt(G.G0 * SR.p / SR.rho);
+        const double uL = SL.p / (G.G9 * SL.rho), uR = SR.p / (G.G9 * SR.rho);
+        const double hL = SL.p / SL.rho + uL + 0.5 * (SL.v[0] * SL.v[0] + SL.v[1] * SL.v[1] + SL.v[2] * SL.v[2]);
+        const double hR = SR.p / SR.rho + uR + 0.5 * (SR.v[0] * SR.v[0] + SR.v[1] * SR.v[1] + SR.v[2] * SR.v[2]);
+        const double vL = SL.v[0] * nv[0] + SL.v[1] * nv[1] + SL.v[2] * nv[2];
+        const double vR = SR.v[0] * nv[0] + SR.v[1] * nv[1] + SR.v[2] * nv[2];
+        if (mode == 1) {
+            riemann_kt_mfv(SL, SR, o, nv, vL, vR, cL, cR, hL, hR);
+            m = 2;
+        } else {
+            m = -riemann_exact_mfv_body(SL, SR, o, G, nv, vL, vR, cL, cR);
+        }
+    }
+    double *y = out + 7 * t;
+    y[0] = o.P_M; y[1] = o.S_M; y[2] = o.Frho;
+    y[3] = o.Fv[0]; y[4] = o.Fv[1]; y[5] = o.Fv[2]; y[6] = o.Fp;
+    mask[t] = m;
+}
+
 inline unsigned nblocks_mfv(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
-
-void launch_mfv_particle_vel(hipStream_t s, const BoxDesc *boxes, const int *box_of, const TreeArgs &tree,
-                             int n_active, const int *active, const double4 *posm, const double4 *velu,
-                             const double *hsml, const double *rho_sum, double *pvel)
-{
-    if (n_active > 0)
-        hipLaunchKernelGGL(mfv_particle_vel_kernel, dim3(nblocks_mfv(n_active)), dim3(kBlock), 0, s, boxes, box_of,
-                           tree, n_active, active, posm, velu, hsml, rho_sum, pvel);
-}
 
 void launch_mfv_preloop(hipStream_t s, const ForceArgs &a, const MfvArgs &m)
 {
@@ -390,6 +378,14 @@ void launch_force_mfv_finish(hipStream_t s, const ForceArgs &a, const MfvArgs &m
 {
     if (a.n_active > 0)
         hipLaunchKernelGGL(force_mfv_finish_kernel, dim3(nblocks_mfv(a.n_active)), dim3(kBlock), 0, s, a, m);
+}
+
+void launch_riemann_states_mfv(hipStream_t s, int mode, const GammaConsts &G, int64_t n, const double *L,
+                               const double *R, const double *normal, const double *limiter, double *out, int *mask)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(riemann_states_mfv_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           mode, G, n, L, R, normal, limiter, out, mask);
 }
 
 }  // namespace gzm

@@ -25,7 +25,7 @@ EXPORTED_SYMBOLS = (
     "gzm_status_string", "gzm_last_error", "gzm_ctx_create", "gzm_ctx_destroy", "gzm_upload",
     "gzm_density", "gzm_gradients", "gzm_hydro_force", "gzm_download", "gzm_restore_initial_state",
     "gzm_neighbor_counts", "gzm_timing_get", "gzm_timing_reset", "gzm_upload_batch", "gzm_run_passes",
-    "gzm_riemann_states", "gzm_riemann_states_noalloc",
+    "gzm_riemann_states", "gzm_riemann_states_noalloc", "gzm_riemann_states_mfv",
 )
 
 
@@ -129,6 +129,7 @@ def load_library():
     L.gzm_run_passes.argtypes = [C.c_void_p, C.c_int32, C.POINTER(PassStats)]
     for f in (L.gzm_riemann_states, L.gzm_riemann_states_noalloc):
         f.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int64, _dp, _dp, _dp, _dp, _dp, _ip]
+    L.gzm_riemann_states_mfv.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_int64, _dp, _dp, _dp, _dp, _dp, _ip]
     for name in EXPORTED_SYMBOLS:
         f = getattr(L, name)
         if name not in ("gzm_status_string", "gzm_last_error"):
@@ -344,6 +345,23 @@ def riemann_states(L, R, normal, limiter=None, mode=0, precision=0, gamma=5.0 / 
     name = "gzm_riemann_states_noalloc" if noalloc else "gzm_riemann_states"
     _check(getattr(lib, name)(device, mode, precision, gamma, n, _p(L), _p(R), _p(nrm), _p(lim), _p(out),
                               _p(mask, _ip)), name)
+    return out, mask
+
+
+def riemann_states_mfv(L, R, normal, limiter=None, mode=0, gamma=5.0 / 3.0, device=0):
+    """The device MFV Riemann solver: mode 0 the whole solver, 1 the KT flux,
+    2 the exact solver with its face sampling; returns (out [n][7] = P_M, S_M,
+    F_rho, F_v[3], F_p; masks)."""
+    L = np.ascontiguousarray(L, dtype=np.float64).reshape(-1, 5)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 5)
+    nrm = np.ascontiguousarray(normal, dtype=np.float64).reshape(-1, 3)
+    n = L.shape[0]
+    lim = None if limiter is None else np.ascontiguousarray(np.broadcast_to(limiter, (n,)), dtype=np.float64)
+    out = np.zeros((n, 7))
+    mask = np.zeros(n, dtype=np.int32)
+    lib = load_library()
+    _check(lib.gzm_riemann_states_mfv(device, mode, gamma, n, _p(L), _p(R), _p(nrm), _p(lim), _p(out),
+                                      _p(mask, _ip)), "gzm_riemann_states_mfv")
     return out, mask
 
 

@@ -278,6 +278,21 @@ gzm_status gzm_riemann_states_noalloc(int32_t device, int32_t mode, int32_t prec
                                       const double *L, const double *R, const double *normal,
                                       const double *limiter, double *out, int32_t *mask);
 
+/* Riemann_solver() of a meshless finite-volume run (reimann.h:286-388 with
+ * the HYDRO_MESHLESS_FINITE_VOLUME branches) on n independent face states
+ * in FP64, the device code of the MFV force loop: mode 0 the whole solver
+ * (HLLC with the four-wave fluxes, the KT flux, the exact solver past the
+ * limiter), 1 the KT flux alone (reimann.h:484-507), 2 the exact solver with
+ * its face sampling alone (reimann.h:685-1030, 1146-1166).  L, R [n][5] =
+ * rho, p, v[3]; normal [n][3]; limiter [n] (mode 0); out [n][7] = P_M, S_M,
+ * F_rho, F_v[3], F_p through a face at rest in the states' frame; mask [n] =
+ * stages used (mode 0: 1 HLLC, 2 KT, 4 exact), or -1234545 on the exact
+ * solver's negative-density exit.  Element-level checks of branches particle
+ * sets rarely reach (vacuum fans, vacuum inside the fan). */
+gzm_status gzm_riemann_states_mfv(int32_t device, int32_t mode, double gamma, int64_t n, const double *L,
+                                  const double *R, const double *normal, const double *limiter, double *out,
+                                  int32_t *mask);
+
 gzm_status gzm_timing_get(gzm_ctx *ctx, gzm_timing *timing);
 gzm_status gzm_timing_reset(gzm_ctx *ctx);
 

@@ -160,3 +160,75 @@ def test_mfv_batch_and_graph_passes():
 def test_mfv_needs_fp64():
     with pytest.raises(gzm.GzmError, match="FP64"):
         gzm.MFMContext(gzm.make_params(method=gzm.METHOD_MFV, precision=1))
+
+
+def _element_states(seed, n):
+    """Random face states: mild, supersonic and strongly separating flows,
+    with random unit normals."""
+    rng = np.random.default_rng(seed)
+    L = np.column_stack([rng.uniform(0.1, 3.0, n), rng.uniform(0.05, 5.0, n), rng.normal(0, 2.0, (n, 3))])
+    R = np.column_stack([rng.uniform(0.1, 3.0, n), rng.uniform(0.05, 5.0, n), rng.normal(0, 2.0, (n, 3))])
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    return L, R, nrm
+
+
+def _vacuum_states():
+    """One side empty (left or right, gas moving either way), and gas flying
+    apart fast enough to open a vacuum inside the fan."""
+    gas = [(1.0, 1.0, u, 0.3, -0.2) for u in (-3.5, -1.0, 0.0, 0.5, 1.5, 4.0)]
+    vac = (0.0, 0.0, 0.0, 0.0, 0.0)
+    L = [g for g in gas] + [vac] * len(gas) + [(1.0, 0.5, -5.0, 0.1, 0.0), (0.5, 1.0, -2.0, 0.0, 0.4)]
+    R = [vac] * len(gas) + [(g[0], g[1], -g[2], g[3], g[4]) for g in gas] + [(1.0, 0.5, 5.0, 0.0, 0.2),
+                                                                           (0.8, 0.3, 6.0, 0.2, 0.0)]
+    n = len(L)
+    nrm = np.tile([1.0, 0.0, 0.0], (n, 1))
+    return np.asarray(L, dtype=np.float64), np.asarray(R, dtype=np.float64), nrm
+
+
+def _oracle_elements(L, R, nrm, mode, limiter=None):
+    out = np.zeros((L.shape[0], 7))
+    mask = np.zeros(L.shape[0], dtype=np.int32)
+    for k in range(L.shape[0]):
+        if mode == 0:
+            r = O.riemann_mfv(L[k], R[k], nrm[k], float(limiter[k]))
+            mask[k] = r["mask"]
+        elif mode == 1:
+            r = O.riemann_kt_mfv(L[k], R[k], nrm[k])
+            mask[k] = 2
+        else:
+            r = O.riemann_exact_mfv(L[k], R[k], nrm[k])
+            mask[k] = -r["mask"]
+        out[k] = [r["P_M"], r["S_M"], r["Frho"], *r["Fv"], r["Fp"]]
+    return out, mask
+
+
+def _close(a, b, tol=1e-12):
+    scale = np.maximum(1.0, np.abs(b))
+    both_nan = np.isnan(a) & np.isnan(b)
+    return np.where(both_nan, True, np.abs(a - b) <= tol * scale)
+
+
+@pytest.mark.parametrize("case", ["hllc", "exact_past_limiter", "kt", "exact_alone", "vacuum_solver",
+                                  "vacuum_exact"])
+def test_device_mfv_riemann_elements_match_oracle(case):
+    """The MFV Riemann solver of the force loop, one face state per thread,
+    against the oracle's: every stage, including the exact solver's vacuum
+    fans and the vacuum inside the fan, which particle sets rarely reach."""
+    if case.startswith("vacuum"):
+        L, R, nrm = _vacuum_states()
+    else:
+        L, R, nrm = _element_states({"hllc": 1, "exact_past_limiter": 2, "kt": 3, "exact_alone": 4}[case], 2000)
+    n = L.shape[0]
+    mode = {"hllc": 0, "exact_past_limiter": 0, "kt": 1, "exact_alone": 2, "vacuum_solver": 0, "vacuum_exact": 2}[case]
+    lim = np.full(n, 1e300) if case == "hllc" else np.zeros(n)
+    print(f"mfv riemann elements {case}: {n} states, mode {mode}", flush=True)
+    out, mask = gzm.riemann_states_mfv(L, R, nrm, lim if mode == 0 else None, mode=mode)
+    ref, rmask = _oracle_elements(L, R, nrm, mode, lim)
+    np.testing.assert_array_equal(mask, rmask)
+    ok = _close(out, ref)
+    assert ok.all(), (np.argwhere(~ok)[:5], out[~ok.all(axis=1)][:3], ref[~ok.all(axis=1)][:3])
+    if case == "exact_past_limiter":
+        assert np.all(mask & 4)
+    if case == "hllc":
+        assert np.all(mask & 1)
