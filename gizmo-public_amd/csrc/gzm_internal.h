@@ -406,7 +406,8 @@ void launch_tree_build(hipStream_t s, const TreeBuild &b, const double *hsml);
 void launch_tree_hmax(hipStream_t s, const TreeBuild &b, const double *hsml);
 // per-cell max hsml (rounded up to FP32) of the cells' slot ranges
 void launch_cell_hmax(hipStream_t s, int ncell, const int2 *range, const double *hsml, float *hmax);
-// MFV force loop (gzm_mfv.hip): the preloop of its outputs, then
+// MFV force loop (gzm_mfv.hip): the preloop that zeroes the active
+// particles' DtMass, dMass and GravWorkTerm, then the loop
 void launch_mfv_preloop(hipStream_t s, const ForceArgs &a, const MfvArgs &m);
 // list kernel with deferral when a.defer.mask is set (then also
 // launch_force_mfv_overflow for the pair sets the list could not hold, and
