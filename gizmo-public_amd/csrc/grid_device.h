@@ -166,7 +166,9 @@ __device__ __forceinline__ double periodic_gap(double x, double lo, double hi, d
 // x -- and, for the pair search, within the hsml of a particle below the
 // node -- in depth-first Morton order.  Conservative by the same 1e-9
 // relative margin as the cell walk; the caller applies the exact predicate.
-// An f returning bool ends the walk by returning false.
+// An f returning bool ends the walk by returning false.  One step per node or
+// leaf (the walk the pair searches of the gradient and force loops use: their
+// tree path is the rare one there, and the form below costs them registers).
 template <typename F>
 __device__ __forceinline__ void tree_walk(const TreeArgs &T, const BoxDesc &G, const double x[3], double R,
                                           bool pairs, F &&f)
@@ -198,6 +200,52 @@ __device__ __forceinline__ void tree_walk(const TreeArgs &T, const BoxDesc &G, c
         } else {
             node = nd.link.x;
         }
+    }
+}
+
+// The same visits as tree_walk (radius R only: the density search), in two
+// nested loops ("while-while"): each lane walks nodes until it reaches a slot
+// range to scan (a bucket or a single leaf), and the lanes then scan their
+// ranges together.  In tree_walk's single loop a wavefront's lanes stand at
+// nodes and buckets in no common rhythm, so nearly every step pays for a
+// bucket scan (the candidate loop with its pair body) with the few lanes that
+// happen to be at a bucket; here every unfinished lane is in the scan.  Each
+// lane visits its candidates in the same order either way.
+template <typename F>
+__device__ __forceinline__ void tree_walk_grouped(const TreeArgs &T, const BoxDesc &G, const double x[3], double R,
+                                                  F &&f)
+{
+    int node = G.root;
+    int steps = 2 * G.count;  // as in tree_walk
+    for (;;) {
+        int beg = 0, end = -1;
+        while (node != kTreeEnd && steps > 0) {
+            steps--;
+            if (node < 0) {
+                beg = end = -node - 1;
+                node = T.leaf_esc[beg];
+                break;
+            }
+            const TreeNode nd = T.nodes[node];
+            const double gx = periodic_gap(x[0], nd.lo.x, nd.hi.x, G.box.L[0]);
+            const double gy = periodic_gap(x[1], nd.lo.y, nd.hi.y, G.box.L[1]);
+            const double gz = periodic_gap(x[2], nd.lo.z, nd.hi.z, G.box.L[2]);
+            const int esc = __float_as_int(nd.hi.w);
+            if (!(gx * gx + gy * gy + gz * gz <= R * R * (1.0 + 1e-9))) {
+                node = esc;
+                continue;
+            }
+            if (nd.link.z - nd.link.y < kTreeBucket) {
+                beg = nd.link.y;
+                end = nd.link.z;
+                node = esc;
+                break;
+            }
+            node = nd.link.x;
+        }
+        if (beg > end) return;
+        for (int j = beg; j <= end; j++)
+            if (!visit_more(f, j)) return;
     }
 }
 

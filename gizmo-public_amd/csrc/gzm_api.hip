@@ -120,8 +120,7 @@ struct gzm_ctx {
     int *d_defer_count = nullptr, *d_defer_off = nullptr, *d_defer_total = nullptr;
     uint8_t *d_defer_raw = nullptr;
     double *d_defer_result = nullptr;
-    int *d_defer_owner = nullptr;      // per deferred pair of a round: active entry and list position
-    uint8_t *d_defer_kpos = nullptr;
+    int2 *d_defer_pair = nullptr;      // per deferred pair of a round: particle and neighbour
     size_t defer_result_cap = 0;   // deferred pairs the result buffer holds
     int64_t last_deferred = 0;     // deferred pairs of the last force call
     // pair list written by the gradient kernel, read by the force kernel
@@ -144,6 +143,8 @@ struct gzm_ctx {
     double4 *d_t_lo = nullptr, *d_t_hi = nullptr;
     float *d_t_wmax = nullptr;
     TreeNode *d_t_nodes = nullptr;
+    int *d_dcache_idx = nullptr, *d_dcache_n = nullptr;  // density candidate cache (tree search)
+    double *d_dcache_r2 = nullptr;
     TreeArgs tree;      // all null in grid mode
     TreeBuild tbuild;
 
@@ -269,11 +270,12 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_list); dev_free(c->d_list_count); dev_free(c->d_list_over);
     dev_free(c->d_box_of);
     dev_free(c->d_defer_mask); dev_free(c->d_defer_count); dev_free(c->d_defer_off); dev_free(c->d_defer_total);
-    dev_free(c->d_defer_raw); dev_free(c->d_defer_result); dev_free(c->d_defer_owner); dev_free(c->d_defer_kpos);
+    dev_free(c->d_defer_raw); dev_free(c->d_defer_result); dev_free(c->d_defer_pair);
     dev_free(c->d_t_child); dev_free(c->d_t_range); dev_free(c->d_t_large);
     dev_free(c->d_t_large_count);
     dev_free(c->d_t_split); dev_free(c->d_t_esc); dev_free(c->d_t_leaf_esc); dev_free(c->d_t_flag);
     dev_free(c->d_t_lo); dev_free(c->d_t_hi); dev_free(c->d_t_wmax); dev_free(c->d_t_nodes);
+    dev_free(c->d_dcache_idx); dev_free(c->d_dcache_n); dev_free(c->d_dcache_r2);
     c->tree = TreeArgs{};
     c->defer_result_cap = 0;
     c->list_entries = 0;
@@ -738,6 +740,9 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
             s = s ? s : dev_alloc(&c->d_t_lo, n);
             s = s ? s : dev_alloc(&c->d_t_hi, n);
             s = s ? s : dev_alloc(&c->d_t_nodes, n);
+            s = s ? s : dev_alloc(&c->d_dcache_idx, (size_t)n * kDensityCacheCap);
+            s = s ? s : dev_alloc(&c->d_dcache_n, n);
+            s = s ? s : dev_alloc(&c->d_dcache_r2, n);
         }
         if (s) { free_particles(c); return s; }
     }
@@ -936,11 +941,14 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     }
     const int na = c->n_active;
     int n_live = 0;
+    // candidate cache of the FP64 tree sweeps (not with FP32 sums: their FP64
+    // redo sweeps restart the iteration of a few particles)
+    const bool dcache = c->tree.nodes && c->d_dcache_idx && c->p.precision != 2;
     if (na > 0) {
         {
             Timed tm(c, kOther);
-            launch_density_init(c->stream, na, c->d_active, c->d_posm, c->d_hsml, c->d_left, c->d_right,
-                                c->p.max_hsml, c->d_flags);
+            launch_density_init(c->stream, na, c->d_active, c->d_posm, dcache ? c->d_dcache_n : nullptr, c->d_hsml,
+                                c->d_left, c->d_right, c->p.max_hsml, c->d_flags);
         }
         s = compact(c, c->d_active, c->d_flags, na, c->d_live, gp ? nullptr : &n_live);
         if (s) return s;
@@ -968,6 +976,11 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     a.num_ngb = c->d_num_ngb; a.dhsml = c->d_dhsml; a.divvel = c->d_divvel; a.rho = c->d_rho;
     a.fce = c->d_fce; a.nvt = c->d_nvt;
     a.pvel = (c->p.method == 1) ? c->d_pvel : nullptr;
+    if (dcache) {
+        a.dcache_idx = c->d_dcache_idx;
+        a.dcache_n = c->d_dcache_n;
+        a.dcache_r2 = c->d_dcache_r2;
+    }
     a.redo = c->d_flags;
     a.budget = c->p.density_budget == 0 ? kDensityBudget : (c->p.density_budget < 0 ? 0 : c->p.density_budget);
     a.heavy_list = c->d_heavy_list;
@@ -1365,12 +1378,10 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
             const size_t want = (size_t)std::min<int64_t>(D, std::max<int64_t>(1 << 20, 2 * (int64_t)c->n));
             if (want > c->defer_result_cap) {
                 dev_free(c->d_defer_result);
-                dev_free(c->d_defer_owner);
-                dev_free(c->d_defer_kpos);
+                dev_free(c->d_defer_pair);
                 c->defer_result_cap = 0;
                 s = dev_alloc(&c->d_defer_result, want * (mfv ? 9 : 5));
-                s = s ? s : dev_alloc(&c->d_defer_owner, want);
-                s = s ? s : dev_alloc(&c->d_defer_kpos, want);
+                s = s ? s : dev_alloc(&c->d_defer_pair, want);
                 if (s) return s;
                 c->defer_result_cap = want;
             }
@@ -1379,8 +1390,7 @@ gzm_status force_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         if (D > 0 && c->defer_result_cap == 0) { set_error("deferred-pair buffer missing while capturing"); return GZM_ERR_STATE; }
         const int64_t chunk = D > 0 ? (int64_t)c->defer_result_cap : 1;
         a.defer.result = c->d_defer_result;
-        a.defer.owner = c->d_defer_owner;
-        a.defer.kpos = c->d_defer_kpos;
+        a.defer.pair = c->d_defer_pair;
         for (int64_t q0 = 0; q0 == 0 || q0 < D; q0 += chunk) {
             const int64_t q1 = std::min<int64_t>(D, q0 + chunk);
             a.defer.q_begin = (int)q0;

@@ -154,6 +154,15 @@ struct Counters {
 // candidates a density-sweep lane walks before it defers its particle to the
 // second, load-balanced launch (gzm_params.density_budget = 0)
 constexpr int kDensityBudget = 512;
+
+// Density candidate cache of the FP64 tree sweeps: the first sweep of a stage
+// walks the tree with radius kDensityCacheFactor * hsml and keeps, per
+// particle, the candidates within that radius in walk order (at most
+// kDensityCacheCap; more marks the particle uncached).  A later sweep whose
+// hsml still lies inside the radius scans that row instead of the tree: the
+// candidates within hsml come out of it in the order the walk would give.
+constexpr double kDensityCacheFactor = 1.2;
+constexpr int kDensityCacheCap = 96;  // a multiple of 4: rows are read as int4
 // a sweep over at most this many particles runs one wavefront per particle:
 // they all fit on the GPU at once, and a walk shared by 64 lanes is shorter
 constexpr int kWaveSweepMax = 4096;
@@ -188,6 +197,13 @@ struct DensitySweepArgs {
     int *heavy_list;
     int *n_heavy;
     int wave_all;          // wavefront-per-particle launch over all live entries (a small sweep)
+    // Density candidate cache (tree search, FP64 sums; NULL otherwise): row
+    // dcache_idx[i * kDensityCacheCap ...] of dcache_n[i] candidates within
+    // sqrt(dcache_r2[i]) of particle i, in walk order; dcache_n[i] above the
+    // cap: no valid row (set for every active particle by the stage's init).
+    int *dcache_idx;
+    int *dcache_n;
+    double *dcache_r2;
     // FP32 density sums (precision 2): a particle that converges with a
     // candidate whose r^2 lies within kEdgeBand of h^2 is appended to
     // edge_list (count n_edge); the stage then redoes its whole iteration in
@@ -254,8 +270,8 @@ struct DeferArgs {
     const int *total;      // sum of count (device)
     double *result;        // per deferred pair of the round: dv/dt terms (3), du/dt term, v_sig
                            // (MFV: 9 per pair, see force_mfv_deferred_kernel)
-    int *owner;            // per deferred pair of the round: its active entry t and list position k,
-    uint8_t *kpos;         //   written by launch_defer_owners before the round's pair kernel
+    int2 *pair;            // per deferred pair of the round: particle i and neighbour j, written by
+                           //   launch_defer_owners before the round's pair kernel
     int q_begin, q_end;    // deferred pairs handled by this round
 };
 
@@ -325,7 +341,7 @@ void launch_box_hmax(hipStream_t s, int n_boxes, BoxDesc *boxes, const double *h
 void launch_flag_unconverged(hipStream_t s, const int *n_live_dev, Counters *ctr);
 void launch_clear_counters(hipStream_t s, Counters *ctr);
 void launch_map_active(hipStream_t s, int n_active, const int *active_caller, const int *inv, int *active_sorted);
-void launch_density_init(hipStream_t s, int n_active, const int *active, const double4 *posm, double *hsml,
+void launch_density_init(hipStream_t s, int n_active, const int *active, const double4 *posm, int *dcache_n, double *hsml,
                          double *left, double *right, double max_hsml, int *live_flag);
 void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a);
 // precision 2: the listed particles (count on the device) restart their hsml

@@ -169,12 +169,13 @@ __global__ void set_flags_kernel(int n_active, const int *active, uint8_t *flags
 // ---------------------------------------------------------------------------
 // density (density.c:252-347 pair sums + 505-850 iteration, fused)
 // ---------------------------------------------------------------------------
-__global__ void density_init_kernel(int n_active, const int *active, const double4 *posm, double *hsml,
-                                    double *left, double *right, double max_hsml, int *live_flag)
+__global__ void density_init_kernel(int n_active, const int *active, const double4 *posm, int *dcache_n,
+                                    double *hsml, double *left, double *right, double max_hsml, int *live_flag)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_active) return;
     int i = active[t];
+    if (dcache_n) dcache_n[i] = kDensityCacheCap + 1;  // no candidate row until this stage's first sweep
     int live = posm[i].w > 0;  // density_isactive: gas with mass > 0 (density.c:85)
     live_flag[t] = live;
     if (!live) return;
@@ -192,12 +193,13 @@ template <bool EDGE = false, bool MFV = false>
 __device__ __forceinline__ bool density_pair(const BoxDesc &G, const double4 &pi, const double4 &vi,
                                              const DensitySweepArgs &a, int j, double h2, double hinv,
                                              double hinv3, double hinv4, DensitySums &S, bool *edge = nullptr,
-                                             double *pv = nullptr)
+                                             double *pv = nullptr, double *r2_out = nullptr)
 {
     const double4 pj = a.posm[j];
     double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
     wrap_dp(G.box, dp);
     const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+    if (r2_out) *r2_out = r2;
     if (EDGE && pj.w > 0 && fabs(r2 - h2) <= kEdgeBand * h2) *edge = true;
     if (!(r2 < h2 && pj.w > 0)) return false;
     const double r = sqrt(r2);
@@ -288,14 +290,49 @@ __global__ __launch_bounds__(MAXB, 4) void density_sweep_kernel(DensitySweepArgs
             if (density_pair<false, MFV>(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S, nullptr, pv)) lane_cnt++;
         };
         if (TREE) {
-            tree_walk(a.tree, G, xs, h, false, [&](int j) -> bool {
-                if (budget && ++visited > budget) {
-                    heavy = true;
-                    return false;
+            const int cap = kDensityCacheCap;
+            int *row = a.dcache_idx ? a.dcache_idx + (size_t)i * cap : nullptr;
+            const int n_row = (row && a.hp.iter > 0) ? a.dcache_n[i] : cap + 1;
+            if (n_row <= cap && h2 <= a.dcache_r2[i]) {
+                // every candidate within h is in the row, in walk order
+                const int4 *row4 = reinterpret_cast<const int4 *>(row);
+                for (int k = 0; k < n_row; k += 4) {
+                    const int4 q = row4[k >> 2];
+                    pair(q.x);
+                    if (k + 1 < n_row) pair(q.y);
+                    if (k + 2 < n_row) pair(q.z);
+                    if (k + 3 < n_row) pair(q.w);
                 }
-                pair(j);
-                return true;
-            });
+            } else if (row && a.hp.iter == 0) {
+                // first sweep: walk the cache radius, keep its candidates
+                const double Rc = kDensityCacheFactor * h, Rc2 = Rc * Rc;
+                int m = 0;
+                tree_walk_grouped(a.tree, G, xs, Rc, [&](int j) -> bool {
+                    if (budget && ++visited > 2 * budget) {
+                        heavy = true;
+                        return false;
+                    }
+                    double r2;
+                    if (density_pair<false, MFV>(G, pi, vi, a, j, h2, hinv, hinv3, hinv4, S, nullptr, pv, &r2))
+                        lane_cnt++;
+                    if (r2 < Rc2) {
+                        if (m < cap) row[m] = j;
+                        m++;
+                    }
+                    return true;
+                });
+                a.dcache_n[i] = heavy ? cap + 1 : m;
+                a.dcache_r2[i] = Rc2;
+            } else {
+                tree_walk_grouped(a.tree, G, xs, h, [&](int j) -> bool {
+                    if (budget && ++visited > budget) {
+                        heavy = true;
+                        return false;
+                    }
+                    pair(j);
+                    return true;
+                });
+            }
         } else {
             // the budget is charged a cell at a time, outside the candidate loop
             bool full[3];
@@ -732,12 +769,10 @@ __global__ __launch_bounds__(kBlock) void force_deferred_kernel(ForceArgs a)
     const int q = a.defer.q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int q_end = min(*a.defer.total, a.defer.q_end);
     const bool valid = q < q_end;
-    const int t = valid ? a.defer.owner[q - a.defer.q_begin] : 0;
+    const int2 ij = valid ? a.defer.pair[q - a.defer.q_begin] : make_int2(0, 0);
     unsigned long long nretry = 0, nexact = 0;
-    for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
-        const int i = a.active[t];
-        const int k = a.defer.kpos[q - a.defer.q_begin];
-        const int j = a.list.idx[(size_t)t * a.list.cap + k];
+    for_box(a.boxes, a.box_of, ij.x, valid, [&](const BoxDesc &G) {
+        const int i = ij.x, j = ij.y;
         const ForceRec ri = a.rec[i];
         PairI<double> I;
         unpack_side(ri, I);
@@ -896,11 +931,11 @@ void launch_map_active(hipStream_t s, int n_active, const int *active_caller, co
     if ((n_active) > 0) hipLaunchKernelGGL(map_active_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, n_active, active_caller, inv,
                        active_sorted);
 }
-void launch_density_init(hipStream_t s, int n_active, const int *active, const double4 *posm, double *hsml,
+void launch_density_init(hipStream_t s, int n_active, const int *active, const double4 *posm, int *dcache_n, double *hsml,
                          double *left, double *right, double max_hsml, int *live_flag)
 {
     if ((n_active) > 0) hipLaunchKernelGGL(density_init_kernel, dim3(nblocks(n_active)), dim3(kBlock), 0, s, n_active, active, posm,
-                       hsml, left, right, max_hsml, live_flag);
+                       dcache_n, hsml, left, right, max_hsml, live_flag);
 }
 void launch_density_sweep(hipStream_t s, const DensitySweepArgs &a)
 {
@@ -970,7 +1005,8 @@ void launch_force(hipStream_t s, const ForceArgs &a)
 namespace {
 
 // one thread per active entry: its deferred pairs q = offset + r (r-th set
-// bit of its mask) that fall in the round get their owner entry and bit
+// bit k of its mask) that fall in the round get their particle and neighbour
+// (i, list entry k), so that the pair kernel starts from both records' indices
 __global__ __launch_bounds__(kBlock) void defer_owners_kernel(ForceArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -984,10 +1020,8 @@ __global__ __launch_bounds__(kBlock) void defer_owners_kernel(ForceArgs a)
         const int k = __builtin_ctzll(m);
         m &= m - 1;
         const int q = q0 + r;
-        if (q >= a.defer.q_begin && q < a.defer.q_end) {
-            a.defer.owner[q - a.defer.q_begin] = t;
-            a.defer.kpos[q - a.defer.q_begin] = (uint8_t)k;
-        }
+        if (q >= a.defer.q_begin && q < a.defer.q_end)
+            a.defer.pair[q - a.defer.q_begin] = make_int2(a.active[t], a.list.idx[(size_t)t * a.list.cap + k]);
     }
 }
 

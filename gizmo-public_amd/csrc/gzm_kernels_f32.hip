@@ -217,7 +217,7 @@ __global__ __launch_bounds__(MAXB) void density_sweep_f32_kernel(DensitySweepArg
             }
         };
         if (TREE) {
-            tree_walk(a.tree, G, xs, h, false, [&](int k) -> bool {
+            tree_walk_grouped(a.tree, G, xs, h, [&](int k) -> bool {
                 if (budget && ++visited > budget) {
                     heavy = true;
                     return false;
@@ -780,12 +780,10 @@ __global__ __launch_bounds__(kBlock, 4) void force_deferred_f32_kernel(ForceArgs
     const int q = a.defer.q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int q_end = min(*a.defer.total, a.defer.q_end);
     const bool valid = q < q_end;
-    const int t = valid ? a.defer.owner[q - a.defer.q_begin] : 0;
+    const int2 ij = valid ? a.defer.pair[q - a.defer.q_begin] : make_int2(0, 0);
     unsigned long long nretry = 0, nexact = 0;
-    for_box(a.boxes, a.box_of, valid ? a.active[t] : 0, valid, [&](const BoxDesc &G) {
-        const int i = a.active[t];
-        const int k = a.defer.kpos[q - a.defer.q_begin];
-        const int j = a.list.idx[(size_t)t * a.list.cap + k];
+    for_box(a.boxes, a.box_of, ij.x, valid, [&](const BoxDesc &G) {
+        const int i = ij.x, j = ij.y;
         const ForceRecF ri = a.recf[i];
         PairI<float> I;
         side_i_f(ri, I);
