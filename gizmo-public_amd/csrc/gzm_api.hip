@@ -145,6 +145,8 @@ struct gzm_ctx {
     TreeNode *d_t_nodes = nullptr;
     int *d_dcache_idx = nullptr, *d_dcache_n = nullptr;  // density candidate cache (tree search)
     double *d_dcache_r2 = nullptr;
+    int *d_brow = nullptr, *d_bcount = nullptr;          // symmetric pair rows of the FP64 gradients
+    bool dcache_all = false;  // every particle's candidate row built since the last upload
     TreeArgs tree;      // all null in grid mode
     TreeBuild tbuild;
 
@@ -276,6 +278,8 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_t_split); dev_free(c->d_t_esc); dev_free(c->d_t_leaf_esc); dev_free(c->d_t_flag);
     dev_free(c->d_t_lo); dev_free(c->d_t_hi); dev_free(c->d_t_wmax); dev_free(c->d_t_nodes);
     dev_free(c->d_dcache_idx); dev_free(c->d_dcache_n); dev_free(c->d_dcache_r2);
+    dev_free(c->d_brow); dev_free(c->d_bcount);
+    c->dcache_all = false;
     c->tree = TreeArgs{};
     c->defer_result_cap = 0;
     c->list_entries = 0;
@@ -743,6 +747,8 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
             s = s ? s : dev_alloc(&c->d_dcache_idx, (size_t)n * kDensityCacheCap);
             s = s ? s : dev_alloc(&c->d_dcache_n, n);
             s = s ? s : dev_alloc(&c->d_dcache_r2, n);
+            s = s ? s : dev_alloc(&c->d_brow, (size_t)n * kPairRowCap);
+            s = s ? s : dev_alloc(&c->d_bcount, n);
         }
         if (s) { free_particles(c); return s; }
     }
@@ -836,6 +842,9 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
         GZM_HIP(hipGetLastError());
         c->tree.nodes = c->d_t_nodes;
         c->tree.leaf_esc = c->d_t_leaf_esc;
+        // new positions: no candidate row is valid
+        launch_fill_int(c->stream, c->d_dcache_n, n, kDensityCacheCap + 1);
+        c->dcache_all = false;
     }
 
     // optional state of particles that may stay inactive
@@ -1143,6 +1152,7 @@ gzm_status density_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
     c->have_gradients = false;
     c->list_valid = false;
     c->frec_fresh = false;
+    if (dcache && na == c->n) c->dcache_all = true;  // every particle went through the first sweep
     if (gp) return GZM_OK;
     Counters h;
     s = read_counters(c, kCtrDensity, &h);
@@ -1216,6 +1226,31 @@ gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         a.list.count = c->d_list_count;
         a.list.overflow = c->d_list_over;
         a.list.cap = c->list_cap;
+    }
+    // symmetric pair rows instead of the pair walk (FP64, tree search, every
+    // particle active, candidate rows of every particle built since the upload)
+    if (c->p.precision == 0 && c->tree.nodes && c->d_brow && c->dcache_all && fuse_records && c->n_active > 0) {
+        SymRowArgs r;
+        r.boxes = c->d_boxes;
+        r.box_of = c->d_box_of;
+        r.tree = c->tree;
+        r.n = c->n;
+        r.posm = c->d_posm;
+        r.hsml = c->d_hsml;
+        r.dcache_idx = c->d_dcache_idx;
+        r.dcache_n = c->d_dcache_n;
+        r.dcache_r2 = c->d_dcache_r2;
+        r.brow = c->d_brow;
+        r.bcount = c->d_bcount;
+        {
+            Timed tm(c, kGradients);
+            launch_sym_rows(c->stream, r);
+        }
+        a.dcache_idx = c->d_dcache_idx;
+        a.dcache_n = c->d_dcache_n;
+        a.dcache_r2 = c->d_dcache_r2;
+        a.brow = c->d_brow;
+        a.bcount = c->d_bcount;
     }
     if (c->n_active > 0) {
         Timed tm(c, kGradients);

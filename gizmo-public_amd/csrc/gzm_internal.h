@@ -163,6 +163,30 @@ constexpr int kDensityBudget = 512;
 // candidates within hsml come out of it in the order the walk would give.
 constexpr double kDensityCacheFactor = 1.2;
 constexpr int kDensityCacheCap = 96;  // a multiple of 4: rows are read as int4
+
+// Symmetric pair rows of the FP64 gradient stage (tree search, every particle
+// active).  The gradient pair set of i is {j : r_ij < h_i or r_ij < h_j}; with
+// positions unchanged since the density stage, the part within i's candidate
+// row radius is in i's row, and the rest (r_ij < h_j only) is in j's row or
+// j's own walk, so each j that finds i there appends itself to i's pair row
+// (at most kPairRowCap; more sends i back to the tree walk).  The tree walk
+// visits slots in ascending order, and so does the gradient kernel's merge of
+// the two sorted rows: same candidates, same order, same sums.
+constexpr int kPairRowCap = 32;
+
+struct SymRowArgs {
+    const BoxDesc *boxes;
+    const int *box_of;
+    TreeArgs tree;
+    int n;
+    const double4 *posm;
+    const double *hsml;
+    const int *dcache_idx;
+    const int *dcache_n;
+    const double *dcache_r2;
+    int *brow;             // kPairRowCap per particle
+    int *bcount;           // zeroed before the scatter
+};
 // a sweep over at most this many particles runs one wavefront per particle:
 // they all fit on the GPU at once, and a walk shared by 64 lanes is shorter
 constexpr int kWaveSweepMax = 4096;
@@ -255,6 +279,12 @@ struct GradientArgs {
     const double *dhsml, *fce;
     double gamma;
     int block;             // threads per workgroup of the gradient kernel (64 .. kMaxBlock)
+    // symmetric pair rows (FP64 kernel; NULL: every particle walks the tree)
+    const int *dcache_idx;
+    const int *dcache_n;
+    const double *dcache_r2;
+    const int *brow;
+    const int *bcount;
 };
 
 // Pairs whose Riemann problem reaches the exact solver are taken out of the
@@ -355,6 +385,7 @@ void launch_density_finalize(hipStream_t s, int n_active, const int *active, con
                              double *pres, double gamma_minus_1, double *pvel);
 void launch_pack_grad(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
                       const double *rho, const double *pres, GradRec *rec);
+void launch_sym_rows(hipStream_t s, const SymRowArgs &a);
 void launch_gradients(hipStream_t s, const GradientArgs &a);
 void launch_pack_force(hipStream_t s, int n, const double4 *posm, const double4 *velu, const double *hsml,
                        const double *rho, const double *pres, const double *dhsml, const double *cn,

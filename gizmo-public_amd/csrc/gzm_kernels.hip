@@ -493,7 +493,65 @@ __device__ __forceinline__ ForceRec make_force_rec(const double4 &p, const doubl
     return r;
 }
 
-template <int BS>
+constexpr int kNoSlot = 0x7fffffff;  // above every sorted slot: the end of a merged row
+
+// Symmetric pair rows, scatter: particle j offers itself to every particle i
+// with 0 < r_ij < h_j that lies outside i's candidate row radius, taking the
+// i from its own candidate row when that covers h_j, else from a walk.  Only
+// particles i whose row covers their own hsml take entries (the others walk).
+__global__ __launch_bounds__(kBlock) void sym_rows_scatter_kernel(SymRowArgs a)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = j < a.n;
+    for_box(a.boxes, a.box_of, valid ? j : 0, valid, [&](const BoxDesc &G) {
+        const double4 pj = a.posm[j];
+        const double h_j = a.hsml[j];
+        if (!(h_j > 0)) return;
+        const double hj2 = h_j * h_j;
+        auto offer = [&](int i) {
+            const double4 pi = a.posm[i];
+            // r_ij as the gradient kernel computes it for i (x_i - x_j)
+            double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+            wrap_dp(G.box, dp);
+            const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+            if (!(r2 < hj2) || r2 <= 0) return;
+            if (a.dcache_n[i] > kDensityCacheCap) return;
+            const double ri2 = a.dcache_r2[i], h_i = a.hsml[i];
+            if (!(h_i * h_i <= ri2) || r2 < ri2) return;  // i walks, or j is in i's row
+            const int k = atomicAdd(&a.bcount[i], 1);
+            if (k < kPairRowCap) a.brow[(size_t)i * kPairRowCap + k] = j;
+        };
+        const int n_j = a.dcache_n[j];
+        if (n_j <= kDensityCacheCap && hj2 <= a.dcache_r2[j]) {
+            const int *row = a.dcache_idx + (size_t)j * kDensityCacheCap;
+            for (int k = 0; k < n_j; k++) offer(row[k]);
+        } else {
+            const double xs[3] = {pj.x, pj.y, pj.z};
+            tree_walk_grouped(a.tree, G, xs, h_j, [&](int i) { offer(i); });
+        }
+    });
+}
+
+// Symmetric pair rows, sort: each row ascending (a handful of entries)
+__global__ __launch_bounds__(kBlock) void sym_rows_sort_kernel(int n, int *brow, const int *bcount)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int c = bcount[i];
+    if (c < 2 || c > kPairRowCap) return;
+    int *r = brow + (size_t)i * kPairRowCap;
+    for (int k = 1; k < c; k++) {
+        const int v = r[k];
+        int m = k - 1;
+        while (m >= 0 && r[m] > v) {
+            r[m + 1] = r[m];
+            m--;
+        }
+        r[m + 1] = v;
+    }
+}
+
+template <int BS, bool ROWS = false>
 __global__ __launch_bounds__(BS) void gradients_kernel(GradientArgs a)
 {
     // Pair-list entries are staged in LDS, column-major (entry c of thread x
@@ -520,7 +578,7 @@ __global__ __launch_bounds__(BS) void gradients_kernel(GradientArgs a)
             kernel_hinv(h_i, hinv, hinv3, hinv4);
             const double h2_i = h_i * h_i;
             const double xs[3] = {ri.xh.x, ri.xh.y, ri.xh.z};
-            for_each_neighbour_candidate(a.tree, G, xs, h_i, true, [&](int j) {
+            auto visit = [&](int j) {
                 const double4 xj = a.rec[j].xh;
                 double dp[3] = {xs[0] - xj.x, xs[1] - xj.y, xs[2] - xj.z};
                 wrap_dp(G.box, dp);
@@ -562,7 +620,29 @@ __global__ __launch_bounds__(BS) void gradients_kernel(GradientArgs a)
                     gv10 += w0 * dv1; gv11 += w1 * dv1; gv12 += w2 * dv1;
                     gv20 += w0 * dv2; gv21 += w1 * dv2; gv22 += w2 * dv2;
                 }
-            });
+            };
+            const int n_a = ROWS ? a.dcache_n[i] : 0;
+            const int n_b = ROWS ? a.bcount[i] : 0;
+            if (ROWS && n_a <= kDensityCacheCap && h2_i <= a.dcache_r2[i] && n_b <= kPairRowCap) {
+                // ascending merge of i's candidate row and its pair row
+                const int *ra = a.dcache_idx + (size_t)i * kDensityCacheCap;
+                const int *rb = a.brow + (size_t)i * kPairRowCap;
+                int ka = 0, kb = 0;
+                int ja = (n_a > 0) ? ra[0] : kNoSlot, jb = (n_b > 0) ? rb[0] : kNoSlot;
+                while (ka < n_a || kb < n_b) {
+                    if (ja < jb) {
+                        visit(ja);
+                        ka++;
+                        ja = (ka < n_a) ? ra[ka] : kNoSlot;
+                    } else {
+                        visit(jb);
+                        kb++;
+                        jb = (kb < n_b) ? rb[kb] : kNoSlot;
+                    }
+                }
+            } else {
+                for_each_neighbour_candidate(a.tree, G, xs, h_i, true, visit);
+            }
             // the partly filled last chunk (entries past `listed` are never read)
             if (list_row && listed < a.list.cap && (listed & (kListChunk - 1)))
                 flush_list_chunk(stage, list_row + (listed & ~(kListChunk - 1)), bs);
@@ -973,10 +1053,24 @@ void launch_pack_grad(hipStream_t s, int n, const double4 *posm, const double4 *
 {
     if ((n) > 0) hipLaunchKernelGGL(pack_grad_kernel, dim3(nblocks(n)), dim3(kBlock), 0, s, n, posm, velu, hsml, rho, pres, rec);
 }
+void launch_sym_rows(hipStream_t s, const SymRowArgs &a)
+{
+    if (a.n <= 0) return;
+    launch_fill_int(s, a.bcount, a.n, 0);
+    hipLaunchKernelGGL(sym_rows_scatter_kernel, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(sym_rows_sort_kernel, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a.n, a.brow, a.bcount);
+}
 void launch_gradients(hipStream_t s, const GradientArgs &a)
 {
     if (a.n_active <= 0) return;
     const dim3 grid(nblocks(a.n_active, a.block)), block(a.block);
+    if (a.brow) {
+        if (a.block == 64) hipLaunchKernelGGL((gradients_kernel<64, true>), grid, block, 0, s, a);
+        else if (a.block == 128) hipLaunchKernelGGL((gradients_kernel<128, true>), grid, block, 0, s, a);
+        else if (a.block == 512) hipLaunchKernelGGL((gradients_kernel<512, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((gradients_kernel<kBlock, true>), grid, block, 0, s, a);
+        return;
+    }
     if (a.block == 64) hipLaunchKernelGGL(gradients_kernel<64>, grid, block, 0, s, a);
     else if (a.block == 128) hipLaunchKernelGGL(gradients_kernel<128>, grid, block, 0, s, a);
     else if (a.block == 512) hipLaunchKernelGGL(gradients_kernel<512>, grid, block, 0, s, a);

@@ -152,3 +152,24 @@ def test_tree_fp32_pass_within_bounds():
         assert es[k]["max"] <= TOL32_DENSITY, (k, es[k])
     for k in ("acc", "dtu"):
         assert es[k]["max"] <= TOL32_FLUX, (k, es[k])
+
+
+@pytest.mark.parametrize("name", ["plummer_clusters_32k", "blast_16"])
+def test_tree_candidate_rows_give_the_walk_bits(name):
+    """The FP64 tree pass, all particles active, takes each gradient pair set
+    from the density stage's candidate rows merged with the symmetric pair rows
+    (no tree walk).  Gradients run on the density outputs uploaded into a fresh
+    context have no rows and walk the tree; both must visit the same pairs in
+    the same (ascending slot) order, so gradients, the pair lists and every
+    force output agree bit for bit."""
+    ic = CASES[name]()
+    prm = params_for(ic, search=1)
+    rows, st_rows = gzm.run_pass(ic, params=prm)
+    state = dict(ic)
+    for k in ("hsml", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "cn", "fce"):
+        state[k] = rows[k]
+    walk, st_walk = gzm.run_pass(state, params=prm, stages=("gradients", "force"))
+    assert st_walk["gradient_interactions"] == st_rows["gradient_interactions"]
+    assert st_walk["force_interactions"] == st_rows["force_interactions"]
+    for k in ("grad_rho", "grad_p", "grad_v", "acc", "dtu", "vsig"):
+        np.testing.assert_array_equal(walk[k], rows[k], err_msg=k)
