@@ -692,6 +692,11 @@ def main_clustered(args, gzm, ics):
     active = ics.random_subset(N, 0.1)
     part = run_mode(gzm, state, args, 0, search=1, active=active)
     log(f"10% active pass: {part['ms_per_step']:.1f} ms")
+    # FP32-bulk (precision 1: FP64 density stage, FP32 gradient and force pairs)
+    full32 = run_mode(gzm, ic, args, 1, search=1)
+    log(f"FP32-bulk full pass: {full32['ms_per_step']:.1f} ms")
+    part32 = run_mode(gzm, state, args, 1, search=1, active=active)
+    log(f"FP32-bulk 10% active pass: {part32['ms_per_step']:.1f} ms")
     h = ic["hsml"]
     result = {
         "metric": "particle-neighbor interactions/s, MFM pass; FP32-batched vs FP64 at stated error",
@@ -739,6 +744,21 @@ def main_clustered(args, gzm, ics):
             "density_sweeps": part["density_sweeps"],
             "kernels_ms_per_pass": part["kernels_ms_per_pass"],
         },
+        "fp32_same_workload": {
+            "precision": 1,
+            "value": full32["value"],
+            "unit": "interactions/s",
+            "ms_per_step": full32["ms_per_step"],
+            "speedup_vs_fp64": full["ms_per_step"] / full32["ms_per_step"],
+            "interactions_equal_fp64": bool(full32["interactions_per_pass"] == full["interactions_per_pass"]),
+            "kernels_ms_per_pass": full32["kernels_ms_per_pass"],
+            "active_10pct": {
+                "value": part32["value"],
+                "ms_per_step": part32["ms_per_step"],
+                "speedup_vs_fp64": part["ms_per_step"] / part32["ms_per_step"],
+                "interactions_equal_fp64": bool(part32["interactions_per_pass"] == part["interactions_per_pass"]),
+            },
+        },
     }
     if args.cpu_repeats > 0:
         # CPU baseline on a bounded sample: a clustered set of --cpu-sample
@@ -766,6 +786,29 @@ def main_clustered(args, gzm, ics):
         result["parity_vs_oracle_on_sample"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
         result["parity_vs_oracle_on_sample"]["interactions_equal"] = bool(
             cpu_int == st_s["density_interactions"] + st_s["gradient_interactions"] + st_s["force_interactions"])
+        # every particle: the oracle (octree) over the whole set, full pass and
+        # the 10 %-active pass on the state the FP64 full pass left
+        from parity import error_stats
+
+        def fmt(es):
+            return {k: (v if isinstance(v, int) else {q: (float(f"{x:.3e}") if isinstance(x, float) else x)
+                                                      for q, x in v.items()})
+                    for k, v in es.items()}
+        t = time.perf_counter()
+        prm_o = O.make_params(box=tuple(ic["box"]), nthreads=cores, tree=1)
+        ref_f, ost_f, _ = O.run_pass(ic, params=prm_o)
+        ref_p, ost_p, _ = O.run_pass(state, active=active, params=prm_o)
+        log(f"CPU oracle on all {N:,} particles (full and 10 % active): {time.perf_counter() - t:.1f} s")
+        int_f = ost_f.dens_interactions + ost_f.grad_interactions + ost_f.force_interactions
+        int_p = ost_p.dens_interactions + ost_p.grad_interactions + ost_p.force_interactions
+        for rec, out, ref_o, n_int, act in ((result, full, ref_f, int_f, None),
+                                            (result["active_10pct"], part, ref_p, int_p, active)):
+            errs = compare_pass(out["out"], ref_o, active=act)
+            rec["parity_vs_oracle"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+            rec["parity_vs_oracle"]["interactions_equal"] = bool(n_int == out["interactions_per_pass"])
+        result["fp32_same_workload"]["error_vs_fp64_oracle"] = fmt(error_stats(full32["out"], ref_f))
+        result["fp32_same_workload"]["active_10pct"]["error_vs_fp64_oracle"] = fmt(
+            error_stats(part32["out"], ref_p, active=active))
     print(json.dumps(result))
 
 

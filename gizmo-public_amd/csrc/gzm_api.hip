@@ -1227,9 +1227,10 @@ gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         a.list.overflow = c->d_list_over;
         a.list.cap = c->list_cap;
     }
-    // symmetric pair rows instead of the pair walk (FP64, tree search, every
-    // particle active, candidate rows of every particle built since the upload)
-    if (c->p.precision == 0 && c->tree.nodes && c->d_brow && c->dcache_all && fuse_records && c->n_active > 0) {
+    // symmetric pair rows instead of the pair walk (tree search, FP64 density
+    // sums, every particle active, candidate rows of every particle built since
+    // the upload)
+    if (c->p.precision != 2 && c->tree.nodes && c->d_brow && c->dcache_all && fuse_records && c->n_active > 0) {
         SymRowArgs r;
         r.boxes = c->d_boxes;
         r.box_of = c->d_box_of;

@@ -279,7 +279,7 @@ struct GradientArgs {
     const double *dhsml, *fce;
     double gamma;
     int block;             // threads per workgroup of the gradient kernel (64 .. kMaxBlock)
-    // symmetric pair rows (FP64 kernel; NULL: every particle walks the tree)
+    // symmetric pair rows (tree search; NULL: every particle walks the tree)
     const int *dcache_idx;
     const int *dcache_n;
     const double *dcache_r2;

@@ -318,6 +318,8 @@ __device__ __forceinline__ ForceRecF make_force_rec_f(const float4 &o, const flo
     return r;
 }
 
+constexpr int kNoSlotF = 0x7fffffff;  // above every sorted slot: the end of a merged row
+
 template <bool TREE, int BS>
 __global__ __launch_bounds__(BS, 4) void gradients_f32_kernel(GradientArgs a)
 {
@@ -423,7 +425,28 @@ __global__ __launch_bounds__(BS, 4) void gradients_f32_kernel(GradientArgs a)
                 }
             };
             if (TREE) {
-                tree_walk(a.tree, G, xs, h_i, true, [&](int k) { visit(k, 0.0f, 0.0f, 0.0f, true); });
+                // symmetric pair rows when the stage built them (as in gradients_kernel)
+                const int n_a = a.brow ? a.dcache_n[i] : 0;
+                const int n_b = a.brow ? a.bcount[i] : 0;
+                if (a.brow && n_a <= kDensityCacheCap && h2_i <= a.dcache_r2[i] && n_b <= kPairRowCap) {
+                    const int *ra = a.dcache_idx + (size_t)i * kDensityCacheCap;
+                    const int *rb = a.brow + (size_t)i * kPairRowCap;
+                    int ka = 0, kb = 0;
+                    int ja = (n_a > 0) ? ra[0] : kNoSlotF, jb = (n_b > 0) ? rb[0] : kNoSlotF;
+                    while (ka < n_a || kb < n_b) {
+                        if (ja < jb) {
+                            visit(ja, 0.0f, 0.0f, 0.0f, true);
+                            ka++;
+                            ja = (ka < n_a) ? ra[ka] : kNoSlotF;
+                        } else {
+                            visit(jb, 0.0f, 0.0f, 0.0f, true);
+                            kb++;
+                            jb = (kb < n_b) ? rb[kb] : kNoSlotF;
+                        }
+                    }
+                } else {
+                    tree_walk(a.tree, G, xs, h_i, true, [&](int k) { visit(k, 0.0f, 0.0f, 0.0f, true); });
+                }
             } else {
                 bool full[3];
                 for_each_cell(G.grid, xs, R, full, [&](int cx, int cy, int cz, int beg, int end) {

@@ -154,16 +154,18 @@ def test_tree_fp32_pass_within_bounds():
         assert es[k]["max"] <= TOL32_FLUX, (k, es[k])
 
 
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("name", ["plummer_clusters_32k", "blast_16"])
-def test_tree_candidate_rows_give_the_walk_bits(name):
-    """The FP64 tree pass, all particles active, takes each gradient pair set
-    from the density stage's candidate rows merged with the symmetric pair rows
-    (no tree walk).  Gradients run on the density outputs uploaded into a fresh
-    context have no rows and walk the tree; both must visit the same pairs in
-    the same (ascending slot) order, so gradients, the pair lists and every
-    force output agree bit for bit."""
+def test_tree_candidate_rows_give_the_walk_bits(name, precision):
+    """A tree pass with FP64 density sums (precisions 0 and 1), all particles
+    active, takes each gradient pair set from the density stage's candidate
+    rows merged with the symmetric pair rows (no tree walk).  Gradients run on
+    the density outputs uploaded into a fresh context have no rows and walk
+    the tree; both must visit the same pairs in the same (ascending slot)
+    order, so gradients, the pair lists and every force output agree bit for
+    bit."""
     ic = CASES[name]()
-    prm = params_for(ic, search=1)
+    prm = params_for(ic, search=1, precision=precision)
     rows, st_rows = gzm.run_pass(ic, params=prm)
     state = dict(ic)
     for k in ("hsml", "num_ngb", "dhsml", "divvel", "density", "pressure", "nvt", "cn", "fce"):
