@@ -145,7 +145,8 @@ struct gzm_ctx {
     TreeNode *d_t_nodes = nullptr;
     int *d_dcache_idx = nullptr, *d_dcache_n = nullptr;  // density candidate cache (tree search)
     double *d_dcache_r2 = nullptr;
-    int *d_brow = nullptr, *d_bcount = nullptr;          // symmetric pair rows of the FP64 gradients
+    int *d_brow = nullptr, *d_bcount = nullptr;          // symmetric pair rows of the gradient stage
+    int *d_bwave = nullptr, *d_bwave_n = nullptr;        //   and the particles that walk for them
     bool dcache_all = false;  // every particle's candidate row built since the last upload
     TreeArgs tree;      // all null in grid mode
     TreeBuild tbuild;
@@ -278,7 +279,7 @@ void free_particles(gzm_ctx *c)
     dev_free(c->d_t_split); dev_free(c->d_t_esc); dev_free(c->d_t_leaf_esc); dev_free(c->d_t_flag);
     dev_free(c->d_t_lo); dev_free(c->d_t_hi); dev_free(c->d_t_wmax); dev_free(c->d_t_nodes);
     dev_free(c->d_dcache_idx); dev_free(c->d_dcache_n); dev_free(c->d_dcache_r2);
-    dev_free(c->d_brow); dev_free(c->d_bcount);
+    dev_free(c->d_brow); dev_free(c->d_bcount); dev_free(c->d_bwave); dev_free(c->d_bwave_n);
     c->dcache_all = false;
     c->tree = TreeArgs{};
     c->defer_result_cap = 0;
@@ -749,6 +750,8 @@ gzm_status upload_common(gzm_ctx *c, const gzm_host_view *v, int n_boxes, const 
             s = s ? s : dev_alloc(&c->d_dcache_r2, n);
             s = s ? s : dev_alloc(&c->d_brow, (size_t)n * kPairRowCap);
             s = s ? s : dev_alloc(&c->d_bcount, n);
+            s = s ? s : dev_alloc(&c->d_bwave, n);
+            s = s ? s : dev_alloc(&c->d_bwave_n, 1);
         }
         if (s) { free_particles(c); return s; }
     }
@@ -1243,6 +1246,8 @@ gzm_status gradients_impl(gzm_ctx *c, const Plan *gp, gzm_pass_stats *stats)
         r.dcache_r2 = c->d_dcache_r2;
         r.brow = c->d_brow;
         r.bcount = c->d_bcount;
+        r.wlist = c->d_bwave;
+        r.wcount = c->d_bwave_n;
         {
             Timed tm(c, kGradients);
             launch_sym_rows(c->stream, r);

@@ -186,6 +186,8 @@ struct SymRowArgs {
     const double *dcache_r2;
     int *brow;             // kPairRowCap per particle
     int *bcount;           // zeroed before the scatter
+    int *wlist;            // particles j whose own row does not cover h_j (walked by a wavefront each)
+    int *wcount;
 };
 // a sweep over at most this many particles runs one wavefront per particle:
 // they all fit on the GPU at once, and a walk shared by 64 lanes is shorter

@@ -495,10 +495,30 @@ __device__ __forceinline__ ForceRec make_force_rec(const double4 &p, const doubl
 
 constexpr int kNoSlot = 0x7fffffff;  // above every sorted slot: the end of a merged row
 
-// Symmetric pair rows, scatter: particle j offers itself to every particle i
-// with 0 < r_ij < h_j that lies outside i's candidate row radius, taking the
-// i from its own candidate row when that covers h_j, else from a walk.  Only
-// particles i whose row covers their own hsml take entries (the others walk).
+// Symmetric pair rows: particle j offers itself to particle i (0 < r_ij < h_j)
+// unless j is in i's candidate row (r_ij below the row's radius) or i's row
+// does not cover its own hsml (then i walks the tree anyway).
+__device__ __forceinline__ void sym_offer(const SymRowArgs &a, const BoxDesc &G, const double4 &pj, double hj2, int j,
+                                          int i)
+{
+    const double4 pi = a.posm[i];
+    // r_ij as the gradient kernel computes it for i (x_i - x_j)
+    double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+    wrap_dp(G.box, dp);
+    const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+    if (!(r2 < hj2) || r2 <= 0) return;
+    const double ri2 = a.dcache_r2[i];
+    if (r2 < ri2) return;
+    if (a.dcache_n[i] > kDensityCacheCap) return;
+    const double h_i = a.hsml[i];
+    if (!(h_i * h_i <= ri2)) return;
+    const int k = atomicAdd(&a.bcount[i], 1);
+    if (k < kPairRowCap) a.brow[(size_t)i * kPairRowCap + k] = j;
+}
+
+// Symmetric pair rows, scatter: each j takes the i from its own candidate row
+// when that covers h_j; the other j (rows past their cap, hsml past the row's
+// radius) are listed for sym_rows_wave_kernel.
 __global__ __launch_bounds__(kBlock) void sym_rows_scatter_kernel(SymRowArgs a)
 {
     const int j = blockIdx.x * kBlock + threadIdx.x;
@@ -508,28 +528,32 @@ __global__ __launch_bounds__(kBlock) void sym_rows_scatter_kernel(SymRowArgs a)
         const double h_j = a.hsml[j];
         if (!(h_j > 0)) return;
         const double hj2 = h_j * h_j;
-        auto offer = [&](int i) {
-            const double4 pi = a.posm[i];
-            // r_ij as the gradient kernel computes it for i (x_i - x_j)
-            double dp[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-            wrap_dp(G.box, dp);
-            const double r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-            if (!(r2 < hj2) || r2 <= 0) return;
-            if (a.dcache_n[i] > kDensityCacheCap) return;
-            const double ri2 = a.dcache_r2[i], h_i = a.hsml[i];
-            if (!(h_i * h_i <= ri2) || r2 < ri2) return;  // i walks, or j is in i's row
-            const int k = atomicAdd(&a.bcount[i], 1);
-            if (k < kPairRowCap) a.brow[(size_t)i * kPairRowCap + k] = j;
-        };
         const int n_j = a.dcache_n[j];
         if (n_j <= kDensityCacheCap && hj2 <= a.dcache_r2[j]) {
             const int *row = a.dcache_idx + (size_t)j * kDensityCacheCap;
-            for (int k = 0; k < n_j; k++) offer(row[k]);
+            for (int k = 0; k < n_j; k++) sym_offer(a, G, pj, hj2, j, row[k]);
         } else {
-            const double xs[3] = {pj.x, pj.y, pj.z};
-            tree_walk_grouped(a.tree, G, xs, h_j, [&](int i) { offer(i); });
+            a.wlist[atomicAdd(a.wcount, 1)] = j;
         }
     });
+}
+
+// The listed j, one wavefront each: its lanes share the tree walk of radius h_j
+// (wave_candidates).  A fixed grid strides over the device-side count.
+__global__ __launch_bounds__(kBlock) void sym_rows_wave_kernel(SymRowArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int n_waves = (int)(gridDim.x * (kBlock / 64));
+    const int n = *a.wcount;
+    for (int u = (int)((blockIdx.x * kBlock + threadIdx.x) >> 6); u < n; u += n_waves) {
+        const int j = a.wlist[u];
+        const BoxDesc &G = a.boxes[a.box_of[j]];
+        const double4 pj = a.posm[j];
+        const double h_j = a.hsml[j];
+        const double hj2 = h_j * h_j;
+        const double xs[3] = {pj.x, pj.y, pj.z};
+        wave_candidates(a.tree, G, xs, h_j, lane, [&](int i) { sym_offer(a, G, pj, hj2, j, i); });
+    }
 }
 
 // Symmetric pair rows, sort: each row ascending (a handful of entries)
@@ -1057,7 +1081,9 @@ void launch_sym_rows(hipStream_t s, const SymRowArgs &a)
 {
     if (a.n <= 0) return;
     launch_fill_int(s, a.bcount, a.n, 0);
+    launch_fill_int(s, a.wcount, 1, 0);
     hipLaunchKernelGGL(sym_rows_scatter_kernel, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(sym_rows_wave_kernel, dim3(2048), dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(sym_rows_sort_kernel, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a.n, a.brow, a.bcount);
 }
 void launch_gradients(hipStream_t s, const GradientArgs &a)
