@@ -323,6 +323,10 @@ __global__ __launch_bounds__(MAXB, 4) void density_sweep_kernel(DensitySweepArgs
                 });
                 a.dcache_n[i] = heavy ? cap + 1 : m;
                 a.dcache_r2[i] = Rc2;
+            } else if (row && budget) {
+                // a later sweep without a usable row: one wavefront for this
+                // particle's walk, so that it does not hold a wavefront of row scans
+                heavy = true;
             } else {
                 tree_walk_grouped(a.tree, G, xs, h, [&](int j) -> bool {
                     if (budget && ++visited > budget) {
