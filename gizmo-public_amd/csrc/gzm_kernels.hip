@@ -304,7 +304,8 @@ __global__ __launch_bounds__(MAXB, 4) void density_sweep_kernel(DensitySweepArgs
                     if (k + 3 < n_row) pair(q.w);
                 }
             } else if (row && a.hp.iter == 0) {
-                // first sweep: walk the cache radius, keep its candidates
+                // first sweep: walk the cache radius, keep its candidates (the
+                // candidate budget doubles with the wider radius)
                 const double Rc = kDensityCacheFactor * h, Rc2 = Rc * Rc;
                 int m = 0;
                 tree_walk_grouped(a.tree, G, xs, Rc, [&](int j) -> bool {
